@@ -1,0 +1,200 @@
+// oracle_capi.cpp -- extern "C" entry points of the CPU oracle (ctypes).
+//
+// TEST INFRASTRUCTURE: loaded only by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py.  See spray_oracle.h for what it restates.
+#include <omp.h>
+
+#include <cstring>
+#include <vector>
+
+#include "spray_oracle.h"
+
+namespace {
+
+template <class Mech, int NF>
+int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, const pelemp_geom* g,
+               int64_t n, double* pos, double* rdata, int64_t* id, const pelemp_fab_view* state,
+               pelemp_fab_view* source, double flow_dt, int isVirt, int isGhost, int do_move,
+               double spray_cfl, int nthreads, int64_t* n_out) {
+  using P = oracle::Particle<NF>;
+  constexpr int NSR = oracle::Comps<NF>::pstateNum;
+  std::vector<P> parts;
+  parts.reserve(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (id[i] <= 0) continue;
+    P p;
+    for (int d = 0; d < 3; ++d) p.pos[d] = pos[3 * i + d];
+    for (int c = 0; c < NSR; ++c) p.rdata[c] = rdata[NSR * i + c];
+    p.id = id[i];
+    parts.push_back(p);
+  }
+  pelemp_fab_view src = *source;
+  const int st = oracle::updateParticles<Mech, NF>(parts, *sd, *comps, *g, *state, src, flow_dt,
+                                                   isVirt != 0, isGhost != 0, do_move != 0,
+                                                   spray_cfl, nthreads);
+  int64_t m = 0;
+  for (const P& p : parts) {
+    if (p.id <= 0) continue;
+    for (int d = 0; d < 3; ++d) pos[3 * m + d] = p.pos[d];
+    for (int c = 0; c < NSR; ++c) rdata[NSR * m + c] = p.rdata[c];
+    id[m] = p.id;
+    ++m;
+  }
+  *n_out = m;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_num_threads(void) { return omp_get_max_threads(); }
+
+// One call of updateParticles on host arrays (see spray_oracle.h).
+int oracle_update(int mech, int nf, const pelemp_spray_data* sd, const pelemp_spray_comps* comps,
+                  const pelemp_geom* g, int64_t n, double* pos, double* rdata, int64_t* id,
+                  const pelemp_fab_view* state, pelemp_fab_view* source, double flow_dt,
+                  int isVirt, int isGhost, int do_move, double spray_cfl, int nthreads,
+                  int64_t* n_out) {
+  if (sd->nfuel != nf) return PELEMP_E_BAD_CONFIG;
+  if (sd->do_breakup != 0 || sd->do_splash != 0) return PELEMP_E_BAD_CONFIG;
+  if (mech == PELEMP_MECH_DODECANE_AIR && nf == 1)
+    return run_update<pelemp::Mech_dodecane_air, 1>(sd, comps, g, n, pos, rdata, id, state, source,
+                                                    flow_dt, isVirt, isGhost, do_move, spray_cfl,
+                                                    nthreads, n_out);
+  if (mech == PELEMP_MECH_DECANE_DODECANE_AIR && nf == 2)
+    return run_update<pelemp::Mech_decane_dodecane_air, 2>(sd, comps, g, n, pos, rdata, id, state,
+                                                           source, flow_dt, isVirt, isGhost,
+                                                           do_move, spray_cfl, nthreads, n_out);
+  if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
+    return run_update<pelemp::Mech_alkane6_air, 6>(sd, comps, g, n, pos, rdata, id, state, source,
+                                                   flow_dt, isVirt, isGhost, do_move, spray_cfl,
+                                                   nthreads, n_out);
+  return PELEMP_E_ARG;
+}
+
+// Host evaluation of the pm:: elementary functions (fn codes as in
+// pelemp_math_eval of include/pelemp_spray.h).
+void oracle_pm_eval(int fn, int64_t n, const double* x, const double* y, double* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const double a = x[i];
+    const double b = y ? y[i] : 0.0;
+    double r = 0.0;
+    switch (fn) {
+      case 0: r = pm::exp(a); break;
+      case 1: r = pm::log(a); break;
+      case 2: r = pm::log1p(a); break;
+      case 3: r = pm::pow(a, b); break;
+      case 4: r = pm::cbrt(a); break;
+      case 5: r = pm::sin(a); break;
+      case 6: r = pm::cos(a); break;
+      case 7: r = pm::atan2(a, b); break;
+      case 8: r = pm::acos(a); break;
+      case 9: r = std::sqrt(a); break;
+      case 10: r = a / b; break;
+      default: r = 0.0;
+    }
+    out[i] = r;
+  }
+}
+
+// Gas properties at (T, rho, Y): cp[NS], h[NS], rhoD[NS], mu, lambda.
+int oracle_gas_props(int mech, double T, double rho, const double* Y, double* cp, double* h,
+                     double* rhoD, double* mu, double* lam) {
+  auto run = [&](auto tag) {
+    using M = decltype(tag);
+    pelemp::GasProps<M>::T2Cpi(T, cp);
+    pelemp::GasProps<M>::T2Hi(T, h);
+    pelemp::GasProps<M>::transport(true, true, true, T, rho, Y, rhoD, *mu, *lam);
+    return 0;
+  };
+  switch (mech) {
+    case PELEMP_MECH_DODECANE_AIR: return run(pelemp::Mech_dodecane_air{});
+    case PELEMP_MECH_DECANE_DODECANE_AIR: return run(pelemp::Mech_decane_dodecane_air{});
+    case PELEMP_MECH_ALKANE6_AIR: return run(pelemp::Mech_alkane6_air{});
+    default: return PELEMP_E_ARG;
+  }
+}
+
+double oracle_calc_heat_coeff(double ratio, double B_M, double B_eps, double C_eps, double Nu_0) {
+  return oracle::calcHeatCoeff(ratio, B_M, B_eps, C_eps, Nu_0);
+}
+
+double oracle_psat(const pelemp_spray_data* sd, double T, int spf) { return oracle::psat(*sd, T, spf); }
+double oracle_rhoL(const pelemp_spray_data* sd, double T, int spf) { return oracle::rhoL(*sd, T, spf); }
+double oracle_muL(const pelemp_spray_data* sd, double T, int spf) { return oracle::muL(*sd, T, spf); }
+
+// calcBoilT for one fuel at pressure p with fuel molecular weight mw.
+double oracle_calc_boil_t(const pelemp_spray_data* sd, double p, double mw_fuel, int spf) {
+  oracle::GasPhaseVals<PELEMP_MAX_FUEL, 1> gpv{};
+  for (int n = 0; n < PELEMP_MAX_FUEL; ++n) gpv.mw[n] = mw_fuel;
+  gpv.p_fluid = p;
+  pelemp_spray_data one = *sd;
+  one.indx[0] = 0;
+  one.critT[0] = sd->critT[spf];
+  one.boilT[0] = sd->boilT[spf];
+  one.ref_latent[0] = sd->ref_latent[spf];
+  double cb = 0.0;
+  oracle::calcBoilT<PELEMP_MAX_FUEL, 1>(one, gpv, &cb);
+  return cb;
+}
+
+// trilinear_interp: ijk, lx (domain-relative), bflags -> idx[8][3], w[8]
+void oracle_trilinear(const int* ijk, const double* lx, const int* bflags, int* idx, double* w) {
+  int id8[8][3];
+  oracle::trilinear_interp(ijk, lx, id8, w, bflags);
+  for (int a = 0; a < 8; ++a)
+    for (int d = 0; d < 3; ++d) idx[3 * a + d] = id8[a][d];
+}
+
+// check_bounds for one position; returns left-domain flag, fills bflags.
+int oracle_check_bounds(const double* pos, const pelemp_geom* g, int* bflags) {
+  return oracle::check_bounds(pos, *g, bflags) ? 1 : 0;
+}
+
+// calculateSpraySource for one droplet in a prescribed gas state.
+// gas: vel[3], T, rho, Y[NS]; particle: rdata[NSR] in/out, id in/out.
+// out_src: mom[3], mass, Ydot[nf], eng (the GasPhaseVals sources).
+double oracle_spray_source_one(int mech, int nf, const pelemp_spray_data* sd, double flow_dt,
+                               const double* gas, double* rdata, int64_t* id, double* out_src,
+                               double* cboil_out) {
+  auto run = [&](auto tag, auto nft) -> double {
+    using M = decltype(tag);
+    constexpr int NF = decltype(nft)::value;
+    constexpr int NS = M::NS;
+    oracle::GasPhaseVals<NS, NF> gpv;
+    gpv.reset();
+    pelemp::GasProps<M>::molecular_weight(gpv.mw);
+    for (int d = 0; d < 3; ++d) gpv.vel_fluid[d] = gas[d];
+    gpv.T_fluid = gas[3];
+    gpv.rho_fluid = gas[4];
+    for (int n = 0; n < NS; ++n) gpv.Y_fluid[n] = gas[5 + n];
+    gpv.define();
+    double cB[NF];
+    oracle::calcBoilT<NS, NF>(*sd, gpv, cB);
+    oracle::Particle<NF> p;
+    for (int c = 0; c < oracle::Comps<NF>::pstateNum; ++c) p.rdata[c] = rdata[c];
+    p.id = *id;
+    for (int d = 0; d < 3; ++d) p.pos[d] = 0.0;
+    const double Re = oracle::calculateSpraySource<M, NF>(flow_dt, gpv, *sd, p, cB);
+    for (int c = 0; c < oracle::Comps<NF>::pstateNum; ++c) rdata[c] = p.rdata[c];
+    *id = p.id;
+    for (int d = 0; d < 3; ++d) out_src[d] = gpv.fluid_mom_src[d];
+    out_src[3] = gpv.fluid_mass_src;
+    for (int s = 0; s < NF; ++s) out_src[4 + s] = gpv.fluid_Y_dot[s];
+    out_src[4 + NF] = gpv.fluid_eng_src;
+    for (int s = 0; s < NF; ++s) cboil_out[s] = cB[s];
+    return Re;
+  };
+  if (mech == PELEMP_MECH_DODECANE_AIR && nf == 1)
+    return run(pelemp::Mech_dodecane_air{}, std::integral_constant<int, 1>{});
+  if (mech == PELEMP_MECH_DECANE_DODECANE_AIR && nf == 2)
+    return run(pelemp::Mech_decane_dodecane_air{}, std::integral_constant<int, 2>{});
+  if (mech == PELEMP_MECH_DECANE_DODECANE_AIR && nf == 1)
+    return run(pelemp::Mech_decane_dodecane_air{}, std::integral_constant<int, 1>{});
+  if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
+    return run(pelemp::Mech_alkane6_air{}, std::integral_constant<int, 6>{});
+  return -1.0;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Build with `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from pelemp_amd import _capi as C
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.oracle_update.restype = ctypes.c_int
+        L.oracle_update.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(C.SprayDataC),
+                                    ctypes.POINTER(C.SprayCompsC), ctypes.POINTER(C.GeomC), ctypes.c_int64, vp, vp,
+                                    vp, ctypes.POINTER(C.FabViewC), ctypes.POINTER(C.FabViewC), ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int64)]
+        L.oracle_pm_eval.restype = None
+        L.oracle_pm_eval.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp]
+        L.oracle_gas_props.restype = ctypes.c_int
+        L.oracle_gas_props.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_calc_heat_coeff.restype = ctypes.c_double
+        L.oracle_calc_heat_coeff.argtypes = [ctypes.c_double] * 5
+        for nm in ("oracle_psat", "oracle_rhoL", "oracle_muL"):
+            getattr(L, nm).restype = ctypes.c_double
+            getattr(L, nm).argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_int]
+        L.oracle_calc_boil_t.restype = ctypes.c_double
+        L.oracle_calc_boil_t.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int]
+        L.oracle_trilinear.restype = None
+        L.oracle_trilinear.argtypes = [vp, vp, vp, vp, vp]
+        L.oracle_check_bounds.restype = ctypes.c_int
+        L.oracle_check_bounds.argtypes = [vp, ctypes.POINTER(C.GeomC), vp]
+        L.oracle_spray_source_one.restype = ctypes.c_double
+        L.oracle_spray_source_one.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(C.SprayDataC),
+                                              ctypes.c_double, vp, vp, vp, vp, vp]
+        L.oracle_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def pm_eval(fn: int, x, y=None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    lib().oracle_pm_eval(fn, x.size, x.ctypes.data, None if yy is None else yy.ctypes.data, out.ctypes.data)
+    return out
+
+
+def gas_props(mech: int, T: float, rho: float, Y):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    ns = Y.size
+    cp, h, rhoD = np.empty(ns), np.empty(ns), np.empty(ns)
+    mu, lam = ctypes.c_double(), ctypes.c_double()
+    st = lib().oracle_gas_props(mech, T, rho, Y.ctypes.data, cp.ctypes.data, h.ctypes.data, rhoD.ctypes.data,
+                                ctypes.byref(mu), ctypes.byref(lam))
+    assert st == 0
+    return dict(cp=cp, h=h, rhoD=rhoD, mu=mu.value, lam=lam.value)
+
+
+def latent_shifted(sdata: dict, mech: int, ns: int) -> dict:
+    """SprayData after spraySetup's latent shift (SpraySetup.cpp:284-290)."""
+    d = dict(sdata)
+    h = gas_props(mech, sdata["ref_T"], 1.0, np.full(ns, 1.0 / ns))["h"]
+    d["latent"] = [sdata["ref_latent"][i] - h[sdata["indx"][i]] for i in range(sdata["nfuel"])]
+    return d
+
+
+def update(mech: int, sdata: dict, comps: dict, geom: dict, pos, rdata, ids, state: np.ndarray, state_lo,
+           state_hi, source: np.ndarray, src_lo, src_hi, flow_dt: float, do_move: bool = True,
+           spray_cfl: float = -1.0, is_virt: bool = False, is_ghost: bool = False, nthreads: int = 0):
+    """One updateParticles call on host arrays; `source` is updated in place.
+    Returns (status, pos, rdata, ids) with the surviving droplets in cell order."""
+    nf = int(sdata["nfuel"])
+    pos = np.array(pos, dtype=np.float64, order="C", copy=True)
+    rdata = np.array(rdata, dtype=np.float64, order="C", copy=True)
+    ids = np.array(ids, dtype=np.int64, order="C", copy=True)
+    n = ids.shape[0]
+    sd = C.spray_data_c(sdata)
+    cc = C.comps_c(comps)
+    gg = C.geom_c(geom)
+    assert state.flags["C_CONTIGUOUS"] and source.flags["C_CONTIGUOUS"]
+    sv = C.fab_view(state.ctypes.data, state_lo, state_hi, state.shape[0])
+    dv = C.fab_view(source.ctypes.data, src_lo, src_hi, source.shape[0])
+    nout = ctypes.c_int64()
+    if nthreads <= 0:
+        nthreads = os.cpu_count() or 1
+    st = lib().oracle_update(mech, nf, ctypes.byref(sd), ctypes.byref(cc), ctypes.byref(gg), n, pos.ctypes.data,
+                             rdata.ctypes.data, ids.ctypes.data, ctypes.byref(sv), ctypes.byref(dv), flow_dt,
+                             int(is_virt), int(is_ghost), int(do_move), float(spray_cfl), int(nthreads),
+                             ctypes.byref(nout))
+    m = nout.value
+    return st, pos[:m].copy(), rdata[:m].copy(), ids[:m].copy()
+
+
+def calc_heat_coeff(ratio, B_M, B_eps=1e-7, C_eps=1e-15, Nu_0=2.0) -> float:
+    return lib().oracle_calc_heat_coeff(ratio, B_M, B_eps, C_eps, Nu_0)
+
+
+def spray_source_one(mech: int, sdata: dict, flow_dt: float, gas, rdata, pid: int = 1):
+    """calculateSpraySource for one droplet in the gas state
+    gas = [u, v, w, T, rho, Y_0..Y_NS-1].  Returns (Re, rdata_out, id_out,
+    sources [mom3, mass, Ydot(nf), eng], cBoilT)."""
+    nf = int(sdata["nfuel"])
+    sd = C.spray_data_c(sdata)
+    gas = np.ascontiguousarray(gas, dtype=np.float64)
+    rd = np.array(rdata, dtype=np.float64, copy=True)
+    pid_c = np.array([pid], dtype=np.int64)
+    src = np.zeros(5 + nf)
+    cb = np.zeros(nf)
+    Re = lib().oracle_spray_source_one(mech, nf, ctypes.byref(sd), flow_dt, gas.ctypes.data, rd.ctypes.data,
+                                       pid_c.ctypes.data, src.ctypes.data, cb.ctypes.data)
+    return Re, rd, int(pid_c[0]), src, cb

@@ -1,0 +1,779 @@
+// spray_oracle.h -- CPU restatement of PeleMP's Lagrangian spray update.
+//
+// TEST INFRASTRUCTURE.  This is the parity oracle for the HIP kernels of
+// pelemp_amd: only tests/, __graft_entry__.smoke() and the cpu_baseline leg
+// of bench.py load it, always as the checker, never as the thing measured.
+//
+// It restates, in the reference's own structure (AoS particles with
+// rdata[SprayComps], one function per reference function, the reference's
+// operation order), the FP64 method of
+//   Source/PP_Spray/SprayFuelData.H:8-212   (SprayData fits, GasPhaseVals)
+//   Source/PP_Spray/Drag.H:4-441            (calcHeatCoeff, calcVaporState,
+//                                            calculateSpraySource)
+//   Source/PP_Spray/SprayInterpolation.H:64-199 (check_bounds,
+//                                            InterpolateGasPhase, trilinear)
+//   Source/PP_Spray/SprayParticles.cpp:142-502  (updateParticles driver)
+// in the PeleLM/PeleLMeX flavour (PELELM_USE_SPRAY: the state holds velocity
+// and temperature directly, no kinetic-energy terms in the energy source).
+// Units are SI throughout, so SprayUnits factors are 1 and min_mass =
+// 1e-16 kg (SprayFuelData.H:66 with mass_conv = 1e-3).
+//
+// Gas properties (molecular weights, cp, h, transport) come from
+// pelemp_amd/csrc/gas_props.h and elementary functions from pm_math.h:
+// these play the role PelePhysics and libm play for the reference, and are
+// pinned by their own tests.
+//
+// Deviations from the reference (documented; each is forced by the
+// reference's behaviour being undefined or unordered):
+//  * Order: at the start of every particle sub-cycle the droplets are stably
+//    sorted by the cell that contains them (tile-major cell key, see
+//    cell_key below); each cell's source contributions are summed in that
+//    order (the reference adds them with unordered FP64 atomics).  Sub-
+//    cycles (num_iter > 1) are done as separate passes over all droplets.
+//  * static_cast<int> of a huge substep ratio (UB in the reference,
+//    Drag.H:290,355) saturates at 1e9 before the +1 / min(nSubMax).
+//  * The periodic wrap that the host solver's Redistribute performs after
+//    the update is done at the end of each pass; the containing cell index
+//    is clamped into the domain.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../include/pelemp_spray.h"
+#include "../pelemp_amd/csrc/gas_props.h"
+#include "../pelemp_amd/csrc/pm_math.h"
+
+namespace oracle {
+
+using pelemp::GasProps;
+constexpr double RU = pelemp::kRU;
+constexpr double PATM = pelemp::kPATM;
+constexpr double kMinMass = 1.0e-16;  // SprayUnits::min_mass in SI
+constexpr double kPI = 3.14159265358979323846;
+
+// SprayComps particle indices (SprayFuelData.H:10-19), 3D
+template <int NF>
+struct Comps {
+  static constexpr int pstateVel = 0;
+  static constexpr int pstateT = 3;
+  static constexpr int pstateDia = 4;
+  static constexpr int pstateY = 5;
+  static constexpr int pstateNumDens = 5 + NF;
+  static constexpr int pstateN0 = 6 + NF;
+  static constexpr int pstateBM1 = 7 + NF;
+  static constexpr int pstateBM2 = 8 + NF;
+  static constexpr int pstateFilmHght = 9 + NF;
+  static constexpr int pstateNum = 10 + NF;
+};
+
+template <int NF>
+struct Particle {
+  double pos[3];
+  double rdata[Comps<NF>::pstateNum];
+  int64_t id;
+};
+
+inline int sat_int(double x) {
+  // static_cast<int> with saturation (see header note)
+  if (!(x == x)) return 0;
+  if (x > 1.0e9) return 1000000000;
+  if (x < -1.0e9) return -1000000000;
+  return static_cast<int>(x);
+}
+
+// ---------------------------------------------------------------------------
+// SprayData fits (SprayFuelData.H:146-211)
+inline double rhoL(const pelemp_spray_data& f, double T, int spf) {
+  const double a = f.rho_coef[4 * spf], b = f.rho_coef[4 * spf + 1];
+  const double c = f.rho_coef[4 * spf + 2], d = f.rho_coef[4 * spf + 3];
+  return a + T * (b + T * (c + T * d));
+}
+inline double lambdaL(const pelemp_spray_data& f, double T, int spf) {
+  const double a = f.lambda_coef[4 * spf], b = f.lambda_coef[4 * spf + 1];
+  const double c = f.lambda_coef[4 * spf + 2], d = f.lambda_coef[4 * spf + 3];
+  return a + T * (b + T * (c + T * d));
+}
+inline double muL(const pelemp_spray_data& f, double T, int spf) {
+  const double a = f.mu_coef[4 * spf], b = f.mu_coef[4 * spf + 1];
+  const double c = f.mu_coef[4 * spf + 2], d = f.mu_coef[4 * spf + 3];
+  return a + ((d / T + c) / T + b) / T;
+}
+inline double psat(const pelemp_spray_data& f, double T, int spf) {
+  const double a = f.psat_coef[4 * spf], b = f.psat_coef[4 * spf + 1];
+  const double c = f.psat_coef[4 * spf + 2], d = f.psat_coef[4 * spf + 3];
+  return d * pm::pow(10., a - b / (T + c));
+}
+
+// Interpolated gas state (GasPhaseVals, SprayFuelData.H:71-112)
+template <int NS, int NF>
+struct GasPhaseVals {
+  double vel_fluid[3];
+  double T_fluid;
+  double rho_fluid;
+  double p_fluid;
+  double Y_fluid[NS];
+  double mw_mix;
+  double mw[NS];
+  double fluid_mom_src[3];
+  double fluid_mass_src;
+  double fluid_Y_dot[NF];
+  double fluid_eng_src;
+
+  void reset() {
+    for (int d = 0; d < 3; ++d) {
+      fluid_mom_src[d] = 0.;
+      vel_fluid[d] = 0.;
+    }
+    T_fluid = 0.;
+    rho_fluid = 0.;
+    fluid_eng_src = 0.;
+    fluid_mass_src = 0.;
+    for (int n = 0; n < NS; ++n) Y_fluid[n] = 0.;
+    for (int n = 0; n < NF; ++n) fluid_Y_dot[n] = 0.;
+  }
+  void define() {
+    mw_mix = 0.;
+    for (int n = 0; n < NS; ++n) mw_mix += Y_fluid[n] / mw[n];
+    p_fluid = rho_fluid * RU * mw_mix * T_fluid;
+    mw_mix = 1. / mw_mix;
+  }
+};
+
+// SprayData::calcBoilT (SprayFuelData.H:180-200)
+template <int NS, int NF>
+void calcBoilT(const pelemp_spray_data& f, const GasPhaseVals<NS, NF>& gpv, double* cBoilT) {
+  for (int spf = 0; spf < NF; ++spf) {
+    const int fspec = f.indx[spf];
+    const double mw_fuel = gpv.mw[fspec];
+    const double Hboil_ref =
+        f.ref_latent[spf] * pm::pow((f.critT[spf] - f.ref_T) / (f.critT[spf] - f.boilT[spf]), -0.38);
+    cBoilT[spf] = 1. / (pm::log(PATM / gpv.p_fluid) * RU / (Hboil_ref * mw_fuel) + 1. / f.boilT[spf]);
+    cBoilT[spf] = std::min(f.critT[spf], cBoilT[spf]);
+  }
+}
+
+// calcHeatCoeff (Drag.H:8-45)
+inline double calcHeatCoeff(double ratio, double B_M, double B_eps, double C_eps, double Nu_0) {
+  if (B_M <= C_eps) return 0.;
+  const int maxIter = 100;
+  const double NU2 = Nu_0 - 2.;
+  const double BM1 = 1. + B_M;
+  double phi = ratio / Nu_0;
+  double B_T_old = pm::pow(BM1, phi) - 1.;
+  double logB = pm::log1p(B_T_old);
+  double invFT = B_T_old / (logB * pm::pow(1. + B_T_old, 0.7));
+  double Nu_num = 2. + NU2 * invFT;
+  phi = ratio / Nu_num;
+  double B_T = pm::pow(BM1, phi) - 1.;
+  double error = std::fabs(B_T - B_T_old);
+  int k = 0;
+  while (k < maxIter && error > B_eps) {
+    B_T_old = B_T;
+    logB = pm::log1p(B_T);
+    invFT = B_T / (logB * pm::pow(1. + B_T, 0.7));
+    Nu_num = 2. + NU2 * invFT;
+    phi = ratio / Nu_num;
+    B_T = pm::pow(BM1, phi) - 1.;
+    error = std::fabs(B_T - B_T_old);
+    k++;
+  }
+  logB = pm::log1p(B_T);
+  invFT = B_T / (logB * pm::pow(1. + B_T, 0.7));
+  Nu_num = 2. + NU2 * invFT;
+  return Nu_num * logB / B_T;
+}
+
+// calcVaporState (Drag.H:47-135)
+template <int NS, int NF>
+void calcVaporState(const pelemp_spray_data& fdat, const GasPhaseVals<NS, NF>& gpv, double rule,
+                    double T_in, double C_eps, double mw_part, const double* Y_l,
+                    const double* h_part, const double* cp_n, const double* cBoilT, double* Y_skin,
+                    double* X_vapor, double* L_fuel, double& B_M, double& sumXVap,
+                    double& cp_skin, double& mw_skin) {
+  double mbar_vap = 0.;
+  for (int spf = 0; spf < NF; ++spf) {
+    const int fspec = fdat.indx[spf];
+    const int fdspec = fdat.dep_indx[spf];
+    const double boilT_ref = fdat.boilT[spf];
+    const double mw_fuel = gpv.mw[fspec];
+    const double T_part = std::min(T_in, 0.99 * cBoilT[spf]);
+    const double part_latent = h_part[fspec] + fdat.latent[spf] - fdat.cp[spf] * (T_part - fdat.ref_T);
+    L_fuel[spf] = part_latent;
+    double pres_sat = 0.;
+    if (fdat.psat_coef[4 * spf + 3] == 0.) {
+      pres_sat = PATM * pm::exp(part_latent * mw_fuel / RU * (1. / boilT_ref - 1. / T_part));
+    } else {
+      pres_sat = psat(fdat, T_part, spf);
+    }
+    const double X_fluid = gpv.Y_fluid[fdspec] * gpv.mw_mix / mw_fuel;
+    if (pres_sat > X_fluid * gpv.p_fluid) {
+      const double Xl = Y_l[spf] * mw_part / mw_fuel;
+      const double Xv = Xl * pres_sat / gpv.p_fluid;
+      mbar_vap += Xv * mw_fuel;
+      sumXVap += Xv;
+      X_vapor[spf] = Xv;
+    } else {
+      X_vapor[spf] = 0.;
+    }
+  }
+  const double totalmwx = mbar_vap + (1. - sumXVap) * gpv.mw_mix;
+  double sumYSkin = 0.;
+  double sumYfFluid = 0.;
+  double sumYVap = 0.;
+  for (int spf = 0; spf < NF; ++spf) {
+    if (X_vapor[spf] > 0.) {
+      const int fspec = fdat.indx[spf];
+      const int fdspec = fdat.dep_indx[spf];
+      const double mw_fuel = gpv.mw[fspec];
+      const double Yfv = X_vapor[spf] * mw_fuel / totalmwx;
+      sumYVap += Yfv;
+      const double gasY = gpv.Y_fluid[fdspec];
+      sumYfFluid += gasY;
+      const double Ysk = Yfv + rule * (gasY - Yfv);
+      Y_skin[fspec] = Ysk;
+      sumYSkin += Ysk;
+    }
+  }
+  const double renorm = (1. - sumYSkin) / (1. - sumYfFluid);
+  for (int n = 0; n < NS; ++n) {
+    if (Y_skin[n] == 0.) Y_skin[n] = gpv.Y_fluid[n] * renorm;
+    cp_skin += Y_skin[n] * cp_n[n];
+    mw_skin += Y_skin[n] / gpv.mw[n];
+  }
+  mw_skin = 1. / mw_skin;
+  B_M = (sumYVap - sumYfFluid) / std::max(C_eps, (1. - sumYVap));
+  B_M = std::min(20., B_M);
+}
+
+// calculateSpraySource (Drag.H:137-441), PELELM_USE_SPRAY flavour.
+template <class Mech, int NF>
+double calculateSpraySource(double flow_dt, GasPhaseVals<Mech::NS, NF>& gpv,
+                            const pelemp_spray_data& fdat, Particle<NF>& p, const double* cBoilT) {
+  constexpr int NS = Mech::NS;
+  using C = Comps<NF>;
+  const double rule = 1. / 3.;
+  const double C_eps = 1.E-15;
+  const double B_eps = 1.E-7;
+  const double min_mass = kMinMass;
+  const int nSubMax = 100;
+  const bool get_Ddiag = fdat.mass_trans != 0;
+  const bool get_lambda = fdat.mass_trans != 0;
+  double Y_skin[NS], h_part[NS], cp_n[NS], Ddiag[NS];
+  double L_fuel[NF] = {0.0}, mi_dot[NF] = {0.0}, Y_part[NF], X_vapor[NF] = {0.0};
+  for (int n = 0; n < NS; ++n) Ddiag[n] = 0.;
+  const double num_ppp = p.rdata[C::pstateNumDens];
+  double vel_part[3] = {p.rdata[C::pstateVel], p.rdata[C::pstateVel + 1], p.rdata[C::pstateVel + 2]};
+  double y_tab = 0.;
+  if (fdat.do_breakup == 1) y_tab = p.rdata[C::pstateBM1];
+  if (fdat.fixed_parts) vel_part[0] = vel_part[1] = vel_part[2] = 0.;
+  double T_part = p.rdata[C::pstateT];
+  double dia_part = p.rdata[C::pstateDia];
+  double rho_part = 0.;
+  for (int spf = 0; spf < NF; ++spf) {
+    Y_part[spf] = p.rdata[C::pstateY + spf];
+    rho_part += Y_part[spf] / rhoL(fdat, std::min(T_part, cBoilT[spf]), spf);
+  }
+  rho_part = 1. / rho_part;
+  double dt = flow_dt;
+  int isub = 1;
+  int nsub = 1;
+  double pmass = kPI / 6. * rho_part * (dia_part * dia_part * dia_part);
+  const double startmass = pmass;
+  double Reyn = 0.;
+  double part_vel_src[3];
+  while (isub <= nsub) {
+    double cp_part = 0., Tboil = 0., mw_part = 0.;
+    for (int spf = 0; spf < NF; ++spf) {
+      Tboil += Y_part[spf] * cBoilT[spf];
+      cp_part += Y_part[spf] * fdat.cp[spf];
+      mw_part += Y_part[spf] / gpv.mw[fdat.indx[spf]];
+    }
+    mw_part = 1. / mw_part;
+    const double delT = std::max(gpv.T_fluid - T_part, 0.);
+    double T_skin = gpv.T_fluid;
+    if (fdat.mass_trans) T_skin = T_part + rule * delT;
+    GasProps<Mech>::T2Cpi(T_skin, cp_n);
+    GasProps<Mech>::T2Hi(T_part, h_part);
+    for (int n = 0; n < NS; ++n) Y_skin[n] = 0.;
+    double cp_skin = 0., mw_skin = 0., B_M = 0., sumXVap = 0.;
+    if (fdat.mass_trans) {
+      calcVaporState<NS, NF>(fdat, gpv, rule, T_part, C_eps, mw_part, Y_part, h_part, cp_n, cBoilT,
+                             Y_skin, X_vapor, L_fuel, B_M, sumXVap, cp_skin, mw_skin);
+    } else {
+      for (int n = 0; n < NS; ++n) {
+        Y_skin[n] = gpv.Y_fluid[n];
+        cp_skin += gpv.Y_fluid[n] * cp_n[n];
+      }
+      mw_skin = gpv.mw_mix;
+    }
+    double lambda_skin = 0., mu_skin = 0.;
+    double rho_skin = gpv.rho_fluid;
+    if (fdat.mass_trans) rho_skin = mw_skin * gpv.p_fluid / (RU * T_skin);
+    GasProps<Mech>::transport(true, get_lambda, get_Ddiag, T_skin, rho_skin, Y_skin, Ddiag, mu_skin,
+                              lambda_skin);
+    double diff_vel[3];
+    for (int d = 0; d < 3; ++d) diff_vel[d] = gpv.vel_fluid[d] - vel_part[d];
+    const double diff_vel_mag =
+        std::sqrt(diff_vel[0] * diff_vel[0] + diff_vel[1] * diff_vel[1] + diff_vel[2] * diff_vel[2]);
+    Reyn = rho_skin * diff_vel_mag * dia_part / mu_skin;
+    const double inv_pmass = 1. / pmass;
+    double drag_coef = 0.;
+    if (Reyn > 1000.) {
+      drag_coef = 0.424;
+    } else if (Reyn > 1.) {
+      drag_coef = 24. / Reyn * (1. + pm::cbrt(Reyn * Reyn) / 6.);
+    } else if (Reyn > 0.) {
+      drag_coef = 24. / Reyn;
+    }
+    drag_coef *= (1. + 2.6432 * y_tab);
+    const double drag_force = 0.125 * rho_skin * drag_coef * kPI * dia_part * dia_part * diff_vel_mag;
+    const double drag_force_p = 0.75 * rho_skin * drag_coef * diff_vel_mag / (dia_part * rho_part);
+    for (int d = 0; d < 3; ++d) part_vel_src[d] = drag_force_p * diff_vel[d] + fdat.body_force[d];
+    if (fdat.mom_trans) {
+      const double fm = num_ppp * drag_force;
+      for (int d = 0; d < 3; ++d) gpv.fluid_mom_src[d] += fm * diff_vel[d];
+      if (isub == 1) {
+        const double inv_tau_var = drag_force_p;
+        nsub = std::min(std::max(nsub, sat_int(flow_dt * inv_tau_var) + 1), nSubMax);
+      }
+    }
+    double part_temp_src = 0.;
+    double m_dot = 0.;
+    if (fdat.mass_trans) {
+      const double Pr_skin = mu_skin * cp_skin / lambda_skin;
+      const double powR = std::max(pm::pow(std::min(400., Reyn), 0.077), 1.);
+      double Nu_num = 0.;
+      double sumL = 0.;
+      const double Nu_0 = 1. + powR * pm::cbrt(1. + Reyn * Pr_skin);
+      if (sumXVap > 0.) {
+        double rhoDtotal = 0., cp_fuel = 0., sumYSkin = 0.;
+        for (int spf = 0; spf < NF; ++spf) {
+          const int fspec = fdat.indx[spf];
+          Ddiag[fspec] *= mw_skin / gpv.mw[fspec];
+          const double X_norm = X_vapor[spf] / sumXVap;
+          Ddiag[fspec] *= X_norm;
+          rhoDtotal += Ddiag[fspec];
+          cp_fuel += Y_skin[fspec] * cp_n[fspec];
+          sumYSkin += Y_skin[fspec];
+        }
+        cp_fuel /= sumYSkin;
+        const double Sc_skin = mu_skin / rhoDtotal;
+        const double logB = pm::log1p(B_M);
+        const double invFM = B_M / (logB * pm::pow(1. + B_M, 0.7));
+        const double Sh_0 = 1. + powR * pm::cbrt(1. + Reyn * Sc_skin);
+        const double Sh_num = 2. + (Sh_0 - 2.) * invFM;
+        const double mdotcoeff = kPI * dia_part * Sh_num * logB;
+        const double ratio = cp_fuel * Sh_num * rhoDtotal / lambda_skin;
+        Nu_num = calcHeatCoeff(ratio, B_M, B_eps, C_eps, Nu_0);
+        for (int spf = 0; spf < NF; ++spf) {
+          const int fspec = fdat.indx[spf];
+          if (X_vapor[spf] > 0.) {
+            mi_dot[spf] = -std::max(Ddiag[fspec] * mdotcoeff, 0.);
+            m_dot += mi_dot[spf];
+            sumL += mi_dot[spf] * L_fuel[spf];
+          }
+        }
+      } else {
+        Nu_num = Nu_0;
+      }
+      const double conv_src = kPI * lambda_skin * dia_part * delT * Nu_num;
+      gpv.fluid_eng_src += num_ppp * conv_src;
+      part_temp_src = (sumL + conv_src) * inv_pmass / cp_part;
+      if (isub == 1 && delT > C_eps) {
+        const double inv_tau_d = -m_dot / (0.2 * pmass);
+        const double inv_tau_T = conv_src * inv_pmass / (cp_part * delT);
+        nsub = std::min(
+            std::max(nsub, std::max(sat_int(flow_dt * inv_tau_d), sat_int(flow_dt * inv_tau_T)) + 1),
+            nSubMax);
+      }
+    }
+    if (isub == 1) dt = flow_dt / static_cast<double>(nsub);
+    const double part_dt = fdat.dtmod * dt;
+    if (!fdat.fixed_parts) {
+      for (int d = 0; d < 3; ++d) vel_part[d] += part_dt * part_vel_src[d];
+    }
+    if (fdat.mass_trans) {
+      const double newT = T_part + part_dt * part_temp_src;
+      const double new_mass = pmass + m_dot * part_dt;
+      if (new_mass > min_mass && newT < Tboil) {
+        T_part = newT;
+        if (NF > 1) {
+          rho_part = 0.;
+          double sumY = 0.;
+          for (int spf = 0; spf < NF; ++spf) {
+            Y_part[spf] = std::min(1., std::max(0., (Y_part[spf] * pmass + mi_dot[spf] * part_dt) / new_mass));
+            if (Y_part[spf] <= 1.E-12) Y_part[spf] = 0.;
+            sumY += Y_part[spf];
+          }
+          for (int spf = 0; spf < NF; ++spf) {
+            Y_part[spf] /= sumY;
+            rho_part += Y_part[spf] / rhoL(fdat, std::min(T_part, cBoilT[spf]), spf);
+          }
+          rho_part = 1. / rho_part;
+        } else {
+          rho_part = rhoL(fdat, std::min(T_part, cBoilT[0]), 0);
+        }
+        pmass = new_mass;
+        dia_part = pm::cbrt(6. * pmass / (kPI * rho_part));
+      } else {
+        pmass = 0.;
+        p.id = -1;
+        nsub = isub;
+        isub = nsub + 1;
+      }
+    }
+    ++isub;
+  }
+  if (nsub > 1) {
+    gpv.fluid_eng_src /= static_cast<double>(nsub);
+    for (int d = 0; d < 3; ++d) gpv.fluid_mom_src[d] /= static_cast<double>(nsub);
+  }
+  const double mdot_total = (pmass - startmass) / (fdat.dtmod * flow_dt);
+  gpv.fluid_mass_src = num_ppp * mdot_total;
+  if (fdat.mom_trans) {
+    for (int d = 0; d < 3; ++d) gpv.fluid_mom_src[d] += num_ppp * vel_part[d] * mdot_total;
+  }
+  for (int spf = 0; spf < NF; ++spf) {
+    const int fdspec = fdat.dep_indx[spf];
+    const double oldY = p.rdata[C::pstateY + spf];
+    const double newY = Y_part[spf];
+    const double midot = (newY * pmass - oldY * startmass) / (fdat.dtmod * flow_dt);
+    gpv.fluid_Y_dot[spf] = num_ppp * midot;
+    gpv.fluid_eng_src += num_ppp * midot * h_part[fdspec];
+    p.rdata[C::pstateY + spf] = Y_part[spf];
+  }
+  for (int d = 0; d < 3; ++d) p.rdata[C::pstateVel + d] = vel_part[d];
+  p.rdata[C::pstateT] = T_part;
+  p.rdata[C::pstateDia] = dia_part;
+  return Reyn;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation (SprayInterpolation.H:64-199)
+
+// check_bounds: flags per direction 0 interior/periodic, -1/+1 outside a
+// lower/upper reflective face, -2/+2 within half a cell of it; returns true
+// when the particle is outside a non-reflective face.
+inline bool check_bounds(const double pos[3], const pelemp_geom& g, int bflags[3]) {
+  for (int hilo = 0; hilo < 2; ++hilo) {
+    const int fact = (hilo == 0) ? 1 : -1;
+    for (int dir = 0; dir < 3; ++dir) {
+      const double dom_loc = (hilo == 0) ? g.plo[dir] : g.phi[dir];
+      const int bflag = (hilo == 0) ? g.bndry_lo[dir] : g.bndry_hi[dir];
+      if (bflag != 0) {
+        const double diff = static_cast<double>(fact) * (pos[dir] - dom_loc);
+        if (diff < 0.) {
+          if (bflag == 1) {
+            bflags[dir] = -fact;
+          } else {
+            return true;
+          }
+        } else if (diff < 0.5 * g.dx[dir]) {
+          bflags[dir] = -2 * fact;
+        }
+      }
+    }
+  }
+  return false;
+}
+
+inline void trilinear_interp(const int ijk[3], const double lx[3], int idx[8][3], double w[8],
+                             const int bflags[3]) {
+  double ssv[2][3];
+  for (int dir = 0; dir < 3; ++dir) {
+    const double sx_hi = lx[dir] - static_cast<double>(ijk[dir]);
+    const double sx_lo = 1. - sx_hi;
+    if (bflags[dir] > 1) {
+      ssv[0][dir] = 1.;
+      ssv[1][dir] = 0.;
+    } else if (bflags[dir] < -1) {
+      ssv[0][dir] = 0.;
+      ssv[1][dir] = 1.;
+    } else {
+      ssv[0][dir] = sx_lo;
+      ssv[1][dir] = sx_hi;
+    }
+  }
+  int cc = 0;
+  for (int kk = -1; kk < 1; kk++) {
+    for (int jj = -1; jj < 1; jj++) {
+      for (int ii = -1; ii < 1; ii++) {
+        idx[cc][0] = ijk[0] + ii;
+        idx[cc][1] = ijk[1] + jj;
+        idx[cc][2] = ijk[2] + kk;
+        w[cc] = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
+        cc++;
+      }
+    }
+  }
+}
+
+// Read-only view of a host FAB
+struct Fab {
+  const double* data;
+  int lo[3], hi[3], n[3];
+  int ncomp;
+  explicit Fab(const pelemp_fab_view& v) : data(v.data), ncomp(v.ncomp) {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = v.lo[d];
+      hi[d] = v.hi[d];
+      n[d] = v.hi[d] - v.lo[d] + 1;
+    }
+  }
+  bool contains(const int c[3]) const {
+    return c[0] >= lo[0] && c[0] <= hi[0] && c[1] >= lo[1] && c[1] <= hi[1] && c[2] >= lo[2] &&
+           c[2] <= hi[2];
+  }
+  int64_t off(const int c[3], int comp) const {
+    return ((static_cast<int64_t>(comp) * n[2] + (c[2] - lo[2])) * n[1] + (c[1] - lo[1])) * n[0] +
+           (c[0] - lo[0]);
+  }
+  double operator()(const int c[3], int comp) const { return data[off(c, comp)]; }
+};
+
+// InterpolateGasPhase, PELELM_USE_SPRAY flavour (SprayInterpolation.H:99-150)
+template <int NS, int NF>
+bool InterpolateGasPhase(GasPhaseVals<NS, NF>& gpv, const Fab& st, const pelemp_spray_comps& SPI,
+                         const int idx[8][3], const double w[8]) {
+  for (int a = 0; a < 8; ++a) {
+    const double cw = w[a];
+    if (cw > 0.) {
+      const int* c = idx[a];
+      if (!st.contains(c)) return false;
+      const double cur_rho = st(c, SPI.rhoIndx);
+      gpv.rho_fluid += cw * cur_rho;
+      const double inv_rho = 1. / cur_rho;
+      for (int n = 0; n < NS; ++n) {
+        const double cur_mf = st(c, SPI.specIndx + n) * inv_rho;
+        gpv.Y_fluid[n] += cw * cur_mf;
+      }
+      for (int dir = 0; dir < 3; ++dir) {
+        const double vel = st(c, SPI.momIndx + dir);
+        gpv.vel_fluid[dir] += cw * vel;
+      }
+      const double T_i = st(c, SPI.utempIndx);
+      gpv.T_fluid += cw * T_i;
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Cell ordering shared with the GPU path: cells are grouped in 4x4x4 tiles;
+// key = tile_linear * 64 + (lz*4 + ly)*4 + lx, tiles in x-fastest order.
+constexpr int kTile = 4;
+inline int64_t cell_key(const int c[3], const pelemp_geom& g) {
+  const int nx = g.domain_hi[0] - g.domain_lo[0] + 1;
+  const int ny = g.domain_hi[1] - g.domain_lo[1] + 1;
+  const int i = c[0] - g.domain_lo[0], j = c[1] - g.domain_lo[1], k = c[2] - g.domain_lo[2];
+  const int ntx = nx / kTile, nty = ny / kTile;
+  const int64_t tile = (static_cast<int64_t>(k / kTile) * nty + (j / kTile)) * ntx + (i / kTile);
+  return tile * (kTile * kTile * kTile) + ((k % kTile) * kTile + (j % kTile)) * kTile + (i % kTile);
+}
+
+// cell containing pos, clamped into the domain
+inline void containing_cell(const double pos[3], const pelemp_geom& g, const double dxi[3], int c[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const double lxc = (pos[d] - g.plo[d]) * dxi[d];
+    int ic = static_cast<int>(std::floor(lxc)) + g.domain_lo[d];
+    ic = std::min(std::max(ic, g.domain_lo[d]), g.domain_hi[d]);
+    c[d] = ic;
+  }
+}
+
+// Periodic wrap standing in for the host solver's Redistribute.
+inline void periodic_wrap(double pos[3], const pelemp_geom& g) {
+  for (int d = 0; d < 3; ++d) {
+    if (g.bndry_lo[d] == 0) {
+      const double L = g.phi[d] - g.plo[d];
+      if (pos[d] >= g.phi[d]) {
+        pos[d] -= L;
+      } else if (pos[d] < g.plo[d]) {
+        pos[d] += L;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// updateParticles driver (SprayParticles.cpp:142-502), drag + evaporation
+// path (no film, breakup or splash; those are rejected by the caller).
+template <class Mech, int NF>
+int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& fdat,
+                    const pelemp_spray_comps& SPI, const pelemp_geom& g,
+                    const pelemp_fab_view& state_v, pelemp_fab_view& source_v, double flow_dt,
+                    bool isVirt, bool isGhost, bool do_move, double spray_cfl_lev, int nthreads) {
+  constexpr int NS = Mech::NS;
+  using C = Comps<NF>;
+  (void)isVirt;
+  const double dxi[3] = {1. / g.dx[0], 1. / g.dx[1], 1. / g.dx[2]};
+  const double vol = g.dx[0] * g.dx[1] * g.dx[2];
+  const double inv_vol = 1. / vol;
+  bool at_bounds = false;
+  for (int d = 0; d < 3; ++d) at_bounds = at_bounds || g.bndry_lo[d] != 0 || g.bndry_hi[d] != 0;
+  const double sub_cfl = 0.5;
+  double sub_dt = flow_dt;
+  int num_iter = 1;
+  if (do_move && spray_cfl_lev > sub_cfl) {
+    num_iter = static_cast<int>(std::ceil(spray_cfl_lev / sub_cfl));
+    sub_dt = flow_dt / static_cast<double>(num_iter);
+  }
+  const Fab st(state_v);
+  Fab srcv(source_v);
+  constexpr int NSRC = 5 + NF;  // mom(3) mass Y(nf) eng, contribution slots
+  std::vector<int> bfl(parts.size() * 3, 0);
+  int status = PELEMP_OK;
+  for (int cur_iter = 0; cur_iter < num_iter; ++cur_iter) {
+    // order: active droplets stably sorted by containing cell
+    const size_t np = parts.size();
+    std::vector<int64_t> key(np);
+    for (size_t i = 0; i < np; ++i) {
+      int c[3];
+      containing_cell(parts[i].pos, g, dxi, c);
+      key[i] = (parts[i].id > 0) ? cell_key(c, g) : INT64_MAX;
+    }
+    std::vector<int64_t> order(np);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    {
+      std::vector<Particle<NF>> sorted;
+      std::vector<int> sbf;
+      sorted.reserve(np);
+      sbf.reserve(np * 3);
+      for (size_t r = 0; r < np; ++r) {
+        if (key[order[r]] == INT64_MAX) break;
+        sorted.push_back(parts[order[r]]);
+        for (int d = 0; d < 3; ++d) sbf.push_back(bfl[3 * order[r] + d]);
+      }
+      parts.swap(sorted);
+      bfl.swap(sbf);
+    }
+    const int64_t n = static_cast<int64_t>(parts.size());
+    std::vector<double> contrib(static_cast<size_t>(n) * NSRC, 0.);
+    std::vector<int64_t> dep_off(n, -1);
+    int err_local = PELEMP_OK;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads) reduction(max : err_local)
+    for (int64_t pid = 0; pid < n; ++pid) {
+      Particle<NF>& p = parts[pid];
+      if (p.id <= 0) continue;
+      GasPhaseVals<NS, NF> gpv;
+      double cBoilT[NF];
+      GasProps<Mech>::molecular_weight(gpv.mw);
+      double lx[3];
+      int ijk[3], ijkc[3];
+      for (int d = 0; d < 3; ++d) {
+        lx[d] = (p.pos[d] - g.plo[d]) * dxi[d] + 0.5;
+        ijk[d] = static_cast<int>(std::floor(lx[d]));
+      }
+      containing_cell(p.pos, g, dxi, ijkc);
+      // lx/ijk are in index space relative to the domain lower corner
+      for (int d = 0; d < 3; ++d) ijk[d] += g.domain_lo[d];
+      int* bflags = &bfl[3 * pid];
+      if (at_bounds) {
+        if (check_bounds(p.pos, g, bflags)) {
+          err_local = std::max(err_local, PELEMP_E_LEFT_DOMAIN);
+          continue;
+        }
+      }
+      const bool is_film = p.rdata[C::pstateFilmHght] > 0.;
+      int idx[8][3];
+      double w[8];
+      {
+        // trilinear weights use lx - ijk in domain-relative index space
+        int ijk_rel[3] = {ijk[0] - g.domain_lo[0], ijk[1] - g.domain_lo[1], ijk[2] - g.domain_lo[2]};
+        trilinear_interp(ijk_rel, lx, idx, w, bflags);
+        for (int a = 0; a < 8; ++a)
+          for (int d = 0; d < 3; ++d) idx[a][d] += g.domain_lo[d];
+      }
+      gpv.reset();
+      if (!InterpolateGasPhase<NS, NF>(gpv, st, SPI, idx, w)) {
+        err_local = std::max(err_local, PELEMP_E_STATE_BOX);
+        continue;
+      }
+      gpv.define();
+      calcBoilT<NS, NF>(fdat, gpv, cBoilT);
+      if (is_film) {
+        err_local = std::max(err_local, PELEMP_E_BAD_CONFIG);  // film path not in this driver
+        continue;
+      }
+      calculateSpraySource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT);
+      const double cur_coef = -inv_vol * sub_dt / flow_dt;
+      if (!srcv.contains(ijkc)) {
+        if (!isGhost) {
+          err_local = std::max(err_local, PELEMP_E_SRC_BOX);
+          continue;
+        }
+      } else {
+        double* cb = &contrib[static_cast<size_t>(pid) * NSRC];
+        if (fdat.mom_trans) {
+          for (int d = 0; d < 3; ++d) cb[d] = cur_coef * gpv.fluid_mom_src[d];
+        }
+        if (fdat.mass_trans) {
+          cb[3] = cur_coef * gpv.fluid_mass_src;
+          for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = cur_coef * gpv.fluid_Y_dot[spf];
+        }
+        cb[4 + NF] = cur_coef * gpv.fluid_eng_src;
+        dep_off[pid] = srcv.off(ijkc, 0);
+      }
+      if (do_move && !fdat.fixed_parts && p.id > 0 && !is_film) {
+        for (int dir = 0; dir < 3; ++dir) {
+          const double cvel = p.rdata[C::pstateVel + dir];
+          p.pos[dir] += sub_dt * cvel;
+        }
+        if (at_bounds) {
+          if (check_bounds(p.pos, g, bflags)) {
+            p.id = -1;
+          }
+          // reflective walls are handled by the wall/splash driver
+        }
+      }
+      periodic_wrap(p.pos, g);
+      if (isGhost) {
+        int c2[3];
+        containing_cell(p.pos, g, dxi, c2);
+        if (!srcv.contains(c2)) p.id = -1;
+      }
+    }
+    if (err_local != PELEMP_OK) {
+      status = err_local;
+      break;
+    }
+    // deterministic per-cell sums in sorted order: contributions of one cell
+    // are contiguous (same key); sum them from zero, then add to the source.
+    const int64_t cstride = static_cast<int64_t>(srcv.n[0]) * srcv.n[1] * srcv.n[2];
+    const int comp_of[NSRC] = {SPI.momSrcIndx, SPI.momSrcIndx + 1, SPI.momSrcIndx + 2, SPI.rhoSrcIndx};
+    int comps[NSRC];
+    for (int q = 0; q < 4; ++q) comps[q] = comp_of[q];
+    for (int spf = 0; spf < NF; ++spf) comps[4 + spf] = SPI.specSrcIndx + spf;
+    comps[4 + NF] = SPI.engSrcIndx;
+    // droplets of one cell are adjacent after the sort and share dep_off
+    std::vector<int64_t> seg_start;
+    for (int64_t i = 0; i < n; ++i) {
+      if (i == 0 || dep_off[i] != dep_off[i - 1]) seg_start.push_back(i);
+    }
+    seg_start.push_back(n);
+    const int64_t nseg = static_cast<int64_t>(seg_start.size()) - 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t s = 0; s < nseg; ++s) {
+      const int64_t a = seg_start[s], b = seg_start[s + 1];
+      if (dep_off[a] < 0) continue;
+      for (int q = 0; q < NSRC; ++q) {
+        // disabled couplings leave their source components untouched
+        if (q < 3 && !fdat.mom_trans) continue;
+        if (q >= 3 && q < 4 + NF && !fdat.mass_trans) continue;
+        double acc = 0.;
+        for (int64_t i = a; i < b; ++i) acc += contrib[static_cast<size_t>(i) * NSRC + q];
+        double* dst = &source_v.data[dep_off[a] + static_cast<int64_t>(comps[q]) * cstride];
+        *dst = *dst + acc;
+      }
+    }
+  }
+  return status;
+}
+
+}  // namespace oracle

@@ -1,0 +1,299 @@
+"""ctypes binding of libpelemp_spray.so (include/pelemp_spray.h).
+
+The structures mirror the C ABI one to one.  Loading fails loudly when the
+library has not been built: there is no CPU fallback for the droplet update.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+MAX_FUEL = 8
+
+OK = 0
+E_STATE_BOX = 1
+E_SRC_BOX = 2
+E_LEFT_DOMAIN = 3
+E_BAD_CONFIG = 4
+E_HIP = 5
+E_ARG = 6
+E_NOMEM = 7
+ERROR_NAMES = {
+    E_STATE_BOX: "state box too small",
+    E_SRC_BOX: "source box too small",
+    E_LEFT_DOMAIN: "particle left the domain",
+    E_BAD_CONFIG: "bad configuration",
+    E_HIP: "HIP error",
+    E_ARG: "invalid argument",
+    E_NOMEM: "out of device memory",
+}
+
+MECH_DODECANE_AIR = 0
+MECH_DECANE_DODECANE_AIR = 1
+MECH_ALKANE6_AIR = 2
+
+FORM_FP64 = 0
+FORM_FP32 = 1
+FORM_SURROGATE = 2
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpelemp_spray.so")
+
+_c_dbl = ctypes.c_double
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+
+
+class SprayDataC(ctypes.Structure):
+    _fields_ = [
+        ("mass_trans", _c_i32), ("mom_trans", _c_i32), ("fixed_parts", _c_i32),
+        ("do_splash", _c_i32), ("do_breakup", _c_i32), ("nfuel", _c_i32),
+        ("min_eb_vfrac", _c_dbl), ("ref_T", _c_dbl), ("sigma", _c_dbl), ("wall_T", _c_dbl),
+        ("theta_c", _c_dbl), ("dtmod", _c_dbl), ("body_force", _c_dbl * 3),
+        ("critT", _c_dbl * MAX_FUEL), ("boilT", _c_dbl * MAX_FUEL), ("cp", _c_dbl * MAX_FUEL),
+        ("latent", _c_dbl * MAX_FUEL), ("ref_latent", _c_dbl * MAX_FUEL),
+        ("psat_coef", _c_dbl * (4 * MAX_FUEL)), ("rho_coef", _c_dbl * (4 * MAX_FUEL)),
+        ("lambda_coef", _c_dbl * (4 * MAX_FUEL)), ("mu_coef", _c_dbl * (4 * MAX_FUEL)),
+        ("indx", _c_i32 * MAX_FUEL), ("dep_indx", _c_i32 * MAX_FUEL),
+    ]
+
+
+class SprayCompsC(ctypes.Structure):
+    _fields_ = [(n, _c_i32) for n in (
+        "rhoIndx", "momIndx", "engIndx", "utempIndx", "specIndx",
+        "rhoSrcIndx", "momSrcIndx", "engSrcIndx", "specSrcIndx")]
+
+
+class FabViewC(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("lo", _c_i32 * 3), ("hi", _c_i32 * 3), ("ncomp", _c_i32)]
+
+
+class GeomC(ctypes.Structure):
+    _fields_ = [
+        ("plo", _c_dbl * 3), ("phi", _c_dbl * 3), ("dx", _c_dbl * 3),
+        ("domain_lo", _c_i32 * 3), ("domain_hi", _c_i32 * 3),
+        ("bndry_lo", _c_i32 * 3), ("bndry_hi", _c_i32 * 3),
+    ]
+
+
+_vp = ctypes.c_void_p
+_PROTOS = {
+    "pelemp_spray_create": (_c_i32, [ctypes.POINTER(_vp), _c_i32, _c_i32, ctypes.POINTER(SprayDataC),
+                                     ctypes.POINTER(SprayCompsC), ctypes.POINTER(GeomC), _c_i64, _c_i32]),
+    "pelemp_spray_destroy": (_c_i32, [_vp]),
+    "pelemp_spray_get_data": (_c_i32, [_vp, ctypes.POINTER(SprayDataC)]),
+    "pelemp_spray_set_particles": (_c_i32, [_vp, _c_i64, _vp, _vp, _vp]),
+    "pelemp_spray_get_particles": (_c_i32, [_vp, ctypes.POINTER(_c_i64), _vp, _vp, _vp]),
+    "pelemp_spray_num_particles": (_c_i32, [_vp, ctypes.POINTER(_c_i64)]),
+    "pelemp_spray_update": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl, _c_dbl,
+                                     _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
+    "pelemp_spray_sync": (_c_i32, [_vp, _vp]),
+    "pelemp_spray_move_kick_drift": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl,
+                                              _c_dbl, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
+    "pelemp_spray_est_timestep": (_c_i32, [_vp, _c_dbl, ctypes.POINTER(_c_dbl), _vp]),
+    "pelemp_spray_set_timing": (_c_i32, [_vp, _c_i32]),
+    "pelemp_spray_get_timing": (_c_i32, [_vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_i64), _c_i32]),
+    "pelemp_spray_device_bytes": (_c_i32, [_vp, ctypes.POINTER(_c_i64)]),
+    "pelemp_math_eval": (_c_i32, [_c_i32, _c_i64, _vp, _vp, _vp]),
+    "pelemp_device_count": (_c_i32, [ctypes.POINTER(_c_i32)]),
+    "pelemp_device_malloc": (_c_i32, [ctypes.POINTER(_vp), _c_i64]),
+    "pelemp_device_free": (_c_i32, [_vp]),
+    "pelemp_memcpy_htod": (_c_i32, [_vp, _vp, _c_i64]),
+    "pelemp_memcpy_dtoh": (_c_i32, [_vp, _vp, _c_i64]),
+    "pelemp_memcpy_dtod": (_c_i32, [_vp, _vp, _c_i64]),
+    "pelemp_memset_device": (_c_i32, [_vp, _c_i32, _c_i64]),
+    "pelemp_device_synchronize": (_c_i32, []),
+    "pelemp_stream_create": (_c_i32, [ctypes.POINTER(_vp)]),
+    "pelemp_stream_destroy": (_c_i32, [_vp]),
+    "pelemp_stream_synchronize": (_c_i32, [_vp]),
+    "pelemp_timer_create": (_c_i32, [ctypes.POINTER(_vp)]),
+    "pelemp_timer_record": (_c_i32, [_vp, _c_i32, _vp]),
+    "pelemp_timer_elapsed": (_c_i32, [_vp, ctypes.POINTER(_c_dbl)]),
+    "pelemp_timer_destroy": (_c_i32, [_vp]),
+    "pelemp_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class PelempError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str = ""):
+        self.code = code
+        super().__init__(f"{where}: {ERROR_NAMES.get(code, code)} ({msg})")
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m pelemp_amd.build` "
+                "(the spray update has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_PROTOS)
+
+
+def check(code: int, where: str) -> None:
+    if code != OK:
+        msg = lib().pelemp_last_error()
+        raise PelempError(code, where, msg.decode() if msg else "")
+
+
+def fill_array(dst, values):
+    for i, v in enumerate(values):
+        dst[i] = v
+
+
+def spray_data_c(d: dict) -> SprayDataC:
+    """Build a SprayDataC from a dict with SprayData field names."""
+    s = SprayDataC()
+    for k in ("mass_trans", "mom_trans", "fixed_parts", "do_splash", "do_breakup", "nfuel"):
+        setattr(s, k, int(d.get(k, 0)))
+    for k in ("min_eb_vfrac", "ref_T", "sigma", "wall_T", "theta_c", "dtmod"):
+        setattr(s, k, float(d.get(k, {"min_eb_vfrac": 0.05, "sigma": -1., "wall_T": -1., "theta_c": -1.,
+                                       "dtmod": 0.5}.get(k, 0.))))
+    fill_array(s.body_force, d.get("body_force", [0., 0., 0.]))
+    for k in ("critT", "boilT", "cp", "latent", "ref_latent", "psat_coef", "rho_coef", "lambda_coef",
+              "mu_coef"):
+        fill_array(getattr(s, k), d.get(k, []))
+    fill_array(s.indx, d.get("indx", []))
+    fill_array(s.dep_indx, d.get("dep_indx", d.get("indx", [])))
+    return s
+
+
+def spray_data_dict(s: SprayDataC) -> dict:
+    out = {}
+    for name, _t in SprayDataC._fields_:
+        v = getattr(s, name)
+        out[name] = list(v) if hasattr(v, "__len__") else v
+    return out
+
+
+def fab_view(ptr: int, lo, hi, ncomp: int) -> FabViewC:
+    v = FabViewC()
+    v.data = ptr
+    fill_array(v.lo, lo)
+    fill_array(v.hi, hi)
+    v.ncomp = ncomp
+    return v
+
+
+def geom_c(g: dict) -> GeomC:
+    c = GeomC()
+    for k in ("plo", "phi", "dx", "domain_lo", "domain_hi", "bndry_lo", "bndry_hi"):
+        fill_array(getattr(c, k), g[k])
+    return c
+
+
+def comps_c(d: dict) -> SprayCompsC:
+    c = SprayCompsC()
+    for name, _t in SprayCompsC._fields_:
+        setattr(c, name, int(d[name]))
+    return c
+
+
+def math_eval(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    """Evaluate the library's device FP64 elementary function `fn` on x (, y)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    check(lib().pelemp_math_eval(fn, x.size, x.ctypes.data, None if yy is None else yy.ctypes.data,
+                                 out.ctypes.data), "pelemp_math_eval")
+    return out
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library's HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        p = ctypes.c_void_p()
+        check(lib().pelemp_device_malloc(ctypes.byref(p), int(nbytes)), "pelemp_device_malloc")
+        self.ptr = int(p.value)
+        self.nbytes = int(nbytes)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().pelemp_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().pelemp_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "pelemp_memcpy_htod")
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        assert a.flags["C_CONTIGUOUS"] and a.nbytes <= self.nbytes
+        check(lib().pelemp_memcpy_dtoh(a.ctypes.data, self.ptr, a.nbytes), "pelemp_memcpy_dtoh")
+        return a
+
+    def zero(self) -> None:
+        check(lib().pelemp_memset_device(self.ptr, 0, self.nbytes), "pelemp_memset_device")
+
+
+class Stream:
+    def __init__(self):
+        s = ctypes.c_void_p()
+        check(lib().pelemp_stream_create(ctypes.byref(s)), "pelemp_stream_create")
+        self.handle = int(s.value)
+
+    def synchronize(self) -> None:
+        check(lib().pelemp_stream_synchronize(self.handle), "pelemp_stream_synchronize")
+
+    def __del__(self):
+        try:
+            lib().pelemp_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP event pair recorded on a stream."""
+
+    def __init__(self):
+        t = ctypes.c_void_p()
+        check(lib().pelemp_timer_create(ctypes.byref(t)), "pelemp_timer_create")
+        self.handle = int(t.value)
+
+    def start(self, stream=None) -> None:
+        check(lib().pelemp_timer_record(self.handle, 0, stream), "pelemp_timer_record")
+
+    def stop(self, stream=None) -> None:
+        check(lib().pelemp_timer_record(self.handle, 1, stream), "pelemp_timer_record")
+
+    def elapsed_ms(self) -> float:
+        ms = ctypes.c_double()
+        check(lib().pelemp_timer_elapsed(self.handle, ctypes.byref(ms)), "pelemp_timer_elapsed")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib().pelemp_timer_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    n = ctypes.c_int32()
+    check(lib().pelemp_device_count(ctypes.byref(n)), "pelemp_device_count")
+    return int(n.value)
+
+
+def device_synchronize() -> None:
+    check(lib().pelemp_device_synchronize(), "pelemp_device_synchronize")

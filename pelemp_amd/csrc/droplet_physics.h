@@ -1,0 +1,528 @@
+// droplet_physics.h -- per-droplet drag / evaporation update for the HIP
+// kernels, templated on the arithmetic type R (double or float).
+//
+// Follows the reference method of Source/PP_Spray/Drag.H:4-441 (drag +
+// Abramzon-Sirignano evaporation with adaptive substeps) and the SprayData
+// fits of SprayFuelData.H:146-211, PeleLMeX flavour (SI units, velocity and
+// temperature read directly from the state, no kinetic-energy terms).
+//
+// For R = double every expression keeps the reference's association order
+// and uses the pm:: elementary functions, so a kernel compiled with
+// -ffp-contract=off reproduces the FP64 method bit for bit.  Droplet-
+// independent subexpressions (Watson-corrected latent heat at boiling times
+// the fuel molecular weight, 1/boilT, fuel molecular weights) are hoisted
+// into FuelConst by the host; hoisting a whole subexpression does not change
+// any rounding.
+//
+// Species indices of the fuels (SprayData::indx / dep_indx) are runtime
+// values; register arrays indexed by them are read and written through
+// unrolled compare-and-select loops (sel_get / sel_set) so that they stay in
+// registers on the GPU.
+//
+// For R = float the same sequence runs in single precision with hardware
+// elementary functions; the caller keeps the source sums in double.
+#pragma once
+
+#include "gas_props.h"
+#include "pelemp_spray.h"
+
+namespace pelemp {
+
+constexpr double kPIc = 3.14159265358979323846;
+constexpr double kMinMass = 1.0e-16;  // SprayUnits::min_mass (kg)
+
+// Saturating float->int for the substep count (the reference's
+// static_cast<int> is undefined beyond INT_MAX, Drag.H:290,355).
+template <class R>
+PM_HD int sat_int(R x) {
+  if (!(x == x)) return 0;
+  if (x > R(1.0e9)) return 1000000000;
+  if (x < R(-1.0e9)) return -1000000000;
+  return static_cast<int>(x);
+}
+
+template <class R>
+PM_HD R rmin(R a, R b) {
+  return (b < a) ? b : a;  // std::min semantics
+}
+template <class R>
+PM_HD R rmax(R a, R b) {
+  return (a < b) ? b : a;  // std::max semantics
+}
+
+template <int N, class R>
+PM_HD R sel_get(const R* a, int i) {
+  R v = a[0];
+#pragma unroll
+  for (int n = 1; n < N; ++n) v = (n == i) ? a[n] : v;
+  return v;
+}
+template <int N, class R>
+PM_HD void sel_set(R* a, int i, R v) {
+#pragma unroll
+  for (int n = 0; n < N; ++n) a[n] = (n == i) ? v : a[n];
+}
+
+// Fuel data hoisted per launch (droplet independent).
+template <class R, int NF>
+struct FuelConst {
+  R mw_fuel[NF];   // molecular weight of fuel species indx[spf]
+  R hboil_mw[NF];  // Hboil_ref * mw_fuel (SprayFuelData.H:190-196)
+  R inv_boilT[NF];  // 1 / boilT
+  R critT[NF];
+  R boilT[NF];
+  R cp[NF];
+  R latent[NF];
+  R rho_coef[4 * NF];
+  R psat_coef[4 * NF];
+  int indx[NF];
+  int dep_indx[NF];
+  bool cc_psat[NF];  // psat_coef[4*spf+3] == 0: Clausius-Clapeyron
+  R ref_T;
+  R dtmod;
+  R body_force[3];
+  R y_tab_coef;  // unused placeholder for alignment
+  bool mass_trans, mom_trans, fixed_parts;
+  int do_breakup;
+};
+
+template <class R, int NF>
+inline FuelConst<R, NF> make_fuel_const(const pelemp_spray_data& f, const double* mw_all) {
+  FuelConst<R, NF> c{};
+  for (int spf = 0; spf < NF; ++spf) {
+    const double mw = mw_all[f.indx[spf]];
+    const double Hboil_ref =
+        f.ref_latent[spf] * pm::pow((f.critT[spf] - f.ref_T) / (f.critT[spf] - f.boilT[spf]), -0.38);
+    c.mw_fuel[spf] = static_cast<R>(mw);
+    c.hboil_mw[spf] = static_cast<R>(Hboil_ref * mw);
+    c.inv_boilT[spf] = static_cast<R>(1. / f.boilT[spf]);
+    c.critT[spf] = static_cast<R>(f.critT[spf]);
+    c.boilT[spf] = static_cast<R>(f.boilT[spf]);
+    c.cp[spf] = static_cast<R>(f.cp[spf]);
+    c.latent[spf] = static_cast<R>(f.latent[spf]);
+    for (int q = 0; q < 4; ++q) {
+      c.rho_coef[4 * spf + q] = static_cast<R>(f.rho_coef[4 * spf + q]);
+      c.psat_coef[4 * spf + q] = static_cast<R>(f.psat_coef[4 * spf + q]);
+    }
+    c.indx[spf] = f.indx[spf];
+    c.dep_indx[spf] = f.dep_indx[spf];
+    c.cc_psat[spf] = (f.psat_coef[4 * spf + 3] == 0.);
+  }
+  c.ref_T = static_cast<R>(f.ref_T);
+  c.dtmod = static_cast<R>(f.dtmod);
+  for (int d = 0; d < 3; ++d) c.body_force[d] = static_cast<R>(f.body_force[d]);
+  c.mass_trans = f.mass_trans != 0;
+  c.mom_trans = f.mom_trans != 0;
+  c.fixed_parts = f.fixed_parts != 0;
+  c.do_breakup = f.do_breakup;
+  return c;
+}
+
+// Interpolated gas state at the droplet (GasPhaseVals, SprayFuelData.H:71-112)
+template <class R, int NS>
+struct GasAt {
+  R vel[3];
+  R T;
+  R rho;
+  R p;
+  R Y[NS];
+  R mw_mix;
+};
+
+// Droplet sources (GasPhaseVals::fluid_*), before the -1/V sub_dt/flow_dt factor
+template <class R, int NF>
+struct DropSrc {
+  R mom[3];
+  R mass;
+  R Ydot[NF];
+  R eng;
+};
+
+// Droplet state the update reads and writes.
+template <class R, int NF>
+struct Drop {
+  R vel[3];
+  R T;
+  R dia;
+  R Y[NF];
+  R numDens;
+  R bm1;  // TAB distortion y (used only when do_breakup == 1)
+  bool alive;
+};
+
+// SprayData::rhoL (SprayFuelData.H:146-153)
+template <class R, int NF>
+PM_HD R rhoL(const FuelConst<R, NF>& f, R T, int spf) {
+  const R a = f.rho_coef[4 * spf], b = f.rho_coef[4 * spf + 1];
+  const R c = f.rho_coef[4 * spf + 2], d = f.rho_coef[4 * spf + 3];
+  return a + T * (b + T * (c + T * d));
+}
+
+// SprayData::psat (SprayFuelData.H:204-211)
+template <class R, int NF>
+PM_HD R psat(const FuelConst<R, NF>& f, R T, int spf) {
+  const R a = f.psat_coef[4 * spf], b = f.psat_coef[4 * spf + 1];
+  const R c = f.psat_coef[4 * spf + 2], d = f.psat_coef[4 * spf + 3];
+  return d * rm::pow(R(10.), a - b / (T + c));
+}
+
+// GasPhaseVals::define (SprayFuelData.H:101-111): mixture MW and pressure
+template <class M, class R>
+PM_HD void gas_define(GasAt<R, M::NS>& g) {
+  R s = R(0);
+#pragma unroll
+  for (int n = 0; n < M::NS; ++n) s += g.Y[n] / static_cast<R>(M::W[n]);
+  g.p = g.rho * static_cast<R>(kRU) * s * g.T;
+  g.mw_mix = R(1) / s;
+}
+
+// SprayData::calcBoilT (SprayFuelData.H:180-200)
+template <int NF, class R>
+PM_HD void calc_boil_t(const FuelConst<R, NF>& f, R p, R* cBoilT) {
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    const R cb = R(1) / (rm::log(static_cast<R>(kPATM) / p) * static_cast<R>(kRU) / f.hboil_mw[spf] +
+                         f.inv_boilT[spf]);
+    cBoilT[spf] = rmin(f.critT[spf], cb);
+  }
+}
+
+// calcHeatCoeff (Drag.H:8-45)
+template <class R>
+PM_HD R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
+  if (B_M <= C_eps) return R(0);
+  const int maxIter = 100;
+  const R NU2 = Nu_0 - R(2);
+  const R BM1 = R(1) + B_M;
+  R phi = ratio / Nu_0;
+  R B_T_old = rm::pow(BM1, phi) - R(1);
+  R logB = rm::log1p(B_T_old);
+  R invFT = B_T_old / (logB * rm::pow(R(1) + B_T_old, R(0.7)));
+  R Nu_num = R(2) + NU2 * invFT;
+  phi = ratio / Nu_num;
+  R B_T = rm::pow(BM1, phi) - R(1);
+  R error = rm::fabs(B_T - B_T_old);
+  int k = 0;
+  while (k < maxIter && error > B_eps) {
+    B_T_old = B_T;
+    logB = rm::log1p(B_T);
+    invFT = B_T / (logB * rm::pow(R(1) + B_T, R(0.7)));
+    Nu_num = R(2) + NU2 * invFT;
+    phi = ratio / Nu_num;
+    B_T = rm::pow(BM1, phi) - R(1);
+    error = rm::fabs(B_T - B_T_old);
+    k++;
+  }
+  logB = rm::log1p(B_T);
+  invFT = B_T / (logB * rm::pow(R(1) + B_T, R(0.7)));
+  Nu_num = R(2) + NU2 * invFT;
+  return Nu_num * logB / B_T;
+}
+
+// calcVaporState (Drag.H:47-135)
+template <class M, int NF, class R>
+PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g, R rule, R T_in,
+                            R C_eps, R mw_part, const R* Y_l, const R* h_part, const R* cp_n,
+                            const R* cBoilT, R* Y_skin, R* X_vapor, R* L_fuel, R& B_M, R& sumXVap,
+                            R& cp_skin, R& mw_skin) {
+  constexpr int NS = M::NS;
+  R mbar_vap = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    const int fspec = fd.indx[spf];
+    const int fdspec = fd.dep_indx[spf];
+    const R boilT_ref = fd.boilT[spf];
+    const R mw_fuel = fd.mw_fuel[spf];
+    const R T_part = rmin(T_in, R(0.99) * cBoilT[spf]);
+    const R part_latent = sel_get<NS>(h_part, fspec) + fd.latent[spf] - fd.cp[spf] * (T_part - fd.ref_T);
+    L_fuel[spf] = part_latent;
+    R pres_sat = R(0);
+    if (fd.cc_psat[spf]) {
+      pres_sat = static_cast<R>(kPATM) *
+                 rm::exp(part_latent * mw_fuel / static_cast<R>(kRU) * (R(1) / boilT_ref - R(1) / T_part));
+    } else {
+      pres_sat = psat<R, NF>(fd, T_part, spf);
+    }
+    const R X_fluid = sel_get<NS>(g.Y, fdspec) * g.mw_mix / mw_fuel;
+    if (pres_sat > X_fluid * g.p) {
+      const R Xl = Y_l[spf] * mw_part / mw_fuel;
+      const R Xv = Xl * pres_sat / g.p;
+      mbar_vap += Xv * mw_fuel;
+      sumXVap += Xv;
+      X_vapor[spf] = Xv;
+    } else {
+      X_vapor[spf] = R(0);
+    }
+  }
+  const R totalmwx = mbar_vap + (R(1) - sumXVap) * g.mw_mix;
+  R sumYSkin = R(0), sumYfFluid = R(0), sumYVap = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    if (X_vapor[spf] > R(0)) {
+      const int fspec = fd.indx[spf];
+      const int fdspec = fd.dep_indx[spf];
+      const R mw_fuel = fd.mw_fuel[spf];
+      const R Yfv = X_vapor[spf] * mw_fuel / totalmwx;
+      sumYVap += Yfv;
+      const R gasY = sel_get<NS>(g.Y, fdspec);
+      sumYfFluid += gasY;
+      const R Ysk = Yfv + rule * (gasY - Yfv);
+      sel_set<NS>(Y_skin, fspec, Ysk);
+      sumYSkin += Ysk;
+    }
+  }
+  const R renorm = (R(1) - sumYSkin) / (R(1) - sumYfFluid);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    if (Y_skin[n] == R(0)) Y_skin[n] = g.Y[n] * renorm;
+    cp_skin += Y_skin[n] * cp_n[n];
+    mw_skin += Y_skin[n] / static_cast<R>(M::W[n]);
+  }
+  mw_skin = R(1) / mw_skin;
+  B_M = (sumYVap - sumYfFluid) / rmax(C_eps, (R(1) - sumYVap));
+  B_M = rmin(R(20), B_M);
+}
+
+// calculateSpraySource (Drag.H:137-441), PELELM_USE_SPRAY flavour.
+// Updates d (vel, T, dia, Y, alive) and returns the sources in s and the
+// Reynolds number of the last substep.
+template <class M, int NF, class R>
+PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
+                     Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s) {
+  constexpr int NS = M::NS;
+  const R rule = R(1.) / R(3.);
+  const R C_eps = R(1.E-15);
+  const R B_eps = R(1.E-7);
+  const R min_mass = R(kMinMass);
+  const int nSubMax = 100;
+  const bool mass_trans = fd.mass_trans;
+  R Y_skin[NS], h_part[NS], cp_n[NS], Ddiag[NS];
+  R L_fuel[NF], mi_dot[NF], Y_part[NF], X_vapor[NF];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) Ddiag[n] = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    L_fuel[spf] = R(0);
+    mi_dot[spf] = R(0);
+    X_vapor[spf] = R(0);
+  }
+  s.mom[0] = s.mom[1] = s.mom[2] = R(0);
+  s.mass = R(0);
+  s.eng = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) s.Ydot[spf] = R(0);
+  const R num_ppp = d.numDens;
+  R vel_part[3] = {d.vel[0], d.vel[1], d.vel[2]};
+  R y_tab = R(0);
+  if (fd.do_breakup == 1) y_tab = d.bm1;
+  if (fd.fixed_parts) vel_part[0] = vel_part[1] = vel_part[2] = R(0);
+  R T_part = d.T;
+  R dia_part = d.dia;
+  R rho_part = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    Y_part[spf] = d.Y[spf];
+    rho_part += Y_part[spf] / rhoL<R, NF>(fd, rmin(T_part, cBoilT[spf]), spf);
+  }
+  rho_part = R(1) / rho_part;
+  R dt = flow_dt;
+  int isub = 1;
+  int nsub = 1;
+  R pmass = static_cast<R>(kPIc / 6.) * rho_part * (dia_part * dia_part * dia_part);
+  const R startmass = pmass;
+  R Reyn = R(0);
+  R part_vel_src[3];
+  while (isub <= nsub) {
+    R cp_part = R(0), Tboil = R(0), mw_part = R(0);
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      Tboil += Y_part[spf] * cBoilT[spf];
+      cp_part += Y_part[spf] * fd.cp[spf];
+      mw_part += Y_part[spf] / fd.mw_fuel[spf];
+    }
+    mw_part = R(1) / mw_part;
+    const R delT = rmax(g.T - T_part, R(0));
+    R T_skin = g.T;
+    if (mass_trans) T_skin = T_part + rule * delT;
+    GasProps<M>::T2Cpi(T_skin, cp_n);
+    GasProps<M>::T2Hi(T_part, h_part);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Y_skin[n] = R(0);
+    R cp_skin = R(0), mw_skin = R(0), B_M = R(0), sumXVap = R(0);
+    if (mass_trans) {
+      calc_vapor_state<M, NF, R>(fd, g, rule, T_part, C_eps, mw_part, Y_part, h_part, cp_n, cBoilT,
+                                 Y_skin, X_vapor, L_fuel, B_M, sumXVap, cp_skin, mw_skin);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        Y_skin[n] = g.Y[n];
+        cp_skin += g.Y[n] * cp_n[n];
+      }
+      mw_skin = g.mw_mix;
+    }
+    R lambda_skin = R(0), mu_skin = R(0);
+    R rho_skin = g.rho;
+    if (mass_trans) rho_skin = mw_skin * g.p / (static_cast<R>(kRU) * T_skin);
+    GasProps<M>::transport(true, mass_trans, mass_trans, T_skin, rho_skin, Y_skin, Ddiag, mu_skin,
+                           lambda_skin);
+    R diff_vel[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) diff_vel[q] = g.vel[q] - vel_part[q];
+    const R diff_vel_mag =
+        rm::sqrt(diff_vel[0] * diff_vel[0] + diff_vel[1] * diff_vel[1] + diff_vel[2] * diff_vel[2]);
+    Reyn = rho_skin * diff_vel_mag * dia_part / mu_skin;
+    const R inv_pmass = R(1) / pmass;
+    R drag_coef = R(0);
+    if (Reyn > R(1000.)) {
+      drag_coef = R(0.424);
+    } else if (Reyn > R(1.)) {
+      drag_coef = R(24.) / Reyn * (R(1.) + rm::cbrt(Reyn * Reyn) / R(6.));
+    } else if (Reyn > R(0.)) {
+      drag_coef = R(24.) / Reyn;
+    }
+    drag_coef *= (R(1.) + R(2.6432) * y_tab);
+    const R drag_force =
+        R(0.125) * rho_skin * drag_coef * static_cast<R>(kPIc) * dia_part * dia_part * diff_vel_mag;
+    const R drag_force_p = R(0.75) * rho_skin * drag_coef * diff_vel_mag / (dia_part * rho_part);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) part_vel_src[q] = drag_force_p * diff_vel[q] + fd.body_force[q];
+    if (fd.mom_trans) {
+      const R fm = num_ppp * drag_force;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) s.mom[q] += fm * diff_vel[q];
+      if (isub == 1) {
+        const R inv_tau_var = drag_force_p;
+        const int c = sat_int(flow_dt * inv_tau_var) + 1;
+        nsub = (c < nsub) ? nsub : c;
+        nsub = (nSubMax < nsub) ? nSubMax : nsub;
+      }
+    }
+    R part_temp_src = R(0);
+    R m_dot = R(0);
+    if (mass_trans) {
+      const R Pr_skin = mu_skin * cp_skin / lambda_skin;
+      const R powR = rmax(rm::pow(rmin(R(400.), Reyn), R(0.077)), R(1.));
+      R Nu_num = R(0);
+      R sumL = R(0);
+      const R Nu_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Pr_skin);
+      if (sumXVap > R(0)) {
+        R rhoDtotal = R(0), cp_fuel = R(0), sumYSkin = R(0);
+        R Dfuel[NF];
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) {
+          const int fspec = fd.indx[spf];
+          R dd = sel_get<NS>(Ddiag, fspec);
+          dd *= mw_skin / fd.mw_fuel[spf];
+          const R X_norm = X_vapor[spf] / sumXVap;
+          dd *= X_norm;
+          sel_set<NS>(Ddiag, fspec, dd);
+          Dfuel[spf] = dd;
+          rhoDtotal += dd;
+          const R ysk = sel_get<NS>(Y_skin, fspec);
+          cp_fuel += ysk * sel_get<NS>(cp_n, fspec);
+          sumYSkin += ysk;
+        }
+        cp_fuel /= sumYSkin;
+        const R Sc_skin = mu_skin / rhoDtotal;
+        const R logB = rm::log1p(B_M);
+        const R invFM = B_M / (logB * rm::pow(R(1.) + B_M, R(0.7)));
+        const R Sh_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Sc_skin);
+        const R Sh_num = R(2.) + (Sh_0 - R(2.)) * invFM;
+        const R mdotcoeff = static_cast<R>(kPIc) * dia_part * Sh_num * logB;
+        const R ratio = cp_fuel * Sh_num * rhoDtotal / lambda_skin;
+        Nu_num = calc_heat_coeff<R>(ratio, B_M, B_eps, C_eps, Nu_0);
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) {
+          if (X_vapor[spf] > R(0)) {
+            // Dfuel[spf] is Ddiag[indx[spf]] after the updates above (fuel
+            // species are distinct)
+            mi_dot[spf] = -rmax(Dfuel[spf] * mdotcoeff, R(0.));
+            m_dot += mi_dot[spf];
+            sumL += mi_dot[spf] * L_fuel[spf];
+          }
+        }
+      } else {
+        Nu_num = Nu_0;
+      }
+      const R conv_src = static_cast<R>(kPIc) * lambda_skin * dia_part * delT * Nu_num;
+      s.eng += num_ppp * conv_src;
+      part_temp_src = (sumL + conv_src) * inv_pmass / cp_part;
+      if (isub == 1 && delT > C_eps) {
+        const R inv_tau_d = -m_dot / (R(0.2) * pmass);
+        const R inv_tau_T = conv_src * inv_pmass / (cp_part * delT);
+        const int a = sat_int(flow_dt * inv_tau_d);
+        const int b = sat_int(flow_dt * inv_tau_T);
+        const int c = ((a < b) ? b : a) + 1;
+        nsub = (c < nsub) ? nsub : c;
+        nsub = (nSubMax < nsub) ? nSubMax : nsub;
+      }
+    }
+    if (isub == 1) dt = flow_dt / static_cast<R>(nsub);
+    const R part_dt = fd.dtmod * dt;
+    if (!fd.fixed_parts) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) vel_part[q] += part_dt * part_vel_src[q];
+    }
+    if (mass_trans) {
+      const R newT = T_part + part_dt * part_temp_src;
+      const R new_mass = pmass + m_dot * part_dt;
+      if (new_mass > min_mass && newT < Tboil) {
+        T_part = newT;
+        if (NF > 1) {
+          rho_part = R(0);
+          R sumY = R(0);
+#pragma unroll
+          for (int spf = 0; spf < NF; ++spf) {
+            Y_part[spf] = rmin(R(1.), rmax(R(0.), (Y_part[spf] * pmass + mi_dot[spf] * part_dt) / new_mass));
+            if (Y_part[spf] <= R(1.E-12)) Y_part[spf] = R(0.);
+            sumY += Y_part[spf];
+          }
+#pragma unroll
+          for (int spf = 0; spf < NF; ++spf) {
+            Y_part[spf] /= sumY;
+            rho_part += Y_part[spf] / rhoL<R, NF>(fd, rmin(T_part, cBoilT[spf]), spf);
+          }
+          rho_part = R(1.) / rho_part;
+        } else {
+          rho_part = rhoL<R, NF>(fd, rmin(T_part, cBoilT[0]), 0);
+        }
+        pmass = new_mass;
+        dia_part = rm::cbrt(R(6.) * pmass / (static_cast<R>(kPIc) * rho_part));
+      } else {
+        pmass = R(0.);
+        d.alive = false;
+        nsub = isub;
+        isub = nsub + 1;
+      }
+    }
+    ++isub;
+  }
+  if (nsub > 1) {
+    s.eng /= static_cast<R>(nsub);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s.mom[q] /= static_cast<R>(nsub);
+  }
+  const R mdot_total = (pmass - startmass) / (fd.dtmod * flow_dt);
+  s.mass = num_ppp * mdot_total;
+  if (fd.mom_trans) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s.mom[q] += num_ppp * vel_part[q] * mdot_total;
+  }
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    const int fdspec = fd.dep_indx[spf];
+    const R oldY = d.Y[spf];
+    const R newY = Y_part[spf];
+    const R midot = (newY * pmass - oldY * startmass) / (fd.dtmod * flow_dt);
+    s.Ydot[spf] = num_ppp * midot;
+    s.eng += num_ppp * midot * sel_get<NS>(h_part, fdspec);
+    d.Y[spf] = Y_part[spf];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) d.vel[q] = vel_part[q];
+  d.T = T_part;
+  d.dia = dia_part;
+  return Reyn;
+}
+
+}  // namespace pelemp

@@ -1,0 +1,181 @@
+// gas_props.h -- gas-phase species properties used by the droplet update.
+//
+// These stand in for the PelePhysics calls the reference makes from the
+// droplet kernel (absent from the reference tree):
+//   eos.molecular_weight   SprayParticles.cpp:332
+//   eos.T2Cpi, eos.T2Hi    Drag.H:220-221, SpraySetup.cpp:286
+//   trans.transport(...)   Drag.H:252-256  ("Simple" mixture-averaged model)
+// All quantities are SI (J/kg/K, J/kg, Pa s, W/m/K, kg/m/s for rho*D), so the
+// SprayUnits conversions of the LMeX flavour are identities here.
+//
+// Models (tables in mech_data.h, generated by mech/gen_mech.py):
+//   cp_k/R  = a0 + a1 T + a2 T^2 + a3 T^3 + a4 T^4               (NASA-7)
+//   h_k/RT  = a0 + a1 T/2 + a2 T^2/3 + a3 T^3/4 + a4 T^4/5 + a5/T
+//   ln mu_k, ln lambda_k, ln D_ij(1 atm) = cubic in ln T
+//   mu      = (sum_k X_k mu_k^6)^(1/6)
+//   lambda  = (sum_k X_k lambda_k^(1/4))^4
+//   rho D_k = rho (sum_{j!=k} Y_j) / (sum_{j!=k} X_j / D_kj(p)),
+//             D_kj(p) = D_kj(1 atm) p_atm / p,  p = rho R T / W_mix
+// The functions are templated on the arithmetic type: double evaluates with
+// the project's pm:: elementary functions (bitwise identical on host and
+// device), float with the gfx950 hardware approximations.
+#pragma once
+
+#include "mech_data.h"
+#include "pm_math.h"
+
+namespace pelemp {
+
+constexpr double kRU = 8.31446261815324;  // J/mol/K
+constexpr double kPATM = 101325.0;        // Pa
+
+// Arithmetic-type dispatch for the elementary functions.
+namespace rm {
+PM_HD double exp(double x) { return pm::exp(x); }
+PM_HD double log(double x) { return pm::log(x); }
+PM_HD double log1p(double x) { return pm::log1p(x); }
+PM_HD double pow(double x, double y) { return pm::pow(x, y); }
+PM_HD double cbrt(double x) { return pm::cbrt(x); }
+PM_HD double sqrt(double x) { return std::sqrt(x); }
+PM_HD double fabs(double x) { return std::fabs(x); }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float exp(float x) { return __expf(x); }
+__device__ inline float log(float x) { return __logf(x); }
+__device__ inline float log1p(float x) { return ::log1pf(x); }
+__device__ inline float pow(float x, float y) {
+  return (x == 0.0f) ? ((y > 0.0f) ? 0.0f : __builtin_huge_valf()) : __expf(y * __logf(x));
+}
+__device__ inline float cbrt(float x) { return ::cbrtf(x); }
+__device__ inline float sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ inline float fabs(float x) { return __builtin_fabsf(x); }
+#else
+inline float exp(float x) { return std::exp(x); }
+inline float log(float x) { return std::log(x); }
+inline float log1p(float x) { return std::log1p(x); }
+inline float pow(float x, float y) { return std::pow(x, y); }
+inline float cbrt(float x) { return std::cbrt(x); }
+inline float sqrt(float x) { return std::sqrt(x); }
+inline float fabs(float x) { return std::fabs(x); }
+#endif
+}  // namespace rm
+
+template <class M>
+struct GasProps {
+  static constexpr int NS = M::NS;
+
+  // Molecular weights [kg/mol] (eos.molecular_weight * mass_conv).
+  template <class R>
+  static PM_HD void molecular_weight(R* mw) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) mw[n] = static_cast<R>(M::W[n]);
+  }
+
+  // Species heat capacities at constant pressure [J/kg/K].
+  template <class R>
+  static PM_HD void T2Cpi(R T, R* cp) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const bool lo = T < static_cast<R>(M::Tmid[n]);
+      const R a0 = static_cast<R>(lo ? M::nasa_lo[7 * n + 0] : M::nasa_hi[7 * n + 0]);
+      const R a1 = static_cast<R>(lo ? M::nasa_lo[7 * n + 1] : M::nasa_hi[7 * n + 1]);
+      const R a2 = static_cast<R>(lo ? M::nasa_lo[7 * n + 2] : M::nasa_hi[7 * n + 2]);
+      const R a3 = static_cast<R>(lo ? M::nasa_lo[7 * n + 3] : M::nasa_hi[7 * n + 3]);
+      const R a4 = static_cast<R>(lo ? M::nasa_lo[7 * n + 4] : M::nasa_hi[7 * n + 4]);
+      const R rw = static_cast<R>(kRU / M::W[n]);
+      cp[n] = rw * (a0 + T * (a1 + T * (a2 + T * (a3 + T * a4))));
+    }
+  }
+
+  // Species specific enthalpies [J/kg] (incl. heat of formation).
+  template <class R>
+  static PM_HD void T2Hi(R T, R* h) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const bool lo = T < static_cast<R>(M::Tmid[n]);
+      // coefficient selects between two compile-time constants
+      const R b0 = static_cast<R>(lo ? M::nasa_lo[7 * n + 0] : M::nasa_hi[7 * n + 0]);
+      const R b1 = static_cast<R>(lo ? M::nasa_lo[7 * n + 1] * 0.5 : M::nasa_hi[7 * n + 1] * 0.5);
+      const R b2 = static_cast<R>(lo ? M::nasa_lo[7 * n + 2] * (1.0 / 3.0)
+                                     : M::nasa_hi[7 * n + 2] * (1.0 / 3.0));
+      const R b3 = static_cast<R>(lo ? M::nasa_lo[7 * n + 3] * 0.25 : M::nasa_hi[7 * n + 3] * 0.25);
+      const R b4 = static_cast<R>(lo ? M::nasa_lo[7 * n + 4] * 0.2 : M::nasa_hi[7 * n + 4] * 0.2);
+      const R b5 = static_cast<R>(lo ? M::nasa_lo[7 * n + 5] : M::nasa_hi[7 * n + 5]);
+      const R rw = static_cast<R>(kRU / M::W[n]);
+      h[n] = rw * (T * (b0 + T * (b1 + T * (b2 + T * (b3 + T * b4)))) + b5);
+    }
+  }
+
+  // Mixture-averaged transport at (T, rho, Y).  Ddiag[k] = rho D_k,mix.
+  template <class R>
+  static PM_HD void transport(bool get_mu, bool get_lam, bool get_D, R T,
+                              R rho, const R* Y, R* Ddiag, R& mu, R& lam) {
+    const R lt = rm::log(T);
+    const R lt2 = lt * lt;
+    const R lt3 = lt2 * lt;
+    R X[NS];
+    R sumYW = R(0);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) sumYW += Y[n] / static_cast<R>(M::W[n]);
+    const R wbar = R(1) / sumYW;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) X[n] = Y[n] / static_cast<R>(M::W[n]) * wbar;
+    if (get_mu) {
+      R s = R(0);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const R f = static_cast<R>(M::fitmu[4 * n]) + static_cast<R>(M::fitmu[4 * n + 1]) * lt +
+                    static_cast<R>(M::fitmu[4 * n + 2]) * lt2 +
+                    static_cast<R>(M::fitmu[4 * n + 3]) * lt3;
+        const R m = rm::exp(f);
+        const R m3 = m * m * m;
+        s += X[n] * (m3 * m3);
+      }
+      mu = rm::cbrt(rm::sqrt(s));
+    }
+    if (get_lam) {
+      R s = R(0);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const R f = static_cast<R>(M::fitlam[4 * n]) + static_cast<R>(M::fitlam[4 * n + 1]) * lt +
+                    static_cast<R>(M::fitlam[4 * n + 2]) * lt2 +
+                    static_cast<R>(M::fitlam[4 * n + 3]) * lt3;
+        const R l = rm::exp(f);
+        s += X[n] * rm::sqrt(rm::sqrt(l));
+      }
+      const R s2 = s * s;
+      lam = s2 * s2;
+    }
+    if (get_D) {
+      // inverse binary diffusivities at 1 atm, pairs i<j row-major
+      constexpr int NP = NS * (NS - 1) / 2;
+      R invD[NP > 0 ? NP : 1];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const R f = static_cast<R>(M::fitdbin[4 * p]) + static_cast<R>(M::fitdbin[4 * p + 1]) * lt +
+                    static_cast<R>(M::fitdbin[4 * p + 2]) * lt2 +
+                    static_cast<R>(M::fitdbin[4 * p + 3]) * lt3;
+        invD[p] = rm::exp(-f);
+      }
+      // p / p_atm from the ideal gas law
+      const R pratio = rho * static_cast<R>(kRU) * T / wbar / static_cast<R>(kPATM);
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        R term1 = R(0);
+        R term2 = R(0);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          if (j == i) continue;
+          const int a = i < j ? i : j;
+          const int b = i < j ? j : i;
+          const int pidx = a * NS - a * (a + 1) / 2 + (b - a - 1);
+          term1 += Y[j];
+          term2 += X[j] * invD[pidx];
+        }
+        Ddiag[i] = rho * term1 / (term2 * pratio);
+      }
+    }
+  }
+};
+
+}  // namespace pelemp

@@ -1,0 +1,45 @@
+// layout.h -- droplet storage layout and cell ordering shared by all kernels.
+//
+// Droplets are stored field-major (structure of arrays): field f of the
+// droplet at storage position i is base[f*cap + i].  Cells are grouped in
+// 4x4x4 tiles and ordered by the tile-major key
+//   key = tile_linear * 64 + (k%4 * 4 + j%4) * 4 + i%4,
+// tiles in x-fastest order, so the droplets of one tile are contiguous once
+// sorted by key.
+#pragma once
+
+#include <cstdint>
+
+#include "pm_math.h"
+
+namespace pelemp {
+
+constexpr uint32_t kDeadKey = 0xffffffffu;
+constexpr int kTile = 4;
+constexpr int kTileCells = kTile * kTile * kTile;
+
+// SoA field indices.  For the FP64 form pos is the absolute position; for
+// the FP32 form it is the position inside the droplet's cell in cell units,
+// in [0,1).
+template <int NF>
+struct Fields {
+  static constexpr int POS = 0;
+  static constexpr int VEL = 3;
+  static constexpr int T = 6;
+  static constexpr int DIA = 7;
+  static constexpr int Y = 8;
+  static constexpr int ND = 8 + NF;
+  static constexpr int FILM = 9 + NF;
+  static constexpr int BM1 = 10 + NF;
+  static constexpr int BM2 = 11 + NF;
+  static constexpr int N0 = 12 + NF;
+  static constexpr int NHOT = 10 + NF;      // without breakup state
+  static constexpr int NHOT_BRK = 13 + NF;  // with breakup state
+};
+
+PM_HD uint32_t tile_key(int i, int j, int k, const int* ntile) {
+  const uint32_t tile = (static_cast<uint32_t>(k / kTile) * ntile[1] + (j / kTile)) * ntile[0] + (i / kTile);
+  return tile * kTileCells + ((k % kTile) * kTile + (j % kTile)) * kTile + (i % kTile);
+}
+
+}  // namespace pelemp

@@ -1,0 +1,171 @@
+// mech_data.h -- generated by pelemp_amd/mech/gen_mech.py; do not edit.
+// Gas-phase species tables (SI units): molecular weight [kg/mol], NASA-7
+// coefficients (cp/R, h/RT), and cubic fits in ln T of ln mu [Pa s],
+// ln lambda [W/m/K] and ln D_ij(1 atm) [m^2/s] (pairs i<j, row-major).
+#pragma once
+
+namespace pelemp {
+
+struct Mech_dodecane_air {
+  static constexpr int NS = 3;
+  // species: O2, N2, NC12H26
+  static constexpr double W[3] = {
+      0x1.0620ab7132724p-5, 0x1.cafb3b7521145p-6, 0x1.5cdb37c99ae93p-3,
+  };
+  static constexpr double Tmid[3] = {
+      0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9,
+  };
+  static constexpr double nasa_lo[21] = {
+      0x1.e42787ae5fa45p+1, -0x1.88c9b66c8c0dfp-9, 0x1.4a6bae1fdae75p-17, -0x1.4ca5927ada53cp-27, 0x1.c883a18b02bcdp-39, -0x1.09fc63497b741p+10, 0x1.d42eb7e3dc88dp+1,
+      0x1.a63b0c4588a05p+1, 0x1.712969da04050p-10, -0x1.09f7a2b098bd4p-18, 0x1.83ae94da0fe59p-28, -0x1.581526048e0d6p-39, -0x1.fe732fec56d5dp+9, 0x1.f9a5ca29845ddp+1,
+      -0x1.6111f126688bap+0, 0x1.1d25ab9299328p-3, -0x1.448e555a0859dp-14, 0x1.16088218d87cfp-26, 0x1.3bc0d0332262ep-42, -0x1.3a102dfa19de0p+15, 0x0.0p+0,
+  };
+  static constexpr double nasa_hi[21] = {
+      0x1.a42a332f575b8p+1, 0x1.84c851ecf574dp-10, -0x1.96ee255022a3dp-21, 0x1.cca1706e20348p-33, -0x1.8667712a69c18p-46, -0x1.101d4b48d3ae7p+10, 0x1.5d01bdd004baap+2,
+      0x1.769c23b7952d2p+1, 0x1.86106ec5d7fc6p-10, -0x1.3132bf1b89c39p-21, 0x1.bc128a9b8a724p-34, -0x1.e6a15668d1fd9p-48, -0x1.cd661b089a027p+9, 0x1.7ec0f88333fc8p+2,
+      -0x1.6111f126688bap+0, 0x1.1d25ab9299328p-3, -0x1.448e555a0859dp-14, 0x1.16088218d87cfp-26, 0x1.3bc0d0332262ep-42, -0x1.3a102dfa19de0p+15, 0x0.0p+0,
+  };
+  static constexpr double fitmu[12] = {
+      -0x1.51d857a6480d2p+4, 0x1.b4e92e840fb95p+1, -0x1.8566b3fe3c6fcp-2, 0x1.1d276cd09057bp-6,
+      -0x1.489f8aed55cb2p+4, 0x1.90f18d636097fp+1, -0x1.60179d81699c4p-2, 0x1.036a58b6f18f8p-6,
+      -0x1.2bd68c93e0b06p+4, 0x1.bb850401e2f9fp-1, 0x1.98f0c9bb4268fp-4, -0x1.3b86cd0b8c289p-7,
+  };
+  static constexpr double fitlam[12] = {
+      -0x1.5593478cc452ep+3, 0x1.c0f28a72b5ecep+0, -0x1.df931a09c188dp-4, 0x1.208d41685923cp-8,
+      -0x1.dfbb4be5dcf2fp+1, -0x1.323b80789f944p+0, 0x1.30a337e08db2fp-2, -0x1.eac46c6a420c9p-7,
+      -0x1.0a8c0cb88b9cep+6, 0x1.8984de524548bp+4, -0x1.a7279b5d7f216p+1, 0x1.402bcc73040c6p-3,
+  };
+  static constexpr double fitdbin[12] = {
+      -0x1.90ca45323fb91p+4, 0x1.cf57ce18e36bdp+1, -0x1.0d1f9243e1ba2p-2, 0x1.80153bce76c01p-7,
+      -0x1.f6d08974581a8p+4, 0x1.59a9a85ffab74p+2, -0x1.e71aad5ac5122p-2, 0x1.4b2053d1f4ee7p-6,
+      -0x1.f361ca9e4c27bp+4, 0x1.56a86d4534424p+2, -0x1.e3ac00abd851ep-2, 0x1.4a94d4ac7dcb0p-6,
+  };
+};
+
+struct Mech_decane_dodecane_air {
+  static constexpr int NS = 4;
+  // species: O2, N2, NC10H22, NC12H26
+  static constexpr double W[4] = {
+      0x1.0620ab7132724p-5, 0x1.cafb3b7521145p-6, 0x1.2366d7a56de34p-3, 0x1.5cdb37c99ae93p-3,
+  };
+  static constexpr double Tmid[4] = {
+      0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9,
+  };
+  static constexpr double nasa_lo[28] = {
+      0x1.e42787ae5fa45p+1, -0x1.88c9b66c8c0dfp-9, 0x1.4a6bae1fdae75p-17, -0x1.4ca5927ada53cp-27, 0x1.c883a18b02bcdp-39, -0x1.09fc63497b741p+10, 0x1.d42eb7e3dc88dp+1,
+      0x1.a63b0c4588a05p+1, 0x1.712969da04050p-10, -0x1.09f7a2b098bd4p-18, 0x1.83ae94da0fe59p-28, -0x1.581526048e0d6p-39, -0x1.fe732fec56d5dp+9, 0x1.f9a5ca29845ddp+1,
+      -0x1.0eaf2aaf1c51ep+0, 0x1.da40affa5ba8ap-4, -0x1.08ba1228c5064p-14, 0x1.a99e0e79bf805p-27, 0x1.263e34fed170bp-41, -0x1.0cd583c9995f6p+15, 0x0.0p+0,
+      -0x1.6111f126688bap+0, 0x1.1d25ab9299328p-3, -0x1.448e555a0859dp-14, 0x1.16088218d87cfp-26, 0x1.3bc0d0332262ep-42, -0x1.3a102dfa19de0p+15, 0x0.0p+0,
+  };
+  static constexpr double nasa_hi[28] = {
+      0x1.a42a332f575b8p+1, 0x1.84c851ecf574dp-10, -0x1.96ee255022a3dp-21, 0x1.cca1706e20348p-33, -0x1.8667712a69c18p-46, -0x1.101d4b48d3ae7p+10, 0x1.5d01bdd004baap+2,
+      0x1.769c23b7952d2p+1, 0x1.86106ec5d7fc6p-10, -0x1.3132bf1b89c39p-21, 0x1.bc128a9b8a724p-34, -0x1.e6a15668d1fd9p-48, -0x1.cd661b089a027p+9, 0x1.7ec0f88333fc8p+2,
+      -0x1.0eaf2aaf1c51ep+0, 0x1.da40affa5ba8ap-4, -0x1.08ba1228c5064p-14, 0x1.a99e0e79bf805p-27, 0x1.263e34fed170bp-41, -0x1.0cd583c9995f6p+15, 0x0.0p+0,
+      -0x1.6111f126688bap+0, 0x1.1d25ab9299328p-3, -0x1.448e555a0859dp-14, 0x1.16088218d87cfp-26, 0x1.3bc0d0332262ep-42, -0x1.3a102dfa19de0p+15, 0x0.0p+0,
+  };
+  static constexpr double fitmu[16] = {
+      -0x1.51d857a6480d2p+4, 0x1.b4e92e840fb95p+1, -0x1.8566b3fe3c6fcp-2, 0x1.1d276cd09057bp-6,
+      -0x1.489f8aed55cb2p+4, 0x1.90f18d636097fp+1, -0x1.60179d81699c4p-2, 0x1.036a58b6f18f8p-6,
+      -0x1.442f9ca7437ebp+4, 0x1.97bcac61b1c9cp+0, -0x1.037e8f99be7a4p-7, -0x1.21611dee5e6d7p-8,
+      -0x1.2bd68c93e0b06p+4, 0x1.bb850401e2f9fp-1, 0x1.98f0c9bb4268fp-4, -0x1.3b86cd0b8c289p-7,
+  };
+  static constexpr double fitlam[16] = {
+      -0x1.5593478cc452ep+3, 0x1.c0f28a72b5ecep+0, -0x1.df931a09c188dp-4, 0x1.208d41685923cp-8,
+      -0x1.dfbb4be5dcf2fp+1, -0x1.323b80789f944p+0, 0x1.30a337e08db2fp-2, -0x1.eac46c6a420c9p-7,
+      -0x1.178e99150a736p+6, 0x1.a26407e330b34p+4, -0x1.c5ac8a051c681p+1, 0x1.58d6d5f6dc020p-3,
+      -0x1.0a8c0cb88b9cep+6, 0x1.8984de524548bp+4, -0x1.a7279b5d7f216p+1, 0x1.402bcc73040c6p-3,
+  };
+  static constexpr double fitdbin[24] = {
+      -0x1.90ca45323fb91p+4, 0x1.cf57ce18e36bdp+1, -0x1.0d1f9243e1ba2p-2, 0x1.80153bce76c01p-7,
+      -0x1.f320c6e11f8f4p+4, 0x1.57d2f1b695bd7p+2, -0x1.e52ae3efc8788p-2, 0x1.4b039a416d2fap-6,
+      -0x1.f6d08974581a8p+4, 0x1.59a9a85ffab74p+2, -0x1.e71aad5ac5122p-2, 0x1.4b2053d1f4ee7p-6,
+      -0x1.ef57dd2fb1c33p+4, 0x1.5414866252a6fp+2, -0x1.dff77442f076fp-2, 0x1.4915e218554c0p-6,
+      -0x1.f361ca9e4c27bp+4, 0x1.56a86d4534424p+2, -0x1.e3ac00abd851ep-2, 0x1.4a94d4ac7dcb0p-6,
+      -0x1.c30cde936895ep+4, 0x1.76268e385294dp+1, -0x1.084ad8ec48309p-4, -0x1.4cc67fdbf0f55p-10,
+  };
+};
+
+struct Mech_alkane6_air {
+  static constexpr int NS = 8;
+  // species: O2, N2, NC8H18, NC10H22, NC12H26, NC13H28, NC14H30, NC16H34
+  static constexpr double W[8] = {
+      0x1.0620ab7132724p-5, 0x1.cafb3b7521145p-6, 0x1.d3e4ef0281ba8p-4, 0x1.2366d7a56de34p-3,
+      0x1.5cdb37c99ae93p-3, 0x1.799567dbb16c2p-3, 0x1.964f97edc7ef2p-3, 0x1.cfc3f811f4f51p-3,
+  };
+  static constexpr double Tmid[8] = {
+      0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9,
+      0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9,
+  };
+  static constexpr double nasa_lo[56] = {
+      0x1.e42787ae5fa45p+1, -0x1.88c9b66c8c0dfp-9, 0x1.4a6bae1fdae75p-17, -0x1.4ca5927ada53cp-27, 0x1.c883a18b02bcdp-39, -0x1.09fc63497b741p+10, 0x1.d42eb7e3dc88dp+1,
+      0x1.a63b0c4588a05p+1, 0x1.712969da04050p-10, -0x1.09f7a2b098bd4p-18, 0x1.83ae94da0fe59p-28, -0x1.581526048e0d6p-39, -0x1.fe732fec56d5dp+9, 0x1.f9a5ca29845ddp+1,
+      -0x1.7898c86f9f94cp-1, 0x1.7a3608cf84e5ep-4, -0x1.99cb9def03487p-15, 0x1.272b18c1cda53p-27, 0x1.ae9c01e413896p-41, -0x1.bf35b33231c1ep+14, 0x0.0p+0,
+      -0x1.0eaf2aaf1c51ep+0, 0x1.da40affa5ba8ap-4, -0x1.08ba1228c5064p-14, 0x1.a99e0e79bf805p-27, 0x1.263e34fed170bp-41, -0x1.0cd583c9995f6p+15, 0x0.0p+0,
+      -0x1.6111f126688bap+0, 0x1.1d25ab9299328p-3, -0x1.448e555a0859dp-14, 0x1.16088218d87cfp-26, 0x1.3bc0d0332262ep-42, -0x1.3a102dfa19de0p+15, 0x0.0p+0,
+      -0x1.8a4354620e5d1p+0, 0x1.3528555d4edddp-3, -0x1.627876f2a9f4bp-14, 0x1.36a53f86d4a9bp-26, 0x1.66c6069bcb3c0p-43, -0x1.50ad83125a1dcp+15, 0x0.0p+0,
+      -0x1.b374b79db43e3p+0, 0x1.4d2aff28048a8p-3, -0x1.8062988b4b95fp-14, 0x1.5741fcf4d0ed1p-26, 0x1.5829b34539189p-45, -0x1.674ad82a9a5d7p+15, 0x0.0p+0,
+      -0x1.02ebbf0a80b64p+1, 0x1.7d3052bd6ff2ep-3, -0x1.bc36dbbc8f125p-14, 0x1.987b77d0ca2f7p-26, -0x1.cb6cc6c3c5af5p-43, -0x1.9485825b1adb1p+15, 0x0.0p+0,
+  };
+  static constexpr double nasa_hi[56] = {
+      0x1.a42a332f575b8p+1, 0x1.84c851ecf574dp-10, -0x1.96ee255022a3dp-21, 0x1.cca1706e20348p-33, -0x1.8667712a69c18p-46, -0x1.101d4b48d3ae7p+10, 0x1.5d01bdd004baap+2,
+      0x1.769c23b7952d2p+1, 0x1.86106ec5d7fc6p-10, -0x1.3132bf1b89c39p-21, 0x1.bc128a9b8a724p-34, -0x1.e6a15668d1fd9p-48, -0x1.cd661b089a027p+9, 0x1.7ec0f88333fc8p+2,
+      -0x1.7898c86f9f94cp-1, 0x1.7a3608cf84e5ep-4, -0x1.99cb9def03487p-15, 0x1.272b18c1cda53p-27, 0x1.ae9c01e413896p-41, -0x1.bf35b33231c1ep+14, 0x0.0p+0,
+      -0x1.0eaf2aaf1c51ep+0, 0x1.da40affa5ba8ap-4, -0x1.08ba1228c5064p-14, 0x1.a99e0e79bf805p-27, 0x1.263e34fed170bp-41, -0x1.0cd583c9995f6p+15, 0x0.0p+0,
+      -0x1.6111f126688bap+0, 0x1.1d25ab9299328p-3, -0x1.448e555a0859dp-14, 0x1.16088218d87cfp-26, 0x1.3bc0d0332262ep-42, -0x1.3a102dfa19de0p+15, 0x0.0p+0,
+      -0x1.8a4354620e5d1p+0, 0x1.3528555d4edddp-3, -0x1.627876f2a9f4bp-14, 0x1.36a53f86d4a9bp-26, 0x1.66c6069bcb3c0p-43, -0x1.50ad83125a1dcp+15, 0x0.0p+0,
+      -0x1.b374b79db43e3p+0, 0x1.4d2aff28048a8p-3, -0x1.8062988b4b95fp-14, 0x1.5741fcf4d0ed1p-26, 0x1.5829b34539189p-45, -0x1.674ad82a9a5d7p+15, 0x0.0p+0,
+      -0x1.02ebbf0a80b64p+1, 0x1.7d3052bd6ff2ep-3, -0x1.bc36dbbc8f125p-14, 0x1.987b77d0ca2f7p-26, -0x1.cb6cc6c3c5af5p-43, -0x1.9485825b1adb1p+15, 0x0.0p+0,
+  };
+  static constexpr double fitmu[32] = {
+      -0x1.51d857a6480d2p+4, 0x1.b4e92e840fb95p+1, -0x1.8566b3fe3c6fcp-2, 0x1.1d276cd09057bp-6,
+      -0x1.489f8aed55cb2p+4, 0x1.90f18d636097fp+1, -0x1.60179d81699c4p-2, 0x1.036a58b6f18f8p-6,
+      -0x1.60536b6755ebap+4, 0x1.38b9e4bb0ae9ep+1, -0x1.153cf8c8e970fp-3, 0x1.d8375bb51b085p-10,
+      -0x1.442f9ca7437ebp+4, 0x1.97bcac61b1c9cp+0, -0x1.037e8f99be7a4p-7, -0x1.21611dee5e6d7p-8,
+      -0x1.2bd68c93e0b06p+4, 0x1.bb850401e2f9fp-1, 0x1.98f0c9bb4268fp-4, -0x1.3b86cd0b8c289p-7,
+      -0x1.218bc3b0b6812p+4, 0x1.1d4272a582899p-1, 0x1.29d808f0d8640p-3, -0x1.8378eb81696b7p-7,
+      -0x1.16808fefc9bccp+4, 0x1.d70f7b9221489p-3, 0x1.8c5ae4614dc08p-3, -0x1.cf2c59e91808bp-7,
+      -0x1.040c004bea5fep+4, -0x1.3f266611f96f5p-2, 0x1.1767436efda09p-2, -0x1.25c85aea09f2bp-6,
+  };
+  static constexpr double fitlam[32] = {
+      -0x1.5593478cc452ep+3, 0x1.c0f28a72b5ecep+0, -0x1.df931a09c188dp-4, 0x1.208d41685923cp-8,
+      -0x1.dfbb4be5dcf2fp+1, -0x1.323b80789f944p+0, 0x1.30a337e08db2fp-2, -0x1.eac46c6a420c9p-7,
+      -0x1.285fedb4846d3p+6, 0x1.c2bd35e67e8afp+4, -0x1.edab4b2d1a03dp+1, 0x1.796558d2f6227p-3,
+      -0x1.178e99150a736p+6, 0x1.a26407e330b34p+4, -0x1.c5ac8a051c681p+1, 0x1.58d6d5f6dc020p-3,
+      -0x1.0a8c0cb88b9cep+6, 0x1.8984de524548bp+4, -0x1.a7279b5d7f216p+1, 0x1.402bcc73040c6p-3,
+      -0x1.05383e2ddf299p+6, 0x1.7f4e468a96e13p+4, -0x1.9aabe5e3ad1d5p+1, 0x1.361de05d72d66p-3,
+      -0x1.000e3bfe276f4p+6, 0x1.757a1e37c9e88p+4, -0x1.8eb7cd2278ffbp+1, 0x1.2c8a461670489p-3,
+      -0x1.eee6c05c15c05p+5, 0x1.652ec0c2c6ccbp+4, -0x1.7af1d47e774fap+1, 0x1.1cba0428cb177p-3,
+  };
+  static constexpr double fitdbin[112] = {
+      -0x1.90ca45323fb91p+4, 0x1.cf57ce18e36bdp+1, -0x1.0d1f9243e1ba2p-2, 0x1.80153bce76c01p-7,
+      -0x1.ee1f8c1f9102bp+4, 0x1.54adf67af59f0p+2, -0x1.e0dd88bf94f2dp-2, 0x1.497bcdeacc407p-6,
+      -0x1.f320c6e11f8f4p+4, 0x1.57d2f1b695bd7p+2, -0x1.e52ae3efc8788p-2, 0x1.4b039a416d2fap-6,
+      -0x1.f6d08974581a8p+4, 0x1.59a9a85ffab74p+2, -0x1.e71aad5ac5122p-2, 0x1.4b2053d1f4ee7p-6,
+      -0x1.f861f3d64f91fp+4, 0x1.5a41b86dcf875p+2, -0x1.e78871c02cab0p-2, 0x1.4ae57cee34714p-6,
+      -0x1.f9d438016405fp+4, 0x1.5ac6d45f8f971p+2, -0x1.e7c12cb7301b3p-2, 0x1.4a7cca2919870p-6,
+      -0x1.fc1b15c4ab49ap+4, 0x1.5b68d61d88d96p+2, -0x1.e7a05ad4091bdp-2, 0x1.4973af53daabcp-6,
+      -0x1.e9ed842153b19p+4, 0x1.500e0a7bced13p+2, -0x1.d98b7a3a61de9p-2, 0x1.45e3dc9cd92d6p-6,
+      -0x1.ef57dd2fb1c33p+4, 0x1.5414866252a6fp+2, -0x1.dff77442f076fp-2, 0x1.4915e218554c0p-6,
+      -0x1.f361ca9e4c27bp+4, 0x1.56a86d4534424p+2, -0x1.e3ac00abd851ep-2, 0x1.4a94d4ac7dcb0p-6,
+      -0x1.f51978a0ee115p+4, 0x1.5790d989799b9p+2, -0x1.e4d9827134e5ep-2, 0x1.4aefaa2b45dbbp-6,
+      -0x1.f6b572e6a578ap+4, 0x1.586b3c638ad60p+2, -0x1.e5dd5f099ff58p-2, 0x1.4b2554e44b3dep-6,
+      -0x1.f94232adc61cep+4, 0x1.599b869c14587p+2, -0x1.e70ebff0e9fe1p-2, 0x1.4b23765f5df05p-6,
+      -0x1.d0a4720b1e670p+4, 0x1.ba3be4828d412p+1, -0x1.29b4cc7c1d6e8p-3, 0x1.62e0afa4bc87bp-9,
+      -0x1.cb0ea828a9a8fp+4, 0x1.9d95f5fd79d2ap+1, -0x1.c793d62cc4362p-4, 0x1.0c3882fcdd614p-10,
+      -0x1.c8d20a4edfebbp+4, 0x1.91a5ad9fac311p+1, -0x1.8d87279d834b2p-4, 0x1.55eeee81d6108p-12,
+      -0x1.c654c0d2b4f63p+4, 0x1.851c410fafefcp+1, -0x1.50b3e07ae5838p-4, -0x1.a6a27992ca7cdp-12,
+      -0x1.c23fee7765892p+4, 0x1.7077dc21c4935p+1, -0x1.d9ab4f081560ap-5, -0x1.a2c72f0bddb84p-10,
+      -0x1.c30cde936895ep+4, 0x1.76268e385294dp+1, -0x1.084ad8ec48309p-4, -0x1.4cc67fdbf0f55p-10,
+      -0x1.c09efc1942cf6p+4, 0x1.698d80eede34fp+1, -0x1.96e7b682797d6p-5, -0x1.059ff8cca60dcp-9,
+      -0x1.bdefe2b16d202p+4, 0x1.5c5a9cace903fp+1, -0x1.17ae7ea057dacp-5, -0x1.6908cf1eb00f0p-9,
+      -0x1.b98e735b540ccp+4, 0x1.46afbb8bf1acdp+1, -0x1.1d876e947032cp-7, -0x1.05b7f08e27f7cp-8,
+      -0x1.ba0f4b44febd8p+4, 0x1.49860ea83a476p+1, -0x1.8a7a4a083a2cfp-7, -0x1.f639e1b053a93p-9,
+      -0x1.b73d085fbad5bp+4, 0x1.3bdc4fa7ceb78p+1, 0x1.038b36e4214e8p-8, -0x1.2e1c9ef77c0e1p-8,
+      -0x1.b2a62366b458ep+4, 0x1.257acb9047b8bp+1, 0x1.ed2a0d5b116ffp-6, -0x1.8141da8ed17acp-8,
+      -0x1.b490e53f5d059p+4, 0x1.2e71bade3017ap+1, 0x1.41c79b8f001d2p-6, -0x1.6002e6b8fd6bdp-8,
+      -0x1.afe572d88c8d6p+4, 0x1.17cb410a182d6p+1, 0x1.7933802d5ea3bp-5, -0x1.b3ddc55862e89p-8,
+      -0x1.ace3907279dcap+4, 0x1.09836e390dd0cp+1, 0x1.009f0dd4259c5p-4, -0x1.e87dc6e297e3bp-8,
+  };
+};
+
+}  // namespace pelemp

@@ -1,0 +1,92 @@
+// spray_ctx.h -- device-resident state of one spray level and the launcher
+// interface shared by the host translation unit and the per-form kernel
+// translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pelemp_spray.h"
+
+namespace pelemp {
+
+// Per-update parameters computed on the host (SprayParticles.cpp:183-229).
+struct PassParams {
+  double flow_dt;
+  double sub_dt;
+  double cur_coef;  // -(1/V) sub_dt / flow_dt
+  int do_move;
+  int is_ghost;
+  int at_bounds;
+  const pelemp_fab_view* state;
+  pelemp_fab_view* source;
+};
+
+struct Ctx;
+
+// Launch one update pass of the given form (implemented in the per-form
+// translation units).  Reads buffer ctx.cur via ctx.perm, writes buffer
+// 1-ctx.cur and ctx.key.
+int launch_update_fp64(Ctx& c, const PassParams& p, hipStream_t s);
+int launch_update_fp32(Ctx& c, const PassParams& p, hipStream_t s);
+
+struct Ctx {
+  int mech = 0;
+  int form = 0;
+  int nf = 1;
+  int ns = 3;
+  pelemp_spray_data sd{};
+  pelemp_spray_comps comps{};
+  pelemp_geom geom{};
+  std::vector<double> mw;  // species molecular weights
+
+  int64_t cap = 0;
+  int ncell[3] = {0, 0, 0};
+  int ntile[3] = {0, 0, 0};
+  int ncells = 0;
+  int ntiles = 0;
+  bool hot_brk = false;  // breakup state (BM1, BM2, N0) carried in the hot SoA
+  int nhot = 0;          // hot fields per droplet
+  size_t wsize = 8;      // bytes per hot field value
+
+  void* fbuf[2] = {nullptr, nullptr};
+  int32_t* slot[2] = {nullptr, nullptr};
+  int cur = 0;
+  int32_t* perm = nullptr;
+  int32_t* cstart = nullptr;  // ncells + 1
+  int32_t* cnt = nullptr;
+  int32_t* arrival = nullptr;
+  uint32_t* key = nullptr;  // key of each storage position of buffer cur
+  int32_t* n_entries = nullptr;  // entries in buffer cur (device scalar)
+  int32_t* block_sum = nullptr;
+  int32_t* total = nullptr;
+  int32_t* err = nullptr;
+  int32_t* scalars = nullptr;  // n_entries, total, err in one allocation
+  double* dscratch = nullptr;  // reduction scratch
+  // cold per-slot data (indexed by slot)
+  int64_t* cold_id = nullptr;
+  double* cold_n0 = nullptr;
+  double* cold_bm1 = nullptr;
+  double* cold_bm2 = nullptr;
+  int64_t nslots = 0;  // slots handed out
+  int64_t host_count = 0;  // active droplets after the last sort (host copy, when known)
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double t_ms[4] = {0, 0, 0, 0};
+  int64_t t_n[4] = {0, 0, 0, 0};
+
+  size_t bytes = 0;
+};
+
+hipEvent_t ctx_event(Ctx& c);
+void ctx_time_begin(Ctx& c, int slot, hipStream_t s, hipEvent_t* ev);
+void ctx_time_end(Ctx& c, int slot, hipStream_t s, hipEvent_t ev0);
+void set_last_error(const std::string& msg);
+
+}  // namespace pelemp

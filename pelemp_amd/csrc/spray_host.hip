@@ -1,0 +1,852 @@
+// spray_host.hip -- C ABI of the spray droplet update (include/pelemp_spray.h).
+//
+// Owns the device-resident droplet set of one AMR level: hot droplet fields
+// in ping-pong SoA buffers sorted by cell (tile-major key), cold per-slot
+// data (id, breakup state when breakup is off), the sort permutation and the
+// CSR cell offsets.  One update pass = fused update kernel + deterministic
+// counting sort; no host synchronisation inside the pass chain.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "droplet_physics.h"
+#include "gas_props.h"
+#include "pm_math.h"
+#include "sort_kernels.h"
+#include "spray_ctx.h"
+#include "update_kernel.h"
+
+struct pelemp_spray_ctx {
+  pelemp::Ctx c;
+};
+
+namespace pelemp {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+#define PM_HIP(expr)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+      return PELEMP_E_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+hipEvent_t ctx_event(Ctx& c) {
+  if (!c.ev_pool.empty()) {
+    hipEvent_t e = c.ev_pool.back();
+    c.ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void ctx_time_begin(Ctx& c, int slot, hipStream_t s, hipEvent_t* ev) {
+  (void)slot;
+  *ev = nullptr;
+  if (!c.timing) return;
+  *ev = ctx_event(c);
+  if (*ev) hipEventRecord(*ev, s);
+}
+
+void ctx_time_end(Ctx& c, int slot, hipStream_t s, hipEvent_t ev0) {
+  if (!c.timing || !ev0) return;
+  hipEvent_t ev1 = ctx_event(c);
+  if (!ev1) return;
+  hipEventRecord(ev1, s);
+  c.ev_pending.push_back({slot, {ev0, ev1}});
+}
+
+static void harvest_timing(Ctx& c) {
+  for (auto& pe : c.ev_pending) {
+    hipEventSynchronize(pe.second.second);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+    c.t_ms[pe.first] += ms;
+    c.t_n[pe.first] += 1;
+    c.ev_pool.push_back(pe.second.first);
+    c.ev_pool.push_back(pe.second.second);
+  }
+  c.ev_pending.clear();
+}
+
+// ---------------------------------------------------------------------------
+// helper kernels
+
+// AoS host-layout particles -> SoA buffer `cur` (storage order = input order)
+template <class R, bool POS64>
+__global__ void __launch_bounds__(256) ingest_kernel(int64_t n, int nf, int nsr, const double* __restrict__ pos,
+                                                     const double* __restrict__ rdata,
+                                                     const int64_t* __restrict__ id, R* __restrict__ f,
+                                                     int64_t cap, int hot_brk, int32_t* __restrict__ slot,
+                                                     int64_t slot_base, int64_t* __restrict__ cold_id,
+                                                     double* __restrict__ cold_n0, double* __restrict__ cold_bm1,
+                                                     double* __restrict__ cold_bm2, uint32_t* __restrict__ key,
+                                                     double plo0, double plo1, double plo2, double dxi0,
+                                                     double dxi1, double dxi2, int nc0, int nc1, int nc2,
+                                                     int nt0, int nt1, int nt2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double plo[3] = {plo0, plo1, plo2};
+  const double dxi[3] = {dxi0, dxi1, dxi2};
+  const int nc[3] = {nc0, nc1, nc2};
+  const int nt[3] = {nt0, nt1, nt2};
+  int c[3];
+  for (int q = 0; q < 3; ++q) {
+    const double p = pos[3 * i + q];
+    const double lxc = (p - plo[q]) * dxi[q];
+    int ic = static_cast<int>(floor(lxc));
+    ic = ic < 0 ? 0 : (ic >= nc[q] ? nc[q] - 1 : ic);
+    c[q] = ic;
+    if constexpr (POS64) {
+      f[q * cap + i] = static_cast<R>(p);
+    } else {
+      double xi = lxc - static_cast<double>(ic);
+      xi = xi < 0. ? 0. : xi;
+      R xr = static_cast<R>(xi);
+      if (xr >= R(1)) xr = R(0.99999994f);
+      f[q * cap + i] = xr;
+    }
+  }
+  const double* r = rdata + static_cast<int64_t>(nsr) * i;
+  // Fields<NF>: VEL 3..5, T 6, DIA 7, Y 8..7+nf, ND 8+nf, FILM 9+nf, BM1 10+nf, BM2 11+nf, N0 12+nf
+  for (int q = 0; q < 3; ++q) f[(3 + q) * cap + i] = static_cast<R>(r[q]);
+  f[6 * cap + i] = static_cast<R>(r[3]);
+  f[7 * cap + i] = static_cast<R>(r[4]);
+  for (int s = 0; s < nf; ++s) f[(8 + s) * cap + i] = static_cast<R>(r[5 + s]);
+  f[(8 + nf) * cap + i] = static_cast<R>(r[5 + nf]);  // numDens
+  f[(9 + nf) * cap + i] = static_cast<R>(r[9 + nf]);  // FilmHght
+  const int64_t sl = slot_base + i;
+  if (hot_brk) {
+    f[(10 + nf) * cap + i] = static_cast<R>(r[7 + nf]);
+    f[(11 + nf) * cap + i] = static_cast<R>(r[8 + nf]);
+    f[(12 + nf) * cap + i] = static_cast<R>(r[6 + nf]);
+  } else {
+    cold_n0[sl] = r[6 + nf];
+    cold_bm1[sl] = r[7 + nf];
+    cold_bm2[sl] = r[8 + nf];
+  }
+  slot[i] = static_cast<int32_t>(sl);
+  cold_id[sl] = id[i];
+  key[i] = tile_key(c[0], c[1], c[2], nt);
+}
+
+__device__ inline void decode_key(uint32_t k, const int* nt, int* c) {
+  const uint32_t tile = k / kTileCells;
+  const uint32_t l = k % kTileCells;
+  const uint32_t txy = static_cast<uint32_t>(nt[0]) * nt[1];
+  c[2] = static_cast<int>((tile / txy) * kTile + l / (kTile * kTile));
+  c[1] = static_cast<int>(((tile % txy) / nt[0]) * kTile + (l / kTile) % kTile);
+  c[0] = static_cast<int>((tile % nt[0]) * kTile + l % kTile);
+}
+
+// sorted view -> host-layout AoS
+template <class R, bool POS64>
+__global__ void __launch_bounds__(256) gather_kernel(const int32_t* __restrict__ total, int nf, int nsr,
+                                                     const int32_t* __restrict__ perm, const R* __restrict__ f,
+                                                     int64_t cap, int hot_brk, const int32_t* __restrict__ slot,
+                                                     const int64_t* __restrict__ cold_id,
+                                                     const double* __restrict__ cold_n0,
+                                                     const double* __restrict__ cold_bm1,
+                                                     const double* __restrict__ cold_bm2,
+                                                     const uint32_t* __restrict__ key, double* __restrict__ pos,
+                                                     double* __restrict__ rdata, int64_t* __restrict__ id,
+                                                     double plo0, double plo1, double plo2, double dx0, double dx1,
+                                                     double dx2, int nt0, int nt1, int nt2) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= *total) return;
+  const int32_t o = perm[s];
+  if constexpr (POS64) {
+    for (int q = 0; q < 3; ++q) pos[3 * s + q] = static_cast<double>(f[q * cap + o]);
+  } else {
+    const int nt[3] = {nt0, nt1, nt2};
+    int c[3];
+    decode_key(key[o], nt, c);
+    const double plo[3] = {plo0, plo1, plo2};
+    const double dx[3] = {dx0, dx1, dx2};
+    for (int q = 0; q < 3; ++q)
+      pos[3 * s + q] = plo[q] + (static_cast<double>(c[q]) + static_cast<double>(f[q * cap + o])) * dx[q];
+  }
+  double* r = rdata + static_cast<int64_t>(nsr) * s;
+  for (int q = 0; q < 3; ++q) r[q] = static_cast<double>(f[(3 + q) * cap + o]);
+  r[3] = static_cast<double>(f[6 * cap + o]);
+  r[4] = static_cast<double>(f[7 * cap + o]);
+  for (int sp = 0; sp < nf; ++sp) r[5 + sp] = static_cast<double>(f[(8 + sp) * cap + o]);
+  r[5 + nf] = static_cast<double>(f[(8 + nf) * cap + o]);
+  r[9 + nf] = static_cast<double>(f[(9 + nf) * cap + o]);
+  const int32_t sl = slot[o];
+  if (hot_brk) {
+    r[7 + nf] = static_cast<double>(f[(10 + nf) * cap + o]);
+    r[8 + nf] = static_cast<double>(f[(11 + nf) * cap + o]);
+    r[6 + nf] = static_cast<double>(f[(12 + nf) * cap + o]);
+  } else {
+    r[6 + nf] = cold_n0[sl];
+    r[7 + nf] = cold_bm1[sl];
+    r[8 + nf] = cold_bm2[sl];
+  }
+  id[s] = cold_id[sl];
+}
+
+// particle-CFL time step: min over droplets of cfl / max_d(|u_d|/dx_d)
+// (SprayParticles.cpp:112-124).  Positive doubles order like their bit
+// patterns, so an integer atomicMin gives the exact minimum.
+template <class R>
+__global__ void __launch_bounds__(256) est_dt_kernel(const int32_t* __restrict__ total,
+                                                     const int32_t* __restrict__ perm, const R* __restrict__ f,
+                                                     int64_t cap, double dxi0, double dxi1, double dxi2, double cfl,
+                                                     unsigned long long* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 1.0e300;
+  const int32_t n = *total;
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int32_t o = perm[s];
+    const double u0 = fabs(static_cast<double>(f[3 * cap + o])) * dxi0;
+    const double u1 = fabs(static_cast<double>(f[4 * cap + o])) * dxi1;
+    const double u2 = fabs(static_cast<double>(f[5 * cap + o])) * dxi2;
+    double mx = u0;
+    mx = (mx < u1) ? u1 : mx;
+    mx = (mx < u2) ? u2 : mx;
+    const double dtp = (mx > 0.) ? (cfl / mx) : 1.E50;
+    m = dtp < m ? dtp : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(w)) {
+      const double o2 = red[threadIdx.x + w];
+      red[threadIdx.x] = o2 < red[threadIdx.x] ? o2 : red[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    unsigned long long bits;
+    const double v = red[0];
+    memcpy(&bits, &v, sizeof(bits));
+    atomicMin(out, bits);
+  }
+}
+
+__global__ void __launch_bounds__(256) math_eval_kernel(int fn, int64_t n, const double* __restrict__ x,
+                                                        const double* __restrict__ y, double* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = x[i];
+  const double b = y ? y[i] : 0.0;
+  double r = 0.0;
+  switch (fn) {
+    case 0: r = pm::exp(a); break;
+    case 1: r = pm::log(a); break;
+    case 2: r = pm::log1p(a); break;
+    case 3: r = pm::pow(a, b); break;
+    case 4: r = pm::cbrt(a); break;
+    case 5: r = pm::sin(a); break;
+    case 6: r = pm::cos(a); break;
+    case 7: r = pm::atan2(a, b); break;
+    case 8: r = pm::acos(a); break;
+    case 9: r = sqrt(a); break;
+    case 10: r = a / b; break;
+    default: r = 0.0;
+  }
+  out[i] = r;
+}
+
+// ---------------------------------------------------------------------------
+
+static int run_sort(Ctx& c, hipStream_t s) {
+  hipEvent_t ev0;
+  ctx_time_begin(c, 1, s, &ev0);
+  // entries in the buffer written last = active count of the previous sort
+  PM_HIP(hipMemcpyAsync(c.n_entries, c.cstart + c.ncells, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  PM_HIP(hipMemsetAsync(c.cnt, 0, sizeof(int32_t) * c.ncells, s));
+  const int grid_s = 2048;
+  hipLaunchKernelGGL(count_keys_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.cnt, c.arrival);
+  const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, c.ncells, c.cstart, c.block_sum);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.block_sum, nb, c.total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total);
+  hipLaunchKernelGGL(scatter_perm_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
+                     c.perm);
+  hipLaunchKernelGGL(order_cells_kernel, dim3((c.ncells + 255) / 256), dim3(256), 0, s, c.cstart, c.ncells,
+                     c.perm);
+  PM_HIP(hipGetLastError());
+  ctx_time_end(c, 1, s, ev0);
+  return PELEMP_OK;
+}
+
+static int alloc(Ctx& c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    set_last_error("hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+    return PELEMP_E_NOMEM;
+  }
+  c.bytes += bytes;
+  return PELEMP_OK;
+}
+
+static void free_ctx(Ctx& c) {
+  void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt, c.arrival, c.key,
+                  c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& pe : c.ev_pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
+}
+
+static int mech_ns(int mech) {
+  switch (mech) {
+    case PELEMP_MECH_DODECANE_AIR: return Mech_dodecane_air::NS;
+    case PELEMP_MECH_DECANE_DODECANE_AIR: return Mech_decane_dodecane_air::NS;
+    case PELEMP_MECH_ALKANE6_AIR: return Mech_alkane6_air::NS;
+    default: return -1;
+  }
+}
+
+static void mech_mw(int mech, std::vector<double>& mw) {
+  mw.clear();
+  auto fill = [&](auto tag) {
+    using M = decltype(tag);
+    for (int n = 0; n < M::NS; ++n) mw.push_back(M::W[n]);
+  };
+  switch (mech) {
+    case PELEMP_MECH_DODECANE_AIR: fill(Mech_dodecane_air{}); break;
+    case PELEMP_MECH_DECANE_DODECANE_AIR: fill(Mech_decane_dodecane_air{}); break;
+    case PELEMP_MECH_ALKANE6_AIR: fill(Mech_alkane6_air{}); break;
+    default: break;
+  }
+}
+
+static void mech_h(int mech, double T, double* h) {
+  auto run = [&](auto tag) {
+    using M = decltype(tag);
+    GasProps<M>::T2Hi(T, h);
+  };
+  switch (mech) {
+    case PELEMP_MECH_DODECANE_AIR: run(Mech_dodecane_air{}); break;
+    case PELEMP_MECH_DECANE_DODECANE_AIR: run(Mech_decane_dodecane_air{}); break;
+    case PELEMP_MECH_ALKANE6_AIR: run(Mech_alkane6_air{}); break;
+    default: break;
+  }
+}
+
+}  // namespace pelemp
+
+using namespace pelemp;
+
+extern "C" {
+
+const char* pelemp_last_error(void) { return g_last_error.c_str(); }
+
+int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, const pelemp_spray_data* sd,
+                        const pelemp_spray_comps* comps, const pelemp_geom* geom, int64_t max_particles,
+                        int32_t apply_latent_shift) {
+  if (!out || !sd || !comps || !geom || max_particles <= 0 || max_particles > 0x7ffffff0LL) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  *out = nullptr;
+  const int ns = mech_ns(mech);
+  if (ns < 0) {
+    set_last_error("unknown mechanism");
+    return PELEMP_E_ARG;
+  }
+  if (form != PELEMP_FORM_FP64 && form != PELEMP_FORM_FP32) {
+    set_last_error("unsupported form");
+    return PELEMP_E_ARG;
+  }
+  if (sd->nfuel < 1 || sd->nfuel > PELEMP_MAX_FUEL) {
+    set_last_error("nfuel out of range");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  for (int i = 0; i < sd->nfuel; ++i) {
+    if (sd->indx[i] < 0 || sd->indx[i] >= ns || sd->dep_indx[i] < 0 || sd->dep_indx[i] >= ns) {
+      set_last_error("fuel species index not in the mechanism");  // SpraySetup.cpp:272-277
+      return PELEMP_E_BAD_CONFIG;
+    }
+  }
+  if (sd->do_breakup != 0 || sd->do_splash != 0) {
+    set_last_error("breakup / splash are not enabled in this build of the update");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  auto* h = new pelemp_spray_ctx();
+  Ctx& c = h->c;
+  c.mech = mech;
+  c.form = form;
+  c.nf = sd->nfuel;
+  c.ns = ns;
+  c.sd = *sd;
+  c.comps = *comps;
+  c.geom = *geom;
+  mech_mw(mech, c.mw);
+  if (apply_latent_shift) {
+    // latent = ref_latent - h_gas,fuel(ref_T)   (SpraySetup.cpp:284-290)
+    std::vector<double> hh(ns);
+    mech_h(mech, sd->ref_T, hh.data());
+    for (int i = 0; i < c.nf; ++i) c.sd.latent[i] = sd->ref_latent[i] - hh[sd->indx[i]];
+  }
+  for (int d = 0; d < 3; ++d) {
+    c.ncell[d] = geom->domain_hi[d] - geom->domain_lo[d] + 1;
+    if (c.ncell[d] <= 0 || c.ncell[d] % kTile != 0) {
+      delete h;
+      set_last_error("domain extent must be a positive multiple of 4 in every direction");
+      return PELEMP_E_ARG;
+    }
+    c.ntile[d] = c.ncell[d] / kTile;
+  }
+  c.ncells = c.ncell[0] * c.ncell[1] * c.ncell[2];
+  c.ntiles = c.ntile[0] * c.ntile[1] * c.ntile[2];
+  c.cap = max_particles;
+  c.hot_brk = false;
+  c.nhot = c.hot_brk ? (13 + c.nf) : (10 + c.nf);
+  c.wsize = (form == PELEMP_FORM_FP64) ? 8 : 4;
+  int st = PELEMP_OK;
+  const size_t fb = c.wsize * c.nhot * static_cast<size_t>(c.cap);
+  st |= alloc(c, &c.fbuf[0], fb);
+  st |= alloc(c, &c.fbuf[1], fb);
+  st |= alloc(c, reinterpret_cast<void**>(&c.slot[0]), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.slot[1]), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.perm), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.cstart), 4 * (static_cast<size_t>(c.ncells) + 1));
+  st |= alloc(c, reinterpret_cast<void**>(&c.cnt), 4 * static_cast<size_t>(c.ncells));
+  st |= alloc(c, reinterpret_cast<void**>(&c.arrival), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.key), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.block_sum),
+              4 * (static_cast<size_t>(c.ncells) / kScanBlock + 2));
+  st |= alloc(c, reinterpret_cast<void**>(&c.scalars), 64);
+  st |= alloc(c, reinterpret_cast<void**>(&c.dscratch), 64);
+  st |= alloc(c, reinterpret_cast<void**>(&c.cold_id), 8 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.cold_n0), 8 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm1), 8 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm2), 8 * c.cap);
+  if (st != PELEMP_OK) {
+    free_ctx(c);
+    delete h;
+    return PELEMP_E_NOMEM;
+  }
+  c.n_entries = c.scalars;
+  c.total = c.scalars + 1;
+  c.err = c.scalars + 2;
+  if (hipMemset(c.scalars, 0, 64) != hipSuccess || hipMemset(c.cstart, 0, 4 * (static_cast<size_t>(c.ncells) + 1)) != hipSuccess) {
+    free_ctx(c);
+    delete h;
+    set_last_error("hipMemset failed");
+    return PELEMP_E_HIP;
+  }
+  *out = h;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_destroy(pelemp_spray_ctx* h) {
+  if (!h) return PELEMP_OK;
+  (void)hipDeviceSynchronize();
+  free_ctx(h->c);
+  delete h;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_get_data(const pelemp_spray_ctx* h, pelemp_spray_data* out) {
+  if (!h || !out) return PELEMP_E_ARG;
+  *out = h->c.sd;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos, const double* rdata,
+                               const int64_t* id) {
+  if (!h || n < 0 || (n > 0 && (!pos || !rdata || !id))) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  const int nsr = 10 + c.nf;
+  // keep only active particles (id > 0)
+  std::vector<double> hp, hr;
+  std::vector<int64_t> hid;
+  hp.reserve(3 * n);
+  hr.reserve(nsr * n);
+  hid.reserve(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (id[i] <= 0) continue;
+    hp.insert(hp.end(), pos + 3 * i, pos + 3 * i + 3);
+    hr.insert(hr.end(), rdata + nsr * i, rdata + nsr * i + nsr);
+    hid.push_back(id[i]);
+  }
+  const int64_t m = static_cast<int64_t>(hid.size());
+  if (m > c.cap) {
+    set_last_error("more particles than the context capacity");
+    return PELEMP_E_ARG;
+  }
+  // domain check (SprayParticles.cpp:347-351 aborts on out-of-domain particles)
+  for (int64_t i = 0; i < m; ++i)
+    for (int d = 0; d < 3; ++d)
+      if (!(hp[3 * i + d] >= c.geom.plo[d] && hp[3 * i + d] <= c.geom.phi[d])) {
+        set_last_error("particle outside the domain");
+        return PELEMP_E_LEFT_DOMAIN;
+      }
+  double *dpos = nullptr, *drd = nullptr;
+  int64_t* did = nullptr;
+  PM_HIP(hipMalloc(&dpos, 8 * std::max<int64_t>(1, 3 * m)));
+  PM_HIP(hipMalloc(&drd, 8 * std::max<int64_t>(1, nsr * m)));
+  PM_HIP(hipMalloc(&did, 8 * std::max<int64_t>(1, m)));
+  if (m > 0) {
+    PM_HIP(hipMemcpy(dpos, hp.data(), 8 * 3 * m, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(drd, hr.data(), 8 * nsr * m, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(did, hid.data(), 8 * m, hipMemcpyHostToDevice));
+  }
+  c.cur = 0;
+  c.nslots = 0;
+  const double dxi[3] = {1. / c.geom.dx[0], 1. / c.geom.dx[1], 1. / c.geom.dx[2]};
+  if (m > 0) {
+    const dim3 grid(static_cast<unsigned>((m + 255) / 256)), block(256);
+    if (c.form == PELEMP_FORM_FP64) {
+      hipLaunchKernelGGL((ingest_kernel<double, true>), grid, block, 0, 0, m, c.nf, nsr, dpos, drd, did,
+                         static_cast<double*>(c.fbuf[0]), c.cap, c.hot_brk ? 1 : 0, c.slot[0], c.nslots,
+                         c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, c.geom.plo[0], c.geom.plo[1],
+                         c.geom.plo[2], dxi[0], dxi[1], dxi[2], c.ncell[0], c.ncell[1], c.ncell[2],
+                         c.ntile[0], c.ntile[1], c.ntile[2]);
+    } else {
+      hipLaunchKernelGGL((ingest_kernel<float, false>), grid, block, 0, 0, m, c.nf, nsr, dpos, drd, did,
+                         static_cast<float*>(c.fbuf[0]), c.cap, c.hot_brk ? 1 : 0, c.slot[0], c.nslots,
+                         c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, c.geom.plo[0], c.geom.plo[1],
+                         c.geom.plo[2], dxi[0], dxi[1], dxi[2], c.ncell[0], c.ncell[1], c.ncell[2],
+                         c.ntile[0], c.ntile[1], c.ntile[2]);
+    }
+    PM_HIP(hipGetLastError());
+  }
+  c.nslots = m;
+  // the sort reads the entry count from cstart[ncells]
+  const int32_t m32 = static_cast<int32_t>(m);
+  PM_HIP(hipMemcpy(c.cstart + c.ncells, &m32, 4, hipMemcpyHostToDevice));
+  PM_HIP(hipMemset(c.err, 0, 4));
+  const int st = run_sort(c, 0);
+  PM_HIP(hipDeviceSynchronize());
+  (void)hipFree(dpos);
+  (void)hipFree(drd);
+  (void)hipFree(did);
+  c.host_count = m;
+  return st;
+}
+
+int pelemp_spray_num_particles(pelemp_spray_ctx* h, int64_t* n) {
+  if (!h || !n) return PELEMP_E_ARG;
+  int32_t t = 0;
+  PM_HIP(hipDeviceSynchronize());
+  PM_HIP(hipMemcpy(&t, h->c.cstart + h->c.ncells, 4, hipMemcpyDeviceToHost));
+  *n = t;
+  h->c.host_count = t;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_get_particles(pelemp_spray_ctx* h, int64_t* n, double* pos, double* rdata, int64_t* id) {
+  if (!h || !n) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  int64_t m = 0;
+  int st = pelemp_spray_num_particles(h, &m);
+  if (st) return st;
+  if (m > *n) {
+    *n = m;
+    set_last_error("output capacity too small");
+    return PELEMP_E_ARG;
+  }
+  const int nsr = 10 + c.nf;
+  double *dpos = nullptr, *drd = nullptr;
+  int64_t* did = nullptr;
+  PM_HIP(hipMalloc(&dpos, 8 * std::max<int64_t>(1, 3 * m)));
+  PM_HIP(hipMalloc(&drd, 8 * std::max<int64_t>(1, nsr * m)));
+  PM_HIP(hipMalloc(&did, 8 * std::max<int64_t>(1, m)));
+  if (m > 0) {
+    const dim3 grid(static_cast<unsigned>((m + 255) / 256)), block(256);
+    if (c.form == PELEMP_FORM_FP64) {
+      hipLaunchKernelGGL((gather_kernel<double, true>), grid, block, 0, 0, c.cstart + c.ncells, c.nf, nsr, c.perm,
+                         static_cast<const double*>(c.fbuf[c.cur]), c.cap, c.hot_brk ? 1 : 0, c.slot[c.cur],
+                         c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, dpos, drd, did, c.geom.plo[0],
+                         c.geom.plo[1], c.geom.plo[2], c.geom.dx[0], c.geom.dx[1], c.geom.dx[2], c.ntile[0],
+                         c.ntile[1], c.ntile[2]);
+    } else {
+      hipLaunchKernelGGL((gather_kernel<float, false>), grid, block, 0, 0, c.cstart + c.ncells, c.nf, nsr, c.perm,
+                         static_cast<const float*>(c.fbuf[c.cur]), c.cap, c.hot_brk ? 1 : 0, c.slot[c.cur],
+                         c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, dpos, drd, did, c.geom.plo[0],
+                         c.geom.plo[1], c.geom.plo[2], c.geom.dx[0], c.geom.dx[1], c.geom.dx[2], c.ntile[0],
+                         c.ntile[1], c.ntile[2]);
+    }
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipDeviceSynchronize());
+    if (pos) PM_HIP(hipMemcpy(pos, dpos, 8 * 3 * m, hipMemcpyDeviceToHost));
+    if (rdata) PM_HIP(hipMemcpy(rdata, drd, 8 * nsr * m, hipMemcpyDeviceToHost));
+    if (id) PM_HIP(hipMemcpy(id, did, 8 * m, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(dpos);
+  (void)hipFree(drd);
+  (void)hipFree(did);
+  *n = m;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,
+                        double flow_dt, double time, int32_t state_ghosts, int32_t source_ghosts, int32_t isVirt,
+                        int32_t isGhost, int32_t do_move, double spray_cfl_lev, void* stream) {
+  (void)time;
+  (void)state_ghosts;
+  (void)source_ghosts;
+  (void)isVirt;
+  if (!h || !state || !source || !state->data || !source->data) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  Ctx& c = h->c;
+  const pelemp_spray_comps& S = c.comps;
+  const int st_need = std::max({S.rhoIndx, S.momIndx + 2, S.specIndx + c.ns - 1, S.utempIndx});
+  const int sr_need = std::max({S.rhoSrcIndx, S.momSrcIndx + 2, S.specSrcIndx + c.nf - 1, S.engSrcIndx});
+  if (state->ncomp <= st_need || source->ncomp <= sr_need) {
+    set_last_error("state or source FAB has too few components for SprayComps");
+    return PELEMP_E_ARG;
+  }
+  if (!(flow_dt > 0.)) {
+    set_last_error("flow_dt must be positive");
+    return PELEMP_E_ARG;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // particle sub-cycles (SprayParticles.cpp:221-229)
+  const double sub_cfl = 0.5;
+  double sub_dt = flow_dt;
+  int num_iter = 1;
+  if (do_move && spray_cfl_lev > sub_cfl) {
+    num_iter = static_cast<int>(std::ceil(spray_cfl_lev / sub_cfl));
+    sub_dt = flow_dt / static_cast<double>(num_iter);
+  }
+  const double vol = c.geom.dx[0] * c.geom.dx[1] * c.geom.dx[2];
+  const double inv_vol = 1. / vol;
+  PassParams p;
+  p.flow_dt = flow_dt;
+  p.sub_dt = sub_dt;
+  p.cur_coef = -inv_vol * sub_dt / flow_dt;
+  p.do_move = do_move ? 1 : 0;
+  p.is_ghost = isGhost ? 1 : 0;
+  p.at_bounds = 0;
+  for (int d = 0; d < 3; ++d)
+    if (c.geom.bndry_lo[d] != 0 || c.geom.bndry_hi[d] != 0) p.at_bounds = 1;
+  p.state = state;
+  p.source = source;
+  for (int it = 0; it < num_iter; ++it) {
+    hipEvent_t ev0;
+    ctx_time_begin(c, 0, s, &ev0);
+    const int st = (c.form == PELEMP_FORM_FP64) ? launch_update_fp64(c, p, s) : launch_update_fp32(c, p, s);
+    if (st != PELEMP_OK) return st;
+    ctx_time_end(c, 0, s, ev0);
+    c.cur = 1 - c.cur;
+    const int st2 = run_sort(c, s);
+    if (st2 != PELEMP_OK) return st2;
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_spray_move_kick_drift(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,
+                                 double dt, double time, int32_t isVirt, int32_t isGhost, int32_t state_ghosts,
+                                 int32_t source_ghosts, int32_t do_move, double spray_cfl_lev, void* stream) {
+  return pelemp_spray_update(h, state, source, dt, time, state_ghosts, source_ghosts, isVirt, isGhost, do_move,
+                             spray_cfl_lev, stream);
+}
+
+// Waits for the stream and returns the sticky device error (0 if none).
+int pelemp_spray_sync(pelemp_spray_ctx* h, void* stream) {
+  if (!h) return PELEMP_E_ARG;
+  PM_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  int32_t e = 0;
+  PM_HIP(hipMemcpy(&e, h->c.err, 4, hipMemcpyDeviceToHost));
+  if (e != 0) {
+    set_last_error("device-side check failed (code " + std::to_string(e) + ")");
+    PM_HIP(hipMemset(h->c.err, 0, 4));
+  }
+  return e;
+}
+
+int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double* dt, void* stream) {
+  if (!h || !dt) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  if (c.sd.fixed_parts) {
+    *dt = -1.;
+    return PELEMP_OK;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned long long init = 0x7fefffffffffffffULL;  // DBL_MAX bits
+  PM_HIP(hipMemcpyAsync(c.dscratch, &init, 8, hipMemcpyHostToDevice, s));
+  const double dxi[3] = {1. / c.geom.dx[0], 1. / c.geom.dx[1], 1. / c.geom.dx[2]};
+  auto* out = reinterpret_cast<unsigned long long*>(c.dscratch);
+  if (c.form == PELEMP_FORM_FP64) {
+    hipLaunchKernelGGL((est_dt_kernel<double>), dim3(1024), dim3(256), 0, s, c.cstart + c.ncells, c.perm,
+                       static_cast<const double*>(c.fbuf[c.cur]), c.cap, dxi[0], dxi[1], dxi[2], cfl, out);
+  } else {
+    hipLaunchKernelGGL((est_dt_kernel<float>), dim3(1024), dim3(256), 0, s, c.cstart + c.ncells, c.perm,
+                       static_cast<const float*>(c.fbuf[c.cur]), c.cap, dxi[0], dxi[1], dxi[2], cfl, out);
+  }
+  PM_HIP(hipGetLastError());
+  unsigned long long bits = 0;
+  PM_HIP(hipMemcpyAsync(&bits, c.dscratch, 8, hipMemcpyDeviceToHost, s));
+  PM_HIP(hipStreamSynchronize(s));
+  double v;
+  memcpy(&v, &bits, 8);
+  int32_t t = 0;
+  PM_HIP(hipMemcpy(&t, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
+  *dt = (t == 0) ? -1. : v;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_set_timing(pelemp_spray_ctx* h, int32_t enable) {
+  if (!h) return PELEMP_E_ARG;
+  h->c.timing = enable != 0;
+  harvest_timing(h->c);
+  for (int i = 0; i < 4; ++i) {
+    h->c.t_ms[i] = 0;
+    h->c.t_n[i] = 0;
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_spray_get_timing(pelemp_spray_ctx* h, double* ms, int64_t* launches, int32_t nslots) {
+  if (!h) return PELEMP_E_ARG;
+  harvest_timing(h->c);
+  for (int i = 0; i < nslots && i < 4; ++i) {
+    if (ms) ms[i] = h->c.t_ms[i];
+    if (launches) launches[i] = h->c.t_n[i];
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_spray_device_bytes(const pelemp_spray_ctx* h, int64_t* bytes) {
+  if (!h || !bytes) return PELEMP_E_ARG;
+  *bytes = static_cast<int64_t>(h->c.bytes);
+  return PELEMP_OK;
+}
+
+int pelemp_math_eval(int32_t fn, int64_t n, const double* x, const double* y, double* out) {
+  if (n <= 0) return PELEMP_OK;
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  PM_HIP(hipMalloc(&dx, 8 * n));
+  PM_HIP(hipMalloc(&dout, 8 * n));
+  PM_HIP(hipMemcpy(dx, x, 8 * n, hipMemcpyHostToDevice));
+  if (y) {
+    PM_HIP(hipMalloc(&dy, 8 * n));
+    PM_HIP(hipMemcpy(dy, y, 8 * n, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(math_eval_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, 0, fn, n, dx, dy,
+                     dout);
+  PM_HIP(hipGetLastError());
+  PM_HIP(hipMemcpy(out, dout, 8 * n, hipMemcpyDeviceToHost));
+  (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  (void)hipFree(dout);
+  return PELEMP_OK;
+}
+
+int pelemp_device_count(int32_t* n) {
+  if (!n) return PELEMP_E_ARG;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return PELEMP_OK;
+}
+
+int pelemp_device_malloc(void** ptr, int64_t bytes) {
+  if (!ptr || bytes < 0) return PELEMP_E_ARG;
+  if (hipMalloc(ptr, static_cast<size_t>(bytes > 0 ? bytes : 8)) != hipSuccess) {
+    set_last_error("hipMalloc failed");
+    return PELEMP_E_NOMEM;
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_device_free(void* ptr) {
+  if (ptr) PM_HIP(hipFree(ptr));
+  return PELEMP_OK;
+}
+
+int pelemp_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+  if (bytes > 0) PM_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+  return PELEMP_OK;
+}
+
+int pelemp_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+  if (bytes > 0) PM_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+  return PELEMP_OK;
+}
+
+int pelemp_memcpy_dtod(void* dst, const void* src, int64_t bytes) {
+  if (bytes > 0) PM_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice));
+  return PELEMP_OK;
+}
+
+int pelemp_memset_device(void* ptr, int32_t value, int64_t bytes) {
+  if (bytes > 0) PM_HIP(hipMemset(ptr, value, static_cast<size_t>(bytes)));
+  return PELEMP_OK;
+}
+
+int pelemp_device_synchronize(void) {
+  PM_HIP(hipDeviceSynchronize());
+  return PELEMP_OK;
+}
+
+int pelemp_stream_create(void** stream) {
+  if (!stream) return PELEMP_E_ARG;
+  hipStream_t s = nullptr;
+  PM_HIP(hipStreamCreate(&s));  // default priority
+  *stream = s;
+  return PELEMP_OK;
+}
+
+int pelemp_stream_destroy(void* stream) {
+  if (stream) PM_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return PELEMP_OK;
+}
+
+int pelemp_stream_synchronize(void* stream) {
+  PM_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return PELEMP_OK;
+}
+
+struct pelemp_timer {
+  hipEvent_t e[2];
+};
+
+int pelemp_timer_create(void** timer) {
+  if (!timer) return PELEMP_E_ARG;
+  auto* t = new pelemp_timer();
+  if (hipEventCreate(&t->e[0]) != hipSuccess || hipEventCreate(&t->e[1]) != hipSuccess) {
+    delete t;
+    set_last_error("hipEventCreate failed");
+    return PELEMP_E_HIP;
+  }
+  *timer = t;
+  return PELEMP_OK;
+}
+
+int pelemp_timer_record(void* timer, int32_t which, void* stream) {
+  if (!timer || which < 0 || which > 1) return PELEMP_E_ARG;
+  PM_HIP(hipEventRecord(static_cast<pelemp_timer*>(timer)->e[which], static_cast<hipStream_t>(stream)));
+  return PELEMP_OK;
+}
+
+int pelemp_timer_elapsed(void* timer, double* ms) {
+  if (!timer || !ms) return PELEMP_E_ARG;
+  auto* t = static_cast<pelemp_timer*>(timer);
+  PM_HIP(hipEventSynchronize(t->e[1]));
+  float f = 0.f;
+  PM_HIP(hipEventElapsedTime(&f, t->e[0], t->e[1]));
+  *ms = f;
+  return PELEMP_OK;
+}
+
+int pelemp_timer_destroy(void* timer) {
+  if (!timer) return PELEMP_OK;
+  auto* t = static_cast<pelemp_timer*>(timer);
+  (void)hipEventDestroy(t->e[0]);
+  (void)hipEventDestroy(t->e[1]);
+  delete t;
+  return PELEMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,405 @@
+// update_kernel.h -- fused per-tile droplet update for gfx950.
+//
+// Replaces the per-particle body of SprayParticleContainer::updateParticles
+// (SprayParticles.cpp:324-482) for the drag/evaporation path: gather of the
+// gas state (trilinear, SprayInterpolation.H:99-199), boiling temperature,
+// calculateSpraySource (Drag.H:137-441), source deposition into the
+// containing cell (SprayParticles.cpp:419-441) and the drift with bounds
+// check and periodic wrap (SprayParticles.cpp:443-479).
+//
+// Work decomposition.  Cells are grouped in 4x4x4 tiles; droplets are kept
+// sorted by (tile, cell) so the droplets of one tile are contiguous.  One
+// 256-thread workgroup owns one tile:
+//   1. it stages the gas state of the tile plus a one-cell halo (6^3 cells)
+//      in LDS, converting rho*Y_k to Y_k once per cell;
+//   2. it walks the tile's droplets in chunks of 256 (one droplet per
+//      thread, storage position via perm), runs the droplet physics in
+//      registers and writes the droplet back at its sorted position;
+//   3. after each chunk, the per-droplet source contributions are reduced
+//      per cell in droplet order by the threads that own (cell, component)
+//      accumulators, so every cell's sum has a fixed association order;
+//   4. finally it adds each cell's sums to the source FAB (owner computes,
+//      no atomics).
+// Workgroups are mapped XCD-contiguously: consecutive tiles (which share
+// halo cells) run on the same XCD and hit the same L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "droplet_physics.h"
+#include "layout.h"
+
+namespace pelemp {
+
+constexpr int kHaloW = kTile + 2;
+constexpr int kHaloCells = kHaloW * kHaloW * kHaloW;
+constexpr int kUpdThreads = 256;
+
+// Error codes written to the sticky error word (first failure wins).
+__device__ inline void set_err(int* err, int code) { atomicCAS(err, 0, code); }
+
+struct FabDev {
+  double* data;
+  int lo[3];
+  int n[3];
+  long long cstride;  // cells per component
+};
+
+template <class R, int NF>
+struct UpdateArgs {
+  const R* __restrict__ fin;
+  R* __restrict__ fout;
+  long long cap;
+  const int32_t* __restrict__ perm;
+  const int32_t* __restrict__ cstart;
+  const int32_t* __restrict__ slot_in;
+  int32_t* __restrict__ slot_out;
+  uint32_t* __restrict__ key_out;
+  FabDev st;
+  int c_rho, c_vel, c_spec, c_T;
+  FabDev src;
+  int c_rsrc, c_msrc, c_ssrc, c_esrc;
+  double plo[3], phi[3], dx[3], dxi[3];
+  int dom_lo[3];
+  int ncell[3];
+  int ntile[3];
+  int bndry_lo[3], bndry_hi[3];
+  FuelConst<R, NF> fc;
+  R flow_dt;
+  R sub_dt;
+  double cur_coef;  // -(1/V) sub_dt / flow_dt
+  R sub_dt_dxi[3];  // FP32 form: sub_dt / dx per direction
+  int do_move;
+  int is_ghost;
+  int* err;
+};
+
+__device__ inline int xcd_tile(int b, int ntiles) {
+  // gfx950 dispatches workgroup b to XCD b % 8; give each XCD a contiguous
+  // range of tiles.
+  if ((ntiles & 7) != 0) return b;
+  const int per = ntiles >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// check_bounds (SprayInterpolation.H:64-97)
+__device__ inline bool dev_check_bounds(const double* pos, const double* plo, const double* phi,
+                                        const double* dx, const int* blo, const int* bhi,
+                                        int* bflags) {
+  for (int hilo = 0; hilo < 2; ++hilo) {
+    const int fact = (hilo == 0) ? 1 : -1;
+    for (int dir = 0; dir < 3; ++dir) {
+      const double dom_loc = (hilo == 0) ? plo[dir] : phi[dir];
+      const int bflag = (hilo == 0) ? blo[dir] : bhi[dir];
+      if (bflag != 0) {
+        const double diff = static_cast<double>(fact) * (pos[dir] - dom_loc);
+        if (diff < 0.) {
+          if (bflag == 1) {
+            bflags[dir] = -fact;
+          } else {
+            return true;
+          }
+        } else if (diff < 0.5 * dx[dir]) {
+          bflags[dir] = -2 * fact;
+        }
+      }
+    }
+  }
+  return false;
+}
+
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
+__global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R, NF> a) {
+  constexpr int NS = M::NS;
+  constexpr int NG = 5 + NS;     // staged gas fields: rho, Y[NS], vel[3], T
+  constexpr int NSRC = 5 + NF;   // mom[3], mass, Ydot[NF], eng
+  constexpr int NITEM = (kTileCells * NSRC + kUpdThreads - 1) / kUpdThreads;
+  using F = Fields<NF>;
+
+  __shared__ R gas[NG * kHaloCells];
+  __shared__ unsigned char gvalid[kHaloCells];
+  __shared__ int32_t cs[kTileCells + 1];
+  __shared__ double contrib[kUpdThreads * NSRC];
+
+  const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int t = threadIdx.x;
+  const int txy = a.ntile[0] * a.ntile[1];
+  const int otz = (tile / txy) * kTile;
+  const int oty = ((tile % txy) / a.ntile[0]) * kTile;
+  const int otx = (tile % a.ntile[0]) * kTile;
+
+  if (t <= kTileCells) cs[t] = a.cstart[tile * kTileCells + t];
+  __syncthreads();
+  const int32_t d0 = cs[0];
+  const int32_t d1 = cs[kTileCells];
+  if (d1 == d0) return;  // uniform: empty tile
+
+  // ---- stage gas halo ----
+  for (int h = t; h < kHaloCells; h += kUpdThreads) {
+    const int hx = h % kHaloW, hy = (h / kHaloW) % kHaloW, hz = h / (kHaloW * kHaloW);
+    const int gx = a.dom_lo[0] + otx + hx - 1;
+    const int gy = a.dom_lo[1] + oty + hy - 1;
+    const int gz = a.dom_lo[2] + otz + hz - 1;
+    const int lx = gx - a.st.lo[0], ly = gy - a.st.lo[1], lz = gz - a.st.lo[2];
+    const bool ok = lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2];
+    gvalid[h] = ok ? 1 : 0;
+    if (ok) {
+      const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
+      const double* S = a.st.data;
+      const double rho = S[a.c_rho * a.st.cstride + c];
+      const double inv_rho = 1. / rho;
+      gas[0 * kHaloCells + h] = static_cast<R>(rho);
+#pragma unroll
+      for (int n = 0; n < NS; ++n)
+        gas[(1 + n) * kHaloCells + h] = static_cast<R>(S[(a.c_spec + n) * a.st.cstride + c] * inv_rho);
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        gas[(1 + NS + q) * kHaloCells + h] = static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
+      gas[(4 + NS) * kHaloCells + h] = static_cast<R>(S[a.c_T * a.st.cstride + c]);
+    }
+  }
+  __syncthreads();
+
+  double acc[NITEM];
+#pragma unroll
+  for (int r = 0; r < NITEM; ++r) acc[r] = 0.;
+
+  const long long cap = a.cap;
+  for (int32_t base = d0; base < d1; base += kUpdThreads) {
+    const int32_t s = base + t;
+    if (s < d1) {
+      const int32_t o = a.perm[s];
+      // local cell of sorted position s: largest c with cs[c] <= s
+      int lc = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1)
+        if (lc + step < kTileCells && cs[lc + step] <= s) lc += step;
+      const int cx = otx + (lc % kTile), cy = oty + ((lc / kTile) % kTile), cz = otz + lc / (kTile * kTile);
+
+      Drop<R, NF> d;
+      R pos[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        pos[q] = a.fin[(F::POS + q) * cap + o];
+        d.vel[q] = a.fin[(F::VEL + q) * cap + o];
+      }
+      d.T = a.fin[F::T * cap + o];
+      d.dia = a.fin[F::DIA * cap + o];
+#pragma unroll
+      for (int spf = 0; spf < NF; ++spf) d.Y[spf] = a.fin[(F::Y + spf) * cap + o];
+      d.numDens = a.fin[F::ND * cap + o];
+      const R film = a.fin[F::FILM * cap + o];
+      d.bm1 = R(0);
+      d.alive = true;
+
+      // trilinear corner cell and weights (SprayInterpolation.H:158-199)
+      int ijk[3];
+      R sx_hi[3];
+      int bflags[3] = {0, 0, 0};
+      if constexpr (POS64) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double lx = (static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q] + 0.5;
+          ijk[q] = static_cast<int>(floor(lx));
+          sx_hi[q] = static_cast<R>(lx - static_cast<double>(ijk[q]));
+        }
+        if constexpr (AT_BOUNDS) {
+          double pd[3] = {static_cast<double>(pos[0]), static_cast<double>(pos[1]), static_cast<double>(pos[2])};
+          if (dev_check_bounds(pd, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags))
+            set_err(a.err, PELEMP_E_LEFT_DOMAIN);
+        }
+      } else {
+        const int c3[3] = {cx, cy, cz};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const R l = pos[q] + R(0.5);
+          const R fl = floor(l);
+          ijk[q] = c3[q] + static_cast<int>(fl);
+          sx_hi[q] = l - fl;
+        }
+      }
+      R ssv[2][3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const R sx_lo = R(1.) - sx_hi[q];
+        if (bflags[q] > 1) {
+          ssv[0][q] = R(1.);
+          ssv[1][q] = R(0.);
+        } else if (bflags[q] < -1) {
+          ssv[0][q] = R(0.);
+          ssv[1][q] = R(1.);
+        } else {
+          ssv[0][q] = sx_lo;
+          ssv[1][q] = sx_hi[q];
+        }
+      }
+      // InterpolateGasPhase (SprayInterpolation.H:99-150), LMeX flavour
+      GasAt<R, NS> g;
+      g.rho = R(0);
+      g.T = R(0);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) g.vel[q] = R(0);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) g.Y[n] = R(0);
+#pragma unroll
+      for (int kk = -1; kk < 1; ++kk) {
+#pragma unroll
+        for (int jj = -1; jj < 1; ++jj) {
+#pragma unroll
+          for (int ii = -1; ii < 1; ++ii) {
+            const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
+            if (cw > R(0.)) {
+              int hx = ijk[0] + ii - otx + 1, hy = ijk[1] + jj - oty + 1, hz = ijk[2] + kk - otz + 1;
+              const bool inr = hx >= 0 && hx < kHaloW && hy >= 0 && hy < kHaloW && hz >= 0 && hz < kHaloW;
+              hx = inr ? hx : 0;
+              hy = inr ? hy : 0;
+              hz = inr ? hz : 0;
+              const int h = (hz * kHaloW + hy) * kHaloW + hx;
+              if (!inr || !gvalid[h]) set_err(a.err, PELEMP_E_STATE_BOX);
+              g.rho += cw * gas[h];
+#pragma unroll
+              for (int n = 0; n < NS; ++n) g.Y[n] += cw * gas[(1 + n) * kHaloCells + h];
+#pragma unroll
+              for (int q = 0; q < 3; ++q) g.vel[q] += cw * gas[(1 + NS + q) * kHaloCells + h];
+              g.T += cw * gas[(4 + NS) * kHaloCells + h];
+            }
+          }
+        }
+      }
+      gas_define<M, R>(g);
+      R cBoilT[NF];
+      calc_boil_t<NF, R>(a.fc, g.p, cBoilT);
+      DropSrc<R, NF> sr;
+      if (film > R(0.)) {
+        set_err(a.err, PELEMP_E_BAD_CONFIG);  // wall-film droplets use the film pass
+        sr.mom[0] = sr.mom[1] = sr.mom[2] = R(0);
+        sr.mass = R(0);
+        sr.eng = R(0);
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) sr.Ydot[spf] = R(0);
+      } else {
+        spray_source<M, NF, R>(a.sub_dt, g, a.fc, d, cBoilT, sr);
+      }
+      // contributions cur_coef * source (SprayParticles.cpp:419-441)
+      double* cb = &contrib[t * NSRC];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cb[q] = a.cur_coef * static_cast<double>(sr.mom[q]);
+      cb[3] = a.cur_coef * static_cast<double>(sr.mass);
+#pragma unroll
+      for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = a.cur_coef * static_cast<double>(sr.Ydot[spf]);
+      cb[4 + NF] = a.cur_coef * static_cast<double>(sr.eng);
+
+      // drift (SprayParticles.cpp:443-479) and new cell key
+      uint32_t key = kDeadKey;
+      if constexpr (POS64) {
+        double p3[3] = {static_cast<double>(pos[0]), static_cast<double>(pos[1]), static_cast<double>(pos[2])};
+        if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) p3[q] += static_cast<double>(a.sub_dt) * static_cast<double>(d.vel[q]);
+          if constexpr (AT_BOUNDS) {
+            if (dev_check_bounds(p3, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags)) d.alive = false;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          if (a.bndry_lo[q] == 0) {
+            const double L = a.phi[q] - a.plo[q];
+            if (p3[q] >= a.phi[q]) {
+              p3[q] -= L;
+            } else if (p3[q] < a.plo[q]) {
+              p3[q] += L;
+            }
+          }
+          pos[q] = static_cast<R>(p3[q]);
+        }
+        if (d.alive) {
+          int c[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            int ic = static_cast<int>(floor((p3[q] - a.plo[q]) * a.dxi[q]));
+            ic = ic < 0 ? 0 : (ic >= a.ncell[q] ? a.ncell[q] - 1 : ic);
+            c[q] = ic;
+          }
+          key = tile_key(c[0], c[1], c[2], a.ntile);
+        }
+      } else {
+        int c[3] = {cx, cy, cz};
+        if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            R xi = pos[q] + a.sub_dt_dxi[q] * d.vel[q];
+            const R sh = floor(xi);
+            xi -= sh;
+            if (xi >= R(1.)) xi = R(0.);  // rounding of a tiny negative xi
+            int ic = c[q] + static_cast<int>(sh);
+            const int n = a.ncell[q];
+            ic = ((ic % n) + n) % n;  // periodic
+            c[q] = ic;
+            pos[q] = xi;
+          }
+        }
+        if (d.alive) key = tile_key(c[0], c[1], c[2], a.ntile);
+      }
+
+      R* fo = a.fout;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        fo[(F::POS + q) * cap + s] = pos[q];
+        fo[(F::VEL + q) * cap + s] = d.vel[q];
+      }
+      fo[F::T * cap + s] = d.T;
+      fo[F::DIA * cap + s] = d.dia;
+#pragma unroll
+      for (int spf = 0; spf < NF; ++spf) fo[(F::Y + spf) * cap + s] = d.Y[spf];
+      fo[F::ND * cap + s] = d.numDens;
+      fo[F::FILM * cap + s] = film;
+      a.slot_out[s] = a.slot_in[o];
+      a.key_out[s] = key;
+    }
+    __syncthreads();
+    // ordered per-cell reduction of this chunk
+    const int32_t chunk_end = (base + kUpdThreads < d1) ? base + kUpdThreads : d1;
+#pragma unroll
+    for (int r = 0; r < NITEM; ++r) {
+      const int item = t + r * kUpdThreads;
+      if (item < kTileCells * NSRC) {
+        const int c = item / NSRC, q = item % NSRC;
+        const int32_t lo = cs[c] > base ? cs[c] : base;
+        const int32_t hi = cs[c + 1] < chunk_end ? cs[c + 1] : chunk_end;
+        double v = acc[r];
+        for (int32_t i = lo; i < hi; ++i) v = v + contrib[(i - base) * NSRC + q];
+        acc[r] = v;
+      }
+    }
+    __syncthreads();
+  }
+
+  // add each cell's sums to the source FAB
+#pragma unroll
+  for (int r = 0; r < NITEM; ++r) {
+    const int item = t + r * kUpdThreads;
+    if (item < kTileCells * NSRC) {
+      const int c = item / NSRC, q = item % NSRC;
+      if (cs[c + 1] > cs[c]) {
+        bool on = true;
+        if (q < 3 && !a.fc.mom_trans) on = false;
+        if (q >= 3 && q < 4 + NF && !a.fc.mass_trans) on = false;
+        if (on) {
+          const int comp = (q < 3) ? a.c_msrc + q : (q == 3) ? a.c_rsrc : (q < 4 + NF) ? a.c_ssrc + (q - 4) : a.c_esrc;
+          const int gx = a.dom_lo[0] + otx + (c % kTile) - a.src.lo[0];
+          const int gy = a.dom_lo[1] + oty + ((c / kTile) % kTile) - a.src.lo[1];
+          const int gz = a.dom_lo[2] + otz + c / (kTile * kTile) - a.src.lo[2];
+          if (gx < 0 || gx >= a.src.n[0] || gy < 0 || gy >= a.src.n[1] || gz < 0 || gz >= a.src.n[2]) {
+            if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
+          } else {
+            double* dst = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
+            *dst = *dst + acc[r];
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace pelemp

@@ -1,0 +1,93 @@
+// update_launch.h -- host-side launch of the fused update kernel for one
+// arithmetic form; included by spray_fp64.hip and spray_fp32.hip, which are
+// compiled with different floating-point contraction settings.
+#pragma once
+
+#include "spray_ctx.h"
+#include "update_kernel.h"
+
+namespace pelemp {
+
+inline FabDev make_fab(const pelemp_fab_view& v) {
+  FabDev f;
+  f.data = v.data;
+  for (int d = 0; d < 3; ++d) {
+    f.lo[d] = v.lo[d];
+    f.n[d] = v.hi[d] - v.lo[d] + 1;
+  }
+  f.cstride = static_cast<long long>(f.n[0]) * f.n[1] * f.n[2];
+  return f;
+}
+
+template <class M, int NF, class R, bool POS64>
+int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
+  UpdateArgs<R, NF> a{};
+  a.fin = static_cast<const R*>(c.fbuf[c.cur]);
+  a.fout = static_cast<R*>(c.fbuf[1 - c.cur]);
+  a.cap = c.cap;
+  a.perm = c.perm;
+  a.cstart = c.cstart;
+  a.slot_in = c.slot[c.cur];
+  a.slot_out = c.slot[1 - c.cur];
+  a.key_out = c.key;
+  a.st = make_fab(*p.state);
+  a.c_rho = c.comps.rhoIndx;
+  a.c_vel = c.comps.momIndx;
+  a.c_spec = c.comps.specIndx;
+  a.c_T = c.comps.utempIndx;
+  a.src = make_fab(*p.source);
+  a.c_rsrc = c.comps.rhoSrcIndx;
+  a.c_msrc = c.comps.momSrcIndx;
+  a.c_ssrc = c.comps.specSrcIndx;
+  a.c_esrc = c.comps.engSrcIndx;
+  for (int d = 0; d < 3; ++d) {
+    a.plo[d] = c.geom.plo[d];
+    a.phi[d] = c.geom.phi[d];
+    a.dx[d] = c.geom.dx[d];
+    a.dxi[d] = 1. / c.geom.dx[d];
+    a.dom_lo[d] = c.geom.domain_lo[d];
+    a.ncell[d] = c.ncell[d];
+    a.ntile[d] = c.ntile[d];
+    a.bndry_lo[d] = c.geom.bndry_lo[d];
+    a.bndry_hi[d] = c.geom.bndry_hi[d];
+    a.sub_dt_dxi[d] = static_cast<R>(p.sub_dt * a.dxi[d]);
+  }
+  a.fc = make_fuel_const<R, NF>(c.sd, c.mw.data());
+  a.flow_dt = static_cast<R>(p.flow_dt);
+  a.sub_dt = static_cast<R>(p.sub_dt);
+  a.cur_coef = p.cur_coef;
+  a.do_move = p.do_move;
+  a.is_ghost = p.is_ghost;
+  a.err = c.err;
+  const dim3 grid(c.ntiles), block(kUpdThreads);
+  if (p.at_bounds) {
+    if constexpr (POS64) {
+      hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, true>), grid, block, 0, s, a);
+    } else {
+      set_last_error("walls are supported by the FP64 form only");
+      return PELEMP_E_BAD_CONFIG;
+    }
+  } else {
+    hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, false>), grid, block, 0, s, a);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_last_error(std::string("update launch: ") + hipGetErrorString(e));
+    return PELEMP_E_HIP;
+  }
+  return PELEMP_OK;
+}
+
+template <class R, bool POS64>
+int dispatch_update(Ctx& c, const PassParams& p, hipStream_t s) {
+  if (c.mech == PELEMP_MECH_DODECANE_AIR && c.nf == 1)
+    return launch_update_t<Mech_dodecane_air, 1, R, POS64>(c, p, s);
+  if (c.mech == PELEMP_MECH_DECANE_DODECANE_AIR && c.nf == 2)
+    return launch_update_t<Mech_decane_dodecane_air, 2, R, POS64>(c, p, s);
+  if (c.mech == PELEMP_MECH_ALKANE6_AIR && c.nf == 6)
+    return launch_update_t<Mech_alkane6_air, 6, R, POS64>(c, p, s);
+  set_last_error("unsupported mechanism / fuel count combination");
+  return PELEMP_E_BAD_CONFIG;
+}
+
+}  // namespace pelemp

@@ -1,0 +1,202 @@
+"""Seeded synthetic spray problems (gas state FAB + droplets) for the
+benchmark configurations and the tests.
+
+The gas recipe follows the configuration text: a periodic box of air
+(Y_O2 = 0.233, Y_N2 = 0.767, Exec/SprayTests/PeleC/abramzon_test/
+prob_parm.H:13-14), temperature T0 (1 + 0.05 sin(2 pi x/L) sin(2 pi y/L)
+sin(2 pi z/L)), velocity (U0, 0, 0) + 0.5 (sin 2 pi y/L, sin 2 pi z/L,
+sin 2 pi x/L) m/s, density from the ideal gas law at pressure p.  Ghost cells
+are periodic copies of interior cells.  Droplets: uniform positions (config
+1) or Poisson counts per cell (config 2), lognormal diameters, T_d = 300 K,
+velocity = local gas velocity + N(0, 1) m/s per component.
+
+State layout (PeleLMeX flavour): rho, velocity(3), rho Y_k (NS), rho h
+(unused by the spray), T.  Spray source layout: rho, momentum(3), rho Y_f
+(nf), energy.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as C
+from .fuels import spray_data
+
+RU = 8.31446261815324
+PATM = 101325.0
+
+_MECH_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mech", "mech_data.json")
+MECH_NAMES = {C.MECH_DODECANE_AIR: "dodecane_air", C.MECH_DECANE_DODECANE_AIR: "decane_dodecane_air",
+              C.MECH_ALKANE6_AIR: "alkane6_air"}
+
+
+def mech_info(mech: int) -> dict:
+    with open(_MECH_JSON) as f:
+        return json.load(f)[MECH_NAMES[mech]]
+
+
+@dataclass
+class SprayProblem:
+    name: str
+    mech: int
+    species: list
+    W: np.ndarray
+    nf: int
+    geom: dict
+    comps: dict
+    sdata: dict
+    state: np.ndarray  # (ncomp, nz, ny, nx) incl. ghosts
+    state_lo: tuple
+    state_hi: tuple
+    src_shape: tuple
+    src_lo: tuple
+    src_hi: tuple
+    pos: np.ndarray  # (n, 3)
+    rdata: np.ndarray  # (n, 10 + nf)
+    ids: np.ndarray  # (n,)
+    flow_dt: float
+    nsteps: int
+    state_ghosts: int = 2
+    source_ghosts: int = 1
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nsr(self) -> int:
+        return 10 + self.nf
+
+    def new_source(self) -> np.ndarray:
+        return np.zeros(self.src_shape, dtype=np.float64)
+
+    @property
+    def n(self) -> int:
+        return int(self.ids.shape[0])
+
+
+def state_comps(ns: int, nf: int) -> dict:
+    return dict(rhoIndx=0, momIndx=1, specIndx=4, engIndx=4 + ns, utempIndx=5 + ns,
+                rhoSrcIndx=0, momSrcIndx=1, specSrcIndx=4, engSrcIndx=4 + nf)
+
+
+def _periodic_fill(a: np.ndarray, g: int) -> np.ndarray:
+    """Pad the last three axes of a by g cells, periodically."""
+    return np.pad(a, [(0, 0)] * (a.ndim - 3) + [(g, g)] * 3, mode="wrap")
+
+
+def gas_velocity(x, y, z, L, U0):
+    tw = 2.0 * math.pi
+    return (U0 + 0.5 * np.sin(tw * y / L[1]), 0.5 * np.sin(tw * z / L[2]), 0.5 * np.sin(tw * x / L[0]))
+
+
+def build_problem(name: str, *, ncell=(32, 32, 32), dx=1.0e-4, p=PATM, T0=1000.0, U0=5.0,
+                  mech=C.MECH_DODECANE_AIR, fuels=("NC12H26",), n_drops=100000, poisson=False,
+                  d_median=20e-6, sigma_ln=0.4, Y_range=None, flow_dt=1e-6, nsteps=100, seed=1234,
+                  state_ghosts=2, source_ghosts=1, sdata_kw=None) -> SprayProblem:
+    info = mech_info(mech)
+    species = info["species"]
+    W = np.array(info["W"])
+    ns = len(species)
+    nf = len(fuels)
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = ncell
+    L = (nx * dx, ny * dx, nz * dx)
+    plo = (0.0, 0.0, 0.0)
+    # interior cell centres
+    xc = (np.arange(nx) + 0.5) * dx
+    yc = (np.arange(ny) + 0.5) * dx
+    zc = (np.arange(nz) + 0.5) * dx
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    tw = 2.0 * math.pi
+    T = T0 * (1.0 + 0.05 * np.sin(tw * X / L[0]) * np.sin(tw * Y / L[1]) * np.sin(tw * Z / L[2]))
+    u, v, w = gas_velocity(X, Y, Z, L, U0)
+    Ygas = np.zeros(ns)
+    Ygas[species.index("O2")] = 0.233
+    Ygas[species.index("N2")] = 0.767
+    wbar = 1.0 / np.sum(Ygas / W)
+    rho = p * wbar / (RU * T)
+    comps = state_comps(ns, nf)
+    ncomp = 6 + ns
+    st = np.zeros((ncomp, nz, ny, nx))
+    st[comps["rhoIndx"]] = rho
+    st[comps["momIndx"]] = u
+    st[comps["momIndx"] + 1] = v
+    st[comps["momIndx"] + 2] = w
+    for k in range(ns):
+        st[comps["specIndx"] + k] = rho * Ygas[k]
+    st[comps["utempIndx"]] = T
+    g = state_ghosts
+    state = np.ascontiguousarray(_periodic_fill(st, g))
+    state_lo = (-g, -g, -g)
+    state_hi = (nx - 1 + g, ny - 1 + g, nz - 1 + g)
+    sg = source_ghosts
+    src_shape = (5 + nf, nz + 2 * sg, ny + 2 * sg, nx + 2 * sg)
+    src_lo = (-sg, -sg, -sg)
+    src_hi = (nx - 1 + sg, ny - 1 + sg, nz - 1 + sg)
+    # droplets
+    if poisson:
+        lam = n_drops / (nx * ny * nz)
+        counts = rng.poisson(lam, size=nx * ny * nz)
+        n = int(counts.sum())
+        cell = np.repeat(np.arange(nx * ny * nz), counts)
+        ci = cell % nx
+        cj = (cell // nx) % ny
+        ck = cell // (nx * ny)
+        frac = rng.random((n, 3))
+        pos = np.stack([(ci + frac[:, 0]) * dx, (cj + frac[:, 1]) * dx, (ck + frac[:, 2]) * dx], axis=1)
+    else:
+        n = int(n_drops)
+        pos = rng.random((n, 3)) * np.array(L)
+    pos = np.minimum(pos, np.nextafter(np.array(L), 0.0))
+    dia = d_median * np.exp(sigma_ln * rng.standard_normal(n))
+    ug = np.stack(gas_velocity(pos[:, 0], pos[:, 1], pos[:, 2], L, U0), axis=1)
+    vel = ug + rng.standard_normal((n, 3))
+    nsr = 10 + nf
+    rdata = np.zeros((n, nsr))
+    rdata[:, 0:3] = vel
+    rdata[:, 3] = 300.0
+    rdata[:, 4] = dia
+    if nf == 1:
+        rdata[:, 5] = 1.0
+    else:
+        lo, hi = Y_range if Y_range is not None else (0.3, 0.7)
+        y1 = rng.uniform(lo, hi, n)
+        rdata[:, 5] = y1
+        rdata[:, 6] = 1.0 - y1
+        if nf > 2:
+            raise ValueError("use build_problem_multi for nf > 2")
+    rdata[:, 5 + nf] = 1.0  # numDens
+    rdata[:, 6 + nf] = 1.0  # N0
+    ids = np.arange(1, n + 1, dtype=np.int64)
+    geom = dict(plo=list(plo), phi=list(L), dx=[dx, dx, dx], domain_lo=[0, 0, 0],
+                domain_hi=[nx - 1, ny - 1, nz - 1], bndry_lo=[0, 0, 0], bndry_hi=[0, 0, 0])
+    kw = dict(sdata_kw or {})
+    sdata = spray_data(list(fuels), species, **kw)
+    return SprayProblem(name=name, mech=mech, species=species, W=W, nf=nf, geom=geom, comps=comps,
+                        sdata=sdata, state=state, state_lo=state_lo, state_hi=state_hi, src_shape=src_shape,
+                        src_lo=src_lo, src_hi=src_hi, pos=np.ascontiguousarray(pos),
+                        rdata=np.ascontiguousarray(rdata), ids=ids, flow_dt=flow_dt, nsteps=nsteps,
+                        state_ghosts=state_ghosts, source_ghosts=source_ghosts,
+                        meta=dict(p=p, T0=T0, U0=U0, seed=seed))
+
+
+def config1(scale: float = 1.0, seed: int = 1234, **kw) -> SprayProblem:
+    """Config 1: 32^3 periodic box, dx = 1e-4 m, air at 1 atm, T0 = 1000 K,
+    1e5 n-dodecane droplets (median 20 um, sigma_ln 0.4), 100 steps of 1e-6 s.
+    `scale` < 1 shrinks the droplet count (box unchanged)."""
+    return build_problem("config1", ncell=(32, 32, 32), p=PATM, T0=1000.0, mech=C.MECH_DODECANE_AIR,
+                         fuels=("NC12H26",), n_drops=int(round(1e5 * scale)), d_median=20e-6, sigma_ln=0.4,
+                         flow_dt=1e-6, nsteps=100, seed=seed, **kw)
+
+
+def config2(ncell: int = 128, drops_per_cell: float = 1.6e7 / 128 ** 3, seed: int = 4321, **kw) -> SprayProblem:
+    """Config 2: 128^3 periodic box at 30 atm, T0 = 900 K, Poisson(~7.63)
+    droplets per cell of an n-decane/n-dodecane liquid (Y_decane ~ U[0.3,0.7]),
+    median 15 um, sigma_ln 0.5, 50 steps of 5e-7 s.  `ncell` < 128 gives the
+    same recipe on a smaller box (for tests)."""
+    n = drops_per_cell * ncell ** 3
+    return build_problem("config2", ncell=(ncell, ncell, ncell), p=30.0 * PATM, T0=900.0,
+                         mech=C.MECH_DECANE_DODECANE_AIR, fuels=("NC10H22", "NC12H26"), n_drops=n, poisson=True,
+                         d_median=15e-6, sigma_ln=0.5, Y_range=(0.3, 0.7), flow_dt=5e-7, nsteps=50, seed=seed, **kw)

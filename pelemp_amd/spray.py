@@ -1,0 +1,203 @@
+"""Host-side mirror of PeleMP's SprayParticleContainer for the droplet update.
+
+Method names and argument meanings follow the reference
+(Source/PP_Spray/SprayParticles.H:38-415):
+
+    updateParticles(level, state, source, flow_dt, time, state_ghosts,
+                    source_ghosts, isVirt, isGhost, do_move, ltransparm,
+                    spray_cfl_lev)                       SprayParticles.H:229-243
+    moveKickDrift(state, source, level, dt, time, isVirtualPart, isGhostPart,
+                  state_ghosts, source_ghosts, do_move, ltransparm,
+                  spray_cfl_lev)                         SprayParticles.H:212-226
+    moveKick(...)                                        SprayParticles.H:196-209
+    estTimestep(level)                                   SprayParticles.H:110
+    getSprayData() / getSprayComps()                     SprayParticles.H:378-384
+
+Every call goes through the C ABI of libpelemp_spray.so (HIP kernels for
+gfx950).  `state` and `source` are device FABs (DeviceFab), the MultiFab of
+one box.  Errors that the reference turns into amrex::Abort raise
+PelempError with the matching code.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _capi as C
+
+
+class DeviceFab:
+    """One FAB in device memory: box [lo, hi] (incl. ghosts), ncomp components,
+    component-major with x fastest (the MultiFab data of one box)."""
+
+    def __init__(self, lo, hi, ncomp: int, data: Optional[np.ndarray] = None):
+        self.lo = tuple(int(v) for v in lo)
+        self.hi = tuple(int(v) for v in hi)
+        self.ncomp = int(ncomp)
+        self.shape = (self.ncomp, self.hi[2] - self.lo[2] + 1, self.hi[1] - self.lo[1] + 1,
+                      self.hi[0] - self.lo[0] + 1)
+        self.buf = C.DeviceBuffer(8 * int(np.prod(self.shape)))
+        if data is not None:
+            if tuple(data.shape) != self.shape:
+                raise ValueError(f"FAB data shape {data.shape} != {self.shape}")
+            self.buf.upload(np.ascontiguousarray(data, dtype=np.float64))
+        else:
+            self.buf.zero()
+        self._view = C.fab_view(self.buf.ptr, self.lo, self.hi, self.ncomp)
+
+    def view(self) -> C.FabViewC:
+        return self._view
+
+    def numpy(self) -> np.ndarray:
+        return self.buf.download(np.empty(self.shape, dtype=np.float64))
+
+    def setVal(self, v: float) -> None:
+        if v != 0.0:
+            self.buf.upload(np.full(self.shape, v, dtype=np.float64))
+        else:
+            self.buf.zero()
+
+
+def _stream_handle(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.handle)
+
+
+class SprayParticleContainer:
+    """Droplets of one AMR level, resident on the GPU."""
+
+    def __init__(self, mech: int, spray_data: dict, comps: dict, geom: dict, max_particles: int,
+                 form: int = C.FORM_FP64, apply_latent_shift: bool = True):
+        self._lib = C.lib()
+        self.mech = mech
+        self.form = form
+        self.nf = int(spray_data["nfuel"])
+        self.geom = dict(geom)
+        self.comps = dict(comps)
+        self._sd = C.spray_data_c(spray_data)
+        self._comps = C.comps_c(comps)
+        self._geom = C.geom_c(geom)
+        h = ctypes.c_void_p()
+        C.check(self._lib.pelemp_spray_create(ctypes.byref(h), mech, form, ctypes.byref(self._sd),
+                                              ctypes.byref(self._comps), ctypes.byref(self._geom),
+                                              int(max_particles), 1 if apply_latent_shift else 0),
+                "pelemp_spray_create")
+        self._h = h
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pelemp_spray_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- data --------------------------------------------------------------
+    @property
+    def NSR(self) -> int:
+        return 10 + self.nf
+
+    def getSprayData(self) -> dict:
+        out = C.SprayDataC()
+        C.check(self._lib.pelemp_spray_get_data(self._h, ctypes.byref(out)), "pelemp_spray_get_data")
+        return C.spray_data_dict(out)
+
+    def getSprayComps(self) -> dict:
+        return dict(self.comps)
+
+    def set_particles(self, pos: np.ndarray, rdata: np.ndarray, ids: np.ndarray) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rdata = np.ascontiguousarray(rdata, dtype=np.float64)
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        n = ids.shape[0]
+        if pos.shape != (n, 3) or rdata.shape != (n, self.NSR):
+            raise ValueError("particle arrays have the wrong shape")
+        C.check(self._lib.pelemp_spray_set_particles(self._h, n, pos.ctypes.data, rdata.ctypes.data,
+                                                     ids.ctypes.data), "pelemp_spray_set_particles")
+
+    def num_particles(self) -> int:
+        n = ctypes.c_int64()
+        C.check(self._lib.pelemp_spray_num_particles(self._h, ctypes.byref(n)), "pelemp_spray_num_particles")
+        return int(n.value)
+
+    def get_particles(self):
+        n = self.num_particles()
+        pos = np.empty((n, 3))
+        rdata = np.empty((n, self.NSR))
+        ids = np.empty(n, dtype=np.int64)
+        nn = ctypes.c_int64(n)
+        C.check(self._lib.pelemp_spray_get_particles(self._h, ctypes.byref(nn), pos.ctypes.data,
+                                                     rdata.ctypes.data, ids.ctypes.data),
+                "pelemp_spray_get_particles")
+        return pos, rdata, ids
+
+    # -- the update ----------------------------------------------------------
+    def updateParticles(self, level: int, state: DeviceFab, source: DeviceFab, flow_dt: float, time: float,
+                        state_ghosts: int, source_ghosts: int, isVirt: bool, isGhost: bool, do_move: bool,
+                        ltransparm=None, spray_cfl_lev: float = -1.0, stream=None, sync: bool = True) -> None:
+        """SprayParticleContainer::updateParticles (SprayParticles.cpp:142-502).
+
+        `ltransparm` is accepted for signature parity; transport data are
+        compiled into the library.  With sync=False the call returns as soon
+        as the kernels are queued; `sync()` then reports device-side errors.
+        """
+        del level, ltransparm
+        sv, dv = state.view(), source.view()
+        h = _stream_handle(stream)
+        C.check(self._lib.pelemp_spray_update(self._h, ctypes.byref(sv), ctypes.byref(dv), float(flow_dt),
+                                              float(time), int(state_ghosts), int(source_ghosts), int(isVirt),
+                                              int(isGhost), int(do_move), float(spray_cfl_lev), h),
+                "pelemp_spray_update")
+        if sync:
+            self.sync(stream)
+
+    def moveKickDrift(self, state: DeviceFab, source: DeviceFab, level: int, dt: float, time: float,
+                      isVirtualPart: bool, isGhostPart: bool, state_ghosts: int, source_ghosts: int,
+                      do_move: bool = True, ltransparm=None, spray_cfl_lev: float = -1.0, stream=None,
+                      sync: bool = True) -> None:
+        """SprayParticleContainer::moveKickDrift (SprayParticles.cpp:59-87)."""
+        self.updateParticles(level, state, source, dt, time, state_ghosts, source_ghosts, isVirtualPart,
+                             isGhostPart, do_move, ltransparm, spray_cfl_lev, stream, sync)
+
+    def moveKick(self, state: DeviceFab, source: DeviceFab, level: int, dt: float, time: float,
+                 isVirtualPart: bool, isGhostPart: bool, state_ghosts: int, source_ghosts: int,
+                 ltransparm=None, spray_cfl_lev: float = -1.0, stream=None, sync: bool = True) -> None:
+        """SprayParticleContainer::moveKick (SprayParticles.cpp:37-57): update without moving."""
+        self.updateParticles(level, state, source, dt, time, state_ghosts, source_ghosts, isVirtualPart,
+                             isGhostPart, False, ltransparm, spray_cfl_lev, stream, sync)
+
+    def sync(self, stream=None) -> None:
+        C.check(self._lib.pelemp_spray_sync(self._h, _stream_handle(stream)), "device check")
+
+    def estTimestep(self, level: int = 0, cfl: float = 0.5, stream=None) -> float:
+        """SprayParticleContainer::estTimestep (SprayParticles.cpp:89-140),
+        without the spray-jet velocity bound."""
+        del level
+        dt = ctypes.c_double()
+        C.check(self._lib.pelemp_spray_est_timestep(self._h, float(cfl), ctypes.byref(dt), _stream_handle(stream)),
+                "pelemp_spray_est_timestep")
+        return float(dt.value)
+
+    # -- timing ---------------------------------------------------------------
+    def set_timing(self, enable: bool) -> None:
+        C.check(self._lib.pelemp_spray_set_timing(self._h, 1 if enable else 0), "pelemp_spray_set_timing")
+
+    def get_timing(self):
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_int64 * 4)()
+        C.check(self._lib.pelemp_spray_get_timing(self._h, ms, n, 4), "pelemp_spray_get_timing")
+        return list(ms), list(n)
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_int64()
+        C.check(self._lib.pelemp_spray_device_bytes(self._h, ctypes.byref(b)), "pelemp_spray_device_bytes")
+        return int(b.value)
