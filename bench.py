@@ -1,0 +1,201 @@
+"""Benchmark of the spray droplet update on one MI355X.
+
+Workload (config 2): 128^3 periodic box, dx = 1e-4 m, air at 30 atm,
+T0 = 900 K with the sinusoidal recipe, ~1.6e7 droplets (Poisson ~7.63 per
+cell) of an n-decane/n-dodecane liquid, median diameter 15 um, sigma_ln 0.5,
+drag + evaporation + full momentum/mass/energy deposition, dt = 5e-7 s,
+dtmod = 1 (one call = kick + drift).  A step is one updateParticles call
+over all droplets with the source FAB zeroed first (transferSource's
+tmpSource.setVal(0), SprayParticles.H:294).
+
+Prints one JSON line.  `value` is the FP64 (reference method) throughput in
+droplet-steps/s; the FP32 form is reported under "forms".  The roofline entry
+is for the dominant kernel (the fused update kernel of the FP64 form):
+achieved = algorithmic bytes per droplet-step x droplets per launch / mean
+kernel time from HIP events on the launch stream.  cpu_baseline times the
+CPU oracle (the FP64 restatement of the reference method) on a bounded
+sample of the same workload on this host's cores.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--forms fp64,fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "droplet-steps/s at 16M drops, plain FP64 vs MLP-surrogate; % of HBM peak"
+HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_bytes(nf: int, ns: int, ppc: float, w: int) -> float:
+    """Compulsory HBM bytes per droplet-step: droplet reads (9+nf) words and
+    writes (8+nf) words of width w; per cell, the gas state (5+NS) FP64 words
+    are read and the sources (5+nf) FP64 words written, shared by ppc
+    droplets."""
+    return (17 + 2 * nf) * w + (5 + ns + 5 + nf) * 8.0 / ppc
+
+
+def run_form(prob, form: int, steps: int, warmup: int, stream):
+    from pelemp_amd import _capi as C
+    from pelemp_amd.spray import DeviceFab, SprayParticleContainer
+
+    cap = int(prob.n * 1.02) + 1024
+    pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, cap, form=form)
+    pc.set_particles(prob.pos, prob.rdata, prob.ids)
+    state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
+    src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+    h = stream.handle
+
+    def step():
+        C.check(C.lib().pelemp_memset_device(src.buf.ptr, 0, src.buf.nbytes), "memset source")
+        pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
+                           True, None, -1.0, stream=h, sync=False)
+
+    for _ in range(warmup):
+        step()
+    pc.sync(h)
+    pc.pass_counts()  # reset the count log
+    pc.set_timing(True)
+    timer = C.Timer()
+    C.device_synchronize()
+    t0 = time.perf_counter()
+    timer.start(h)
+    for _ in range(steps):
+        step()
+    timer.stop(h)
+    stream.synchronize()
+    C.device_synchronize()
+    wall = time.perf_counter() - t0
+    pc.sync(h)
+    ms_total = timer.elapsed_ms()
+    counts = pc.pass_counts()
+    kms, kn = pc.get_timing()
+    pc.set_timing(False)
+    droplet_steps = int(sum(counts))
+    res = dict(droplet_steps=droplet_steps, ms_total=ms_total, wall_s=wall, counts=counts,
+               update_ms=kms[0] / max(kn[0], 1), update_launches=kn[0], sort_ms=kms[1] / max(kn[1], 1),
+               rate=droplet_steps / (ms_total * 1e-3), n_end=pc.num_particles())
+    pc.close()
+    return res
+
+
+def cpu_baseline(prob, target_s: float = 12.0):
+    """Time the CPU oracle on a bounded sample of the workload."""
+    from oracle import oracle_py as O
+
+    nthreads = min(os.cpu_count() or 1, 16)
+    sd = O.latent_shifted(prob.sdata, prob.mech, len(prob.species))
+
+    def timed(stride: int, nsteps: int):
+        pos, rd, ids = prob.pos[::stride], prob.rdata[::stride], prob.ids[::stride]
+        t0 = time.perf_counter()
+        nsum = 0
+        for _ in range(nsteps):
+            src = prob.new_source()
+            nsum += ids.shape[0]
+            st, pos, rd, ids = O.update(prob.mech, sd, prob.comps, prob.geom, pos, rd, ids, prob.state,
+                                        prob.state_lo, prob.state_hi, src, prob.src_lo, prob.src_hi, prob.flow_dt,
+                                        nthreads=nthreads)
+            assert st == 0
+        return nsum, time.perf_counter() - t0
+
+    # calibrate on ~1/64 of the droplets, then size the sample for ~target_s
+    n_cal, t_cal = timed(64, 1)
+    per = t_cal / max(n_cal, 1)
+    want = max(1, int(target_s / per))
+    stride = max(1, int(prob.n // want))
+    nsteps = 1
+    if stride == 1:
+        nsteps = max(1, min(4, int(want // prob.n)))
+    n_s, t_s = timed(stride, nsteps)
+    return dict(value=n_s / t_s, unit="droplet-steps/s", cores=nthreads, kind="port",
+                sample=f"{nsteps} step(s) of every {stride}th droplet of the workload "
+                       f"({n_s // nsteps} of {prob.n} droplets, full 128^3 gas field), "
+                       f"FP64 oracle built with g++ -O3 -march=x86-64-v3 -fopenmp, {t_s:.1f} s",
+                seconds=t_s)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--ncell", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by spatial domain decomposition, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from pelemp_amd import _capi as C
+    from pelemp_amd import problem as P
+
+    t0 = time.perf_counter()
+    prob = P.config2(ncell=args.ncell)
+    print(f"# problem built: {prob.n} droplets on {args.ncell}^3 cells in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    ncells = args.ncell ** 3
+    ppc = prob.n / ncells
+    ns = len(prob.species)
+    stream = C.Stream()
+    forms = {}
+    for name in args.forms.split(","):
+        form = {"fp64": C.FORM_FP64, "fp32": C.FORM_FP32}[name]
+        r = run_form(prob, form, args.steps, args.warmup, stream)
+        w = 8 if form == C.FORM_FP64 else 4
+        b = algorithmic_bytes(prob.nf, ns, ppc, w)
+        droplets_per_launch = r["droplet_steps"] / max(r["update_launches"], 1)
+        achieved = b * droplets_per_launch / (r["update_ms"] * 1e-3) / 1e9
+        forms[name] = dict(value=r["rate"], ms_per_step=r["ms_total"] / args.steps, update_kernel_ms=r["update_ms"],
+                           sort_ms=r["sort_ms"], bytes_per_droplet_step=b, achieved_GBs=achieved,
+                           hbm_frac=achieved / HBM_PEAK_GBS, droplets_end=r["n_end"],
+                           droplet_steps=r["droplet_steps"])
+        print(f"# {name}: {r['rate']:.4g} droplet-steps/s, {r['ms_total'] / args.steps:.3f} ms/step, "
+              f"update kernel {r['update_ms']:.3f} ms, sort {r['sort_ms']:.3f} ms, {achieved:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    main_form = "fp64" if "fp64" in forms else next(iter(forms))
+    mf = forms[main_form]
+    out = {
+        "metric": METRIC,
+        "value": mf["value"],
+        "unit": "droplet-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mf["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_form == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config2: {args.ncell}^3 periodic box dx=1e-4 m, 30 atm, T0=900 K, "
+                               f"{prob.n} n-decane/n-dodecane droplets (Poisson {ppc:.2f}/cell), "
+                               "drag+evaporation+deposition, dt=5e-7 s, dtmod=1",
+                   "droplets": prob.n, "cells": ncells, "form": main_form},
+        "roofline": {"bound": "hbm", "achieved": mf["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": mf["hbm_frac"], "traffic": None,
+                     "kernel": "spray_update_kernel", "bytes_per_droplet_step": mf["bytes_per_droplet_step"]},
+        "forms": forms,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(prob, args.cpu_seconds)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        print(f"# cpu baseline: {cb['value']:.4g} droplet-steps/s on {cb['cores']} threads ({cb['seconds']:.1f} s)",
+              file=sys.stderr, flush=True)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

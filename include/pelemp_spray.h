@@ -172,6 +172,12 @@ int pelemp_spray_move_kick_drift(pelemp_spray_ctx* ctx,
 int pelemp_spray_est_timestep(pelemp_spray_ctx* ctx, double cfl, double* dt,
                               void* stream);
 
+/* Active droplets at the start of each update pass since the previous call
+ * (at most 4096 passes are logged); waits for the device.  The sum is the
+ * droplet-step count of those passes. */
+int pelemp_spray_pass_counts(pelemp_spray_ctx* ctx, int64_t* out, int32_t max,
+                             int32_t* n_out);
+
 /* Kernel timing: when enabled, the update records HIP events around its
  * kernels; the accumulated milliseconds and launch counts are returned per
  * kernel slot (0 update, 1 sort, 2 reorder). */

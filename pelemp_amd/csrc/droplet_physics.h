@@ -148,6 +148,11 @@ struct Drop {
   R numDens;
   R bm1;  // TAB distortion y (used only when do_breakup == 1)
   bool alive;
+  // mass bookkeeping form only: droplet mass on entry and the mass change of
+  // each liquid species over the call (kept separately so that changes far
+  // below the precision of R are not lost in mass - startmass)
+  R mass;
+  R dmk[NF];
 };
 
 // SprayData::rhoL (SprayFuelData.H:146-153)
@@ -203,7 +208,7 @@ PM_HD R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
   R B_T = rm::pow(BM1, phi) - R(1);
   R error = rm::fabs(B_T - B_T_old);
   int k = 0;
-  while (k < maxIter && error > B_eps) {
+  while (k < maxIter && error > B_eps) {  // B_eps: see spray_source
     B_T_old = B_T;
     logB = rm::log1p(B_T);
     invFT = B_T / (logB * rm::pow(R(1) + B_T, R(0.7)));
@@ -286,13 +291,22 @@ PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g
 // calculateSpraySource (Drag.H:137-441), PELELM_USE_SPRAY flavour.
 // Updates d (vel, T, dia, Y, alive) and returns the sources in s and the
 // Reynolds number of the last substep.
-template <class M, int NF, class R>
+//
+// MASSB (mass bookkeeping form, used with R = float): the droplet mass comes
+// in as d.mass, the per-species mass changes sum_substeps mi_dot part_dt are
+// returned in d.dmk, and the mass/species sources are built from those sums
+// instead of from (pmass - startmass); in exact arithmetic the two agree for
+// the explicit Euler update of Drag.H:371-400.
+template <class M, int NF, class R, bool MASSB = false>
 PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
                      Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s) {
   constexpr int NS = M::NS;
   const R rule = R(1.) / R(3.);
   const R C_eps = R(1.E-15);
-  const R B_eps = R(1.E-7);
+  // The fixed-point tolerance of calcHeatCoeff (1e-7, Drag.H:155) is below
+  // single-precision resolution for B_T of order one; the single-precision
+  // form uses 1e-6 (an absolute tolerance, as in the reference).
+  const R B_eps = (sizeof(R) == 4) ? R(1.E-6) : R(1.E-7);
   const R min_mass = R(kMinMass);
   const int nSubMax = 100;
   const bool mass_trans = fd.mass_trans;
@@ -328,7 +342,15 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
   R dt = flow_dt;
   int isub = 1;
   int nsub = 1;
-  R pmass = static_cast<R>(kPIc / 6.) * rho_part * (dia_part * dia_part * dia_part);
+  R pmass;
+  if constexpr (MASSB) {
+    pmass = d.mass;
+    dia_part = rm::cbrt(R(6.) * pmass / (static_cast<R>(kPIc) * rho_part));
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) d.dmk[spf] = R(0);
+  } else {
+    pmass = static_cast<R>(kPIc / 6.) * rho_part * (dia_part * dia_part * dia_part);
+  }
   const R startmass = pmass;
   R Reyn = R(0);
   R part_vel_src[3];
@@ -468,6 +490,10 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
       const R new_mass = pmass + m_dot * part_dt;
       if (new_mass > min_mass && newT < Tboil) {
         T_part = newT;
+        if constexpr (MASSB) {
+#pragma unroll
+          for (int spf = 0; spf < NF; ++spf) d.dmk[spf] += mi_dot[spf] * part_dt;
+        }
         if (NF > 1) {
           rho_part = R(0);
           R sumY = R(0);
@@ -491,6 +517,10 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
       } else {
         pmass = R(0.);
         d.alive = false;
+        if constexpr (MASSB) {
+#pragma unroll
+          for (int spf = 0; spf < NF; ++spf) d.dmk[spf] = -d.Y[spf] * startmass;
+        }
         nsub = isub;
         isub = nsub + 1;
       }
@@ -501,6 +531,31 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
     s.eng /= static_cast<R>(nsub);
 #pragma unroll
     for (int q = 0; q < 3; ++q) s.mom[q] /= static_cast<R>(nsub);
+  }
+  if constexpr (MASSB) {
+    R dm = R(0);
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) dm += d.dmk[spf];
+    const R inv_dt = R(1) / (fd.dtmod * flow_dt);
+    const R mdot_total = dm * inv_dt;
+    s.mass = num_ppp * mdot_total;
+    if (fd.mom_trans) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) s.mom[q] += num_ppp * vel_part[q] * mdot_total;
+    }
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      const int fdspec = fd.dep_indx[spf];
+      const R midot = d.dmk[spf] * inv_dt;
+      s.Ydot[spf] = num_ppp * midot;
+      s.eng += num_ppp * midot * sel_get<NS>(h_part, fdspec);
+      d.Y[spf] = Y_part[spf];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) d.vel[q] = vel_part[q];
+    d.T = T_part;
+    d.dia = dia_part;
+    return Reyn;
   }
   const R mdot_total = (pmass - startmass) / (fd.dtmod * flow_dt);
   s.mass = num_ppp * mdot_total;

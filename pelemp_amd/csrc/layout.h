@@ -37,6 +37,19 @@ struct Fields {
   static constexpr int NHOT_BRK = 13 + NF;  // with breakup state
 };
 
+// FP32 form: single-precision fields plus nf double-precision liquid species
+// masses per droplet in a separate array (mass bookkeeping stays FP64; the
+// diameter is written for output only and recomputed from the mass).
+struct Fields32 {
+  static constexpr int POS = 0;  // position inside the cell, [0,1)
+  static constexpr int VEL = 3;
+  static constexpr int T = 6;
+  static constexpr int DIA = 7;
+  static constexpr int ND = 8;
+  static constexpr int FILM = 9;
+  static constexpr int NHOT = 10;
+};
+
 PM_HD uint32_t tile_key(int i, int j, int k, const int* ntile) {
   const uint32_t tile = (static_cast<uint32_t>(k / kTile) * ntile[1] + (j / kTile)) * ntile[0] + (i / kTile);
   return tile * kTileCells + ((k % kTile) * kTile + (j % kTile)) * kTile + (i % kTile);

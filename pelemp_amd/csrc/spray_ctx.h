@@ -53,6 +53,7 @@ struct Ctx {
   size_t wsize = 8;      // bytes per hot field value
 
   void* fbuf[2] = {nullptr, nullptr};
+  double* mbuf[2] = {nullptr, nullptr};  // FP32 form: liquid species masses
   int32_t* slot[2] = {nullptr, nullptr};
   int cur = 0;
   int32_t* perm = nullptr;
@@ -73,6 +74,10 @@ struct Ctx {
   double* cold_bm2 = nullptr;
   int64_t nslots = 0;  // slots handed out
   int64_t host_count = 0;  // active droplets after the last sort (host copy, when known)
+  // active droplets at the start of every pass since the last read
+  static constexpr int kCountLog = 4096;
+  int32_t* count_log = nullptr;
+  int64_t npass = 0;
 
   // timing
   bool timing = false;

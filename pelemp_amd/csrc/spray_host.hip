@@ -194,6 +194,93 @@ __global__ void __launch_bounds__(256) gather_kernel(const int32_t* __restrict__
   id[s] = cold_id[sl];
 }
 
+// FP32 form: host-layout particles -> Fields32 + species masses.  `mass` is
+// the droplet mass pi/6 rho_l(T) d^3 computed on the host.
+__global__ void __launch_bounds__(256) ingest32_kernel(int64_t n, int nf, int nsr, const double* __restrict__ pos,
+                                                       const double* __restrict__ rdata,
+                                                       const double* __restrict__ mass,
+                                                       const int64_t* __restrict__ id, float* __restrict__ f,
+                                                       double* __restrict__ mk, int64_t cap,
+                                                       int32_t* __restrict__ slot, int64_t slot_base,
+                                                       int64_t* __restrict__ cold_id, double* __restrict__ cold_n0,
+                                                       double* __restrict__ cold_bm1, double* __restrict__ cold_bm2,
+                                                       uint32_t* __restrict__ key, double plo0, double plo1,
+                                                       double plo2, double dxi0, double dxi1, double dxi2, int nc0,
+                                                       int nc1, int nc2, int nt0, int nt1, int nt2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  using G = Fields32;
+  const double plo[3] = {plo0, plo1, plo2};
+  const double dxi[3] = {dxi0, dxi1, dxi2};
+  const int nc[3] = {nc0, nc1, nc2};
+  const int nt[3] = {nt0, nt1, nt2};
+  int c[3];
+  for (int q = 0; q < 3; ++q) {
+    const double lxc = (pos[3 * i + q] - plo[q]) * dxi[q];
+    int ic = static_cast<int>(floor(lxc));
+    ic = ic < 0 ? 0 : (ic >= nc[q] ? nc[q] - 1 : ic);
+    c[q] = ic;
+    double xi = lxc - static_cast<double>(ic);
+    xi = xi < 0. ? 0. : xi;
+    float xr = static_cast<float>(xi);
+    if (xr >= 1.f) xr = 0.99999994f;
+    f[(G::POS + q) * cap + i] = xr;
+  }
+  const double* r = rdata + static_cast<int64_t>(nsr) * i;
+  for (int q = 0; q < 3; ++q) f[(G::VEL + q) * cap + i] = static_cast<float>(r[q]);
+  f[G::T * cap + i] = static_cast<float>(r[3]);
+  f[G::DIA * cap + i] = static_cast<float>(r[4]);
+  f[G::ND * cap + i] = static_cast<float>(r[5 + nf]);
+  f[G::FILM * cap + i] = static_cast<float>(r[9 + nf]);
+  for (int s = 0; s < nf; ++s) mk[s * cap + i] = r[5 + s] * mass[i];
+  const int64_t sl = slot_base + i;
+  cold_n0[sl] = r[6 + nf];
+  cold_bm1[sl] = r[7 + nf];
+  cold_bm2[sl] = r[8 + nf];
+  slot[i] = static_cast<int32_t>(sl);
+  cold_id[sl] = id[i];
+  key[i] = tile_key(c[0], c[1], c[2], nt);
+}
+
+__global__ void __launch_bounds__(256) gather32_kernel(const int32_t* __restrict__ total, int nf, int nsr,
+                                                       const int32_t* __restrict__ perm, const float* __restrict__ f,
+                                                       const double* __restrict__ mk, int64_t cap,
+                                                       const int32_t* __restrict__ slot,
+                                                       const int64_t* __restrict__ cold_id,
+                                                       const double* __restrict__ cold_n0,
+                                                       const double* __restrict__ cold_bm1,
+                                                       const double* __restrict__ cold_bm2,
+                                                       const uint32_t* __restrict__ key, double* __restrict__ pos,
+                                                       double* __restrict__ rdata, int64_t* __restrict__ id,
+                                                       double plo0, double plo1, double plo2, double dx0, double dx1,
+                                                       double dx2, int nt0, int nt1, int nt2) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= *total) return;
+  using G = Fields32;
+  const int32_t o = perm[s];
+  const int nt[3] = {nt0, nt1, nt2};
+  int c[3];
+  decode_key(key[o], nt, c);
+  const double plo[3] = {plo0, plo1, plo2};
+  const double dx[3] = {dx0, dx1, dx2};
+  for (int q = 0; q < 3; ++q)
+    pos[3 * s + q] = plo[q] + (static_cast<double>(c[q]) + static_cast<double>(f[(G::POS + q) * cap + o])) * dx[q];
+  double* r = rdata + static_cast<int64_t>(nsr) * s;
+  for (int q = 0; q < 3; ++q) r[q] = static_cast<double>(f[(G::VEL + q) * cap + o]);
+  r[3] = static_cast<double>(f[G::T * cap + o]);
+  r[4] = static_cast<double>(f[G::DIA * cap + o]);
+  double m = 0.;
+  for (int sp = 0; sp < nf; ++sp) m += mk[sp * cap + o];
+  for (int sp = 0; sp < nf; ++sp) r[5 + sp] = (m > 0.) ? mk[sp * cap + o] / m : 0.;
+  r[5 + nf] = static_cast<double>(f[G::ND * cap + o]);
+  r[9 + nf] = static_cast<double>(f[G::FILM * cap + o]);
+  const int32_t sl = slot[o];
+  r[6 + nf] = cold_n0[sl];
+  r[7 + nf] = cold_bm1[sl];
+  r[8 + nf] = cold_bm2[sl];
+  id[s] = cold_id[sl];
+}
+
 // particle-CFL time step: min over droplets of cfl / max_d(|u_d|/dx_d)
 // (SprayParticles.cpp:112-124).  Positive doubles order like their bit
 // patterns, so an integer atomicMin gives the exact minimum.
@@ -291,8 +378,10 @@ static int alloc(Ctx& c, void** p, size_t bytes) {
 }
 
 static void free_ctx(Ctx& c) {
-  void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt, c.arrival, c.key,
-                  c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2};
+  void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.mbuf[0], c.mbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt,
+                  c.arrival, c.key,
+                  c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
+                  c.count_log};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -406,12 +495,21 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.ntiles = c.ntile[0] * c.ntile[1] * c.ntile[2];
   c.cap = max_particles;
   c.hot_brk = false;
-  c.nhot = c.hot_brk ? (13 + c.nf) : (10 + c.nf);
-  c.wsize = (form == PELEMP_FORM_FP64) ? 8 : 4;
+  if (form == PELEMP_FORM_FP64) {
+    c.nhot = c.hot_brk ? (13 + c.nf) : (10 + c.nf);
+    c.wsize = 8;
+  } else {
+    c.nhot = Fields32::NHOT;
+    c.wsize = 4;
+  }
   int st = PELEMP_OK;
   const size_t fb = c.wsize * c.nhot * static_cast<size_t>(c.cap);
   st |= alloc(c, &c.fbuf[0], fb);
   st |= alloc(c, &c.fbuf[1], fb);
+  if (form != PELEMP_FORM_FP64) {
+    st |= alloc(c, reinterpret_cast<void**>(&c.mbuf[0]), 8 * static_cast<size_t>(c.nf) * c.cap);
+    st |= alloc(c, reinterpret_cast<void**>(&c.mbuf[1]), 8 * static_cast<size_t>(c.nf) * c.cap);
+  }
   st |= alloc(c, reinterpret_cast<void**>(&c.slot[0]), 4 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.slot[1]), 4 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.perm), 4 * c.cap);
@@ -427,6 +525,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_n0), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm1), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm2), 8 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.count_log), 4 * Ctx::kCountLog);
   if (st != PELEMP_OK) {
     free_ctx(c);
     delete h;
@@ -500,6 +599,7 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
   }
   c.cur = 0;
   c.nslots = 0;
+  c.npass = 0;
   const double dxi[3] = {1. / c.geom.dx[0], 1. / c.geom.dx[1], 1. / c.geom.dx[2]};
   if (m > 0) {
     const dim3 grid(static_cast<unsigned>((m + 255) / 256)), block(256);
@@ -510,11 +610,29 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
                          c.geom.plo[2], dxi[0], dxi[1], dxi[2], c.ncell[0], c.ncell[1], c.ncell[2],
                          c.ntile[0], c.ntile[1], c.ntile[2]);
     } else {
-      hipLaunchKernelGGL((ingest_kernel<float, false>), grid, block, 0, 0, m, c.nf, nsr, dpos, drd, did,
-                         static_cast<float*>(c.fbuf[0]), c.cap, c.hot_brk ? 1 : 0, c.slot[0], c.nslots,
-                         c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, c.geom.plo[0], c.geom.plo[1],
-                         c.geom.plo[2], dxi[0], dxi[1], dxi[2], c.ncell[0], c.ncell[1], c.ncell[2],
-                         c.ntile[0], c.ntile[1], c.ntile[2]);
+      // droplet mass pi/6 rho_l(T) d^3 with the mixture liquid density
+      // 1/sum(Y/rhoL(T)) (Drag.H:190-199 at T below the boiling point)
+      std::vector<double> hm(m);
+      for (int64_t i = 0; i < m; ++i) {
+        const double* r = &hr[nsr * i];
+        const double T = r[3];
+        double inv = 0.;
+        for (int s = 0; s < c.nf; ++s) {
+          const double* a = &c.sd.rho_coef[4 * s];
+          inv += r[5 + s] / (a[0] + T * (a[1] + T * (a[2] + T * a[3])));
+        }
+        hm[i] = kPIc / 6. * (1. / inv) * (r[4] * r[4] * r[4]);
+      }
+      double* dm = nullptr;
+      PM_HIP(hipMalloc(&dm, 8 * m));
+      PM_HIP(hipMemcpy(dm, hm.data(), 8 * m, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(ingest32_kernel, grid, block, 0, 0, m, c.nf, nsr, dpos, drd, dm, did,
+                         static_cast<float*>(c.fbuf[0]), c.mbuf[0], c.cap, c.slot[0], c.nslots, c.cold_id,
+                         c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, c.geom.plo[0], c.geom.plo[1], c.geom.plo[2],
+                         dxi[0], dxi[1], dxi[2], c.ncell[0], c.ncell[1], c.ncell[2], c.ntile[0], c.ntile[1],
+                         c.ntile[2]);
+      PM_HIP(hipDeviceSynchronize());
+      (void)hipFree(dm);
     }
     PM_HIP(hipGetLastError());
   }
@@ -568,8 +686,8 @@ int pelemp_spray_get_particles(pelemp_spray_ctx* h, int64_t* n, double* pos, dou
                          c.geom.plo[1], c.geom.plo[2], c.geom.dx[0], c.geom.dx[1], c.geom.dx[2], c.ntile[0],
                          c.ntile[1], c.ntile[2]);
     } else {
-      hipLaunchKernelGGL((gather_kernel<float, false>), grid, block, 0, 0, c.cstart + c.ncells, c.nf, nsr, c.perm,
-                         static_cast<const float*>(c.fbuf[c.cur]), c.cap, c.hot_brk ? 1 : 0, c.slot[c.cur],
+      hipLaunchKernelGGL(gather32_kernel, grid, block, 0, 0, c.cstart + c.ncells, c.nf, nsr, c.perm,
+                         static_cast<const float*>(c.fbuf[c.cur]), c.mbuf[c.cur], c.cap, c.slot[c.cur],
                          c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, c.key, dpos, drd, did, c.geom.plo[0],
                          c.geom.plo[1], c.geom.plo[2], c.geom.dx[0], c.geom.dx[1], c.geom.dx[2], c.ntile[0],
                          c.ntile[1], c.ntile[2]);
@@ -633,6 +751,11 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
   p.state = state;
   p.source = source;
   for (int it = 0; it < num_iter; ++it) {
+    if (c.npass < Ctx::kCountLog) {
+      PM_HIP(hipMemcpyAsync(c.count_log + c.npass, c.cstart + c.ncells, sizeof(int32_t),
+                            hipMemcpyDeviceToDevice, s));
+    }
+    ++c.npass;
     hipEvent_t ev0;
     ctx_time_begin(c, 0, s, &ev0);
     const int st = (c.form == PELEMP_FORM_FP64) ? launch_update_fp64(c, p, s) : launch_update_fp32(c, p, s);
@@ -693,6 +816,19 @@ int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double* dt, void*
   int32_t t = 0;
   PM_HIP(hipMemcpy(&t, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
   *dt = (t == 0) ? -1. : v;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_pass_counts(pelemp_spray_ctx* h, int64_t* out, int32_t max, int32_t* n_out) {
+  if (!h || !n_out || (max > 0 && !out)) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  PM_HIP(hipDeviceSynchronize());
+  const int64_t m = std::min<int64_t>(std::min<int64_t>(c.npass, Ctx::kCountLog), max);
+  std::vector<int32_t> tmp(static_cast<size_t>(std::max<int64_t>(m, 1)));
+  if (m > 0) PM_HIP(hipMemcpy(tmp.data(), c.count_log, 4 * m, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < m; ++i) out[i] = tmp[i];
+  *n_out = static_cast<int32_t>(m);
+  c.npass = 0;
   return PELEMP_OK;
 }
 

@@ -49,6 +49,8 @@ template <class R, int NF>
 struct UpdateArgs {
   const R* __restrict__ fin;
   R* __restrict__ fout;
+  const double* __restrict__ min;  // FP32 form: species masses (nf x cap)
+  double* __restrict__ mout;
   long long cap;
   const int32_t* __restrict__ perm;
   const int32_t* __restrict__ cstart;
@@ -179,17 +181,42 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
 
       Drop<R, NF> d;
       R pos[3];
+      R film;
+      double mk[NF];  // FP32 form: liquid species masses
+      if constexpr (POS64) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        pos[q] = a.fin[(F::POS + q) * cap + o];
-        d.vel[q] = a.fin[(F::VEL + q) * cap + o];
+        for (int q = 0; q < 3; ++q) {
+          pos[q] = a.fin[(F::POS + q) * cap + o];
+          d.vel[q] = a.fin[(F::VEL + q) * cap + o];
+        }
+        d.T = a.fin[F::T * cap + o];
+        d.dia = a.fin[F::DIA * cap + o];
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) d.Y[spf] = a.fin[(F::Y + spf) * cap + o];
+        d.numDens = a.fin[F::ND * cap + o];
+        film = a.fin[F::FILM * cap + o];
+      } else {
+        using G = Fields32;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          pos[q] = a.fin[(G::POS + q) * cap + o];
+          d.vel[q] = a.fin[(G::VEL + q) * cap + o];
+        }
+        d.T = a.fin[G::T * cap + o];
+        d.numDens = a.fin[G::ND * cap + o];
+        film = a.fin[G::FILM * cap + o];
+        double m = 0.;
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) {
+          mk[spf] = a.min[spf * cap + o];
+          m += mk[spf];
+        }
+        const double im = 1. / m;
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) d.Y[spf] = static_cast<R>(mk[spf] * im);
+        d.mass = static_cast<R>(m);
+        d.dia = R(0);
       }
-      d.T = a.fin[F::T * cap + o];
-      d.dia = a.fin[F::DIA * cap + o];
-#pragma unroll
-      for (int spf = 0; spf < NF; ++spf) d.Y[spf] = a.fin[(F::Y + spf) * cap + o];
-      d.numDens = a.fin[F::ND * cap + o];
-      const R film = a.fin[F::FILM * cap + o];
       d.bm1 = R(0);
       d.alive = true;
 
@@ -279,7 +306,7 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
 #pragma unroll
         for (int spf = 0; spf < NF; ++spf) sr.Ydot[spf] = R(0);
       } else {
-        spray_source<M, NF, R>(a.sub_dt, g, a.fc, d, cBoilT, sr);
+        spray_source<M, NF, R, !POS64>(a.sub_dt, g, a.fc, d, cBoilT, sr);
       }
       // contributions cur_coef * source (SprayParticles.cpp:419-441)
       double* cb = &contrib[t * NSRC];
@@ -343,17 +370,45 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
       }
 
       R* fo = a.fout;
+      if constexpr (POS64) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        fo[(F::POS + q) * cap + s] = pos[q];
-        fo[(F::VEL + q) * cap + s] = d.vel[q];
+        for (int q = 0; q < 3; ++q) {
+          fo[(F::POS + q) * cap + s] = pos[q];
+          fo[(F::VEL + q) * cap + s] = d.vel[q];
+        }
+        fo[F::T * cap + s] = d.T;
+        fo[F::DIA * cap + s] = d.dia;
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) fo[(F::Y + spf) * cap + s] = d.Y[spf];
+        fo[F::ND * cap + s] = d.numDens;
+        fo[F::FILM * cap + s] = film;
+      } else {
+        using G = Fields32;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          fo[(G::POS + q) * cap + s] = pos[q];
+          fo[(G::VEL + q) * cap + s] = d.vel[q];
+        }
+        fo[G::T * cap + s] = d.T;
+        fo[G::DIA * cap + s] = d.dia;
+        fo[G::ND * cap + s] = d.numDens;
+        fo[G::FILM * cap + s] = film;
+        // FP64 mass bookkeeping: accumulate the single-precision increments
+        double m = 0.;
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) {
+          double v = mk[spf] + static_cast<double>(d.dmk[spf]);
+          v = v > 0. ? v : 0.;
+          mk[spf] = v;
+          m += v;
+        }
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) {
+          // Y <= 1e-12 is set to zero (Drag.H:385-387)
+          const double v = (mk[spf] <= 1.e-12 * m) ? 0. : mk[spf];
+          a.mout[spf * cap + s] = v;
+        }
       }
-      fo[F::T * cap + s] = d.T;
-      fo[F::DIA * cap + s] = d.dia;
-#pragma unroll
-      for (int spf = 0; spf < NF; ++spf) fo[(F::Y + spf) * cap + s] = d.Y[spf];
-      fo[F::ND * cap + s] = d.numDens;
-      fo[F::FILM * cap + s] = film;
       a.slot_out[s] = a.slot_in[o];
       a.key_out[s] = key;
     }

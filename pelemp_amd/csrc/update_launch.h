@@ -24,6 +24,8 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   UpdateArgs<R, NF> a{};
   a.fin = static_cast<const R*>(c.fbuf[c.cur]);
   a.fout = static_cast<R*>(c.fbuf[1 - c.cur]);
+  a.min = c.mbuf[c.cur];
+  a.mout = c.mbuf[1 - c.cur];
   a.cap = c.cap;
   a.perm = c.perm;
   a.cstart = c.cstart;

@@ -188,6 +188,13 @@ class SprayParticleContainer:
         return float(dt.value)
 
     # -- timing ---------------------------------------------------------------
+    def pass_counts(self) -> list:
+        """Active droplets at the start of each update pass since the last call."""
+        buf = (ctypes.c_int64 * 4096)()
+        n = ctypes.c_int32()
+        C.check(self._lib.pelemp_spray_pass_counts(self._h, buf, 4096, ctypes.byref(n)), "pelemp_spray_pass_counts")
+        return [int(buf[i]) for i in range(n.value)]
+
     def set_timing(self, enable: bool) -> None:
         C.check(self._lib.pelemp_spray_set_timing(self._h, 1 if enable else 0), "pelemp_spray_set_timing")
 
