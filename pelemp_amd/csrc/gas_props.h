@@ -14,8 +14,12 @@
 //   ln mu_k, ln lambda_k, ln D_ij(1 atm) = cubic in ln T
 //   mu      = (sum_k X_k mu_k^6)^(1/6)
 //   lambda  = (sum_k X_k lambda_k^(1/4))^4
-//   rho D_k = rho (sum_{j!=k} Y_j) / (sum_{j!=k} X_j / D_kj(p)),
+//   Ddiag_k = rho D_k W_k / W_mix,
+//   D_k     = (sum_{j!=k} Y_j) / (sum_{j!=k} X_j / D_kj(p)),
 //             D_kj(p) = D_kj(1 atm) p_atm / p,  p = rho R T / W_mix
+// Ddiag carries the factor W_k/W_mix (diffusion flux written with mole-
+// fraction gradients); the droplet model removes it again when it converts
+// Ddiag of the fuel with mw_skin / mw_fuel (Drag.H:312-314).
 // The functions are templated on the arithmetic type: double evaluates with
 // the project's pm:: elementary functions (bitwise identical on host and
 // device), float with the gfx950 hardware approximations.
@@ -172,7 +176,7 @@ struct GasProps {
           term1 += Y[j];
           term2 += X[j] * invD[pidx];
         }
-        Ddiag[i] = rho * term1 / (term2 * pratio);
+        Ddiag[i] = rho * term1 / (term2 * pratio) * (static_cast<R>(M::W[i]) / wbar);
       }
     }
   }
