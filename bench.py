@@ -121,6 +121,37 @@ def cpu_baseline(prob, target_s: float = 12.0):
                 seconds=t_s)
 
 
+def calibrate_copy() -> int:
+    """Three device-to-device copies of 1 GiB: the reference kernel for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    from pelemp_amd import _capi as C
+
+    n = 1 << 30
+    a, b = C.DeviceBuffer(n), C.DeviceBuffer(n)
+    a.zero()
+    b.zero()
+    for _ in range(3):
+        C.check(C.lib().pelemp_memcpy_dtod(b.ptr, a.ptr, n), "copy")
+    C.device_synchronize()
+    print("copied 3 x 1 GiB", file=sys.stderr)
+    return 0
+
+
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def measured_traffic(form: str):
+    """HBM bytes per launch of the update kernel from the committed rocprofv3
+    PMC summary (profiles/spray_update_config2_hbm_traffic.md), or None."""
+    path = os.path.join(PROFILES, "spray_update_config2_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch[{form}]"):
+            return float(line.split("=")[1].split()[0])
+    return None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,11 +161,15 @@ def main(argv=None):
     ap.add_argument("--ncell", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the 1 GiB copy kernels used to calibrate PMC byte counters")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by spatial domain decomposition, see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy()
 
     from pelemp_amd import _capi as C
     from pelemp_amd import problem as P
@@ -158,7 +193,7 @@ def main(argv=None):
         forms[name] = dict(value=r["rate"], ms_per_step=r["ms_total"] / args.steps, update_kernel_ms=r["update_ms"],
                            sort_ms=r["sort_ms"], bytes_per_droplet_step=b, achieved_GBs=achieved,
                            hbm_frac=achieved / HBM_PEAK_GBS, droplets_end=r["n_end"],
-                           droplet_steps=r["droplet_steps"])
+                           droplet_steps=r["droplet_steps"], traffic=measured_traffic(name))
         print(f"# {name}: {r['rate']:.4g} droplet-steps/s, {r['ms_total'] / args.steps:.3f} ms/step, "
               f"update kernel {r['update_ms']:.3f} ms, sort {r['sort_ms']:.3f} ms, {achieved:.0f} GB/s",
               file=sys.stderr, flush=True)
@@ -182,7 +217,7 @@ def main(argv=None):
                                "drag+evaporation+deposition, dt=5e-7 s, dtmod=1",
                    "droplets": prob.n, "cells": ncells, "form": main_form},
         "roofline": {"bound": "hbm", "achieved": mf["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": mf["hbm_frac"], "traffic": None,
+                     "frac": mf["hbm_frac"], "traffic": mf["traffic"],
                      "kernel": "spray_update_kernel", "bytes_per_droplet_step": mf["bytes_per_droplet_step"]},
         "forms": forms,
     }

@@ -14,10 +14,8 @@
 // into FuelConst by the host; hoisting a whole subexpression does not change
 // any rounding.
 //
-// Species indices of the fuels (SprayData::indx / dep_indx) are runtime
-// values; register arrays indexed by them are read and written through
-// unrolled compare-and-select loops (sel_get / sel_set) so that they stay in
-// registers on the GPU.
+// Species indices of the fuels are compile-time (fuel_species below), so
+// the per-species register arrays are never indexed dynamically.
 //
 // For R = float the same sequence runs in single precision with hardware
 // elementary functions; the caller keeps the source sums in double.
@@ -50,17 +48,14 @@ PM_HD R rmax(R a, R b) {
   return (a < b) ? b : a;  // std::max semantics
 }
 
-template <int N, class R>
-PM_HD R sel_get(const R* a, int i) {
-  R v = a[0];
-#pragma unroll
-  for (int n = 1; n < N; ++n) v = (n == i) ? a[n] : v;
-  return v;
-}
-template <int N, class R>
-PM_HD void sel_set(R* a, int i, R v) {
-#pragma unroll
-  for (int n = 0; n < N; ++n) a[n] = (n == i) ? v : a[n];
+// Gas species index of liquid fuel spf.  Every compiled mechanism lists its
+// NF fuel species last and the vapour of each fuel is deposited into its own
+// species (indx == dep_indx), which the context checks at creation; the
+// index is then a compile-time constant and the per-species arrays stay in
+// registers.
+template <int NS, int NF>
+PM_HD constexpr int fuel_species(int spf) {
+  return NS - NF + spf;
 }
 
 // Fuel data hoisted per launch (droplet independent).
@@ -157,7 +152,7 @@ struct Drop {
 
 // SprayData::rhoL (SprayFuelData.H:146-153)
 template <class R, int NF>
-PM_HD R rhoL(const FuelConst<R, NF>& f, R T, int spf) {
+PM_HDI R rhoL(const FuelConst<R, NF>& f, R T, int spf) {
   const R a = f.rho_coef[4 * spf], b = f.rho_coef[4 * spf + 1];
   const R c = f.rho_coef[4 * spf + 2], d = f.rho_coef[4 * spf + 3];
   return a + T * (b + T * (c + T * d));
@@ -165,7 +160,7 @@ PM_HD R rhoL(const FuelConst<R, NF>& f, R T, int spf) {
 
 // SprayData::psat (SprayFuelData.H:204-211)
 template <class R, int NF>
-PM_HD R psat(const FuelConst<R, NF>& f, R T, int spf) {
+PM_HDI R psat(const FuelConst<R, NF>& f, R T, int spf) {
   const R a = f.psat_coef[4 * spf], b = f.psat_coef[4 * spf + 1];
   const R c = f.psat_coef[4 * spf + 2], d = f.psat_coef[4 * spf + 3];
   return d * rm::pow(R(10.), a - b / (T + c));
@@ -173,7 +168,7 @@ PM_HD R psat(const FuelConst<R, NF>& f, R T, int spf) {
 
 // GasPhaseVals::define (SprayFuelData.H:101-111): mixture MW and pressure
 template <class M, class R>
-PM_HD void gas_define(GasAt<R, M::NS>& g) {
+PM_HDI void gas_define(GasAt<R, M::NS>& g) {
   R s = R(0);
 #pragma unroll
   for (int n = 0; n < M::NS; ++n) s += g.Y[n] / static_cast<R>(M::W[n]);
@@ -183,7 +178,7 @@ PM_HD void gas_define(GasAt<R, M::NS>& g) {
 
 // SprayData::calcBoilT (SprayFuelData.H:180-200)
 template <int NF, class R>
-PM_HD void calc_boil_t(const FuelConst<R, NF>& f, R p, R* cBoilT) {
+PM_HDI void calc_boil_t(const FuelConst<R, NF>& f, R p, R* cBoilT) {
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
     const R cb = R(1) / (rm::log(static_cast<R>(kPATM) / p) * static_cast<R>(kRU) / f.hboil_mw[spf] +
@@ -194,7 +189,7 @@ PM_HD void calc_boil_t(const FuelConst<R, NF>& f, R p, R* cBoilT) {
 
 // calcHeatCoeff (Drag.H:8-45)
 template <class R>
-PM_HD R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
+PM_HDI R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
   if (B_M <= C_eps) return R(0);
   const int maxIter = 100;
   const R NU2 = Nu_0 - R(2);
@@ -226,7 +221,7 @@ PM_HD R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
 
 // calcVaporState (Drag.H:47-135)
 template <class M, int NF, class R>
-PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g, R rule, R T_in,
+PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g, R rule, R T_in,
                             R C_eps, R mw_part, const R* Y_l, const R* h_part, const R* cp_n,
                             const R* cBoilT, R* Y_skin, R* X_vapor, R* L_fuel, R& B_M, R& sumXVap,
                             R& cp_skin, R& mw_skin) {
@@ -234,12 +229,12 @@ PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g
   R mbar_vap = R(0);
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
-    const int fspec = fd.indx[spf];
-    const int fdspec = fd.dep_indx[spf];
+    const int fspec = fuel_species<NS, NF>(spf);
+    const int fdspec = fspec;
     const R boilT_ref = fd.boilT[spf];
     const R mw_fuel = fd.mw_fuel[spf];
     const R T_part = rmin(T_in, R(0.99) * cBoilT[spf]);
-    const R part_latent = sel_get<NS>(h_part, fspec) + fd.latent[spf] - fd.cp[spf] * (T_part - fd.ref_T);
+    const R part_latent = h_part[fspec] + fd.latent[spf] - fd.cp[spf] * (T_part - fd.ref_T);
     L_fuel[spf] = part_latent;
     R pres_sat = R(0);
     if (fd.cc_psat[spf]) {
@@ -248,7 +243,7 @@ PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g
     } else {
       pres_sat = psat<R, NF>(fd, T_part, spf);
     }
-    const R X_fluid = sel_get<NS>(g.Y, fdspec) * g.mw_mix / mw_fuel;
+    const R X_fluid = g.Y[fdspec] * g.mw_mix / mw_fuel;
     if (pres_sat > X_fluid * g.p) {
       const R Xl = Y_l[spf] * mw_part / mw_fuel;
       const R Xv = Xl * pres_sat / g.p;
@@ -264,15 +259,15 @@ PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
     if (X_vapor[spf] > R(0)) {
-      const int fspec = fd.indx[spf];
-      const int fdspec = fd.dep_indx[spf];
+      const int fspec = fuel_species<NS, NF>(spf);
+      const int fdspec = fspec;
       const R mw_fuel = fd.mw_fuel[spf];
       const R Yfv = X_vapor[spf] * mw_fuel / totalmwx;
       sumYVap += Yfv;
-      const R gasY = sel_get<NS>(g.Y, fdspec);
+      const R gasY = g.Y[fdspec];
       sumYfFluid += gasY;
       const R Ysk = Yfv + rule * (gasY - Yfv);
-      sel_set<NS>(Y_skin, fspec, Ysk);
+      Y_skin[fspec] = Ysk;
       sumYSkin += Ysk;
     }
   }
@@ -298,7 +293,7 @@ PM_HD void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g
 // instead of from (pmass - startmass); in exact arithmetic the two agree for
 // the explicit Euler update of Drag.H:371-400.
 template <class M, int NF, class R, bool MASSB = false>
-PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
+PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
                      Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s) {
   constexpr int NS = M::NS;
   const R rule = R(1.) / R(3.);
@@ -432,16 +427,16 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
         R Dfuel[NF];
 #pragma unroll
         for (int spf = 0; spf < NF; ++spf) {
-          const int fspec = fd.indx[spf];
-          R dd = sel_get<NS>(Ddiag, fspec);
+          const int fspec = fuel_species<NS, NF>(spf);
+          R dd = Ddiag[fspec];
           dd *= mw_skin / fd.mw_fuel[spf];
           const R X_norm = X_vapor[spf] / sumXVap;
           dd *= X_norm;
-          sel_set<NS>(Ddiag, fspec, dd);
+          Ddiag[fspec] = dd;
           Dfuel[spf] = dd;
           rhoDtotal += dd;
-          const R ysk = sel_get<NS>(Y_skin, fspec);
-          cp_fuel += ysk * sel_get<NS>(cp_n, fspec);
+          const R ysk = Y_skin[fspec];
+          cp_fuel += ysk * cp_n[fspec];
           sumYSkin += ysk;
         }
         cp_fuel /= sumYSkin;
@@ -545,10 +540,10 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
     }
 #pragma unroll
     for (int spf = 0; spf < NF; ++spf) {
-      const int fdspec = fd.dep_indx[spf];
+      const int fdspec = fuel_species<NS, NF>(spf);
       const R midot = d.dmk[spf] * inv_dt;
       s.Ydot[spf] = num_ppp * midot;
-      s.eng += num_ppp * midot * sel_get<NS>(h_part, fdspec);
+      s.eng += num_ppp * midot * h_part[fdspec];
       d.Y[spf] = Y_part[spf];
     }
 #pragma unroll
@@ -565,12 +560,12 @@ PM_HD R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>
   }
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
-    const int fdspec = fd.dep_indx[spf];
+    const int fdspec = fuel_species<NS, NF>(spf);
     const R oldY = d.Y[spf];
     const R newY = Y_part[spf];
     const R midot = (newY * pmass - oldY * startmass) / (fd.dtmod * flow_dt);
     s.Ydot[spf] = num_ppp * midot;
-    s.eng += num_ppp * midot * sel_get<NS>(h_part, fdspec);
+    s.eng += num_ppp * midot * h_part[fdspec];
     d.Y[spf] = Y_part[spf];
   }
 #pragma unroll

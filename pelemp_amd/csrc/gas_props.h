@@ -70,14 +70,14 @@ struct GasProps {
 
   // Molecular weights [kg/mol] (eos.molecular_weight * mass_conv).
   template <class R>
-  static PM_HD void molecular_weight(R* mw) {
+  static PM_HDI void molecular_weight(R* mw) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) mw[n] = static_cast<R>(M::W[n]);
   }
 
   // Species heat capacities at constant pressure [J/kg/K].
   template <class R>
-  static PM_HD void T2Cpi(R T, R* cp) {
+  static PM_HDI void T2Cpi(R T, R* cp) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const bool lo = T < static_cast<R>(M::Tmid[n]);
@@ -93,7 +93,7 @@ struct GasProps {
 
   // Species specific enthalpies [J/kg] (incl. heat of formation).
   template <class R>
-  static PM_HD void T2Hi(R T, R* h) {
+  static PM_HDI void T2Hi(R T, R* h) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const bool lo = T < static_cast<R>(M::Tmid[n]);
@@ -112,7 +112,7 @@ struct GasProps {
 
   // Mixture-averaged transport at (T, rho, Y).  Ddiag[k] = rho D_k,mix.
   template <class R>
-  static PM_HD void transport(bool get_mu, bool get_lam, bool get_D, R T,
+  static PM_HDI void transport(bool get_mu, bool get_lam, bool get_D, R T,
                               R rho, const R* Y, R* Ddiag, R& mu, R& lam) {
     const R lt = rm::log(T);
     const R lt2 = lt * lt;

@@ -48,8 +48,12 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PM_HD __host__ __device__ inline
+// Functions that take pointers to per-droplet register arrays must be
+// inlined: an out-of-line call forces those arrays into scratch memory.
+#define PM_HDI __host__ __device__ __attribute__((always_inline)) inline
 #else
 #define PM_HD inline
+#define PM_HDI inline
 #endif
 
 namespace pm {

@@ -60,6 +60,9 @@ struct Ctx {
   int32_t* cstart = nullptr;  // ncells + 1
   int32_t* cnt = nullptr;
   int32_t* arrival = nullptr;
+  int32_t* mov_lo = nullptr;    // per cell: droplets arriving from lower keys
+  int32_t* mov_hi = nullptr;    // per cell: droplets arriving from higher keys
+  int32_t* stay_cnt = nullptr;  // per cell: droplets that stayed
   uint32_t* key = nullptr;  // key of each storage position of buffer cur
   int32_t* n_entries = nullptr;  // entries in buffer cur (device scalar)
   int32_t* block_sum = nullptr;

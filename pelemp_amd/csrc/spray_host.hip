@@ -367,6 +367,26 @@ static int run_sort(Ctx& c, hipStream_t s) {
   return PELEMP_OK;
 }
 
+// Re-sort after an update pass (incremental form, see sort_kernels.h).
+static int run_sort_incremental(Ctx& c, hipStream_t s) {
+  hipEvent_t ev0;
+  ctx_time_begin(c, 1, s, &ev0);
+  PM_HIP(hipMemcpyAsync(c.n_entries, c.cstart + c.ncells, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  const unsigned gc = static_cast<unsigned>((c.ncells + 255) / 256);
+  hipLaunchKernelGGL(sum3_kernel, dim3(gc), dim3(256), 0, s, c.mov_lo, c.stay_cnt, c.mov_hi, c.ncells, c.cnt);
+  const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, c.ncells, c.cstart, c.block_sum);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.block_sum, nb, c.total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total);
+  hipLaunchKernelGGL(scatter_incr_kernel, dim3(2048), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
+                     c.mov_lo, c.stay_cnt, c.perm);
+  hipLaunchKernelGGL(order_movers_kernel, dim3(gc), dim3(256), 0, s, c.cstart, c.mov_lo, c.stay_cnt, c.mov_hi,
+                     c.ncells, c.perm);
+  PM_HIP(hipGetLastError());
+  ctx_time_end(c, 1, s, ev0);
+  return PELEMP_OK;
+}
+
 static int alloc(Ctx& c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (hipMalloc(p, bytes) != hipSuccess) {
@@ -379,7 +399,7 @@ static int alloc(Ctx& c, void** p, size_t bytes) {
 
 static void free_ctx(Ctx& c) {
   void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.mbuf[0], c.mbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt,
-                  c.arrival, c.key,
+                  c.arrival, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
                   c.count_log};
   for (void* p : ptrs)
@@ -461,6 +481,13 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
       set_last_error("fuel species index not in the mechanism");  // SpraySetup.cpp:272-277
       return PELEMP_E_BAD_CONFIG;
     }
+    // the kernels address fuel species at compile time: the fuels are the
+    // last nfuel species of the mechanism and deposit into themselves
+    if (sd->indx[i] != ns - sd->nfuel + i || sd->dep_indx[i] != sd->indx[i]) {
+      set_last_error("fuel species must be the last nfuel species of the mechanism, in order, "
+                     "with dep_indx == indx");
+      return PELEMP_E_BAD_CONFIG;
+    }
   }
   if (sd->do_breakup != 0 || sd->do_splash != 0) {
     set_last_error("breakup / splash are not enabled in this build of the update");
@@ -516,6 +543,8 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.cstart), 4 * (static_cast<size_t>(c.ncells) + 1));
   st |= alloc(c, reinterpret_cast<void**>(&c.cnt), 4 * static_cast<size_t>(c.ncells));
   st |= alloc(c, reinterpret_cast<void**>(&c.arrival), 4 * c.cap);
+  // mov_lo, stay_cnt, mov_hi in one allocation so one memset clears them
+  st |= alloc(c, reinterpret_cast<void**>(&c.mov_lo), 3 * 4 * static_cast<size_t>(c.ncells));
   st |= alloc(c, reinterpret_cast<void**>(&c.key), 4 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.block_sum),
               4 * (static_cast<size_t>(c.ncells) / kScanBlock + 2));
@@ -531,6 +560,8 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
     delete h;
     return PELEMP_E_NOMEM;
   }
+  c.stay_cnt = c.mov_lo + c.ncells;
+  c.mov_hi = c.mov_lo + 2 * static_cast<size_t>(c.ncells);
   c.n_entries = c.scalars;
   c.total = c.scalars + 1;
   c.err = c.scalars + 2;
@@ -756,13 +787,14 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
                             hipMemcpyDeviceToDevice, s));
     }
     ++c.npass;
+    PM_HIP(hipMemsetAsync(c.mov_lo, 0, 3 * 4 * static_cast<size_t>(c.ncells), s));
     hipEvent_t ev0;
     ctx_time_begin(c, 0, s, &ev0);
     const int st = (c.form == PELEMP_FORM_FP64) ? launch_update_fp64(c, p, s) : launch_update_fp32(c, p, s);
     if (st != PELEMP_OK) return st;
     ctx_time_end(c, 0, s, ev0);
     c.cur = 1 - c.cur;
-    const int st2 = run_sort(c, s);
+    const int st2 = run_sort_incremental(c, s);
     if (st2 != PELEMP_OK) return st2;
   }
   return PELEMP_OK;

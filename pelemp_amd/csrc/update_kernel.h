@@ -57,6 +57,11 @@ struct UpdateArgs {
   const int32_t* __restrict__ slot_in;
   int32_t* __restrict__ slot_out;
   uint32_t* __restrict__ key_out;
+  // incremental re-sort bookkeeping (see sort_kernels.h)
+  int32_t* __restrict__ arrival_out;
+  int32_t* __restrict__ mov_lo;
+  int32_t* __restrict__ mov_hi;
+  int32_t* __restrict__ stay_cnt;
   FabDev st;
   int c_rho, c_vel, c_spec, c_T;
   FabDev src;
@@ -85,7 +90,7 @@ __device__ inline int xcd_tile(int b, int ntiles) {
 }
 
 // check_bounds (SprayInterpolation.H:64-97)
-__device__ inline bool dev_check_bounds(const double* pos, const double* plo, const double* phi,
+__device__ __attribute__((always_inline)) inline bool dev_check_bounds(const double* pos, const double* plo, const double* phi,
                                         const double* dx, const int* blo, const int* bhi,
                                         int* bflags) {
   for (int hilo = 0; hilo < 2; ++hilo) {
@@ -110,8 +115,17 @@ __device__ inline bool dev_check_bounds(const double* pos, const double* plo, co
   return false;
 }
 
+// Occupancy target: the FP64 form needs ~265 registers per lane at full
+// freedom (1 wave/SIMD); it is compiled for 2 workgroups per CU (2 waves per
+// SIMD, 256 registers) at the cost of a few spilled values.
+template <class R>
+struct UpdOccupancy {
+  static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : 1;
+};
+
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
-__global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R, NF> a) {
+__global__ void __launch_bounds__(kUpdThreads, UpdOccupancy<R>::kMinBlocks)
+    spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NS = M::NS;
   constexpr int NG = 5 + NS;     // staged gas fields: rho, Y[NS], vel[3], T
   constexpr int NSRC = 5 + NF;   // mom[3], mass, Ydot[NF], eng
@@ -122,6 +136,7 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
   __shared__ unsigned char gvalid[kHaloCells];
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ double contrib[kUpdThreads * NSRC];
+  __shared__ unsigned char stayed[kUpdThreads];
 
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -166,6 +181,7 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
   double acc[NITEM];
 #pragma unroll
   for (int r = 0; r < NITEM; ++r) acc[r] = 0.;
+  int32_t srank = 0;  // threads t < 64: stayers of cell t seen so far
 
   const long long cap = a.cap;
   for (int32_t base = d0; base < d1; base += kUpdThreads) {
@@ -411,8 +427,26 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
       }
       a.slot_out[s] = a.slot_in[o];
       a.key_out[s] = key;
+      // classify for the incremental re-sort
+      const uint32_t k_old = static_cast<uint32_t>(tile) * kTileCells + lc;
+      stayed[t] = (key == k_old) ? 1 : 0;
+      if (key != kDeadKey && key != k_old) {
+        // relative to the new cell `key`, the droplet comes from a cell with a
+        // lower key (it precedes the stayers) or a higher key (it follows them)
+        const int32_t arr = (k_old < key) ? (atomicAdd(&a.mov_lo[key], 1) | (1 << 30))
+                                          : (atomicAdd(&a.mov_hi[key], 1) | (2 << 30));
+        a.arrival_out[s] = arr;
+      }
     }
     __syncthreads();
+    // stayers: rank within their cell in position order (cell owner threads)
+    if (t < kTileCells) {
+      const int32_t cend = (base + kUpdThreads < d1) ? base + kUpdThreads : d1;
+      const int32_t lo = cs[t] > base ? cs[t] : base;
+      const int32_t hi = cs[t + 1] < cend ? cs[t + 1] : cend;
+      for (int32_t i = lo; i < hi; ++i)
+        if (stayed[i - base]) a.arrival_out[i] = srank++;
+    }
     // ordered per-cell reduction of this chunk
     const int32_t chunk_end = (base + kUpdThreads < d1) ? base + kUpdThreads : d1;
 #pragma unroll
@@ -429,6 +463,8 @@ __global__ void __launch_bounds__(kUpdThreads) spray_update_kernel(UpdateArgs<R,
     }
     __syncthreads();
   }
+
+  if (t < kTileCells) a.stay_cnt[tile * kTileCells + t] = srank;
 
   // add each cell's sums to the source FAB
 #pragma unroll

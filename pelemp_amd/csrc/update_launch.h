@@ -32,6 +32,10 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.slot_in = c.slot[c.cur];
   a.slot_out = c.slot[1 - c.cur];
   a.key_out = c.key;
+  a.arrival_out = c.arrival;
+  a.mov_lo = c.mov_lo;
+  a.mov_hi = c.mov_hi;
+  a.stay_cnt = c.stay_cnt;
   a.st = make_fab(*p.state);
   a.c_rho = c.comps.rhoIndx;
   a.c_vel = c.comps.momIndx;
