@@ -224,7 +224,7 @@ template <class M, int NF, class R>
 PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g, R rule, R T_in,
                             R C_eps, R mw_part, const R* Y_l, const R* h_part, const R* cp_n,
                             const R* cBoilT, R* Y_skin, R* X_vapor, R* L_fuel, R& B_M, R& sumXVap,
-                            R& cp_skin, R& mw_skin) {
+                            R& cp_skin, R& mw_skin, R* sumYVap_out = nullptr) {
   constexpr int NS = M::NS;
   R mbar_vap = R(0);
 #pragma unroll
@@ -271,6 +271,7 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
       sumYSkin += Ysk;
     }
   }
+  if (sumYVap_out) *sumYVap_out = sumYVap;
   const R renorm = (R(1) - sumYSkin) / (R(1) - sumYfFluid);
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
@@ -281,6 +282,168 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
   mw_skin = R(1) / mw_skin;
   B_M = (sumYVap - sumYfFluid) / rmax(C_eps, (R(1) - sumYVap));
   B_M = rmin(R(20), B_M);
+}
+
+// Rates of one substep of calculateSpraySource (Drag.H:204-347): drag,
+// evaporation and heat transfer of a droplet in the gas state g.  This is the
+// plain method's "rate evaluation" (the part the learned surrogate replaces).
+template <class M, int NF, class R>
+struct SubRates {
+  R Reyn;
+  R drag_force;    // 0.125 rho_s C_D pi d^2 |du|  (force / |du| ... per droplet)
+  R drag_force_p;  // drag acceleration per unit relative velocity [1/s]
+  R diff_vel[3];
+  R m_dot;       // total mass rate of the droplet (<= 0) [kg/s]
+  R mi_dot[NF];  // per liquid species
+  R sumL;        // sum_k mi_dot_k L_k [W]
+  R conv_src;    // convective heat into the droplet [W]
+  R part_temp_src;  // dT_d/dt [K/s]
+  R cp_part, Tboil, delT, inv_pmass;
+  R h_part[M::NS];  // gas enthalpies at T_d (used by the species energy term)
+  // transfer groups (the quantities the learned surrogate predicts):
+  R ShRhoD;  // Sh* x sum_k rho D_k            [kg/(m s)]   (mass transfer)
+  R lamNu;   // lambda_skin x Nu_eff           [W/(m K)]    (heat transfer)
+  R sumYVap;  // fuel vapour mass fraction at the surface
+  R B_M;      // Spalding mass transfer number
+};
+
+template <class M, int NF, class R>
+PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, R T_part, R dia_part,
+                          R pmass, R rho_part, const R* Y_part, const R* vel_part, R y_tab,
+                          const R* cBoilT, SubRates<M, NF, R>& o) {
+  constexpr int NS = M::NS;
+  const R rule = R(1.) / R(3.);
+  const R C_eps = R(1.E-15);
+  // The fixed-point tolerance of calcHeatCoeff (1e-7, Drag.H:155) is below
+  // single-precision resolution for B_T of order one; the single-precision
+  // form uses 1e-6 (an absolute tolerance, as in the reference).
+  const R B_eps = (sizeof(R) == 4) ? R(1.E-6) : R(1.E-7);
+  const bool mass_trans = fd.mass_trans;
+  R Y_skin[NS], cp_n[NS], Ddiag[NS];
+  R L_fuel[NF], X_vapor[NF];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) Ddiag[n] = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    L_fuel[spf] = R(0);
+    X_vapor[spf] = R(0);
+    o.mi_dot[spf] = R(0);
+  }
+  R cp_part = R(0), Tboil = R(0), mw_part = R(0);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    Tboil += Y_part[spf] * cBoilT[spf];
+    cp_part += Y_part[spf] * fd.cp[spf];
+    mw_part += Y_part[spf] / fd.mw_fuel[spf];
+  }
+  mw_part = R(1) / mw_part;
+  const R delT = rmax(g.T - T_part, R(0));
+  R T_skin = g.T;
+  if (mass_trans) T_skin = T_part + rule * delT;
+  GasProps<M>::T2Cpi(T_skin, cp_n);
+  GasProps<M>::T2Hi(T_part, o.h_part);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) Y_skin[n] = R(0);
+  R cp_skin = R(0), mw_skin = R(0), B_M = R(0), sumXVap = R(0);
+  o.sumYVap = R(0);
+  o.ShRhoD = R(0);
+  if (mass_trans) {
+    calc_vapor_state<M, NF, R>(fd, g, rule, T_part, C_eps, mw_part, Y_part, o.h_part, cp_n, cBoilT, Y_skin,
+                               X_vapor, L_fuel, B_M, sumXVap, cp_skin, mw_skin, &o.sumYVap);
+  } else {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      Y_skin[n] = g.Y[n];
+      cp_skin += g.Y[n] * cp_n[n];
+    }
+    mw_skin = g.mw_mix;
+  }
+  R lambda_skin = R(0), mu_skin = R(0);
+  R rho_skin = g.rho;
+  if (mass_trans) rho_skin = mw_skin * g.p / (static_cast<R>(kRU) * T_skin);
+  GasProps<M>::transport(true, mass_trans, mass_trans, T_skin, rho_skin, Y_skin, Ddiag, mu_skin, lambda_skin);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) o.diff_vel[q] = g.vel[q] - vel_part[q];
+  const R diff_vel_mag = rm::sqrt(o.diff_vel[0] * o.diff_vel[0] + o.diff_vel[1] * o.diff_vel[1] +
+                                  o.diff_vel[2] * o.diff_vel[2]);
+  const R Reyn = rho_skin * diff_vel_mag * dia_part / mu_skin;
+  const R inv_pmass = R(1) / pmass;
+  R drag_coef = R(0);
+  if (Reyn > R(1000.)) {
+    drag_coef = R(0.424);
+  } else if (Reyn > R(1.)) {
+    drag_coef = R(24.) / Reyn * (R(1.) + rm::cbrt(Reyn * Reyn) / R(6.));
+  } else if (Reyn > R(0.)) {
+    drag_coef = R(24.) / Reyn;
+  }
+  drag_coef *= (R(1.) + R(2.6432) * y_tab);
+  o.drag_force = R(0.125) * rho_skin * drag_coef * static_cast<R>(kPIc) * dia_part * dia_part * diff_vel_mag;
+  o.drag_force_p = R(0.75) * rho_skin * drag_coef * diff_vel_mag / (dia_part * rho_part);
+  R part_temp_src = R(0);
+  R m_dot = R(0);
+  R sumL = R(0);
+  R conv_src = R(0);
+  if (mass_trans) {
+    const R Pr_skin = mu_skin * cp_skin / lambda_skin;
+    const R powR = rmax(rm::pow(rmin(R(400.), Reyn), R(0.077)), R(1.));
+    R Nu_num = R(0);
+    const R Nu_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Pr_skin);
+    if (sumXVap > R(0)) {
+      R rhoDtotal = R(0), cp_fuel = R(0), sumYSkin = R(0);
+      R Dfuel[NF];
+#pragma unroll
+      for (int spf = 0; spf < NF; ++spf) {
+        const int fspec = fuel_species<NS, NF>(spf);
+        R dd = Ddiag[fspec];
+        dd *= mw_skin / fd.mw_fuel[spf];
+        const R X_norm = X_vapor[spf] / sumXVap;
+        dd *= X_norm;
+        Ddiag[fspec] = dd;
+        Dfuel[spf] = dd;
+        rhoDtotal += dd;
+        const R ysk = Y_skin[fspec];
+        cp_fuel += ysk * cp_n[fspec];
+        sumYSkin += ysk;
+      }
+      cp_fuel /= sumYSkin;
+      const R Sc_skin = mu_skin / rhoDtotal;
+      const R logB = rm::log1p(B_M);
+      const R invFM = B_M / (logB * rm::pow(R(1.) + B_M, R(0.7)));
+      const R Sh_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Sc_skin);
+      const R Sh_num = R(2.) + (Sh_0 - R(2.)) * invFM;
+      const R mdotcoeff = static_cast<R>(kPIc) * dia_part * Sh_num * logB;
+      const R ratio = cp_fuel * Sh_num * rhoDtotal / lambda_skin;
+      Nu_num = calc_heat_coeff<R>(ratio, B_M, B_eps, C_eps, Nu_0);
+      o.ShRhoD = Sh_num * rhoDtotal;
+#pragma unroll
+      for (int spf = 0; spf < NF; ++spf) {
+        if (X_vapor[spf] > R(0)) {
+          // Dfuel[spf] is Ddiag[indx[spf]] after the updates above (fuel
+          // species are distinct)
+          o.mi_dot[spf] = -rmax(Dfuel[spf] * mdotcoeff, R(0.));
+          m_dot += o.mi_dot[spf];
+          sumL += o.mi_dot[spf] * L_fuel[spf];
+        }
+      }
+    } else {
+      Nu_num = Nu_0;
+    }
+    conv_src = static_cast<R>(kPIc) * lambda_skin * dia_part * delT * Nu_num;
+    part_temp_src = (sumL + conv_src) * inv_pmass / cp_part;
+    o.lamNu = lambda_skin * Nu_num;
+  } else {
+    o.lamNu = R(0);
+  }
+  o.B_M = B_M;
+  o.Reyn = Reyn;
+  o.m_dot = m_dot;
+  o.sumL = sumL;
+  o.conv_src = conv_src;
+  o.part_temp_src = part_temp_src;
+  o.cp_part = cp_part;
+  o.Tboil = Tboil;
+  o.delT = delT;
+  o.inv_pmass = inv_pmass;
 }
 
 // calculateSpraySource (Drag.H:137-441), PELELM_USE_SPRAY flavour.
@@ -296,25 +459,12 @@ template <class M, int NF, class R, bool MASSB = false>
 PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
                      Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s) {
   constexpr int NS = M::NS;
-  const R rule = R(1.) / R(3.);
   const R C_eps = R(1.E-15);
-  // The fixed-point tolerance of calcHeatCoeff (1e-7, Drag.H:155) is below
-  // single-precision resolution for B_T of order one; the single-precision
-  // form uses 1e-6 (an absolute tolerance, as in the reference).
-  const R B_eps = (sizeof(R) == 4) ? R(1.E-6) : R(1.E-7);
   const R min_mass = R(kMinMass);
   const int nSubMax = 100;
   const bool mass_trans = fd.mass_trans;
-  R Y_skin[NS], h_part[NS], cp_n[NS], Ddiag[NS];
-  R L_fuel[NF], mi_dot[NF], Y_part[NF], X_vapor[NF];
-#pragma unroll
-  for (int n = 0; n < NS; ++n) Ddiag[n] = R(0);
-#pragma unroll
-  for (int spf = 0; spf < NF; ++spf) {
-    L_fuel[spf] = R(0);
-    mi_dot[spf] = R(0);
-    X_vapor[spf] = R(0);
-  }
+  R Y_part[NF];
+  SubRates<M, NF, R> o;
   s.mom[0] = s.mom[1] = s.mom[2] = R(0);
   s.mass = R(0);
   s.eng = R(0);
@@ -350,123 +500,30 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
   R Reyn = R(0);
   R part_vel_src[3];
   while (isub <= nsub) {
-    R cp_part = R(0), Tboil = R(0), mw_part = R(0);
+    substep_rates<M, NF, R>(g, fd, T_part, dia_part, pmass, rho_part, Y_part, vel_part, y_tab, cBoilT, o);
+    Reyn = o.Reyn;
+    const R Tboil = o.Tboil;
+    const R* mi_dot = o.mi_dot;
 #pragma unroll
-    for (int spf = 0; spf < NF; ++spf) {
-      Tboil += Y_part[spf] * cBoilT[spf];
-      cp_part += Y_part[spf] * fd.cp[spf];
-      mw_part += Y_part[spf] / fd.mw_fuel[spf];
-    }
-    mw_part = R(1) / mw_part;
-    const R delT = rmax(g.T - T_part, R(0));
-    R T_skin = g.T;
-    if (mass_trans) T_skin = T_part + rule * delT;
-    GasProps<M>::T2Cpi(T_skin, cp_n);
-    GasProps<M>::T2Hi(T_part, h_part);
-#pragma unroll
-    for (int n = 0; n < NS; ++n) Y_skin[n] = R(0);
-    R cp_skin = R(0), mw_skin = R(0), B_M = R(0), sumXVap = R(0);
-    if (mass_trans) {
-      calc_vapor_state<M, NF, R>(fd, g, rule, T_part, C_eps, mw_part, Y_part, h_part, cp_n, cBoilT,
-                                 Y_skin, X_vapor, L_fuel, B_M, sumXVap, cp_skin, mw_skin);
-    } else {
-#pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        Y_skin[n] = g.Y[n];
-        cp_skin += g.Y[n] * cp_n[n];
-      }
-      mw_skin = g.mw_mix;
-    }
-    R lambda_skin = R(0), mu_skin = R(0);
-    R rho_skin = g.rho;
-    if (mass_trans) rho_skin = mw_skin * g.p / (static_cast<R>(kRU) * T_skin);
-    GasProps<M>::transport(true, mass_trans, mass_trans, T_skin, rho_skin, Y_skin, Ddiag, mu_skin,
-                           lambda_skin);
-    R diff_vel[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) diff_vel[q] = g.vel[q] - vel_part[q];
-    const R diff_vel_mag =
-        rm::sqrt(diff_vel[0] * diff_vel[0] + diff_vel[1] * diff_vel[1] + diff_vel[2] * diff_vel[2]);
-    Reyn = rho_skin * diff_vel_mag * dia_part / mu_skin;
-    const R inv_pmass = R(1) / pmass;
-    R drag_coef = R(0);
-    if (Reyn > R(1000.)) {
-      drag_coef = R(0.424);
-    } else if (Reyn > R(1.)) {
-      drag_coef = R(24.) / Reyn * (R(1.) + rm::cbrt(Reyn * Reyn) / R(6.));
-    } else if (Reyn > R(0.)) {
-      drag_coef = R(24.) / Reyn;
-    }
-    drag_coef *= (R(1.) + R(2.6432) * y_tab);
-    const R drag_force =
-        R(0.125) * rho_skin * drag_coef * static_cast<R>(kPIc) * dia_part * dia_part * diff_vel_mag;
-    const R drag_force_p = R(0.75) * rho_skin * drag_coef * diff_vel_mag / (dia_part * rho_part);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) part_vel_src[q] = drag_force_p * diff_vel[q] + fd.body_force[q];
+    for (int q = 0; q < 3; ++q) part_vel_src[q] = o.drag_force_p * o.diff_vel[q] + fd.body_force[q];
     if (fd.mom_trans) {
-      const R fm = num_ppp * drag_force;
+      const R fm = num_ppp * o.drag_force;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) s.mom[q] += fm * diff_vel[q];
+      for (int q = 0; q < 3; ++q) s.mom[q] += fm * o.diff_vel[q];
       if (isub == 1) {
-        const R inv_tau_var = drag_force_p;
+        const R inv_tau_var = o.drag_force_p;
         const int c = sat_int(flow_dt * inv_tau_var) + 1;
         nsub = (c < nsub) ? nsub : c;
         nsub = (nSubMax < nsub) ? nSubMax : nsub;
       }
     }
-    R part_temp_src = R(0);
-    R m_dot = R(0);
+    const R part_temp_src = o.part_temp_src;
+    const R m_dot = o.m_dot;
     if (mass_trans) {
-      const R Pr_skin = mu_skin * cp_skin / lambda_skin;
-      const R powR = rmax(rm::pow(rmin(R(400.), Reyn), R(0.077)), R(1.));
-      R Nu_num = R(0);
-      R sumL = R(0);
-      const R Nu_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Pr_skin);
-      if (sumXVap > R(0)) {
-        R rhoDtotal = R(0), cp_fuel = R(0), sumYSkin = R(0);
-        R Dfuel[NF];
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) {
-          const int fspec = fuel_species<NS, NF>(spf);
-          R dd = Ddiag[fspec];
-          dd *= mw_skin / fd.mw_fuel[spf];
-          const R X_norm = X_vapor[spf] / sumXVap;
-          dd *= X_norm;
-          Ddiag[fspec] = dd;
-          Dfuel[spf] = dd;
-          rhoDtotal += dd;
-          const R ysk = Y_skin[fspec];
-          cp_fuel += ysk * cp_n[fspec];
-          sumYSkin += ysk;
-        }
-        cp_fuel /= sumYSkin;
-        const R Sc_skin = mu_skin / rhoDtotal;
-        const R logB = rm::log1p(B_M);
-        const R invFM = B_M / (logB * rm::pow(R(1.) + B_M, R(0.7)));
-        const R Sh_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Sc_skin);
-        const R Sh_num = R(2.) + (Sh_0 - R(2.)) * invFM;
-        const R mdotcoeff = static_cast<R>(kPIc) * dia_part * Sh_num * logB;
-        const R ratio = cp_fuel * Sh_num * rhoDtotal / lambda_skin;
-        Nu_num = calc_heat_coeff<R>(ratio, B_M, B_eps, C_eps, Nu_0);
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) {
-          if (X_vapor[spf] > R(0)) {
-            // Dfuel[spf] is Ddiag[indx[spf]] after the updates above (fuel
-            // species are distinct)
-            mi_dot[spf] = -rmax(Dfuel[spf] * mdotcoeff, R(0.));
-            m_dot += mi_dot[spf];
-            sumL += mi_dot[spf] * L_fuel[spf];
-          }
-        }
-      } else {
-        Nu_num = Nu_0;
-      }
-      const R conv_src = static_cast<R>(kPIc) * lambda_skin * dia_part * delT * Nu_num;
-      s.eng += num_ppp * conv_src;
-      part_temp_src = (sumL + conv_src) * inv_pmass / cp_part;
-      if (isub == 1 && delT > C_eps) {
+      s.eng += num_ppp * o.conv_src;
+      if (isub == 1 && o.delT > C_eps) {
         const R inv_tau_d = -m_dot / (R(0.2) * pmass);
-        const R inv_tau_T = conv_src * inv_pmass / (cp_part * delT);
+        const R inv_tau_T = o.conv_src * o.inv_pmass / (o.cp_part * o.delT);
         const int a = sat_int(flow_dt * inv_tau_d);
         const int b = sat_int(flow_dt * inv_tau_T);
         const int c = ((a < b) ? b : a) + 1;
@@ -543,7 +600,7 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
       const int fdspec = fuel_species<NS, NF>(spf);
       const R midot = d.dmk[spf] * inv_dt;
       s.Ydot[spf] = num_ppp * midot;
-      s.eng += num_ppp * midot * h_part[fdspec];
+      s.eng += num_ppp * midot * o.h_part[fdspec];
       d.Y[spf] = Y_part[spf];
     }
 #pragma unroll
@@ -565,7 +622,7 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
     const R newY = Y_part[spf];
     const R midot = (newY * pmass - oldY * startmass) / (fd.dtmod * flow_dt);
     s.Ydot[spf] = num_ppp * midot;
-    s.eng += num_ppp * midot * h_part[fdspec];
+    s.eng += num_ppp * midot * o.h_part[fdspec];
     d.Y[spf] = Y_part[spf];
   }
 #pragma unroll
