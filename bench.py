@@ -41,12 +41,20 @@ def algorithmic_bytes(nf: int, ns: int, ppc: float, w: int) -> float:
     return (17 + 2 * nf) * w + (5 + ns + 5 + nf) * 8.0 / ppc
 
 
-def run_form(prob, form: int, steps: int, warmup: int, stream):
+def run_form(prob, form: int, steps: int, warmup: int, stream, acc_steps: int = 0):
+    import ctypes
+
     from pelemp_amd import _capi as C
     from pelemp_amd.spray import DeviceFab, SprayParticleContainer
 
     cap = int(prob.n * 1.02) + 1024
     pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, cap, form=form)
+    net = None
+    if form == C.FORM_SURROGATE:
+        from pelemp_amd import surrogate as S
+
+        net = S.Surrogate.load(S.default_path(prob.mech))
+        net.install(pc)
     pc.set_particles(prob.pos, prob.rdata, prob.ids)
     state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
     src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
@@ -81,6 +89,22 @@ def run_form(prob, form: int, steps: int, warmup: int, stream):
     res = dict(droplet_steps=droplet_steps, ms_total=ms_total, wall_s=wall, counts=counts,
                update_ms=kms[0] / max(kn[0], 1), update_launches=kn[0], sort_ms=kms[1] / max(kn[1], 1),
                rate=droplet_steps / (ms_total * 1e-3), n_end=pc.num_particles())
+    if net is not None:
+        ns, nfb = ctypes.c_int64(), ctypes.c_int64()
+        C.check(C.lib().pelemp_spray_surrogate_counts(pc._h, ctypes.byref(ns), ctypes.byref(nfb)), "counts")
+        res["surrogate_share"] = ns.value / max(1, ns.value + nfb.value)
+    if acc_steps > 0:
+        # untimed accuracy pass from the initial droplets: evaporated mass =
+        # sum over steps of the mass source (accumulated, += semantics) * V * dt
+        pc.set_particles(prob.pos, prob.rdata, prob.ids)
+        src.setVal(0.0)
+        for _ in range(acc_steps):
+            pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
+                               True, None, -1.0, stream=h, sync=False)
+        pc.sync(h)
+        V = float(np.prod(prob.geom["dx"]))
+        res["evaporated_mass"] = float(src.numpy()[prob.comps["rhoSrcIndx"]].sum()) * V * prob.flow_dt
+        res["acc_steps"] = acc_steps
     pc.close()
     return res
 
@@ -157,7 +181,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--forms", default="fp64,fp32,surrogate")
+    ap.add_argument("--acc-steps", type=int, default=10,
+                    help="untimed steps per form for the evaporated-mass comparison (0: skip)")
     ap.add_argument("--ncell", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -184,8 +210,8 @@ def main(argv=None):
     stream = C.Stream()
     forms = {}
     for name in args.forms.split(","):
-        form = {"fp64": C.FORM_FP64, "fp32": C.FORM_FP32}[name]
-        r = run_form(prob, form, args.steps, args.warmup, stream)
+        form = {"fp64": C.FORM_FP64, "fp32": C.FORM_FP32, "surrogate": C.FORM_SURROGATE}[name]
+        r = run_form(prob, form, args.steps, args.warmup, stream, args.acc_steps)
         w = 8 if form == C.FORM_FP64 else 4
         b = algorithmic_bytes(prob.nf, ns, ppc, w)
         droplets_per_launch = r["droplet_steps"] / max(r["update_launches"], 1)
@@ -194,9 +220,19 @@ def main(argv=None):
                            sort_ms=r["sort_ms"], bytes_per_droplet_step=b, achieved_GBs=achieved,
                            hbm_frac=achieved / HBM_PEAK_GBS, droplets_end=r["n_end"],
                            droplet_steps=r["droplet_steps"], traffic=measured_traffic(name))
+        for k in ("surrogate_share", "evaporated_mass", "acc_steps"):
+            if k in r:
+                forms[name][k] = r[k]
         print(f"# {name}: {r['rate']:.4g} droplet-steps/s, {r['ms_total'] / args.steps:.3f} ms/step, "
               f"update kernel {r['update_ms']:.3f} ms, sort {r['sort_ms']:.3f} ms, {achieved:.0f} GB/s",
               file=sys.stderr, flush=True)
+    if "fp64" in forms and "evaporated_mass" in forms["fp64"]:
+        ref = forms["fp64"]["evaporated_mass"]
+        for name, f in forms.items():
+            if "evaporated_mass" in f:
+                f["evaporated_mass_rel_err_vs_fp64"] = abs(f["evaporated_mass"] - ref) / abs(ref)
+        for name, f in forms.items():
+            f["speedup_vs_fp64"] = f["value"] / forms["fp64"]["value"]
     main_form = "fp64" if "fp64" in forms else next(iter(forms))
     mf = forms[main_form]
     out = {

@@ -188,6 +188,116 @@ int pelemp_spray_get_timing(pelemp_spray_ctx* ctx, double* ms, int64_t* launches
 /* Bytes of device memory held by the context. */
 int pelemp_spray_device_bytes(const pelemp_spray_ctx* ctx, int64_t* bytes);
 
+/* ---- Learned surrogate of the substep rate evaluation -------------------
+ * No counterpart in the reference: the surrogate replaces the film-property
+ * / Spalding-number evaluation inside the substep of calculateSpraySource
+ * (Drag.H:213-347) for PELEMP_FORM_SURROGATE.  Network: 8 inputs -> 4 hidden
+ * layers of 64 (SiLU) -> 3 outputs; FP32 master parameters, 13251 floats,
+ * per linear layer L = 0..4 the weights W_L[out][in] (row-major) followed by
+ * the biases b_L[out]; fan-in 8,64,64,64,64, fan-out 64,64,64,64,3.
+ *
+ * Inputs (before normalisation): ln d [m], T_d [K], ln(|u_gas - u_d| +
+ * 0.01 m/s), T_gas [K], ln p [Pa], fuel mass fraction of the film gas (1/3
+ * rule between the surface vapour and the gas), liquid mass fractions Y_l0,
+ * Y_l1.  Outputs (after de-normalisation):
+ *   z0 = ln(sum_k rho D_k)   film diffusivity of the vapour  -> dm/dt
+ *   z1 = ln(lambda Nu*)      heat-transfer group incl. the B_T fixed point
+ *                            -> Q = pi d (T_gas - T_d) e^z1, dT_d/dt
+ *   z2 = ln(mu_skin)         film viscosity -> Re, drag correction C_D Re/24
+ * The surface vapour state, B_M, latent heats, film density, Re, Sc, the
+ * piecewise drag coefficient and the Sherwood number stay exact closed
+ * forms, so dm/dt = -pi d Sh*(Re, Sc, B_M) e^z0 ln(1 + B_M). */
+#define PELEMP_SURR_NPARAM 13251
+#define PELEMP_SURR_NIN 8
+#define PELEMP_SURR_NOUT 3
+
+typedef struct pelemp_surrogate_norm {
+  float in_center[8];
+  float in_scale[8];   /* x_n = (x - in_center) * in_scale */
+  float out_mean[3];
+  float out_std[3];    /* z = out_mean + out_std * y_n */
+  /* training hull; droplets outside it take the plain FP32 path */
+  float d_min, d_max;         /* m */
+  float Tg_min, Tg_max;       /* K */
+  float p_min, p_max;         /* Pa */
+  float urel_max;             /* m/s */
+  float Yfg_max;              /* fuel vapour mass fraction of the gas */
+  float Td_min;               /* K */
+  float Td_frac_max;          /* T_d <= Td_frac_max * sum_k Y_l,k T_boil,k */
+  float BM_min;               /* Spalding number floor */
+  float indicator_max;        /* error indicator: flow_dt / tau_{drag,mass,T}
+                                 of the predicted rates must stay below it */
+} pelemp_surrogate_norm;
+
+/* Training samples generated by the plain FP64 rate evaluation (the same
+ * device code as the FP64 update) at n counter-based random points of the
+ * hull: x[n*8] raw (un-normalised) inputs, t[n*3] targets z0..z2, w[n]
+ * 1 for a usable sample, 0 otherwise (surface vapour below the gas, B_M
+ * under the floor).  Device pointers; asynchronous on `stream`. */
+int pelemp_surrogate_sample(int32_t mech, const pelemp_spray_data* sd,
+                            int32_t apply_latent_shift,
+                            const pelemp_surrogate_norm* hull, int64_t n,
+                            uint64_t seed, float* x, float* t, float* w,
+                            void* stream);
+
+/* FP32 forward pass of the network on n raw inputs (normalised inside with
+ * `norm`): y[n*3] normalised outputs.  quant != 0 rounds the weights and each
+ * layer's input to bfloat16 (round to nearest even), as the MFMA inference
+ * does.  Device pointers. */
+int pelemp_mlp_forward(const float* params, const pelemp_surrogate_norm* norm,
+                       int32_t quant, int64_t n, const float* x, float* y,
+                       void* stream);
+
+/* Loss L = (1/n) sum_i w_i |y_i - (t_i - out_mean)/out_std|^2 over one batch
+ * of n <= 65536 samples and its gradient with respect to the 13251
+ * parameters (straight-through for the bfloat16 rounding when quant != 0).
+ * grad: device, loss: host.  Waits for the device. */
+int pelemp_mlp_loss_grad(const float* params, const pelemp_surrogate_norm* norm,
+                         int32_t quant, int64_t n, const float* x,
+                         const float* t, const float* w, float* grad,
+                         double* loss, void* stream);
+
+typedef struct pelemp_mlp_train_cfg {
+  int32_t epochs;
+  int32_t batch;        /* <= 65536 */
+  int32_t quant;        /* train through the bfloat16 rounding */
+  int32_t verbose;      /* print the loss of every epoch to stderr */
+  double lr;            /* Adam step size, cosine-decayed to lr_final */
+  double lr_final;
+  double beta1, beta2, eps;
+  uint64_t seed;        /* batch composition per epoch */
+} pelemp_mlp_train_cfg;
+
+/* Adam training of `params` (device, in/out) on n samples, with the
+ * project's forward / backward / gradient-reduction kernels.  epoch_loss
+ * (host, cfg->epochs entries) receives the mean training loss per epoch. */
+int pelemp_mlp_train(float* params, const pelemp_surrogate_norm* norm,
+                     int64_t n, const float* x, const float* t, const float* w,
+                     const pelemp_mlp_train_cfg* cfg, double* epoch_loss,
+                     void* stream);
+
+/* Inference with the bfloat16 MFMA kernel (the code the surrogate update
+ * runs): params on the host, x/y on the device as in pelemp_mlp_forward. */
+int pelemp_mlp_infer_bf16(const float* params_host,
+                          const pelemp_surrogate_norm* norm, int64_t n,
+                          const float* x, float* y, void* stream);
+
+/* Install a trained network (host parameters) in a context created with
+ * PELEMP_FORM_SURROGATE. */
+int pelemp_spray_set_surrogate(pelemp_spray_ctx* ctx, const float* params_host,
+                               const pelemp_surrogate_norm* norm);
+
+/* Droplet-steps taken by the surrogate and by the plain fallback since the
+ * previous call (waits for the device). */
+int pelemp_spray_surrogate_counts(pelemp_spray_ctx* ctx, int64_t* n_surrogate,
+                                  int64_t* n_fallback);
+
+/* Self-test hook: D = A (32x16) * B (16x32) with one bfloat16 MFMA
+ * instruction; A, B row-major bfloat16 bit patterns, D row-major float.
+ * Shows the operand layout the surrogate kernel assumes. */
+int pelemp_mfma_probe(const uint16_t* a_host, const uint16_t* b_host,
+                      float* d_host);
+
 /* Self-test hook: evaluate the library's FP64 elementary functions on the
  * device (fn: 0 exp, 1 log, 2 log1p, 3 pow, 4 cbrt, 5 sin, 6 cos, 7 atan2,
  * 8 acos, 9 sqrt, 10 div) over n host values; used to show the device and

@@ -77,7 +77,32 @@ class GeomC(ctypes.Structure):
     ]
 
 
+SURR_NPARAM = 13251
+SURR_NIN = 8
+SURR_NOUT = 3
+_c_flt = ctypes.c_float
+
+
+class SurrogateNormC(ctypes.Structure):
+    _fields_ = [
+        ("in_center", _c_flt * SURR_NIN), ("in_scale", _c_flt * SURR_NIN),
+        ("out_mean", _c_flt * SURR_NOUT), ("out_std", _c_flt * SURR_NOUT),
+        ("d_min", _c_flt), ("d_max", _c_flt), ("Tg_min", _c_flt), ("Tg_max", _c_flt),
+        ("p_min", _c_flt), ("p_max", _c_flt), ("urel_max", _c_flt), ("Yfg_max", _c_flt),
+        ("Td_min", _c_flt), ("Td_frac_max", _c_flt), ("BM_min", _c_flt), ("indicator_max", _c_flt),
+    ]
+
+
+class MlpTrainCfgC(ctypes.Structure):
+    _fields_ = [
+        ("epochs", _c_i32), ("batch", _c_i32), ("quant", _c_i32), ("verbose", _c_i32),
+        ("lr", _c_dbl), ("lr_final", _c_dbl), ("beta1", _c_dbl), ("beta2", _c_dbl), ("eps", _c_dbl),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 _vp = ctypes.c_void_p
+_PNORM = ctypes.POINTER(SurrogateNormC)
 _PROTOS = {
     "pelemp_spray_create": (_c_i32, [ctypes.POINTER(_vp), _c_i32, _c_i32, ctypes.POINTER(SprayDataC),
                                      ctypes.POINTER(SprayCompsC), ctypes.POINTER(GeomC), _c_i64, _c_i32]),
@@ -96,6 +121,16 @@ _PROTOS = {
     "pelemp_spray_set_timing": (_c_i32, [_vp, _c_i32]),
     "pelemp_spray_get_timing": (_c_i32, [_vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_i64), _c_i32]),
     "pelemp_spray_device_bytes": (_c_i32, [_vp, ctypes.POINTER(_c_i64)]),
+    "pelemp_surrogate_sample": (_c_i32, [_c_i32, ctypes.POINTER(SprayDataC), _c_i32, _PNORM, _c_i64,
+                                         ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "pelemp_mlp_forward": (_c_i32, [_vp, _PNORM, _c_i32, _c_i64, _vp, _vp, _vp]),
+    "pelemp_mlp_loss_grad": (_c_i32, [_vp, _PNORM, _c_i32, _c_i64, _vp, _vp, _vp, _vp,
+                                      ctypes.POINTER(_c_dbl), _vp]),
+    "pelemp_mlp_train": (_c_i32, [_vp, _PNORM, _c_i64, _vp, _vp, _vp, ctypes.POINTER(MlpTrainCfgC), _vp, _vp]),
+    "pelemp_mlp_infer_bf16": (_c_i32, [_vp, _PNORM, _c_i64, _vp, _vp, _vp]),
+    "pelemp_spray_set_surrogate": (_c_i32, [_vp, _vp, _PNORM]),
+    "pelemp_spray_surrogate_counts": (_c_i32, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
+    "pelemp_mfma_probe": (_c_i32, [_vp, _vp, _vp]),
     "pelemp_math_eval": (_c_i32, [_c_i32, _c_i64, _vp, _vp, _vp]),
     "pelemp_device_count": (_c_i32, [ctypes.POINTER(_c_i32)]),
     "pelemp_device_malloc": (_c_i32, [ctypes.POINTER(_vp), _c_i64]),

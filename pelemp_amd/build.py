@@ -32,6 +32,7 @@ UNITS = {
     "spray_host.hip": ["-ffp-contract=off"],
     "spray_fp64.hip": ["-ffp-contract=off"],
     "spray_fp32.hip": ["-ffp-contract=fast"],
+    "spray_surrogate.hip": ["-ffp-contract=off"],
 }
 
 

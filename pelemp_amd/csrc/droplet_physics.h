@@ -300,9 +300,11 @@ struct SubRates {
   R part_temp_src;  // dT_d/dt [K/s]
   R cp_part, Tboil, delT, inv_pmass;
   R h_part[M::NS];  // gas enthalpies at T_d (used by the species energy term)
-  // transfer groups (the quantities the learned surrogate predicts):
+  // film properties and transfer groups (what the learned surrogate sees):
   R ShRhoD;  // Sh* x sum_k rho D_k            [kg/(m s)]   (mass transfer)
   R lamNu;   // lambda_skin x Nu_eff           [W/(m K)]    (heat transfer)
+  R rhoD;    // sum_k rho D_k of the vapour    [kg/(m s)]
+  R mu_skin;  // film viscosity                [Pa s]
   R sumYVap;  // fuel vapour mass fraction at the surface
   R B_M;      // Spalding mass transfer number
 };
@@ -347,6 +349,7 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
   R cp_skin = R(0), mw_skin = R(0), B_M = R(0), sumXVap = R(0);
   o.sumYVap = R(0);
   o.ShRhoD = R(0);
+  o.rhoD = R(0);
   if (mass_trans) {
     calc_vapor_state<M, NF, R>(fd, g, rule, T_part, C_eps, mw_part, Y_part, o.h_part, cp_n, cBoilT, Y_skin,
                                X_vapor, L_fuel, B_M, sumXVap, cp_skin, mw_skin, &o.sumYVap);
@@ -415,6 +418,7 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
       const R ratio = cp_fuel * Sh_num * rhoDtotal / lambda_skin;
       Nu_num = calc_heat_coeff<R>(ratio, B_M, B_eps, C_eps, Nu_0);
       o.ShRhoD = Sh_num * rhoDtotal;
+      o.rhoD = rhoDtotal;
 #pragma unroll
       for (int spf = 0; spf < NF; ++spf) {
         if (X_vapor[spf] > R(0)) {
@@ -435,6 +439,7 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
     o.lamNu = R(0);
   }
   o.B_M = B_M;
+  o.mu_skin = mu_skin;
   o.Reyn = Reyn;
   o.m_dot = m_dot;
   o.sumL = sumL;
@@ -455,9 +460,13 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
 // returned in d.dmk, and the mass/species sources are built from those sums
 // instead of from (pmass - startmass); in exact arithmetic the two agree for
 // the explicit Euler update of Drag.H:371-400.
+//
+// pre (surrogate form): rates of the first substep predicted by the learned
+// surrogate; when given they replace substep_rates for that substep.
 template <class M, int NF, class R, bool MASSB = false>
 PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
-                     Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s) {
+                     Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s,
+                     const SubRates<M, NF, R>* pre = nullptr) {
   constexpr int NS = M::NS;
   const R C_eps = R(1.E-15);
   const R min_mass = R(kMinMass);
@@ -500,7 +509,11 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
   R Reyn = R(0);
   R part_vel_src[3];
   while (isub <= nsub) {
-    substep_rates<M, NF, R>(g, fd, T_part, dia_part, pmass, rho_part, Y_part, vel_part, y_tab, cBoilT, o);
+    if (pre != nullptr && isub == 1) {
+      o = *pre;
+    } else {
+      substep_rates<M, NF, R>(g, fd, T_part, dia_part, pmass, rho_part, Y_part, vel_part, y_tab, cBoilT, o);
+    }
     Reyn = o.Reyn;
     const R Tboil = o.Tboil;
     const R* mi_dot = o.mi_dot;

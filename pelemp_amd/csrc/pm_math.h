@@ -310,4 +310,15 @@ PM_HD double acos(double x) {
   return 2.0 * pm::atan(std::sqrt(1.0 - x) / std::sqrt(1.0 + x));
 }
 
+// asin(x) = atan( x / sqrt((1-x)(1+x)) ), exact +-pi/2 at |x| = 1
+PM_HD double asin(double x) {
+  if (!(x >= -1.0 && x <= 1.0)) return __builtin_nan("");
+  if (x == 1.0) return kPiHalf;
+  if (x == -1.0) return -kPiHalf;
+  return pm::atan(x / std::sqrt((1.0 - x) * (1.0 + x)));
+}
+
+// tan(x) = sin(x) / cos(x)
+PM_HD double tan(double x) { return pm::sin(x) / pm::cos(x); }
+
 }  // namespace pm

@@ -32,6 +32,12 @@ struct Ctx;
 // 1-ctx.cur and ctx.key.
 int launch_update_fp64(Ctx& c, const PassParams& p, hipStream_t s);
 int launch_update_fp32(Ctx& c, const PassParams& p, hipStream_t s);
+int launch_update_surr(Ctx& c, const PassParams& p, hipStream_t s);
+
+// Mechanism tables (spray_host.hip).
+int mech_ns(int mech);
+void mech_mw(int mech, std::vector<double>& mw);
+void mech_h(int mech, double T, double* h);
 
 struct Ctx {
   int mech = 0;
@@ -81,6 +87,13 @@ struct Ctx {
   static constexpr int kCountLog = 4096;
   int32_t* count_log = nullptr;
   int64_t npass = 0;
+
+  // surrogate form: packed network (SurrWeights), its normalisation, and the
+  // droplet-step counters [surrogate, fallback]
+  void* surr_w = nullptr;
+  bool has_surr = false;
+  pelemp_surrogate_norm surr_norm{};
+  unsigned long long* surr_cnt = nullptr;
 
   // timing
   bool timing = false;

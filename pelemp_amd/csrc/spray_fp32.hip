@@ -8,4 +8,7 @@ namespace pelemp {
 int launch_update_fp32(Ctx& c, const PassParams& p, hipStream_t s) {
   return dispatch_update<float, false>(c, p, s);
 }
+int launch_update_surr(Ctx& c, const PassParams& p, hipStream_t s) {
+  return dispatch_update<float, false, true>(c, p, s);
+}
 }  // namespace pelemp

@@ -25,7 +25,7 @@ struct pelemp_spray_ctx {
 
 namespace pelemp {
 
-static thread_local std::string g_last_error;
+thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 #define PM_HIP(expr)                                                           \
@@ -401,7 +401,7 @@ static void free_ctx(Ctx& c) {
   void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.mbuf[0], c.mbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt,
                   c.arrival, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
-                  c.count_log};
+                  c.count_log, c.surr_w, c.surr_cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -411,7 +411,7 @@ static void free_ctx(Ctx& c) {
   for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
 }
 
-static int mech_ns(int mech) {
+int mech_ns(int mech) {
   switch (mech) {
     case PELEMP_MECH_DODECANE_AIR: return Mech_dodecane_air::NS;
     case PELEMP_MECH_DECANE_DODECANE_AIR: return Mech_decane_dodecane_air::NS;
@@ -420,7 +420,7 @@ static int mech_ns(int mech) {
   }
 }
 
-static void mech_mw(int mech, std::vector<double>& mw) {
+void mech_mw(int mech, std::vector<double>& mw) {
   mw.clear();
   auto fill = [&](auto tag) {
     using M = decltype(tag);
@@ -434,7 +434,7 @@ static void mech_mw(int mech, std::vector<double>& mw) {
   }
 }
 
-static void mech_h(int mech, double T, double* h) {
+void mech_h(int mech, double T, double* h) {
   auto run = [&](auto tag) {
     using M = decltype(tag);
     GasProps<M>::T2Hi(T, h);
@@ -468,9 +468,15 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
     set_last_error("unknown mechanism");
     return PELEMP_E_ARG;
   }
-  if (form != PELEMP_FORM_FP64 && form != PELEMP_FORM_FP32) {
+  if (form != PELEMP_FORM_FP64 && form != PELEMP_FORM_FP32 && form != PELEMP_FORM_SURROGATE) {
     set_last_error("unsupported form");
     return PELEMP_E_ARG;
+  }
+  if (form == PELEMP_FORM_SURROGATE &&
+      !((mech == PELEMP_MECH_DODECANE_AIR && sd->nfuel == 1) ||
+        (mech == PELEMP_MECH_DECANE_DODECANE_AIR && sd->nfuel == 2))) {
+    set_last_error("the surrogate form supports mechanisms 0 (one fuel) and 1 (two fuels)");
+    return PELEMP_E_BAD_CONFIG;
   }
   if (sd->nfuel < 1 || sd->nfuel > PELEMP_MAX_FUEL) {
     set_last_error("nfuel out of range");
@@ -555,6 +561,10 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm1), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm2), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.count_log), 4 * Ctx::kCountLog);
+  if (form == PELEMP_FORM_SURROGATE) {
+    st |= alloc(c, &c.surr_w, sizeof(SurrWeights));
+    st |= alloc(c, reinterpret_cast<void**>(&c.surr_cnt), 2 * sizeof(unsigned long long));
+  }
   if (st != PELEMP_OK) {
     free_ctx(c);
     delete h;
@@ -565,7 +575,9 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.n_entries = c.scalars;
   c.total = c.scalars + 1;
   c.err = c.scalars + 2;
-  if (hipMemset(c.scalars, 0, 64) != hipSuccess || hipMemset(c.cstart, 0, 4 * (static_cast<size_t>(c.ncells) + 1)) != hipSuccess) {
+  if (c.surr_cnt && hipMemset(c.surr_cnt, 0, 2 * sizeof(unsigned long long)) != hipSuccess) st = PELEMP_E_HIP;
+  if (st != PELEMP_OK || hipMemset(c.scalars, 0, 64) != hipSuccess ||
+      hipMemset(c.cstart, 0, 4 * (static_cast<size_t>(c.ncells) + 1)) != hipSuccess) {
     free_ctx(c);
     delete h;
     set_last_error("hipMemset failed");
@@ -759,6 +771,10 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
     set_last_error("flow_dt must be positive");
     return PELEMP_E_ARG;
   }
+  if (c.form == PELEMP_FORM_SURROGATE && !c.has_surr) {
+    set_last_error("surrogate form: no network installed (pelemp_spray_set_surrogate)");
+    return PELEMP_E_BAD_CONFIG;
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   // particle sub-cycles (SprayParticles.cpp:221-229)
   const double sub_cfl = 0.5;
@@ -790,7 +806,9 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
     PM_HIP(hipMemsetAsync(c.mov_lo, 0, 3 * 4 * static_cast<size_t>(c.ncells), s));
     hipEvent_t ev0;
     ctx_time_begin(c, 0, s, &ev0);
-    const int st = (c.form == PELEMP_FORM_FP64) ? launch_update_fp64(c, p, s) : launch_update_fp32(c, p, s);
+    const int st = (c.form == PELEMP_FORM_FP64)   ? launch_update_fp64(c, p, s)
+                   : (c.form == PELEMP_FORM_FP32) ? launch_update_fp32(c, p, s)
+                                                  : launch_update_surr(c, p, s);
     if (st != PELEMP_OK) return st;
     ctx_time_end(c, 0, s, ev0);
     c.cur = 1 - c.cur;
@@ -805,6 +823,50 @@ int pelemp_spray_move_kick_drift(pelemp_spray_ctx* h, const pelemp_fab_view* sta
                                  int32_t source_ghosts, int32_t do_move, double spray_cfl_lev, void* stream) {
   return pelemp_spray_update(h, state, source, dt, time, state_ghosts, source_ghosts, isVirt, isGhost, do_move,
                              spray_cfl_lev, stream);
+}
+
+int pelemp_spray_set_surrogate(pelemp_spray_ctx* h, const float* params_host, const pelemp_surrogate_norm* norm) {
+  if (!h || !params_host || !norm) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  Ctx& c = h->c;
+  if (c.form != PELEMP_FORM_SURROGATE) {
+    set_last_error("the context was not created with PELEMP_FORM_SURROGATE");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  for (int q = 0; q < kSurrOut; ++q)
+    if (!(norm->out_std[q] > 0.f)) {
+      set_last_error("surrogate norm: out_std must be positive");
+      return PELEMP_E_ARG;
+    }
+  std::vector<SurrWeights> w(1);
+  surr_pack(params_host, w[0]);
+  PM_HIP(hipDeviceSynchronize());  // no update may be reading the old network
+  PM_HIP(hipMemcpy(c.surr_w, w.data(), sizeof(SurrWeights), hipMemcpyHostToDevice));
+  c.surr_norm = *norm;
+  c.has_surr = true;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_surrogate_counts(pelemp_spray_ctx* h, int64_t* n_surrogate, int64_t* n_fallback) {
+  if (!h || !n_surrogate || !n_fallback) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  Ctx& c = h->c;
+  if (c.form != PELEMP_FORM_SURROGATE) {
+    *n_surrogate = 0;
+    *n_fallback = 0;
+    return PELEMP_OK;
+  }
+  PM_HIP(hipDeviceSynchronize());
+  unsigned long long v[2] = {0, 0};
+  PM_HIP(hipMemcpy(v, c.surr_cnt, sizeof(v), hipMemcpyDeviceToHost));
+  PM_HIP(hipMemset(c.surr_cnt, 0, sizeof(v)));
+  *n_surrogate = static_cast<int64_t>(v[0]);
+  *n_fallback = static_cast<int64_t>(v[1]);
+  return PELEMP_OK;
 }
 
 // Waits for the stream and returns the sticky device error (0 if none).

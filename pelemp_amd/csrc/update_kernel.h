@@ -22,12 +22,21 @@
 //      no atomics).
 // Workgroups are mapped XCD-contiguously: consecutive tiles (which share
 // halo cells) run on the same XCD and hit the same L2.
+//
+// Surrogate form: the first substep's rates come from the learned network
+// (surrogate.h, surrogate_mlp.h) for droplets inside its hull and below its
+// error indicator; every wavefront evaluates the network for its 64
+// droplets with MFMA between the gather and the update, and the other
+// droplets take the plain FP32 substep.  The packed network is staged in
+// LDS once per workgroup, and the workgroups loop over tiles (grid sized to
+// the machine) so the staging is amortised.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "droplet_physics.h"
 #include "layout.h"
+#include "surrogate_mlp.h"
 
 namespace pelemp {
 
@@ -43,6 +52,11 @@ struct FabDev {
   int lo[3];
   int n[3];
   long long cstride;  // cells per component
+};
+
+// Normalisation of the surrogate inputs / outputs as used in the kernel.
+struct SurrKNorm {
+  float c[kSurrIn], s[kSurrIn], mean[kSurrOut], sd[kSurrOut];
 };
 
 template <class R, int NF>
@@ -79,6 +93,11 @@ struct UpdateArgs {
   int do_move;
   int is_ghost;
   int* err;
+  // surrogate form
+  const SurrWeights* __restrict__ surr_w;
+  SurrKNorm sn;
+  pelemp_surrogate_norm hull;
+  unsigned long long* surr_cnt;  // [surrogate, fallback] droplet-steps
 };
 
 __device__ inline int xcd_tile(int b, int ntiles) {
@@ -115,31 +134,378 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
   return false;
 }
 
-// Occupancy target: the FP64 form needs ~265 registers per lane at full
-// freedom (1 wave/SIMD); it is compiled for 2 workgroups per CU (2 waves per
-// SIMD, 256 registers) at the cost of a few spilled values.
-template <class R>
+// Occupancy targets (workgroups of 4 waves per CU = waves per SIMD): the
+// FP64 form needs ~265 registers per lane at full freedom; it is compiled
+// for 2 waves per SIMD (256 registers) at the cost of a few spilled values.
+// The FP32 form with up to two liquid species fits 3 waves per SIMD (168
+// registers) without spills; with more species it keeps 1.
+template <class R, int NF>
 struct UpdOccupancy {
-  static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : 1;
+  static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : (NF <= 2 ? 3 : 1);
 };
 
+// Per-droplet values carried from the gather to the update.
+template <class M, int NF, class R>
+struct DropLocal {
+  int32_t o;  // storage position in the input buffer
+  int lc, cx, cy, cz;
+  Drop<R, NF> d;
+  R pos[3];
+  R film;
+  double mk[NF];  // FP32 form: liquid species masses
+  GasAt<R, M::NS> g;
+  R cBoilT[NF];
+  int bflags[3];
+};
+
+// Load droplet s of the tile and gather its gas state.
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
-__global__ void __launch_bounds__(kUpdThreads, UpdOccupancy<R>::kMinBlocks)
-    spray_update_kernel(UpdateArgs<R, NF> a) {
+__device__ __attribute__((always_inline)) inline void drop_gather(const UpdateArgs<R, NF>& a, const R* gas,
+                                                                  const unsigned char* gvalid, const int32_t* cs,
+                                                                  int otx, int oty, int otz, int32_t s,
+                                                                  DropLocal<M, NF, R>& L) {
   constexpr int NS = M::NS;
-  constexpr int NG = 5 + NS;     // staged gas fields: rho, Y[NS], vel[3], T
+  using F = Fields<NF>;
+  const long long cap = a.cap;
+  const int32_t o = a.perm[s];
+  L.o = o;
+  // local cell of sorted position s: largest c with cs[c] <= s
+  int lc = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1)
+    if (lc + step < kTileCells && cs[lc + step] <= s) lc += step;
+  L.lc = lc;
+  L.cx = otx + (lc % kTile);
+  L.cy = oty + ((lc / kTile) % kTile);
+  L.cz = otz + lc / (kTile * kTile);
+
+  Drop<R, NF>& d = L.d;
+  R* pos = L.pos;
+  if constexpr (POS64) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pos[q] = a.fin[(F::POS + q) * cap + o];
+      d.vel[q] = a.fin[(F::VEL + q) * cap + o];
+    }
+    d.T = a.fin[F::T * cap + o];
+    d.dia = a.fin[F::DIA * cap + o];
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) d.Y[spf] = a.fin[(F::Y + spf) * cap + o];
+    d.numDens = a.fin[F::ND * cap + o];
+    L.film = a.fin[F::FILM * cap + o];
+  } else {
+    using G = Fields32;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pos[q] = a.fin[(G::POS + q) * cap + o];
+      d.vel[q] = a.fin[(G::VEL + q) * cap + o];
+    }
+    d.T = a.fin[G::T * cap + o];
+    d.numDens = a.fin[G::ND * cap + o];
+    L.film = a.fin[G::FILM * cap + o];
+    double m = 0.;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      L.mk[spf] = a.min[spf * cap + o];
+      m += L.mk[spf];
+    }
+    const double im = 1. / m;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) d.Y[spf] = static_cast<R>(L.mk[spf] * im);
+    d.mass = static_cast<R>(m);
+    d.dia = R(0);
+  }
+  d.bm1 = R(0);
+  d.alive = true;
+
+  // trilinear corner cell and weights (SprayInterpolation.H:158-199)
+  int ijk[3];
+  R sx_hi[3];
+  int* bflags = L.bflags;
+  bflags[0] = bflags[1] = bflags[2] = 0;
+  if constexpr (POS64) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double lx = (static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q] + 0.5;
+      ijk[q] = static_cast<int>(floor(lx));
+      sx_hi[q] = static_cast<R>(lx - static_cast<double>(ijk[q]));
+    }
+    if constexpr (AT_BOUNDS) {
+      double pd[3] = {static_cast<double>(pos[0]), static_cast<double>(pos[1]), static_cast<double>(pos[2])};
+      if (dev_check_bounds(pd, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags))
+        set_err(a.err, PELEMP_E_LEFT_DOMAIN);
+    }
+  } else {
+    const int c3[3] = {L.cx, L.cy, L.cz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const R l = pos[q] + R(0.5);
+      const R fl = floor(l);
+      ijk[q] = c3[q] + static_cast<int>(fl);
+      sx_hi[q] = l - fl;
+    }
+  }
+  R ssv[2][3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const R sx_lo = R(1.) - sx_hi[q];
+    if (bflags[q] > 1) {
+      ssv[0][q] = R(1.);
+      ssv[1][q] = R(0.);
+    } else if (bflags[q] < -1) {
+      ssv[0][q] = R(0.);
+      ssv[1][q] = R(1.);
+    } else {
+      ssv[0][q] = sx_lo;
+      ssv[1][q] = sx_hi[q];
+    }
+  }
+  // InterpolateGasPhase (SprayInterpolation.H:99-150), LMeX flavour
+  GasAt<R, NS>& g = L.g;
+  g.rho = R(0);
+  g.T = R(0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) g.vel[q] = R(0);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) g.Y[n] = R(0);
+#pragma unroll
+  for (int kk = -1; kk < 1; ++kk) {
+#pragma unroll
+    for (int jj = -1; jj < 1; ++jj) {
+#pragma unroll
+      for (int ii = -1; ii < 1; ++ii) {
+        const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
+        if (cw > R(0.)) {
+          int hx = ijk[0] + ii - otx + 1, hy = ijk[1] + jj - oty + 1, hz = ijk[2] + kk - otz + 1;
+          const bool inr = hx >= 0 && hx < kHaloW && hy >= 0 && hy < kHaloW && hz >= 0 && hz < kHaloW;
+          hx = inr ? hx : 0;
+          hy = inr ? hy : 0;
+          hz = inr ? hz : 0;
+          const int h = (hz * kHaloW + hy) * kHaloW + hx;
+          if (!inr || !gvalid[h]) set_err(a.err, PELEMP_E_STATE_BOX);
+          g.rho += cw * gas[h];
+#pragma unroll
+          for (int n = 0; n < NS; ++n) g.Y[n] += cw * gas[(1 + n) * kHaloCells + h];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) g.vel[q] += cw * gas[(1 + NS + q) * kHaloCells + h];
+          g.T += cw * gas[(4 + NS) * kHaloCells + h];
+        }
+      }
+    }
+  }
+  gas_define<M, R>(g);
+  calc_boil_t<NF, R>(a.fc, g.p, L.cBoilT);
+}
+
+// Droplet physics, source contribution, drift, store and re-sort
+// classification of droplet s (storage position s of the output buffer).
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
+__device__ __attribute__((always_inline)) inline void drop_update(const UpdateArgs<R, NF>& a, int tile, int t,
+                                                                  int32_t s, DropLocal<M, NF, R>& L,
+                                                                  const SubRates<M, NF, R>* pre, double* contrib,
+                                                                  unsigned char* stayed) {
+  constexpr int NSRC = 5 + NF;
+  using F = Fields<NF>;
+  const long long cap = a.cap;
+  Drop<R, NF>& d = L.d;
+  R* pos = L.pos;
+  const R film = L.film;
+  int* bflags = L.bflags;
+  DropSrc<R, NF> sr;
+  if (film > R(0.)) {
+    set_err(a.err, PELEMP_E_BAD_CONFIG);  // wall-film droplets use the film pass
+    sr.mom[0] = sr.mom[1] = sr.mom[2] = R(0);
+    sr.mass = R(0);
+    sr.eng = R(0);
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) sr.Ydot[spf] = R(0);
+  } else {
+    spray_source<M, NF, R, !POS64>(a.sub_dt, L.g, a.fc, d, L.cBoilT, sr, pre);
+  }
+  // contributions cur_coef * source (SprayParticles.cpp:419-441)
+  double* cb = &contrib[t * NSRC];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) cb[q] = a.cur_coef * static_cast<double>(sr.mom[q]);
+  cb[3] = a.cur_coef * static_cast<double>(sr.mass);
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = a.cur_coef * static_cast<double>(sr.Ydot[spf]);
+  cb[4 + NF] = a.cur_coef * static_cast<double>(sr.eng);
+
+  // drift (SprayParticles.cpp:443-479) and new cell key
+  uint32_t key = kDeadKey;
+  if constexpr (POS64) {
+    double p3[3] = {static_cast<double>(pos[0]), static_cast<double>(pos[1]), static_cast<double>(pos[2])};
+    if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) p3[q] += static_cast<double>(a.sub_dt) * static_cast<double>(d.vel[q]);
+      if constexpr (AT_BOUNDS) {
+        if (dev_check_bounds(p3, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags)) d.alive = false;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (a.bndry_lo[q] == 0) {
+        const double Lq = a.phi[q] - a.plo[q];
+        if (p3[q] >= a.phi[q]) {
+          p3[q] -= Lq;
+        } else if (p3[q] < a.plo[q]) {
+          p3[q] += Lq;
+        }
+      }
+      pos[q] = static_cast<R>(p3[q]);
+    }
+    if (d.alive) {
+      int c[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        int ic = static_cast<int>(floor((p3[q] - a.plo[q]) * a.dxi[q]));
+        ic = ic < 0 ? 0 : (ic >= a.ncell[q] ? a.ncell[q] - 1 : ic);
+        c[q] = ic;
+      }
+      key = tile_key(c[0], c[1], c[2], a.ntile);
+    }
+  } else {
+    int c[3] = {L.cx, L.cy, L.cz};
+    if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        R xi = pos[q] + a.sub_dt_dxi[q] * d.vel[q];
+        const R sh = floor(xi);
+        xi -= sh;
+        if (xi >= R(1.)) xi = R(0.);  // rounding of a tiny negative xi
+        int ic = c[q] + static_cast<int>(sh);
+        const int n = a.ncell[q];
+        ic = ((ic % n) + n) % n;  // periodic
+        c[q] = ic;
+        pos[q] = xi;
+      }
+    }
+    if (d.alive) key = tile_key(c[0], c[1], c[2], a.ntile);
+  }
+
+  R* fo = a.fout;
+  if constexpr (POS64) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      fo[(F::POS + q) * cap + s] = pos[q];
+      fo[(F::VEL + q) * cap + s] = d.vel[q];
+    }
+    fo[F::T * cap + s] = d.T;
+    fo[F::DIA * cap + s] = d.dia;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) fo[(F::Y + spf) * cap + s] = d.Y[spf];
+    fo[F::ND * cap + s] = d.numDens;
+    fo[F::FILM * cap + s] = film;
+  } else {
+    using G = Fields32;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      fo[(G::POS + q) * cap + s] = pos[q];
+      fo[(G::VEL + q) * cap + s] = d.vel[q];
+    }
+    fo[G::T * cap + s] = d.T;
+    fo[G::DIA * cap + s] = d.dia;
+    fo[G::ND * cap + s] = d.numDens;
+    fo[G::FILM * cap + s] = film;
+    // FP64 mass bookkeeping: accumulate the single-precision increments
+    double m = 0.;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      double v = L.mk[spf] + static_cast<double>(d.dmk[spf]);
+      v = v > 0. ? v : 0.;
+      L.mk[spf] = v;
+      m += v;
+    }
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      // Y <= 1e-12 is set to zero (Drag.H:385-387)
+      const double v = (L.mk[spf] <= 1.e-12 * m) ? 0. : L.mk[spf];
+      a.mout[spf * cap + s] = v;
+    }
+  }
+  a.slot_out[s] = a.slot_in[L.o];
+  a.key_out[s] = key;
+  // classify for the incremental re-sort
+  const uint32_t k_old = static_cast<uint32_t>(tile) * kTileCells + L.lc;
+  stayed[t] = (key == k_old) ? 1 : 0;
+  if (key != kDeadKey && key != k_old) {
+    // relative to the new cell `key`, the droplet comes from a cell with a
+    // lower key (it precedes the stayers) or a higher key (it follows them)
+    const int32_t arr = (k_old < key) ? (atomicAdd(&a.mov_lo[key], 1) | (1 << 30))
+                                      : (atomicAdd(&a.mov_hi[key], 1) | (2 << 30));
+    a.arrival_out[s] = arr;
+  }
+}
+
+// Surrogate stage of one chunk: hull test, network (whole wavefront) and
+// predicted rates.  Returns whether this lane's droplet uses them.
+template <class M, int NF>
+__device__ __attribute__((always_inline)) inline bool surr_stage_drop(const UpdateArgs<float, NF>& a,
+                                                                      const SurrWeights& W, bool active,
+                                                                      DropLocal<M, NF, float>& L,
+                                                                      SubRates<M, NF, float>& o) {
+  bool use = active && a.fc.mass_trans && !(L.film > 0.f);
+  float xn[kSurrIn];
+  VaporLite<float, NF> v;
+  float rho_part = 1.f, pmass = 1.f, dia = 1.f;
+  float vel_part[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < kSurrIn; ++q) xn[q] = 0.f;
+  if (use) {
+    const Drop<float, NF>& d = L.d;
+    float rinv = 0.f, mwinv = 0.f, Tb = 0.f;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      rinv += d.Y[spf] / rhoL<float, NF>(a.fc, rmin(d.T, L.cBoilT[spf]), spf);
+      mwinv += d.Y[spf] / a.fc.mw_fuel[spf];
+      Tb += d.Y[spf] * L.cBoilT[spf];
+    }
+    rho_part = 1.f / rinv;
+    pmass = d.mass;
+    dia = rm::cbrt(6.f * pmass / (static_cast<float>(kPIc) * rho_part));
+    if (!a.fc.fixed_parts) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) vel_part[q] = d.vel[q];
+    }
+    vapor_lite<M, NF, float>(a.fc, L.g, d.T, 1.f / mwinv, d.Y, L.cBoilT, v);
+    float du2 = 0.f;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float dv = L.g.vel[q] - vel_part[q];
+      du2 += dv * dv;
+    }
+    const float urel = rm::sqrt(du2);
+    use = v.ok && surr_in_hull<float>(a.hull, dia, d.T, Tb, urel, L.g.T, L.g.p, v.Yfg, v.B_M);
+    if (use) {
+      float x[kSurrIn];
+      surr_features<float>(dia, d.T, urel, L.g.T, L.g.p, v.Ysk_f, d.Y[0], (NF > 1) ? d.Y[NF - 1] : 0.f, x);
+#pragma unroll
+      for (int q = 0; q < kSurrIn; ++q) xn[q] = (x[q] - a.sn.c[q]) * a.sn.s[q];
+    }
+  }
+  float yn[kSurrOut];
+  surr_mlp(W, xn, yn);  // every lane of the wavefront
+  if (use) {
+    float z[kSurrOut];
+#pragma unroll
+    for (int q = 0; q < kSurrOut; ++q) z[q] = a.sn.mean[q] + a.sn.sd[q] * yn[q];
+    surr_rates<M, NF, float>(z, v, L.g, a.fc, L.d.T, dia, pmass, rho_part, L.d.Y, vel_part, 0.f, L.cBoilT, o);
+    const float ind = surr_indicator<M, NF, float>(o, a.sub_dt, pmass, a.fc.mom_trans);
+    use = (ind < a.hull.indicator_max) && (o.part_temp_src == o.part_temp_src);
+  }
+  return use;
+}
+
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR>
+__device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, R* gas,
+                                                                   unsigned char* gvalid, int32_t* cs,
+                                                                   double* contrib, unsigned char* stayed,
+                                                                   const SurrWeights* sw, unsigned& n_surr,
+                                                                   unsigned& n_fb) {
+  constexpr int NS = M::NS;
   constexpr int NSRC = 5 + NF;   // mom[3], mass, Ydot[NF], eng
   constexpr int NITEM = (kTileCells * NSRC + kUpdThreads - 1) / kUpdThreads;
-  using F = Fields<NF>;
 
-  __shared__ R gas[NG * kHaloCells];
-  __shared__ unsigned char gvalid[kHaloCells];
-  __shared__ int32_t cs[kTileCells + 1];
-  __shared__ double contrib[kUpdThreads * NSRC];
-  __shared__ unsigned char stayed[kUpdThreads];
-
-  const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
-  const int tile = xcd_tile(blockIdx.x, ntiles);
   const int t = threadIdx.x;
   const int txy = a.ntile[0] * a.ntile[1];
   const int otz = (tile / txy) * kTile;
@@ -150,7 +516,10 @@ __global__ void __launch_bounds__(kUpdThreads, UpdOccupancy<R>::kMinBlocks)
   __syncthreads();
   const int32_t d0 = cs[0];
   const int32_t d1 = cs[kTileCells];
-  if (d1 == d0) return;  // uniform: empty tile
+  if (d1 == d0) {  // uniform: empty tile
+    if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
+    return;
+  }
 
   // ---- stage gas halo ----
   for (int h = t; h < kHaloCells; h += kUpdThreads) {
@@ -183,259 +552,24 @@ __global__ void __launch_bounds__(kUpdThreads, UpdOccupancy<R>::kMinBlocks)
   for (int r = 0; r < NITEM; ++r) acc[r] = 0.;
   int32_t srank = 0;  // threads t < 64: stayers of cell t seen so far
 
-  const long long cap = a.cap;
   for (int32_t base = d0; base < d1; base += kUpdThreads) {
     const int32_t s = base + t;
-    if (s < d1) {
-      const int32_t o = a.perm[s];
-      // local cell of sorted position s: largest c with cs[c] <= s
-      int lc = 0;
-#pragma unroll
-      for (int step = 32; step >= 1; step >>= 1)
-        if (lc + step < kTileCells && cs[lc + step] <= s) lc += step;
-      const int cx = otx + (lc % kTile), cy = oty + ((lc / kTile) % kTile), cz = otz + lc / (kTile * kTile);
-
-      Drop<R, NF> d;
-      R pos[3];
-      R film;
-      double mk[NF];  // FP32 form: liquid species masses
-      if constexpr (POS64) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          pos[q] = a.fin[(F::POS + q) * cap + o];
-          d.vel[q] = a.fin[(F::VEL + q) * cap + o];
-        }
-        d.T = a.fin[F::T * cap + o];
-        d.dia = a.fin[F::DIA * cap + o];
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) d.Y[spf] = a.fin[(F::Y + spf) * cap + o];
-        d.numDens = a.fin[F::ND * cap + o];
-        film = a.fin[F::FILM * cap + o];
-      } else {
-        using G = Fields32;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          pos[q] = a.fin[(G::POS + q) * cap + o];
-          d.vel[q] = a.fin[(G::VEL + q) * cap + o];
-        }
-        d.T = a.fin[G::T * cap + o];
-        d.numDens = a.fin[G::ND * cap + o];
-        film = a.fin[G::FILM * cap + o];
-        double m = 0.;
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) {
-          mk[spf] = a.min[spf * cap + o];
-          m += mk[spf];
-        }
-        const double im = 1. / m;
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) d.Y[spf] = static_cast<R>(mk[spf] * im);
-        d.mass = static_cast<R>(m);
-        d.dia = R(0);
+    if constexpr (!SURR) {
+      if (s < d1) {
+        DropLocal<M, NF, R> L;
+        drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, L);
+        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, L, nullptr, contrib, stayed);
       }
-      d.bm1 = R(0);
-      d.alive = true;
-
-      // trilinear corner cell and weights (SprayInterpolation.H:158-199)
-      int ijk[3];
-      R sx_hi[3];
-      int bflags[3] = {0, 0, 0};
-      if constexpr (POS64) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const double lx = (static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q] + 0.5;
-          ijk[q] = static_cast<int>(floor(lx));
-          sx_hi[q] = static_cast<R>(lx - static_cast<double>(ijk[q]));
-        }
-        if constexpr (AT_BOUNDS) {
-          double pd[3] = {static_cast<double>(pos[0]), static_cast<double>(pos[1]), static_cast<double>(pos[2])};
-          if (dev_check_bounds(pd, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags))
-            set_err(a.err, PELEMP_E_LEFT_DOMAIN);
-        }
-      } else {
-        const int c3[3] = {cx, cy, cz};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const R l = pos[q] + R(0.5);
-          const R fl = floor(l);
-          ijk[q] = c3[q] + static_cast<int>(fl);
-          sx_hi[q] = l - fl;
-        }
-      }
-      R ssv[2][3];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const R sx_lo = R(1.) - sx_hi[q];
-        if (bflags[q] > 1) {
-          ssv[0][q] = R(1.);
-          ssv[1][q] = R(0.);
-        } else if (bflags[q] < -1) {
-          ssv[0][q] = R(0.);
-          ssv[1][q] = R(1.);
-        } else {
-          ssv[0][q] = sx_lo;
-          ssv[1][q] = sx_hi[q];
-        }
-      }
-      // InterpolateGasPhase (SprayInterpolation.H:99-150), LMeX flavour
-      GasAt<R, NS> g;
-      g.rho = R(0);
-      g.T = R(0);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) g.vel[q] = R(0);
-#pragma unroll
-      for (int n = 0; n < NS; ++n) g.Y[n] = R(0);
-#pragma unroll
-      for (int kk = -1; kk < 1; ++kk) {
-#pragma unroll
-        for (int jj = -1; jj < 1; ++jj) {
-#pragma unroll
-          for (int ii = -1; ii < 1; ++ii) {
-            const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
-            if (cw > R(0.)) {
-              int hx = ijk[0] + ii - otx + 1, hy = ijk[1] + jj - oty + 1, hz = ijk[2] + kk - otz + 1;
-              const bool inr = hx >= 0 && hx < kHaloW && hy >= 0 && hy < kHaloW && hz >= 0 && hz < kHaloW;
-              hx = inr ? hx : 0;
-              hy = inr ? hy : 0;
-              hz = inr ? hz : 0;
-              const int h = (hz * kHaloW + hy) * kHaloW + hx;
-              if (!inr || !gvalid[h]) set_err(a.err, PELEMP_E_STATE_BOX);
-              g.rho += cw * gas[h];
-#pragma unroll
-              for (int n = 0; n < NS; ++n) g.Y[n] += cw * gas[(1 + n) * kHaloCells + h];
-#pragma unroll
-              for (int q = 0; q < 3; ++q) g.vel[q] += cw * gas[(1 + NS + q) * kHaloCells + h];
-              g.T += cw * gas[(4 + NS) * kHaloCells + h];
-            }
-          }
-        }
-      }
-      gas_define<M, R>(g);
-      R cBoilT[NF];
-      calc_boil_t<NF, R>(a.fc, g.p, cBoilT);
-      DropSrc<R, NF> sr;
-      if (film > R(0.)) {
-        set_err(a.err, PELEMP_E_BAD_CONFIG);  // wall-film droplets use the film pass
-        sr.mom[0] = sr.mom[1] = sr.mom[2] = R(0);
-        sr.mass = R(0);
-        sr.eng = R(0);
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) sr.Ydot[spf] = R(0);
-      } else {
-        spray_source<M, NF, R, !POS64>(a.sub_dt, g, a.fc, d, cBoilT, sr);
-      }
-      // contributions cur_coef * source (SprayParticles.cpp:419-441)
-      double* cb = &contrib[t * NSRC];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) cb[q] = a.cur_coef * static_cast<double>(sr.mom[q]);
-      cb[3] = a.cur_coef * static_cast<double>(sr.mass);
-#pragma unroll
-      for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = a.cur_coef * static_cast<double>(sr.Ydot[spf]);
-      cb[4 + NF] = a.cur_coef * static_cast<double>(sr.eng);
-
-      // drift (SprayParticles.cpp:443-479) and new cell key
-      uint32_t key = kDeadKey;
-      if constexpr (POS64) {
-        double p3[3] = {static_cast<double>(pos[0]), static_cast<double>(pos[1]), static_cast<double>(pos[2])};
-        if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
-#pragma unroll
-          for (int q = 0; q < 3; ++q) p3[q] += static_cast<double>(a.sub_dt) * static_cast<double>(d.vel[q]);
-          if constexpr (AT_BOUNDS) {
-            if (dev_check_bounds(p3, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags)) d.alive = false;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          if (a.bndry_lo[q] == 0) {
-            const double L = a.phi[q] - a.plo[q];
-            if (p3[q] >= a.phi[q]) {
-              p3[q] -= L;
-            } else if (p3[q] < a.plo[q]) {
-              p3[q] += L;
-            }
-          }
-          pos[q] = static_cast<R>(p3[q]);
-        }
-        if (d.alive) {
-          int c[3];
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            int ic = static_cast<int>(floor((p3[q] - a.plo[q]) * a.dxi[q]));
-            ic = ic < 0 ? 0 : (ic >= a.ncell[q] ? a.ncell[q] - 1 : ic);
-            c[q] = ic;
-          }
-          key = tile_key(c[0], c[1], c[2], a.ntile);
-        }
-      } else {
-        int c[3] = {cx, cy, cz};
-        if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            R xi = pos[q] + a.sub_dt_dxi[q] * d.vel[q];
-            const R sh = floor(xi);
-            xi -= sh;
-            if (xi >= R(1.)) xi = R(0.);  // rounding of a tiny negative xi
-            int ic = c[q] + static_cast<int>(sh);
-            const int n = a.ncell[q];
-            ic = ((ic % n) + n) % n;  // periodic
-            c[q] = ic;
-            pos[q] = xi;
-          }
-        }
-        if (d.alive) key = tile_key(c[0], c[1], c[2], a.ntile);
-      }
-
-      R* fo = a.fout;
-      if constexpr (POS64) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          fo[(F::POS + q) * cap + s] = pos[q];
-          fo[(F::VEL + q) * cap + s] = d.vel[q];
-        }
-        fo[F::T * cap + s] = d.T;
-        fo[F::DIA * cap + s] = d.dia;
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) fo[(F::Y + spf) * cap + s] = d.Y[spf];
-        fo[F::ND * cap + s] = d.numDens;
-        fo[F::FILM * cap + s] = film;
-      } else {
-        using G = Fields32;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          fo[(G::POS + q) * cap + s] = pos[q];
-          fo[(G::VEL + q) * cap + s] = d.vel[q];
-        }
-        fo[G::T * cap + s] = d.T;
-        fo[G::DIA * cap + s] = d.dia;
-        fo[G::ND * cap + s] = d.numDens;
-        fo[G::FILM * cap + s] = film;
-        // FP64 mass bookkeeping: accumulate the single-precision increments
-        double m = 0.;
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) {
-          double v = mk[spf] + static_cast<double>(d.dmk[spf]);
-          v = v > 0. ? v : 0.;
-          mk[spf] = v;
-          m += v;
-        }
-#pragma unroll
-        for (int spf = 0; spf < NF; ++spf) {
-          // Y <= 1e-12 is set to zero (Drag.H:385-387)
-          const double v = (mk[spf] <= 1.e-12 * m) ? 0. : mk[spf];
-          a.mout[spf * cap + s] = v;
-        }
-      }
-      a.slot_out[s] = a.slot_in[o];
-      a.key_out[s] = key;
-      // classify for the incremental re-sort
-      const uint32_t k_old = static_cast<uint32_t>(tile) * kTileCells + lc;
-      stayed[t] = (key == k_old) ? 1 : 0;
-      if (key != kDeadKey && key != k_old) {
-        // relative to the new cell `key`, the droplet comes from a cell with a
-        // lower key (it precedes the stayers) or a higher key (it follows them)
-        const int32_t arr = (k_old < key) ? (atomicAdd(&a.mov_lo[key], 1) | (1 << 30))
-                                          : (atomicAdd(&a.mov_hi[key], 1) | (2 << 30));
-        a.arrival_out[s] = arr;
+    } else {
+      DropLocal<M, NF, R> L;
+      const bool active = s < d1;
+      if (active) drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, L);
+      SubRates<M, NF, R> o;
+      const bool use = surr_stage_drop<M, NF>(a, *sw, active, L, o);
+      if (active) {
+        n_surr += use ? 1u : 0u;
+        n_fb += use ? 0u : 1u;
+        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, L, use ? &o : nullptr, contrib, stayed);
       }
     }
     __syncthreads();
@@ -490,6 +624,55 @@ __global__ void __launch_bounds__(kUpdThreads, UpdOccupancy<R>::kMinBlocks)
         }
       }
     }
+  }
+  if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
+}
+
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
+__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks))
+    spray_update_kernel(UpdateArgs<R, NF> a) {
+  constexpr int NG = 5 + M::NS;  // staged gas fields: rho, Y[NS], vel[3], T
+  constexpr int NSRC = 5 + NF;
+  __shared__ R gas[NG * kHaloCells];
+  __shared__ unsigned char gvalid[kHaloCells];
+  __shared__ int32_t cs[kTileCells + 1];
+  __shared__ double contrib[kUpdThreads * NSRC];
+  __shared__ unsigned char stayed[kUpdThreads];
+  const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
+  unsigned n_surr = 0, n_fb = 0;
+  process_tile<M, NF, R, POS64, AT_BOUNDS, false>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib, stayed,
+                                                  nullptr, n_surr, n_fb);
+}
+
+// Surrogate form: gridDim.x workgroups (a multiple of 8) loop over the
+// tiles; the network is staged once per workgroup.
+template <class M, int NF>
+__global__ void __launch_bounds__(kUpdThreads, 2) spray_update_surr_kernel(UpdateArgs<float, NF> a) {
+  constexpr int NG = 5 + M::NS;
+  constexpr int NSRC = 5 + NF;
+  __shared__ float gas[NG * kHaloCells];
+  __shared__ unsigned char gvalid[kHaloCells];
+  __shared__ int32_t cs[kTileCells + 1];
+  __shared__ double contrib[kUpdThreads * NSRC];
+  __shared__ unsigned char stayed[kUpdThreads];
+  __shared__ SurrWeights sw;
+  surr_stage(sw, a.surr_w, threadIdx.x, kUpdThreads);
+  __syncthreads();
+  const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
+  unsigned n_surr = 0, n_fb = 0;
+  for (int b = blockIdx.x; b < ntiles; b += gridDim.x)
+    process_tile<M, NF, float, false, false, true>(a, xcd_tile(b, ntiles), gas, gvalid, cs, contrib, stayed, &sw,
+                                                   n_surr, n_fb);
+  // one pair of counter updates per wavefront
+  unsigned long long s1 = n_surr, s2 = n_fb;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    s1 += __shfl_xor(s1, off);
+    s2 += __shfl_xor(s2, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&a.surr_cnt[0], s1);
+    atomicAdd(&a.surr_cnt[1], s2);
   }
 }
 

@@ -19,7 +19,7 @@ inline FabDev make_fab(const pelemp_fab_view& v) {
   return f;
 }
 
-template <class M, int NF, class R, bool POS64>
+template <class M, int NF, class R, bool POS64, bool SURR = false>
 int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   UpdateArgs<R, NF> a{};
   a.fin = static_cast<const R*>(c.fbuf[c.cur]);
@@ -66,7 +66,27 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.is_ghost = p.is_ghost;
   a.err = c.err;
   const dim3 grid(c.ntiles), block(kUpdThreads);
-  if (p.at_bounds) {
+  if constexpr (SURR) {
+    const pelemp_surrogate_norm& n = c.surr_norm;
+    for (int q = 0; q < kSurrIn; ++q) {
+      a.sn.c[q] = n.in_center[q];
+      a.sn.s[q] = n.in_scale[q];
+    }
+    for (int q = 0; q < kSurrOut; ++q) {
+      a.sn.mean[q] = n.out_mean[q];
+      a.sn.sd[q] = n.out_std[q];
+    }
+    a.hull = n;
+    a.surr_w = static_cast<const SurrWeights*>(c.surr_w);
+    a.surr_cnt = c.surr_cnt;
+    if (p.at_bounds) {
+      set_last_error("walls are supported by the FP64 form only");
+      return PELEMP_E_BAD_CONFIG;
+    }
+    // two workgroups per CU on 256 CUs, each looping over tiles
+    const int nwg = c.ntiles < 512 ? c.ntiles : 512;
+    hipLaunchKernelGGL((spray_update_surr_kernel<M, NF>), dim3(nwg), block, 0, s, a);
+  } else if (p.at_bounds) {
     if constexpr (POS64) {
       hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, true>), grid, block, 0, s, a);
     } else {
@@ -84,8 +104,16 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   return PELEMP_OK;
 }
 
-template <class R, bool POS64>
+template <class R, bool POS64, bool SURR = false>
 int dispatch_update(Ctx& c, const PassParams& p, hipStream_t s) {
+  if constexpr (SURR) {
+    if (c.mech == PELEMP_MECH_DODECANE_AIR && c.nf == 1)
+      return launch_update_t<Mech_dodecane_air, 1, float, false, true>(c, p, s);
+    if (c.mech == PELEMP_MECH_DECANE_DODECANE_AIR && c.nf == 2)
+      return launch_update_t<Mech_decane_dodecane_air, 2, float, false, true>(c, p, s);
+    set_last_error("the surrogate form supports mechanisms 0 (one fuel) and 1 (two fuels)");
+    return PELEMP_E_BAD_CONFIG;
+  }
   if (c.mech == PELEMP_MECH_DODECANE_AIR && c.nf == 1)
     return launch_update_t<Mech_dodecane_air, 1, R, POS64>(c, p, s);
   if (c.mech == PELEMP_MECH_DECANE_DODECANE_AIR && c.nf == 2)
