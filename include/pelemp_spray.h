@@ -127,6 +127,20 @@ int pelemp_spray_destroy(pelemp_spray_ctx* ctx);
 /* The SprayData actually in use (after the latent shift). */
 int pelemp_spray_get_data(const pelemp_spray_ctx* ctx, pelemp_spray_data* out);
 
+/* Container-level parameters of the breakup and splash models, static
+ * members of SprayParticleContainer in the reference (SprayParticles.H:
+ * 388-405, defaults SpraySetup.cpp:15-19, read at :144-160).  rng_seed keys
+ * the counter-based numbers that replace amrex::Random() for the azimuth of
+ * breakup children (SpraySB.cpp:153). */
+typedef struct pelemp_spray_params {
+  double max_parcel_size;       /* max_parcel_size, default 100 */
+  double breakup_parcel_factor; /* breakup_parcel_factor, default 0.5 */
+  double khrt_B0, khrt_B1, khrt_C3; /* 0.61, 7, 1 (KH-RT, not enabled) */
+  uint64_t rng_seed;            /* default 0 */
+} pelemp_spray_params;
+int pelemp_spray_set_params(pelemp_spray_ctx* ctx, const pelemp_spray_params* p);
+int pelemp_spray_get_params(const pelemp_spray_ctx* ctx, pelemp_spray_params* out);
+
 /* Particle transfer, host AoS meaning (see header comment):
  * pos[n*3], rdata[n*NSR], id[n].  Inactive ids (<= 0) are dropped. */
 int pelemp_spray_set_particles(pelemp_spray_ctx* ctx, int64_t n,
@@ -180,7 +194,8 @@ int pelemp_spray_pass_counts(pelemp_spray_ctx* ctx, int64_t* out, int32_t max,
 
 /* Kernel timing: when enabled, the update records HIP events around its
  * kernels; the accumulated milliseconds and launch counts are returned per
- * kernel slot (0 update, 1 sort, 2 reorder). */
+ * kernel slot (0 update, 1 sort, 2 film thickness pass, 3 secondary-droplet
+ * pass). */
 int pelemp_spray_set_timing(pelemp_spray_ctx* ctx, int32_t enable);
 int pelemp_spray_get_timing(pelemp_spray_ctx* ctx, double* ms, int64_t* launches,
                             int32_t nslots);

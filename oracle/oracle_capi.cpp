@@ -9,30 +9,57 @@
 
 #include "spray_oracle.h"
 
+// Breakup / splash parameters of one call (in: container parameters, the id
+// of the next new droplet and the capacity of the output arrays; out: next id).
+extern "C" struct oracle_branch_params {
+  double max_ppp;
+  double ppp_fact;
+  uint64_t seed;
+  int64_t next_id;
+  int64_t cap;
+};
+
 namespace {
 
 template <class Mech, int NF>
 int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, const pelemp_geom* g,
                int64_t n, double* pos, double* rdata, int64_t* id, const pelemp_fab_view* state,
                pelemp_fab_view* source, double flow_dt, int isVirt, int isGhost, int do_move,
-               double spray_cfl, int nthreads, int64_t* n_out) {
+               double spray_cfl, int nthreads, oracle_branch_params* bpp, int64_t* n_out) {
   using P = oracle::Particle<NF>;
   constexpr int NSR = oracle::Comps<NF>::pstateNum;
   std::vector<P> parts;
   parts.reserve(n);
+  int64_t max_id = 0;
   for (int64_t i = 0; i < n; ++i) {
     if (id[i] <= 0) continue;
     P p;
     for (int d = 0; d < 3; ++d) p.pos[d] = pos[3 * i + d];
     for (int c = 0; c < NSR; ++c) p.rdata[c] = rdata[NSR * i + c];
     p.id = id[i];
+    max_id = std::max(max_id, p.id);
     parts.push_back(p);
+  }
+  oracle::BranchParams bp;
+  int64_t cap = n;
+  if (bpp) {
+    bp.max_ppp = bpp->max_ppp;
+    bp.ppp_fact = bpp->ppp_fact;
+    bp.seed = bpp->seed;
+    bp.next_id = std::max(bpp->next_id, max_id + 1);
+    cap = bpp->cap;
+  } else {
+    bp.next_id = max_id + 1;
   }
   pelemp_fab_view src = *source;
   const int st = oracle::updateParticles<Mech, NF>(parts, *sd, *comps, *g, *state, src, flow_dt,
                                                    isVirt != 0, isGhost != 0, do_move != 0,
-                                                   spray_cfl, nthreads);
+                                                   spray_cfl, nthreads, bp);
   int64_t m = 0;
+  for (const P& p : parts)
+    if (p.id > 0) ++m;
+  if (m > cap) return PELEMP_E_ARG;
+  m = 0;
   for (const P& p : parts) {
     if (p.id <= 0) continue;
     for (int d = 0; d < 3; ++d) pos[3 * m + d] = p.pos[d];
@@ -41,6 +68,7 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
     ++m;
   }
   *n_out = m;
+  if (bpp) bpp->next_id = bp.next_id;
   return st;
 }
 
@@ -55,21 +83,21 @@ int oracle_update(int mech, int nf, const pelemp_spray_data* sd, const pelemp_sp
                   const pelemp_geom* g, int64_t n, double* pos, double* rdata, int64_t* id,
                   const pelemp_fab_view* state, pelemp_fab_view* source, double flow_dt,
                   int isVirt, int isGhost, int do_move, double spray_cfl, int nthreads,
-                  int64_t* n_out) {
+                  oracle_branch_params* bp, int64_t* n_out) {
   if (sd->nfuel != nf) return PELEMP_E_BAD_CONFIG;
-  if (sd->do_breakup != 0 || sd->do_splash != 0) return PELEMP_E_BAD_CONFIG;
+  if (sd->do_breakup == 2) return PELEMP_E_BAD_CONFIG;  // KH-RT is not restated
   if (mech == PELEMP_MECH_DODECANE_AIR && nf == 1)
     return run_update<pelemp::Mech_dodecane_air, 1>(sd, comps, g, n, pos, rdata, id, state, source,
                                                     flow_dt, isVirt, isGhost, do_move, spray_cfl,
-                                                    nthreads, n_out);
+                                                    nthreads, bp, n_out);
   if (mech == PELEMP_MECH_DECANE_DODECANE_AIR && nf == 2)
     return run_update<pelemp::Mech_decane_dodecane_air, 2>(sd, comps, g, n, pos, rdata, id, state,
                                                            source, flow_dt, isVirt, isGhost,
-                                                           do_move, spray_cfl, nthreads, n_out);
+                                                           do_move, spray_cfl, nthreads, bp, n_out);
   if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
     return run_update<pelemp::Mech_alkane6_air, 6>(sd, comps, g, n, pos, rdata, id, state, source,
                                                    flow_dt, isVirt, isGhost, do_move, spray_cfl,
-                                                   nthreads, n_out);
+                                                   nthreads, bp, n_out);
   return PELEMP_E_ARG;
 }
 
@@ -195,6 +223,45 @@ double oracle_spray_source_one(int mech, int nf, const pelemp_spray_data* sd, do
   if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
     return run(pelemp::Mech_alkane6_air{}, std::integral_constant<int, 6>{});
   return -1.0;
+}
+
+// updateBreakupTAB for one single-fuel droplet (mechanism 0) in a gas state
+// gas = vel[3], T, rho, Y[NS]; rdata in/out; returns the accumulated
+// tangential velocity.  cboil: boiling temperature used.
+double oracle_tab_update(const pelemp_spray_data* sd, double Reyn, double dt, const double* gas, double* rdata,
+                         double* cboil_out) {
+  using M = pelemp::Mech_dodecane_air;
+  constexpr int NS = M::NS;
+  oracle::GasPhaseVals<NS, 1> gpv;
+  gpv.reset();
+  pelemp::GasProps<M>::molecular_weight(gpv.mw);
+  for (int d = 0; d < 3; ++d) gpv.vel_fluid[d] = gas[d];
+  gpv.T_fluid = gas[3];
+  gpv.rho_fluid = gas[4];
+  for (int n = 0; n < NS; ++n) gpv.Y_fluid[n] = gas[5 + n];
+  gpv.define();
+  double cB[1];
+  oracle::calcBoilT<NS, 1>(*sd, gpv, cB);
+  oracle::Particle<1> p;
+  for (int c = 0; c < oracle::Comps<1>::pstateNum; ++c) p.rdata[c] = rdata[c];
+  p.id = 1;
+  for (int d = 0; d < 3; ++d) p.pos[d] = 0.0;
+  const double U = oracle::updateBreakupTAB<NS, 1>(Reyn, dt, cB, gpv, *sd, p);
+  for (int c = 0; c < oracle::Comps<1>::pstateNum; ++c) rdata[c] = p.rdata[c];
+  cboil_out[0] = cB[0];
+  return U;
+}
+
+// splash_criteria (AhamedSplash.H:125-197): returns the splash_type;
+// out = [N_SB, ms, dm].
+int oracle_splash_criteria(double Kv, double Tstar, double alpha, double Kcrit, double del_film, double* out) {
+  int nsb = oracle::SB_NONE;
+  double ms = 0., dm = 0.;
+  const int t = oracle::splash_criteria(Kv, Tstar, alpha, Kcrit, del_film, nsb, ms, dm);
+  out[0] = nsb;
+  out[1] = ms;
+  out[2] = dm;
+  return t;
 }
 
 }  // extern "C"

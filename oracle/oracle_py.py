@@ -24,6 +24,12 @@ def build() -> str:
     return LIB_PATH
 
 
+class BranchParamsC(ctypes.Structure):
+    """oracle_branch_params (oracle_capi.cpp)."""
+    _fields_ = [("max_ppp", ctypes.c_double), ("ppp_fact", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("next_id", ctypes.c_int64), ("cap", ctypes.c_int64)]
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -36,7 +42,7 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(C.SprayCompsC), ctypes.POINTER(C.GeomC), ctypes.c_int64, vp, vp,
                                     vp, ctypes.POINTER(C.FabViewC), ctypes.POINTER(C.FabViewC), ctypes.c_double,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                    ctypes.POINTER(ctypes.c_int64)]
+                                    ctypes.POINTER(BranchParamsC), ctypes.POINTER(ctypes.c_int64)]
         L.oracle_pm_eval.restype = None
         L.oracle_pm_eval.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp]
         L.oracle_gas_props.restype = ctypes.c_int
@@ -57,6 +63,10 @@ def lib() -> ctypes.CDLL:
         L.oracle_spray_source_one.restype = ctypes.c_double
         L.oracle_spray_source_one.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(C.SprayDataC),
                                               ctypes.c_double, vp, vp, vp, vp, vp]
+        L.oracle_tab_update.restype = ctypes.c_double
+        L.oracle_tab_update.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_double, vp, vp, vp]
+        L.oracle_splash_criteria.restype = ctypes.c_int
+        L.oracle_splash_criteria.argtypes = [ctypes.c_double] * 5 + [vp]
         L.oracle_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -91,14 +101,29 @@ def latent_shifted(sdata: dict, mech: int, ns: int) -> dict:
 
 def update(mech: int, sdata: dict, comps: dict, geom: dict, pos, rdata, ids, state: np.ndarray, state_lo,
            state_hi, source: np.ndarray, src_lo, src_hi, flow_dt: float, do_move: bool = True,
-           spray_cfl: float = -1.0, is_virt: bool = False, is_ghost: bool = False, nthreads: int = 0):
+           spray_cfl: float = -1.0, is_virt: bool = False, is_ghost: bool = False, nthreads: int = 0,
+           branch: dict | None = None):
     """One updateParticles call on host arrays; `source` is updated in place.
-    Returns (status, pos, rdata, ids) with the surviving droplets in cell order."""
+    Returns (status, pos, rdata, ids) with the surviving droplets in cell order
+    (then the secondary droplets created by breakup / splash).  `branch`
+    (max_ppp, ppp_fact, seed, next_id, cap) sets the breakup / splash
+    parameters; its next_id is updated."""
     nf = int(sdata["nfuel"])
-    pos = np.array(pos, dtype=np.float64, order="C", copy=True)
-    rdata = np.array(rdata, dtype=np.float64, order="C", copy=True)
-    ids = np.array(ids, dtype=np.int64, order="C", copy=True)
-    n = ids.shape[0]
+    n = np.asarray(ids).shape[0]
+    bp = BranchParamsC(max_ppp=100.0, ppp_fact=0.5, seed=0, next_id=0, cap=n)
+    if branch is not None:
+        for k in ("max_ppp", "ppp_fact", "seed", "next_id", "cap"):
+            if k in branch:
+                setattr(bp, k, branch[k])
+    cap = max(int(bp.cap), n)
+    bp.cap = cap
+    pos_out = np.zeros((cap, 3))
+    pos_out[:n] = np.asarray(pos, dtype=np.float64)
+    rd_out = np.zeros((cap, np.asarray(rdata).shape[1]))
+    rd_out[:n] = np.asarray(rdata, dtype=np.float64)
+    id_out = np.zeros(cap, dtype=np.int64)
+    id_out[:n] = np.asarray(ids, dtype=np.int64)
+    pos, rdata, ids = pos_out, rd_out, id_out
     sd = C.spray_data_c(sdata)
     cc = C.comps_c(comps)
     gg = C.geom_c(geom)
@@ -111,7 +136,9 @@ def update(mech: int, sdata: dict, comps: dict, geom: dict, pos, rdata, ids, sta
     st = lib().oracle_update(mech, nf, ctypes.byref(sd), ctypes.byref(cc), ctypes.byref(gg), n, pos.ctypes.data,
                              rdata.ctypes.data, ids.ctypes.data, ctypes.byref(sv), ctypes.byref(dv), flow_dt,
                              int(is_virt), int(is_ghost), int(do_move), float(spray_cfl), int(nthreads),
-                             ctypes.byref(nout))
+                             ctypes.byref(bp), ctypes.byref(nout))
+    if branch is not None:
+        branch["next_id"] = int(bp.next_id)
     m = nout.value
     return st, pos[:m].copy(), rdata[:m].copy(), ids[:m].copy()
 
@@ -134,3 +161,21 @@ def spray_source_one(mech: int, sdata: dict, flow_dt: float, gas, rdata, pid: in
     Re = lib().oracle_spray_source_one(mech, nf, ctypes.byref(sd), flow_dt, gas.ctypes.data, rd.ctypes.data,
                                        pid_c.ctypes.data, src.ctypes.data, cb.ctypes.data)
     return Re, rd, int(pid_c[0]), src, cb
+
+
+def tab_update(sdata: dict, Reyn: float, dt: float, gas, rdata):
+    """updateBreakupTAB for one n-dodecane droplet (mechanism 0) in the gas
+    state [u, v, w, T, rho, Y...]; returns (Utan_total, rdata_out, cBoilT)."""
+    sd = C.spray_data_c(sdata)
+    gas = np.ascontiguousarray(gas, dtype=np.float64)
+    rd = np.array(rdata, dtype=np.float64, copy=True)
+    cb = np.zeros(1)
+    U = lib().oracle_tab_update(ctypes.byref(sd), Reyn, dt, gas.ctypes.data, rd.ctypes.data, cb.ctypes.data)
+    return U, rd, float(cb[0])
+
+
+def splash_criteria(Kv: float, Tstar: float, alpha: float, Kcrit: float, del_film: float):
+    """splash_criteria: (splash_type, N_SB, ms, dm)."""
+    out = np.zeros(3)
+    t = lib().oracle_splash_criteria(Kv, Tstar, alpha, Kcrit, del_film, out.ctypes.data)
+    return t, int(out[0]), float(out[1]), float(out[2])

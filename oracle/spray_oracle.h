@@ -587,35 +587,69 @@ inline void containing_cell(const double pos[3], const pelemp_geom& g, const dou
 }
 
 // Periodic wrap standing in for the host solver's Redistribute.
+// (A wall-film droplet placed at the impact point of a grazing droplet can
+// lie several periods away, AhamedSplash.H:238, so the shift is a whole
+// number of periods; within one period it is a single +-L.)
 inline void periodic_wrap(double pos[3], const pelemp_geom& g) {
   for (int d = 0; d < 3; ++d) {
-    if (g.bndry_lo[d] == 0) {
+    if (g.bndry_lo[d] == 0 && (pos[d] >= g.phi[d] || pos[d] < g.plo[d])) {
       const double L = g.phi[d] - g.plo[d];
-      if (pos[d] >= g.phi[d]) {
-        pos[d] -= L;
-      } else if (pos[d] < g.plo[d]) {
-        pos[d] += L;
-      }
+      double n = std::floor((pos[d] - g.plo[d]) / L);
+      if (pos[d] >= g.phi[d] && n < 1.) n = 1.;
+      if (pos[d] < g.plo[d] && n > -1.) n = -1.;
+      pos[d] -= n * L;
     }
   }
 }
 
+}  // namespace oracle
+
+#include "spray_oracle_branches.h"
+
+namespace oracle {
+
 // ---------------------------------------------------------------------------
-// updateParticles driver (SprayParticles.cpp:142-502), drag + evaporation
-// path (no film, breakup or splash; those are rejected by the caller).
+// updateParticles driver (SprayParticles.cpp:142-502) with the wall, film,
+// splash and TAB branches.
+
+// Container-level parameters of the breakup / splash models
+// (SprayParticleContainer statics, SpraySetup.cpp:15-19) and the id of the
+// next droplet to create (in/out).
+struct BranchParams {
+  double max_ppp = 100.;
+  double ppp_fact = 0.5;
+  uint64_t seed = 0;
+  int64_t next_id = 1;
+};
+
+// per-droplet values the reference keeps in registers across sub-cycles
+template <int NF>
+struct Carry {
+  int bflags[3] = {0, 0, 0};
+  double Utan_total = 0.;
+  double Reyn_d = 0.;
+  int64_t parent_id = 0;  // id before any deletion in this call
+  SBRecord<NF> rec;
+};
+
 template <class Mech, int NF>
 int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& fdat,
                     const pelemp_spray_comps& SPI, const pelemp_geom& g,
                     const pelemp_fab_view& state_v, pelemp_fab_view& source_v, double flow_dt,
-                    bool isVirt, bool isGhost, bool do_move, double spray_cfl_lev, int nthreads) {
+                    bool isVirt, bool isGhost, bool do_move, double spray_cfl_lev, int nthreads,
+                    BranchParams& bp) {
   constexpr int NS = Mech::NS;
   using C = Comps<NF>;
-  (void)isVirt;
+  const bool isActive = !(isVirt || isGhost);
+  const bool do_splash = fdat.do_splash != 0 && isActive && do_move;
+  const bool do_breakup = fdat.do_breakup > 0;
   const double dxi[3] = {1. / g.dx[0], 1. / g.dx[1], 1. / g.dx[2]};
   const double vol = g.dx[0] * g.dx[1] * g.dx[2];
   const double inv_vol = 1. / vol;
   bool at_bounds = false;
   for (int d = 0; d < 3; ++d) at_bounds = at_bounds || g.bndry_lo[d] != 0 || g.bndry_hi[d] != 0;
+  const bool do_splash_box = do_splash && at_bounds;
+  const bool make_new_drops = (do_breakup || do_splash_box) && isActive && do_move;
   const double sub_cfl = 0.5;
   double sub_dt = flow_dt;
   int num_iter = 1;
@@ -626,7 +660,16 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
   const Fab st(state_v);
   Fab srcv(source_v);
   constexpr int NSRC = 5 + NF;  // mom(3) mass Y(nf) eng, contribution slots
-  std::vector<int> bfl(parts.size() * 3, 0);
+  std::vector<Carry<NF>> carry(parts.size());
+  // film thickness per cell from the film particles at the start of the call
+  // (fillFilmFab, SprayParticles.cpp:275-288), summed in cell-sorted order
+  const int ncx = g.domain_hi[0] - g.domain_lo[0] + 1, ncy = g.domain_hi[1] - g.domain_lo[1] + 1;
+  const int ncz = g.domain_hi[2] - g.domain_lo[2] + 1;
+  std::vector<double> wf;
+  auto wf_index = [&](const int c[3]) {
+    return (static_cast<int64_t>(c[2] - g.domain_lo[2]) * ncy + (c[1] - g.domain_lo[1])) * ncx +
+           (c[0] - g.domain_lo[0]);
+  };
   int status = PELEMP_OK;
   for (int cur_iter = 0; cur_iter < num_iter; ++cur_iter) {
     // order: active droplets stably sorted by containing cell
@@ -642,16 +685,30 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
     {
       std::vector<Particle<NF>> sorted;
-      std::vector<int> sbf;
+      std::vector<Carry<NF>> sc;
       sorted.reserve(np);
-      sbf.reserve(np * 3);
+      sc.reserve(np);
       for (size_t r = 0; r < np; ++r) {
-        if (key[order[r]] == INT64_MAX) break;
+        if (key[order[r]] == INT64_MAX) {
+          // a deleted droplet keeps its record for the secondary droplets
+          if (carry[order[r]].rec.type == SB_NONE) continue;
+        }
         sorted.push_back(parts[order[r]]);
-        for (int d = 0; d < 3; ++d) sbf.push_back(bfl[3 * order[r] + d]);
+        sc.push_back(carry[order[r]]);
       }
       parts.swap(sorted);
-      bfl.swap(sbf);
+      carry.swap(sc);
+    }
+    if (cur_iter == 0 && do_splash_box) {
+      wf.assign(static_cast<size_t>(ncx) * ncy * ncz, 0.);
+      const double face_area = g.dx[0] * g.dx[1] * g.dx[2];
+      for (const Particle<NF>& p : parts) {
+        if (p.id > 0 && p.rdata[C::pstateFilmHght] > 0.) {
+          int c[3];
+          containing_cell(p.pos, g, dxi, c);
+          wf[wf_index(c)] += film_fab_contribution<NF>(p, face_area);
+        }
+      }
     }
     const int64_t n = static_cast<int64_t>(parts.size());
     std::vector<double> contrib(static_cast<size_t>(n) * NSRC, 0.);
@@ -661,6 +718,8 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     for (int64_t pid = 0; pid < n; ++pid) {
       Particle<NF>& p = parts[pid];
       if (p.id <= 0) continue;
+      Carry<NF>& cy = carry[pid];
+      cy.parent_id = p.id;
       GasPhaseVals<NS, NF> gpv;
       double cBoilT[NF];
       GasProps<Mech>::molecular_weight(gpv.mw);
@@ -673,8 +732,8 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       containing_cell(p.pos, g, dxi, ijkc);
       // lx/ijk are in index space relative to the domain lower corner
       for (int d = 0; d < 3; ++d) ijk[d] += g.domain_lo[d];
-      int* bflags = &bfl[3 * pid];
-      if (at_bounds) {
+      int* bflags = cy.bflags;
+      if (at_bounds && cur_iter == 0) {
         if (check_bounds(p.pos, g, bflags)) {
           err_local = std::max(err_local, PELEMP_E_LEFT_DOMAIN);
           continue;
@@ -698,10 +757,15 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       gpv.define();
       calcBoilT<NS, NF>(fdat, gpv, cBoilT);
       if (is_film) {
-        err_local = std::max(err_local, PELEMP_E_BAD_CONFIG);  // film path not in this driver
-        continue;
+        calculateFilmSource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT);
+      } else {
+        cy.Reyn_d = calculateSpraySource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT);
       }
-      calculateSpraySource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT);
+      if (p.id > 0 && do_breakup) {
+        if (fdat.do_breakup == 1) cy.Utan_total += updateBreakupTAB<NS, NF>(cy.Reyn_d, sub_dt, cBoilT, gpv, fdat, p);
+        if (cur_iter == num_iter - 1 && fdat.do_breakup == 1 && make_new_drops)
+          splitDropletTAB<NF>(p, bp.max_ppp, cy.Utan_total, cy.rec);
+      }
       const double cur_coef = -inv_vol * sub_dt / flow_dt;
       if (!srcv.contains(ijkc)) {
         if (!isGhost) {
@@ -721,6 +785,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         dep_off[pid] = srcv.off(ijkc, 0);
       }
       if (do_move && !fdat.fixed_parts && p.id > 0 && !is_film) {
+        const double pos0[3] = {p.pos[0], p.pos[1], p.pos[2]};
         for (int dir = 0; dir < 3; ++dir) {
           const double cvel = p.rdata[C::pstateVel + dir];
           p.pos[dir] += sub_dt * cvel;
@@ -728,8 +793,18 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         if (at_bounds) {
           if (check_bounds(p.pos, g, bflags)) {
             p.id = -1;
+          } else {
+            // the film thickness array covers the domain cells; a droplet
+            // that started behind a wall sees none (the reference reads its
+            // ghost cell, which holds no film)
+            bool inside = true;
+            for (int d = 0; d < 3; ++d) {
+              const int ic = static_cast<int>(std::floor((pos0[d] - g.plo[d]) * dxi[d])) + g.domain_lo[d];
+              inside = inside && ic >= g.domain_lo[d] && ic <= g.domain_hi[d];
+            }
+            const double film_h = (do_splash_box && inside) ? wf[wf_index(ijkc)] : 0.;
+            impose_wall<NF>(do_splash_box, p, fdat, g, bflags, cBoilT, gpv.p_fluid, cy.rec, film_h);
           }
-          // reflective walls are handled by the wall/splash driver
         }
       }
       periodic_wrap(p.pos, g);
@@ -772,6 +847,18 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         *dst = *dst + acc;
       }
     }
+  }
+  if (status == PELEMP_OK && make_new_drops) {
+    // secondary droplets (CreateSBDroplets, SprayParticles.cpp:484-497),
+    // appended in parent order
+    std::vector<Particle<NF>> born;
+    for (size_t i = 0; i < parts.size(); ++i) {
+      if (carry[i].rec.type != SB_NONE)
+        create_sb_droplets<NF>(carry[i].rec, carry[i].parent_id, fdat, sub_dt, bp.ppp_fact, bp.seed, bp.next_id, born);
+    }
+    // the periodic shift of Redistribute applies to the new droplets too
+    for (Particle<NF>& c : born) periodic_wrap(c.pos, g);
+    parts.insert(parts.end(), born.begin(), born.end());
   }
   return status;
 }

@@ -101,9 +101,16 @@ class MlpTrainCfgC(ctypes.Structure):
     ]
 
 
+class SprayParamsC(ctypes.Structure):
+    _fields_ = [("max_parcel_size", _c_dbl), ("breakup_parcel_factor", _c_dbl), ("khrt_B0", _c_dbl),
+                ("khrt_B1", _c_dbl), ("khrt_C3", _c_dbl), ("rng_seed", ctypes.c_uint64)]
+
+
 _vp = ctypes.c_void_p
 _PNORM = ctypes.POINTER(SurrogateNormC)
 _PROTOS = {
+    "pelemp_spray_set_params": (_c_i32, [_vp, ctypes.POINTER(SprayParamsC)]),
+    "pelemp_spray_get_params": (_c_i32, [_vp, ctypes.POINTER(SprayParamsC)]),
     "pelemp_spray_create": (_c_i32, [ctypes.POINTER(_vp), _c_i32, _c_i32, ctypes.POINTER(SprayDataC),
                                      ctypes.POINTER(SprayCompsC), ctypes.POINTER(GeomC), _c_i64, _c_i32]),
     "pelemp_spray_destroy": (_c_i32, [_vp]),

@@ -42,6 +42,12 @@ PM_HD double pow(double x, double y) { return pm::pow(x, y); }
 PM_HD double cbrt(double x) { return pm::cbrt(x); }
 PM_HD double sqrt(double x) { return std::sqrt(x); }
 PM_HD double fabs(double x) { return std::fabs(x); }
+PM_HD double sin(double x) { return pm::sin(x); }
+PM_HD double cos(double x) { return pm::cos(x); }
+PM_HD double tan(double x) { return pm::tan(x); }
+PM_HD double asin(double x) { return pm::asin(x); }
+PM_HD double acos(double x) { return pm::acos(x); }
+PM_HD double atan2(double y, double x) { return pm::atan2(y, x); }
 
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline float exp(float x) { return __expf(x); }

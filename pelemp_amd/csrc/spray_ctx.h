@@ -21,6 +21,9 @@ struct PassParams {
   int do_move;
   int is_ghost;
   int at_bounds;
+  int branches;        // film / breakup / splash kernel variant
+  int make_new_drops;  // SprayParticles.cpp:298-299
+  int do_splash_box;   // splash on (walls present)
   const pelemp_fab_view* state;
   pelemp_fab_view* source;
 };
@@ -87,6 +90,21 @@ struct Ctx {
   static constexpr int kCountLog = 4096;
   int32_t* count_log = nullptr;
   int64_t npass = 0;
+
+  // branches (FP64 form with breakup / splash / films): container
+  // parameters, film thickness per cell, secondary-droplet records and
+  // the counters of new droplets (device scalars: next id, next slot)
+  pelemp_spray_params params{};
+  bool branches = false;
+  double* wf = nullptr;
+  int32_t* sb_type = nullptr;
+  double* sb_data = nullptr;
+  int32_t* sb_count = nullptr;   // children per position
+  int64_t* sb_offset = nullptr;  // their exclusive scan
+  int64_t* sb_block = nullptr;   // scan block sums over positions
+  int64_t* sb_ids = nullptr;    // [next id, next slot, children of the last pass]
+  int32_t* n_new = nullptr;     // entries after the children pass
+  int64_t slot_cap = 0;         // capacity of the cold (per-slot) arrays
 
   // surrogate form: packed network (SurrWeights), its normalisation, and the
   // droplet-step counters [surrogate, fallback]

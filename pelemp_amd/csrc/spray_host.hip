@@ -18,6 +18,7 @@
 #include "sort_kernels.h"
 #include "spray_ctx.h"
 #include "update_kernel.h"
+#include "branch_kernels.h"
 
 struct pelemp_spray_ctx {
   pelemp::Ctx c;
@@ -367,11 +368,79 @@ static int run_sort(Ctx& c, hipStream_t s) {
   return PELEMP_OK;
 }
 
+// Film thickness per cell from the film droplets of buffer cur.
+static int run_film_fab(Ctx& c, hipStream_t s) {
+  const double face_area = c.geom.dx[0] * c.geom.dx[1] * c.geom.dx[2];  // SprayParticles.cpp:281
+  const dim3 grid(static_cast<unsigned>((c.ncells + 255) / 256)), block(256);
+  auto run = [&](auto nft) {
+    constexpr int NF = decltype(nft)::value;
+    hipLaunchKernelGGL((film_fab_kernel<NF>), grid, block, 0, s, c.cstart, c.perm,
+                       static_cast<const double*>(c.fbuf[c.cur]), c.cap, c.ncell[0], c.ncell[1], c.ncell[2],
+                       c.ntile[0], c.ntile[1], face_area, c.wf);
+  };
+  switch (c.nf) {
+    case 1: run(std::integral_constant<int, 1>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 6: run(std::integral_constant<int, 6>{}); break;
+    default: set_last_error("unsupported fuel count"); return PELEMP_E_BAD_CONFIG;
+  }
+  PM_HIP(hipGetLastError());
+  return PELEMP_OK;
+}
+
+// Secondary droplets of the records the update pass wrote into buffer
+// 1 - cur: counts, exclusive scan, creation, counters.
+static int run_children(Ctx& c, const PassParams& p, hipStream_t s) {
+  const int32_t* n_ptr = c.cstart + c.ncells;  // entries written by the update pass
+  int32_t* count = c.sb_count;
+  int64_t* offset = c.sb_offset;
+  const unsigned nb = static_cast<unsigned>((c.cap + kScanBlock - 1) / kScanBlock);
+  SBGeom gm;
+  for (int d = 0; d < 3; ++d) {
+    gm.plo[d] = c.geom.plo[d];
+    gm.phi[d] = c.geom.phi[d];
+    gm.dxi[d] = 1. / c.geom.dx[d];
+    gm.bndry_lo[d] = c.geom.bndry_lo[d];
+    gm.ncell[d] = c.ncell[d];
+    gm.ntile[d] = c.ntile[d];
+  }
+  auto run = [&](auto tag, auto nft) {
+    using M = decltype(tag);
+    constexpr int NF = decltype(nft)::value;
+    hipLaunchKernelGGL((sb_count_kernel<NF>), dim3(2048), dim3(256), 0, s, n_ptr, c.sb_type, c.sb_data, c.cap,
+                       c.params.breakup_parcel_factor, count);
+    hipLaunchKernelGGL(sb_scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, count, offset, n_ptr, c.sb_block);
+    hipLaunchKernelGGL(sb_scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.sb_block, static_cast<int32_t>(nb));
+    hipLaunchKernelGGL(sb_scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, offset, n_ptr, c.sb_block);
+    hipLaunchKernelGGL((sb_create_kernel<NF>), dim3(2048), dim3(256), 0, s, n_ptr, c.sb_type, c.sb_data, offset,
+                       c.sb_ids, make_fuel_const<double, NF>(c.sd, c.mw.data()), p.sub_dt,
+                       c.params.breakup_parcel_factor, c.params.rng_seed, static_cast<double*>(c.fbuf[1 - c.cur]),
+                       c.cap, c.slot[1 - c.cur], c.cold_id, c.slot_cap, c.key, c.arrival, c.mov_hi, gm, c.err);
+    hipLaunchKernelGGL(sb_total_kernel, dim3(1), dim3(64), 0, s, n_ptr, offset, count, c.sb_ids, c.n_new, c.cap);
+    (void)sizeof(M);
+  };
+  if (c.mech == PELEMP_MECH_DODECANE_AIR && c.nf == 1) {
+    run(Mech_dodecane_air{}, std::integral_constant<int, 1>{});
+  } else if (c.mech == PELEMP_MECH_DECANE_DODECANE_AIR && c.nf == 2) {
+    run(Mech_decane_dodecane_air{}, std::integral_constant<int, 2>{});
+  } else if (c.mech == PELEMP_MECH_ALKANE6_AIR && c.nf == 6) {
+    run(Mech_alkane6_air{}, std::integral_constant<int, 6>{});
+  } else {
+    set_last_error("unsupported mechanism / fuel count combination");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  PM_HIP(hipGetLastError());
+  return PELEMP_OK;
+}
+
 // Re-sort after an update pass (incremental form, see sort_kernels.h).
-static int run_sort_incremental(Ctx& c, hipStream_t s) {
+// n_src: device count of entries in the new buffer when it differs from the
+// previous sort's count (secondary droplets appended).
+static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src = nullptr) {
   hipEvent_t ev0;
   ctx_time_begin(c, 1, s, &ev0);
-  PM_HIP(hipMemcpyAsync(c.n_entries, c.cstart + c.ncells, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  PM_HIP(hipMemcpyAsync(c.n_entries, n_src ? n_src : c.cstart + c.ncells, sizeof(int32_t),
+                        hipMemcpyDeviceToDevice, s));
   const unsigned gc = static_cast<unsigned>((c.ncells + 255) / 256);
   hipLaunchKernelGGL(sum3_kernel, dim3(gc), dim3(256), 0, s, c.mov_lo, c.stay_cnt, c.mov_hi, c.ncells, c.cnt);
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
@@ -401,7 +470,8 @@ static void free_ctx(Ctx& c) {
   void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.mbuf[0], c.mbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt,
                   c.arrival, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
-                  c.count_log, c.surr_w, c.surr_cnt};
+                  c.count_log, c.surr_w, c.surr_cnt, c.wf, c.sb_type, c.sb_data, c.sb_count, c.sb_offset,
+                  c.sb_block, c.sb_ids, c.n_new};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -495,8 +565,20 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
       return PELEMP_E_BAD_CONFIG;
     }
   }
-  if (sd->do_breakup != 0 || sd->do_splash != 0) {
-    set_last_error("breakup / splash are not enabled in this build of the update");
+  if (sd->do_breakup == 2) {
+    set_last_error("KH-RT breakup (do_breakup = 2) is not enabled in this build of the update");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  if ((sd->do_breakup != 0 || sd->do_splash != 0) && form != PELEMP_FORM_FP64) {
+    set_last_error("breakup / splash / wall films are supported by the FP64 form only");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  if (sd->do_splash != 0 && !(sd->sigma > 0. && sd->wall_T > 0. && sd->theta_c > 0.)) {
+    set_last_error("splash needs fuel_sigma, wall_temp and contact_angle");  // SpraySetup.cpp:157-183
+    return PELEMP_E_BAD_CONFIG;
+  }
+  if (sd->do_breakup == 1 && !(sd->sigma > 0.)) {
+    set_last_error("breakup needs fuel_sigma");  // SpraySetup.cpp:157-165
     return PELEMP_E_BAD_CONFIG;
   }
   auto* h = new pelemp_spray_ctx();
@@ -527,7 +609,20 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.ncells = c.ncell[0] * c.ncell[1] * c.ncell[2];
   c.ntiles = c.ntile[0] * c.ntile[1] * c.ntile[2];
   c.cap = max_particles;
-  c.hot_brk = false;
+  // the branch kernel (films, TAB, wall reflection / splash) runs when
+  // breakup or splash is on or a face is a reflective wall; the breakup
+  // state is then carried in the hot fields
+  bool reflective = false;
+  for (int d = 0; d < 3; ++d) reflective = reflective || geom->bndry_lo[d] == 1 || geom->bndry_hi[d] == 1;
+  c.branches = (form == PELEMP_FORM_FP64) && (sd->do_breakup == 1 || sd->do_splash != 0 || reflective);
+  c.hot_brk = c.branches;
+  c.slot_cap = c.branches ? 4 * c.cap : c.cap;
+  c.params.max_parcel_size = 100.;
+  c.params.breakup_parcel_factor = 0.5;
+  c.params.khrt_B0 = 0.61;
+  c.params.khrt_B1 = 7.;
+  c.params.khrt_C3 = 1.;
+  c.params.rng_seed = 0;
   if (form == PELEMP_FORM_FP64) {
     c.nhot = c.hot_brk ? (13 + c.nf) : (10 + c.nf);
     c.wsize = 8;
@@ -556,7 +651,17 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
               4 * (static_cast<size_t>(c.ncells) / kScanBlock + 2));
   st |= alloc(c, reinterpret_cast<void**>(&c.scalars), 64);
   st |= alloc(c, reinterpret_cast<void**>(&c.dscratch), 64);
-  st |= alloc(c, reinterpret_cast<void**>(&c.cold_id), 8 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.cold_id), 8 * c.slot_cap);
+  if (c.branches) {
+    st |= alloc(c, reinterpret_cast<void**>(&c.wf), 8 * static_cast<size_t>(c.ncells));
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_type), 4 * c.cap);
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_data), 8 * static_cast<size_t>(kSBFields + c.nf) * c.cap);
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_count), 4 * static_cast<size_t>(c.cap));
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_offset), 8 * static_cast<size_t>(c.cap));
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_block), 8 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_ids), 8 * 4);
+    st |= alloc(c, reinterpret_cast<void**>(&c.n_new), 64);
+  }
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_n0), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm1), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm2), 8 * c.cap);
@@ -680,6 +785,14 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
     PM_HIP(hipGetLastError());
   }
   c.nslots = m;
+  if (c.branches) {
+    // counters of new droplets: next id (past the largest id handed in),
+    // next slot, children of the last pass
+    int64_t max_id = 0;
+    for (int64_t i = 0; i < m; ++i) max_id = std::max(max_id, hid[i]);
+    const int64_t ids[4] = {max_id + 1, m, 0, 0};
+    PM_HIP(hipMemcpy(c.sb_ids, ids, sizeof(ids), hipMemcpyHostToDevice));
+  }
   // the sort reads the entry count from cstart[ncells]
   const int32_t m32 = static_cast<int32_t>(m);
   PM_HIP(hipMemcpy(c.cstart + c.ncells, &m32, 4, hipMemcpyHostToDevice));
@@ -797,6 +910,17 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
     if (c.geom.bndry_lo[d] != 0 || c.geom.bndry_hi[d] != 0) p.at_bounds = 1;
   p.state = state;
   p.source = source;
+  // branch flags (SprayParticles.cpp:162-164,289-299); the splash model
+  // runs where walls are (tile_at_bndry, WallFunctions.H:7-35)
+  const bool isActive = !(isVirt || isGhost);
+  const bool do_splash = c.sd.do_splash != 0 && isActive && do_move;
+  p.branches = c.branches ? 1 : 0;
+  p.do_splash_box = (do_splash && p.at_bounds) ? 1 : 0;
+  p.make_new_drops = ((c.sd.do_breakup > 0 || p.do_splash_box) && isActive && do_move) ? 1 : 0;
+  if (c.branches && num_iter > 1) {
+    set_last_error("breakup / splash / films need a single particle sub-cycle (spray_cfl_lev <= 0.5)");
+    return PELEMP_E_BAD_CONFIG;
+  }
   for (int it = 0; it < num_iter; ++it) {
     if (c.npass < Ctx::kCountLog) {
       PM_HIP(hipMemcpyAsync(c.count_log + c.npass, c.cstart + c.ncells, sizeof(int32_t),
@@ -804,6 +928,13 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
     }
     ++c.npass;
     PM_HIP(hipMemsetAsync(c.mov_lo, 0, 3 * 4 * static_cast<size_t>(c.ncells), s));
+    if (p.do_splash_box) {
+      hipEvent_t evf;
+      ctx_time_begin(c, 2, s, &evf);
+      const int st = run_film_fab(c, s);
+      if (st != PELEMP_OK) return st;
+      ctx_time_end(c, 2, s, evf);
+    }
     hipEvent_t ev0;
     ctx_time_begin(c, 0, s, &ev0);
     const int st = (c.form == PELEMP_FORM_FP64)   ? launch_update_fp64(c, p, s)
@@ -811,8 +942,15 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
                                                   : launch_update_surr(c, p, s);
     if (st != PELEMP_OK) return st;
     ctx_time_end(c, 0, s, ev0);
+    if (p.make_new_drops) {
+      hipEvent_t evc;
+      ctx_time_begin(c, 3, s, &evc);
+      const int stc = run_children(c, p, s);
+      if (stc != PELEMP_OK) return stc;
+      ctx_time_end(c, 3, s, evc);
+    }
     c.cur = 1 - c.cur;
-    const int st2 = run_sort_incremental(c, s);
+    const int st2 = run_sort_incremental(c, s, p.make_new_drops ? c.n_new : nullptr);
     if (st2 != PELEMP_OK) return st2;
   }
   return PELEMP_OK;
@@ -823,6 +961,29 @@ int pelemp_spray_move_kick_drift(pelemp_spray_ctx* h, const pelemp_fab_view* sta
                                  int32_t source_ghosts, int32_t do_move, double spray_cfl_lev, void* stream) {
   return pelemp_spray_update(h, state, source, dt, time, state_ghosts, source_ghosts, isVirt, isGhost, do_move,
                              spray_cfl_lev, stream);
+}
+
+int pelemp_spray_set_params(pelemp_spray_ctx* h, const pelemp_spray_params* p) {
+  if (!h || !p) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  // SpraySetup.cpp:144-160
+  if (!(p->max_parcel_size > 0.) || p->breakup_parcel_factor < 0. || p->breakup_parcel_factor > 1.) {
+    set_last_error("max_parcel_size must be positive and breakup_parcel_factor in [0, 1]");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  h->c.params = *p;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_get_params(const pelemp_spray_ctx* h, pelemp_spray_params* out) {
+  if (!h || !out) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  *out = h->c.params;
+  return PELEMP_OK;
 }
 
 int pelemp_spray_set_surrogate(pelemp_spray_ctx* h, const float* params_host, const pelemp_surrogate_norm* norm) {

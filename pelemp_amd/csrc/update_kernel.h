@@ -34,6 +34,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "branches.h"
 #include "droplet_physics.h"
 #include "layout.h"
 #include "surrogate_mlp.h"
@@ -98,7 +99,20 @@ struct UpdateArgs {
   SurrKNorm sn;
   pelemp_surrogate_norm hull;
   unsigned long long* surr_cnt;  // [surrogate, fallback] droplet-steps
+  // branches (FP64 form): films, TAB, walls/splash, secondary-droplet records
+  LiquidExtra lx;
+  const double* __restrict__ wf;  // film thickness per cell (splash)
+  int32_t* __restrict__ sb_type;  // record type per output position
+  double* __restrict__ sb_data;   // record fields [kSBFields + NF][cap]
+  int do_tab;                     // do_breakup == 1
+  int make_new_drops;
+  int do_splash_box;
+  double max_ppp;
 };
+
+// fields of a secondary-droplet record (SBData.H:7-19): norm(3) vel(3)
+// loc(3) T0 ref_dia num_dens phi1 phi2 phi3, then Y0[NF]
+constexpr int kSBFields = 15;
 
 __device__ inline int xcd_tile(int b, int ntiles) {
   // gfx950 dispatches workgroup b to XCD b % 8; give each XCD a contiguous
@@ -152,6 +166,7 @@ struct DropLocal {
   Drop<R, NF> d;
   R pos[3];
   R film;
+  R bm2, n0;      // breakup state (FP64 form with breakup / splash)
   double mk[NF];  // FP32 form: liquid species masses
   GasAt<R, M::NS> g;
   R cBoilT[NF];
@@ -159,7 +174,7 @@ struct DropLocal {
 };
 
 // Load droplet s of the tile and gather its gas state.
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false>
 __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateArgs<R, NF>& a, const R* gas,
                                                                   const unsigned char* gvalid, const int32_t* cs,
                                                                   int otx, int oty, int otz, int32_t s,
@@ -193,6 +208,11 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
     for (int spf = 0; spf < NF; ++spf) d.Y[spf] = a.fin[(F::Y + spf) * cap + o];
     d.numDens = a.fin[F::ND * cap + o];
     L.film = a.fin[F::FILM * cap + o];
+    if constexpr (BR) {
+      d.bm1 = a.fin[F::BM1 * cap + o];
+      L.bm2 = a.fin[F::BM2 * cap + o];
+      L.n0 = a.fin[F::N0 * cap + o];
+    }
   } else {
     using G = Fields32;
 #pragma unroll
@@ -215,7 +235,7 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
     d.mass = static_cast<R>(m);
     d.dia = R(0);
   }
-  d.bm1 = R(0);
+  if constexpr (!BR) d.bm1 = R(0);
   d.alive = true;
 
   // trilinear corner cell and weights (SprayInterpolation.H:158-199)
@@ -278,6 +298,31 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
         if (cw > R(0.)) {
           int hx = ijk[0] + ii - otx + 1, hy = ijk[1] + jj - oty + 1, hz = ijk[2] + kk - otz + 1;
           const bool inr = hx >= 0 && hx < kHaloW && hy >= 0 && hy < kHaloW && hz >= 0 && hz < kHaloW;
+          if constexpr (AT_BOUNDS) {
+            // a droplet more than half a cell behind a reflective wall (kept
+            // in the boundary cell) reaches past the halo: read the state's
+            // ghost cells directly, with the staging's arithmetic
+            if (!inr) {
+              const int lx = a.dom_lo[0] + ijk[0] + ii - a.st.lo[0];
+              const int ly = a.dom_lo[1] + ijk[1] + jj - a.st.lo[1];
+              const int lz = a.dom_lo[2] + ijk[2] + kk - a.st.lo[2];
+              if (!(lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2])) {
+                set_err(a.err, PELEMP_E_STATE_BOX);
+                continue;
+              }
+              const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
+              const double* S = a.st.data;
+              const double rho = S[a.c_rho * a.st.cstride + c];
+              const double inv_rho = 1. / rho;
+              g.rho += cw * static_cast<R>(rho);
+#pragma unroll
+              for (int n = 0; n < NS; ++n) g.Y[n] += cw * static_cast<R>(S[(a.c_spec + n) * a.st.cstride + c] * inv_rho);
+#pragma unroll
+              for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
+              g.T += cw * static_cast<R>(S[a.c_T * a.st.cstride + c]);
+              continue;
+            }
+          }
           hx = inr ? hx : 0;
           hy = inr ? hy : 0;
           hz = inr ? hz : 0;
@@ -437,6 +482,151 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
   }
 }
 
+// FP64 form with the branches (SprayParticles.cpp:354-479): film source or
+// droplet source, TAB update and split, deposition, drift, wall reflection /
+// splash, store of the breakup state and of the secondary-droplet record.
+template <class M, int NF, bool AT_BOUNDS>
+__device__ __attribute__((always_inline)) inline void drop_update_br(const UpdateArgs<double, NF>& a, int tile,
+                                                                     int t, int32_t s, DropLocal<M, NF, double>& L,
+                                                                     double* contrib, unsigned char* stayed) {
+  constexpr int NSRC = 5 + NF;
+  using F = Fields<NF>;
+  const long long cap = a.cap;
+  Drop<double, NF>& d = L.d;
+  const bool is_film = L.film > 0.;
+  BDrop<NF> b;
+  b.alive = true;
+  b.film = L.film;
+  b.numDens = d.numDens;
+  b.N0 = L.n0;
+  b.bm2 = L.bm2;
+  SBRec<NF> rec;
+  rec.type = kSBNone;
+  DropSrc<double, NF> sr;
+  double Reyn_d = 0.;
+  if (is_film) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      b.pos[q] = L.pos[q];
+      b.vel[q] = d.vel[q];
+    }
+    b.T = d.T;
+    b.dia = d.dia;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) b.Y[spf] = d.Y[spf];
+    b.bm1 = d.bm1;
+    film_source<M, NF>(a.sub_dt, L.g, a.fc, a.lx, b, L.cBoilT, sr);
+  } else {
+    Reyn_d = spray_source<M, NF, double, false>(a.sub_dt, L.g, a.fc, d, L.cBoilT, sr);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      b.pos[q] = L.pos[q];
+      b.vel[q] = d.vel[q];
+    }
+    b.T = d.T;
+    b.dia = d.dia;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) b.Y[spf] = d.Y[spf];
+    b.bm1 = d.bm1;
+    b.alive = d.alive;
+  }
+  if (b.alive && a.do_tab) {
+    const double Utan = tab_update<M::NS, NF>(Reyn_d, a.sub_dt, L.cBoilT, L.g, a.fc, a.lx, b);
+    if (a.make_new_drops) tab_split<NF>(b, a.max_ppp, Utan, rec);
+  }
+  // contributions cur_coef * source (SprayParticles.cpp:419-441)
+  double* cb = &contrib[t * NSRC];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) cb[q] = a.cur_coef * sr.mom[q];
+  cb[3] = a.cur_coef * sr.mass;
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = a.cur_coef * sr.Ydot[spf];
+  cb[4 + NF] = a.cur_coef * sr.eng;
+
+  // drift, bounds and walls (SprayParticles.cpp:443-479), periodic wrap
+  if (a.do_move && !a.fc.fixed_parts && b.alive && !is_film) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b.pos[q] += a.sub_dt * b.vel[q];
+    if constexpr (AT_BOUNDS) {
+      if (dev_check_bounds(b.pos, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, L.bflags)) {
+        b.alive = false;
+      } else {
+        // film thickness of the cell the droplet started in; a droplet that
+        // started behind a wall sees none (the reference reads its ghost
+        // cell, which holds no film)
+        bool inside = true;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int ic = static_cast<int>(floor((L.pos[q] - a.plo[q]) * a.dxi[q]));
+          inside = inside && ic >= 0 && ic < a.ncell[q];
+        }
+        const double film_h = (a.do_splash_box && inside)
+                                  ? a.wf[(static_cast<long long>(L.cz) * a.ncell[1] + L.cy) * a.ncell[0] + L.cx]
+                                  : 0.;
+        impose_wall<NF>(a.do_splash_box != 0, b, a.fc, a.lx, a.plo, a.dx, L.bflags, L.cBoilT, L.g.p, rec, film_h);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (a.bndry_lo[q] == 0) b.pos[q] = wrap_periodic(b.pos[q], a.plo[q], a.phi[q]);
+  uint32_t key = kDeadKey;
+  if (b.alive) {
+    int c[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int ic = static_cast<int>(floor((b.pos[q] - a.plo[q]) * a.dxi[q]));
+      ic = ic < 0 ? 0 : (ic >= a.ncell[q] ? a.ncell[q] - 1 : ic);
+      c[q] = ic;
+    }
+    key = tile_key(c[0], c[1], c[2], a.ntile);
+  }
+  double* fo = a.fout;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    fo[(F::POS + q) * cap + s] = b.pos[q];
+    fo[(F::VEL + q) * cap + s] = b.vel[q];
+  }
+  fo[F::T * cap + s] = b.T;
+  fo[F::DIA * cap + s] = b.dia;
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) fo[(F::Y + spf) * cap + s] = b.Y[spf];
+  fo[F::ND * cap + s] = b.numDens;
+  fo[F::FILM * cap + s] = b.film;
+  fo[F::BM1 * cap + s] = b.bm1;
+  fo[F::BM2 * cap + s] = b.bm2;
+  fo[F::N0 * cap + s] = b.N0;
+  // secondary-droplet record of this position
+  a.sb_type[s] = rec.type;
+  if (rec.type != kSBNone) {
+    double* r = a.sb_data;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      r[(0 + q) * cap + s] = rec.norm[q];
+      r[(3 + q) * cap + s] = rec.vel[q];
+      r[(6 + q) * cap + s] = rec.loc[q];
+    }
+    r[9 * cap + s] = rec.T0;
+    r[10 * cap + s] = rec.ref_dia;
+    r[11 * cap + s] = rec.num_dens;
+    r[12 * cap + s] = rec.phi1;
+    r[13 * cap + s] = rec.phi2;
+    r[14 * cap + s] = rec.phi3;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) r[(kSBFields + spf) * cap + s] = rec.Y0[spf];
+  }
+  a.slot_out[s] = a.slot_in[L.o];
+  a.key_out[s] = key;
+  // classify for the incremental re-sort
+  const uint32_t k_old = static_cast<uint32_t>(tile) * kTileCells + L.lc;
+  stayed[t] = (key == k_old) ? 1 : 0;
+  if (key != kDeadKey && key != k_old) {
+    const int32_t arr = (k_old < key) ? (atomicAdd(&a.mov_lo[key], 1) | (1 << 30))
+                                      : (atomicAdd(&a.mov_hi[key], 1) | (2 << 30));
+    a.arrival_out[s] = arr;
+  }
+}
+
 // Surrogate stage of one chunk: hull test, network (whole wavefront) and
 // predicted rates.  Returns whether this lane's droplet uses them.
 template <class M, int NF>
@@ -496,7 +686,7 @@ __device__ __attribute__((always_inline)) inline bool surr_stage_drop(const Upda
   return use;
 }
 
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false>
 __device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, R* gas,
                                                                    unsigned char* gvalid, int32_t* cs,
                                                                    double* contrib, unsigned char* stayed,
@@ -554,7 +744,13 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 
   for (int32_t base = d0; base < d1; base += kUpdThreads) {
     const int32_t s = base + t;
-    if constexpr (!SURR) {
+    if constexpr (BR) {
+      if (s < d1) {
+        DropLocal<M, NF, R> L;
+        drop_gather<M, NF, R, true, AT_BOUNDS, true>(a, gas, gvalid, cs, otx, oty, otz, s, L);
+        drop_update_br<M, NF, AT_BOUNDS>(a, tile, t, s, L, contrib, stayed);
+      }
+    } else if constexpr (!SURR) {
       if (s < d1) {
         DropLocal<M, NF, R> L;
         drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, L);
@@ -628,7 +824,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
   if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
 }
 
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false>
 __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NG = 5 + M::NS;  // staged gas fields: rho, Y[NS], vel[3], T
@@ -640,8 +836,8 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
   __shared__ unsigned char stayed[kUpdThreads];
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
-  process_tile<M, NF, R, POS64, AT_BOUNDS, false>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib, stayed,
-                                                  nullptr, n_surr, n_fb);
+  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib,
+                                                      stayed, nullptr, n_surr, n_fb);
 }
 
 // Surrogate form: gridDim.x workgroups (a multiple of 8) loop over the

@@ -86,6 +86,31 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
     // two workgroups per CU on 256 CUs, each looping over tiles
     const int nwg = c.ntiles < 512 ? c.ntiles : 512;
     hipLaunchKernelGGL((spray_update_surr_kernel<M, NF>), dim3(nwg), block, 0, s, a);
+  } else if (p.branches) {
+    if constexpr (POS64) {
+      for (int q = 0; q < 4 * PELEMP_MAX_FUEL; ++q) {
+        a.lx.lambda_coef[q] = c.sd.lambda_coef[q];
+        a.lx.mu_coef[q] = c.sd.mu_coef[q];
+      }
+      a.lx.sigma = c.sd.sigma;
+      a.lx.wall_T = c.sd.wall_T;
+      a.lx.theta_c = c.sd.theta_c;
+      a.wf = c.wf;
+      a.sb_type = c.sb_type;
+      a.sb_data = c.sb_data;
+      a.do_tab = (c.sd.do_breakup == 1) ? 1 : 0;
+      a.make_new_drops = p.make_new_drops;
+      a.do_splash_box = p.do_splash_box;
+      a.max_ppp = c.params.max_parcel_size;
+      if (p.at_bounds) {
+        hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true>), grid, block, 0, s, a);
+      } else {
+        hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true>), grid, block, 0, s, a);
+      }
+    } else {
+      set_last_error("breakup, splash and wall films are supported by the FP64 form only");
+      return PELEMP_E_BAD_CONFIG;
+    }
   } else if (p.at_bounds) {
     if constexpr (POS64) {
       hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, true>), grid, block, 0, s, a);

@@ -182,6 +182,80 @@ def build_problem(name: str, *, ncell=(32, 32, 32), dx=1.0e-4, p=PATM, T0=1000.0
                         meta=dict(p=p, T0=T0, U0=U0, seed=seed))
 
 
+def config3(ncell=(128, 128, 256), drops_per_cell: float = 3.0e7 / (128 * 128 * 256), z_frac: float = 1.0,
+            T0: float = 600.0, p: float = 5.0 * PATM, wall_T: float = 450.0, theta_c_deg: float = 30.0,
+            num_dens: float = 50.0, du_max: float = 150.0, seed: int = 777, flow_dt: float = 5e-7,
+            nsteps: int = 100) -> SprayProblem:
+    """Config 3: slab with a reflective wall at z = 0 (isothermal, wall_T =
+    450 K), x/y periodic, outflow at z_hi; gas at temperature T0 (600 K) and
+    pressure p (5 atm; both unspecified by the configuration and recorded
+    here) moving at -20 m/s toward the wall; n-dodecane droplets with
+    lognormal diameters (median 40 um, sigma_ln 0.6) whose relative velocity
+    to the gas gives Weber numbers log-uniform in [1, 300], capped at du_max
+    (150 m/s, so a droplet crosses less than one cell per step: the wall
+    check of the method assumes it, WallFunctions.H:110-120), directed toward
+    the wall (uniform on the lower hemisphere).  TAB breakup,
+    splash/deposition and wall films on; contact angle theta_c_deg (30 deg,
+    recorded); parcels of num_dens droplets.  `z_frac` < 1 places the
+    droplets in the lowest fraction of the slab."""
+    mech = C.MECH_DODECANE_AIR
+    info = mech_info(mech)
+    species = info["species"]
+    W = np.array(info["W"])
+    ns, nf = len(species), 1
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = ncell
+    dx = 1.0e-4
+    L = (nx * dx, ny * dx, nz * dx)
+    Ygas = np.zeros(ns)
+    Ygas[species.index("O2")] = 0.233
+    Ygas[species.index("N2")] = 0.767
+    wbar = 1.0 / np.sum(Ygas / W)
+    rho = p * wbar / (RU * T0)
+    ug = np.array([0.0, 0.0, -20.0])
+    comps = state_comps(ns, nf)
+    g = 2
+    shape = (6 + ns, nz + 2 * g, ny + 2 * g, nx + 2 * g)
+    state = np.zeros(shape)
+    state[comps["rhoIndx"]] = rho
+    for d in range(3):
+        state[comps["momIndx"] + d] = ug[d]
+    for k in range(ns):
+        state[comps["specIndx"] + k] = rho * Ygas[k]
+    state[comps["utempIndx"]] = T0
+    sg = 1
+    n = int(round(drops_per_cell * nx * ny * nz * z_frac))
+    pos = rng.random((n, 3)) * np.array([L[0], L[1], L[2] * z_frac])
+    pos = np.minimum(pos, np.nextafter(np.array(L), 0.0))
+    dia = 40e-6 * np.exp(0.6 * rng.standard_normal(n))
+    sdata = spray_data(["NC12H26"], species, do_breakup=1, do_splash=True, wall_T=wall_T, theta_c_deg=theta_c_deg)
+    We = np.exp(rng.uniform(0.0, math.log(300.0), n))
+    du = np.minimum(np.sqrt(We * sdata["sigma"] / (rho * dia)), du_max)
+    # direction uniform on the hemisphere pointing to the wall
+    v = rng.standard_normal((n, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    v[:, 2] = -np.abs(v[:, 2])
+    vel = ug[None, :] + du[:, None] * v
+    rdata = np.zeros((n, 10 + nf))
+    rdata[:, 0:3] = vel
+    rdata[:, 3] = 300.0
+    rdata[:, 4] = dia
+    rdata[:, 5] = 1.0
+    rdata[:, 5 + nf] = num_dens
+    rdata[:, 6 + nf] = num_dens
+    geom = dict(plo=[0.0, 0.0, 0.0], phi=list(L), dx=[dx, dx, dx], domain_lo=[0, 0, 0],
+                domain_hi=[nx - 1, ny - 1, nz - 1], bndry_lo=[0, 0, 1], bndry_hi=[0, 0, -1])
+    return SprayProblem(name="config3", mech=mech, species=species, W=W, nf=nf, geom=geom, comps=comps,
+                        sdata=sdata, state=np.ascontiguousarray(state), state_lo=(-g, -g, -g),
+                        state_hi=(nx - 1 + g, ny - 1 + g, nz - 1 + g), src_shape=(5 + nf, nz + 2 * sg, ny + 2 * sg,
+                                                                                   nx + 2 * sg),
+                        src_lo=(-sg, -sg, -sg), src_hi=(nx - 1 + sg, ny - 1 + sg, nz - 1 + sg),
+                        pos=np.ascontiguousarray(pos), rdata=np.ascontiguousarray(rdata),
+                        ids=np.arange(1, n + 1, dtype=np.int64), flow_dt=flow_dt, nsteps=nsteps, state_ghosts=g,
+                        source_ghosts=sg, meta=dict(p=p, T0=T0, wall_T=wall_T, theta_c_deg=theta_c_deg,
+                                                    num_dens=num_dens, seed=seed))
+
+
 def config1(scale: float = 1.0, seed: int = 1234, **kw) -> SprayProblem:
     """Config 1: 32^3 periodic box, dx = 1e-4 m, air at 1 atm, T0 = 1000 K,
     1e5 n-dodecane droplets (median 20 um, sigma_ln 0.4), 100 steps of 1e-6 s.

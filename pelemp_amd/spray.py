@@ -114,6 +114,17 @@ class SprayParticleContainer:
     def getSprayComps(self) -> dict:
         return dict(self.comps)
 
+    def set_params(self, max_parcel_size: float = 100.0, breakup_parcel_factor: float = 0.5,
+                   rng_seed: int = 0) -> None:
+        """Container parameters of the breakup / splash models (max_parcel_size,
+        breakup_parcel_factor; SpraySetup.cpp:15-19,144-160)."""
+        p = C.SprayParamsC()
+        C.check(self._lib.pelemp_spray_get_params(self._h, ctypes.byref(p)), "pelemp_spray_get_params")
+        p.max_parcel_size = float(max_parcel_size)
+        p.breakup_parcel_factor = float(breakup_parcel_factor)
+        p.rng_seed = int(rng_seed)
+        C.check(self._lib.pelemp_spray_set_params(self._h, ctypes.byref(p)), "pelemp_spray_set_params")
+
     def set_particles(self, pos: np.ndarray, rdata: np.ndarray, ids: np.ndarray) -> None:
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         rdata = np.ascontiguousarray(rdata, dtype=np.float64)
