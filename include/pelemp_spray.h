@@ -44,6 +44,8 @@ extern "C" {
 #define PELEMP_E_HIP 5         /* HIP runtime error */
 #define PELEMP_E_ARG 6         /* invalid argument / size */
 #define PELEMP_E_NOMEM 7       /* device allocation failed */
+#define PELEMP_E_BAD_CHILD 8   /* SpraySB.cpp:187-190: a secondary droplet
+                                  without a valid position */
 
 /* Mirror of SprayData (SprayFuelData.H:115-212), fixed at PELEMP_MAX_FUEL
  * fuel slots; the first nfuel are used. */
@@ -135,8 +137,12 @@ int pelemp_spray_get_data(const pelemp_spray_ctx* ctx, pelemp_spray_data* out);
 typedef struct pelemp_spray_params {
   double max_parcel_size;       /* max_parcel_size, default 100 */
   double breakup_parcel_factor; /* breakup_parcel_factor, default 0.5 */
-  double khrt_B0, khrt_B1, khrt_C3; /* 0.61, 7, 1 (KH-RT, not enabled) */
+  double khrt_B0, khrt_B1, khrt_C3; /* KHRT_B0/B1/C3, defaults 0.61, 7, 1 */
   uint64_t rng_seed;            /* default 0 */
+  /* KH-RT: mean injected parcel mass sum_jets N d^3 / n_jets, which the
+   * reference takes from its spray jets (SprayParticles.cpp:230-239);
+   * default 0, as with no jets */
+  double avg_inject_mass;
 } pelemp_spray_params;
 int pelemp_spray_set_params(pelemp_spray_ctx* ctx, const pelemp_spray_params* p);
 int pelemp_spray_get_params(const pelemp_spray_ctx* ctx, pelemp_spray_params* out);

@@ -17,6 +17,10 @@ extern "C" struct oracle_branch_params {
   uint64_t seed;
   int64_t next_id;
   int64_t cap;
+  double khrt_B0;
+  double khrt_B1;
+  double khrt_C3;
+  double avg_inject_mass;
 };
 
 namespace {
@@ -47,6 +51,10 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
     bp.ppp_fact = bpp->ppp_fact;
     bp.seed = bpp->seed;
     bp.next_id = std::max(bpp->next_id, max_id + 1);
+    bp.khrt_B0 = bpp->khrt_B0;
+    bp.khrt_B1 = bpp->khrt_B1;
+    bp.khrt_C3 = bpp->khrt_C3;
+    bp.avg_inject_mass = bpp->avg_inject_mass;
     cap = bpp->cap;
   } else {
     bp.next_id = max_id + 1;
@@ -85,7 +93,6 @@ int oracle_update(int mech, int nf, const pelemp_spray_data* sd, const pelemp_sp
                   int isVirt, int isGhost, int do_move, double spray_cfl, int nthreads,
                   oracle_branch_params* bp, int64_t* n_out) {
   if (sd->nfuel != nf) return PELEMP_E_BAD_CONFIG;
-  if (sd->do_breakup == 2) return PELEMP_E_BAD_CONFIG;  // KH-RT is not restated
   if (mech == PELEMP_MECH_DODECANE_AIR && nf == 1)
     return run_update<pelemp::Mech_dodecane_air, 1>(sd, comps, g, n, pos, rdata, id, state, source,
                                                     flow_dt, isVirt, isGhost, do_move, spray_cfl,
@@ -250,6 +257,45 @@ double oracle_tab_update(const pelemp_spray_data* sd, double Reyn, double dt, co
   for (int c = 0; c < oracle::Comps<1>::pstateNum; ++c) rdata[c] = p.rdata[c];
   cboil_out[0] = cB[0];
   return U;
+}
+
+// updateBreakupKHRT for one n-dodecane droplet (mechanism 0); kp = [B0, B1,
+// C3, avg_inject_mass], pos[3] in/out of the record; out = [record type,
+// loc 3, vel 3, norm 3, num_dens, ref_dia, T0, Utan]; returns the id
+// (1, or -1 when the parcel was replaced by its children).
+int oracle_khrt_update(const pelemp_spray_data* sd, double Reyn, double dt, const double* gas, const double* kp,
+                       int make_new_drops, const double* pos, double* rdata, double* out, double* cboil_out) {
+  using M = pelemp::Mech_dodecane_air;
+  constexpr int NS = M::NS;
+  oracle::GasPhaseVals<NS, 1> gpv;
+  gpv.reset();
+  pelemp::GasProps<M>::molecular_weight(gpv.mw);
+  for (int d = 0; d < 3; ++d) gpv.vel_fluid[d] = gas[d];
+  gpv.T_fluid = gas[3];
+  gpv.rho_fluid = gas[4];
+  for (int n = 0; n < NS; ++n) gpv.Y_fluid[n] = gas[5 + n];
+  gpv.define();
+  double cB[1];
+  oracle::calcBoilT<NS, 1>(*sd, gpv, cB);
+  oracle::Particle<1> p;
+  for (int c = 0; c < oracle::Comps<1>::pstateNum; ++c) p.rdata[c] = rdata[c];
+  p.id = 1;
+  for (int d = 0; d < 3; ++d) p.pos[d] = pos[d];
+  oracle::SBRecord<1> rf;
+  oracle::updateBreakupKHRT<NS, 1>(p, Reyn, dt, cB, kp[3], kp[0], kp[1], kp[2], gpv, *sd, rf, make_new_drops != 0);
+  for (int c = 0; c < oracle::Comps<1>::pstateNum; ++c) rdata[c] = p.rdata[c];
+  out[0] = rf.type;
+  for (int d = 0; d < 3; ++d) {
+    out[1 + d] = rf.loc[d];
+    out[4 + d] = rf.vel[d];
+    out[7 + d] = rf.norm[d];
+  }
+  out[10] = rf.num_dens;
+  out[11] = rf.ref_dia;
+  out[12] = rf.T0;
+  out[13] = rf.phi1;
+  cboil_out[0] = cB[0];
+  return static_cast<int>(p.id);
 }
 
 // splash_criteria (AhamedSplash.H:125-197): returns the splash_type;

@@ -27,7 +27,8 @@ def build() -> str:
 class BranchParamsC(ctypes.Structure):
     """oracle_branch_params (oracle_capi.cpp)."""
     _fields_ = [("max_ppp", ctypes.c_double), ("ppp_fact", ctypes.c_double), ("seed", ctypes.c_uint64),
-                ("next_id", ctypes.c_int64), ("cap", ctypes.c_int64)]
+                ("next_id", ctypes.c_int64), ("cap", ctypes.c_int64), ("khrt_B0", ctypes.c_double),
+                ("khrt_B1", ctypes.c_double), ("khrt_C3", ctypes.c_double), ("avg_inject_mass", ctypes.c_double)]
 
 
 def lib() -> ctypes.CDLL:
@@ -65,6 +66,9 @@ def lib() -> ctypes.CDLL:
                                               ctypes.c_double, vp, vp, vp, vp, vp]
         L.oracle_tab_update.restype = ctypes.c_double
         L.oracle_tab_update.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_double, vp, vp, vp]
+        L.oracle_khrt_update.restype = ctypes.c_int
+        L.oracle_khrt_update.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_double, vp, vp,
+                                         ctypes.c_int, vp, vp, vp, vp]
         L.oracle_splash_criteria.restype = ctypes.c_int
         L.oracle_splash_criteria.argtypes = [ctypes.c_double] * 5 + [vp]
         L.oracle_num_threads.restype = ctypes.c_int
@@ -106,13 +110,16 @@ def update(mech: int, sdata: dict, comps: dict, geom: dict, pos, rdata, ids, sta
     """One updateParticles call on host arrays; `source` is updated in place.
     Returns (status, pos, rdata, ids) with the surviving droplets in cell order
     (then the secondary droplets created by breakup / splash).  `branch`
-    (max_ppp, ppp_fact, seed, next_id, cap) sets the breakup / splash
-    parameters; its next_id is updated."""
+    (max_ppp, ppp_fact, seed, next_id, cap, khrt_B0, khrt_B1, khrt_C3,
+    avg_inject_mass) sets the breakup / splash parameters; its next_id is
+    updated."""
     nf = int(sdata["nfuel"])
     n = np.asarray(ids).shape[0]
-    bp = BranchParamsC(max_ppp=100.0, ppp_fact=0.5, seed=0, next_id=0, cap=n)
+    bp = BranchParamsC(max_ppp=100.0, ppp_fact=0.5, seed=0, next_id=0, cap=n, khrt_B0=0.61, khrt_B1=7.0,
+                       khrt_C3=1.0, avg_inject_mass=0.0)
     if branch is not None:
-        for k in ("max_ppp", "ppp_fact", "seed", "next_id", "cap"):
+        for k in ("max_ppp", "ppp_fact", "seed", "next_id", "cap", "khrt_B0", "khrt_B1", "khrt_C3",
+                  "avg_inject_mass"):
             if k in branch:
                 setattr(bp, k, branch[k])
     cap = max(int(bp.cap), n)
@@ -172,6 +179,23 @@ def tab_update(sdata: dict, Reyn: float, dt: float, gas, rdata):
     cb = np.zeros(1)
     U = lib().oracle_tab_update(ctypes.byref(sd), Reyn, dt, gas.ctypes.data, rd.ctypes.data, cb.ctypes.data)
     return U, rd, float(cb[0])
+
+
+def khrt_update(sdata: dict, Reyn: float, dt: float, gas, rdata, pos=(0.0, 0.0, 0.0), B0=0.61, B1=7.0, C3=1.0,
+                avg_inject_mass=0.0, make_new_drops=True):
+    """updateBreakupKHRT for one n-dodecane droplet (mechanism 0) in the gas
+    state [u, v, w, T, rho, Y...]; returns (id, rdata_out, record, cBoilT) with
+    record = [type, loc 3, vel 3, norm 3, num_dens, ref_dia, T0, Utan]."""
+    sd = C.spray_data_c(sdata)
+    gas = np.ascontiguousarray(gas, dtype=np.float64)
+    rd = np.array(rdata, dtype=np.float64, copy=True)
+    kp = np.array([B0, B1, C3, avg_inject_mass], dtype=np.float64)
+    ps = np.ascontiguousarray(pos, dtype=np.float64)
+    out = np.zeros(14)
+    cb = np.zeros(1)
+    pid = lib().oracle_khrt_update(ctypes.byref(sd), Reyn, dt, gas.ctypes.data, kp.ctypes.data, int(make_new_drops),
+                                   ps.ctypes.data, rd.ctypes.data, out.ctypes.data, cb.ctypes.data)
+    return pid, rd, out, float(cb[0])
 
 
 def splash_criteria(Kv: float, Tstar: float, alpha: float, Kcrit: float, del_film: float):

@@ -620,6 +620,12 @@ struct BranchParams {
   double ppp_fact = 0.5;
   uint64_t seed = 0;
   int64_t next_id = 1;
+  // KH-RT constants (SpraySetup.cpp:17-19) and the mean injected parcel
+  // mass N d^3 of the jets (SprayParticles.cpp:230-239)
+  double khrt_B0 = 0.61;
+  double khrt_B1 = 7.;
+  double khrt_C3 = 1.;
+  double avg_inject_mass = 0.;
 };
 
 // per-droplet values the reference keeps in registers across sub-cycles
@@ -763,8 +769,16 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       }
       if (p.id > 0 && do_breakup) {
         if (fdat.do_breakup == 1) cy.Utan_total += updateBreakupTAB<NS, NF>(cy.Reyn_d, sub_dt, cBoilT, gpv, fdat, p);
-        if (cur_iter == num_iter - 1 && fdat.do_breakup == 1 && make_new_drops)
-          splitDropletTAB<NF>(p, bp.max_ppp, cy.Utan_total, cy.rec);
+        if (cur_iter == num_iter - 1) {
+          // the KH-RT update also runs for the TAB model when no new
+          // droplets can be made (SprayParticles.cpp:401-411)
+          if (fdat.do_breakup == 1 && make_new_drops) {
+            splitDropletTAB<NF>(p, bp.max_ppp, cy.Utan_total, cy.rec);
+          } else {
+            updateBreakupKHRT<NS, NF>(p, cy.Reyn_d, fdat.dtmod * flow_dt, cBoilT, bp.avg_inject_mass, bp.khrt_B0,
+                                      bp.khrt_B1, bp.khrt_C3, gpv, fdat, cy.rec, make_new_drops);
+          }
+        }
       }
       const double cur_coef = -inv_vol * sub_dt / flow_dt;
       if (!srcv.contains(ijkc)) {
@@ -856,6 +870,11 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       if (carry[i].rec.type != SB_NONE)
         create_sb_droplets<NF>(carry[i].rec, carry[i].parent_id, fdat, sub_dt, bp.ppp_fact, bp.seed, bp.next_id, born);
     }
+    // a child with a non-finite position cannot be placed (the reference
+    // aborts, "Bad breakup particle", SpraySB.cpp:187-190): a KH-stripped
+    // wall film has no velocity, so its children's directions are 0/0
+    for (const Particle<NF>& c : born)
+      if (!(std::isfinite(c.pos[0]) && std::isfinite(c.pos[1]) && std::isfinite(c.pos[2]))) status = PELEMP_E_BAD_CHILD;
     // the periodic shift of Redistribute applies to the new droplets too
     for (Particle<NF>& c : born) periodic_wrap(c.pos, g);
     parts.insert(parts.end(), born.begin(), born.end());

@@ -12,6 +12,7 @@
 //                     get_splash_vels, get_ms_theta, splash_criteria,
 //                     droplet_splashing
 //   Source/PP_Spray/WallFunctions.H:89-234             check_wall, impose_wall
+//   Source/PP_Spray/BreakupSplash/ReitzKHRT.H:7-178    updateBreakupKHRT
 //   Source/PP_Spray/SpraySB.cpp:9-207                  CreateSBDroplets
 // (Cartesian walls only; the EB branches are not part of this build.)
 //
@@ -348,6 +349,128 @@ void splitDropletTAB(Particle<NF>& p, double max_num_ppp, double Utan_total, SBR
     for (int spf = 0; spf < NF; ++spf) rf.Y0[spf] = p.rdata[C::pstateY + spf];
     p.id = -1;
   }
+}
+
+// ---------------------------------------------------------------------------
+// KH-RT wave breakup (ReitzKHRT.H:7-178).  BM1 holds the shed mass (in
+// units of N0 r^3), BM2 the RT clock (negative: RT off until the first KH
+// event).  dt is dtmod * flow_dt (SprayParticles.cpp:407-410).
+template <int NS, int NF>
+void updateBreakupKHRT(Particle<NF>& p, double Reyn_d, double dt, const double* cBoilT, double avg_inject_mass,
+                       double B0, double B1, double C3, const GasPhaseVals<NS, NF>& gpv,
+                       const pelemp_spray_data& fdat, SBRecord<NF>& rf, bool make_new_drops) {
+  using C = Comps<NF>;
+  const double mthresh_jet = 0.03;
+  const double mthresh_drop = 0.9;
+  const double We_crit = 6.;
+  const double A1 = 0.188;
+  const double vel_part[3] = {p.rdata[C::pstateVel], p.rdata[C::pstateVel + 1], p.rdata[C::pstateVel + 2]};
+  const double T_part = p.rdata[C::pstateT];
+  const double dia_part = p.rdata[C::pstateDia];
+  double num_dens = p.rdata[C::pstateNumDens];
+  double rho_part = 0., mu_part = 0.;
+  for (int spf = 0; spf < NF; ++spf) {
+    const double minT = std::min(T_part, cBoilT[spf]);
+    const double Y_part = p.rdata[C::pstateY + spf];
+    rho_part += Y_part / rhoL(fdat, minT, spf);
+    mu_part += Y_part * muL(fdat, minT, spf);
+  }
+  rho_part = 1. / rho_part;
+  const double min_rad = 4. * pm::cbrt(kMinMass * 3. / (4. * kPI * rho_part));
+  double shed_mass = p.rdata[C::pstateBM1];
+  double N0 = p.rdata[C::pstateN0];
+  double rt_time = p.rdata[C::pstateBM2];
+  const double sigma = fdat.sigma;
+  double rad_part = 0.5 * dia_part;
+  if (rad_part < min_rad) return;
+  const double rad_part3 = pm::pow(rad_part, 3.);
+  double diff_vel[3];
+  for (int d = 0; d < 3; ++d) diff_vel[d] = gpv.vel_fluid[d] - vel_part[d];
+  const double vel_mag = vlen(diff_vel);
+  const double vel_sq = vel_mag * vel_mag;
+  const double C_D = tab_drag_coef(Reyn_d);  // same piecewise law, ReitzKHRT.H:72-79
+  const double We_g = gpv.rho_fluid * vel_sq * rad_part / sigma;
+  const double Z = mu_part / std::sqrt(rho_part * rad_part * sigma);
+  const double T = Z * std::sqrt(We_g);
+  bool breakupRT = false;
+  if (rt_time >= 0.) {
+    const double a = 3. / 8. * C_D * gpv.rho_fluid * vel_sq / (rho_part * rad_part);
+    const double momdiff = a * (rho_part - gpv.rho_fluid);
+    const double omega_RT = std::sqrt(2. / (3. * std::sqrt(3. * sigma)) * momdiff * std::sqrt(momdiff) /
+                                      (rho_part + gpv.rho_fluid));
+    const double KRT = std::sqrt(momdiff / (3. * sigma));
+    const double lam_RT = C3 * kPI / KRT;
+    if (rad_part < lam_RT) {
+      rt_time = 0.;
+    } else {
+      rt_time += dt;
+    }
+    const double tau_RT = 1. / omega_RT;
+    if (rt_time > tau_RT && lam_RT < rad_part) {
+      rt_time = 0.;
+      num_dens *= pm::pow(rad_part / lam_RT, 3.);
+      rad_part = lam_RT;
+      breakupRT = true;
+    }
+  }
+  if (!breakupRT) {
+    const double lam_KH = rad_part * 9.02 * (1. + 0.45 * std::sqrt(Z)) * (1. + 0.4 * pm::pow(T, 0.7)) /
+                          pm::pow(1. + 0.865 * pm::pow(We_g, 1.67), 0.6);
+    const double omega_KH = (0.34 + 0.385 * We_g * std::sqrt(We_g)) / ((1. + Z) * (1. + 1.4 * pm::pow(T, 0.6))) *
+                            std::sqrt(sigma / (rho_part * rad_part3));
+    const double tau_KH = 3.726 * B1 * rad_part / (lam_KH * omega_KH);
+    double rs = B0 * lam_KH;
+    if (rs < rad_part && We_g > We_crit) {
+      const double rp = (dt / tau_KH * rs + rad_part) / (1. + dt / tau_KH);
+      const double rp3 = rp * rp * rp;
+      double rs3 = rs * rs * rs;
+      double np = num_dens * rad_part3 / rp3;
+      rt_time = 0.;
+      shed_mass += N0 * (rad_part3 - rp3);
+      double ns = shed_mass / rs3;
+      bool create_child_drops = false;
+      if (make_new_drops) {
+        const double rel_shed_drop = shed_mass / (num_dens * rad_part3);
+        const double rel_shed_jet = 8. * shed_mass / avg_inject_mass;
+        if (rel_shed_drop > mthresh_drop) {
+          create_child_drops = true;
+          shed_mass = num_dens * rad_part3;
+          rs = std::max(rs, min_rad);
+          rs3 = rs * rs * rs;
+          ns = shed_mass / rs3;
+          p.id = -1;
+        } else if (rel_shed_jet > mthresh_jet && rs > min_rad && ns > num_dens) {
+          create_child_drops = true;
+          np = (np * rp3 - ns * rs3) / rp3;
+          N0 = np;
+          shed_mass = 0.;
+        }
+      }
+      if (create_child_drops) {
+        const double Utan = A1 * omega_KH * lam_KH;
+        rf.type = SB_KH;
+        for (int d = 0; d < 3; ++d) {
+          rf.loc[d] = p.pos[d];
+          rf.vel[d] = vel_part[d];
+          rf.norm[d] = vel_part[d] / vel_mag;  // droplet velocity over the relative speed, as written
+        }
+        rf.num_dens = ns;
+        rf.ref_dia = 2. * rs;
+        rf.T0 = T_part;
+        rf.phi1 = Utan;
+        rf.phi2 = 0.;
+        rf.phi3 = 0.;
+        for (int spf = 0; spf < NF; ++spf) rf.Y0[spf] = p.rdata[C::pstateY + spf];
+      }
+      rad_part = rp;
+      num_dens = np;
+    }
+  }
+  p.rdata[C::pstateDia] = 2. * rad_part;
+  p.rdata[C::pstateNumDens] = num_dens;
+  p.rdata[C::pstateN0] = N0;
+  p.rdata[C::pstateBM1] = shed_mass;
+  p.rdata[C::pstateBM2] = rt_time;
 }
 
 // ---------------------------------------------------------------------------

@@ -20,6 +20,7 @@ E_BAD_CONFIG = 4
 E_HIP = 5
 E_ARG = 6
 E_NOMEM = 7
+E_BAD_CHILD = 8
 ERROR_NAMES = {
     E_STATE_BOX: "state box too small",
     E_SRC_BOX: "source box too small",
@@ -28,6 +29,7 @@ ERROR_NAMES = {
     E_HIP: "HIP error",
     E_ARG: "invalid argument",
     E_NOMEM: "out of device memory",
+    E_BAD_CHILD: "secondary droplet without a valid position",
 }
 
 MECH_DODECANE_AIR = 0
@@ -103,7 +105,8 @@ class MlpTrainCfgC(ctypes.Structure):
 
 class SprayParamsC(ctypes.Structure):
     _fields_ = [("max_parcel_size", _c_dbl), ("breakup_parcel_factor", _c_dbl), ("khrt_B0", _c_dbl),
-                ("khrt_B1", _c_dbl), ("khrt_C3", _c_dbl), ("rng_seed", ctypes.c_uint64)]
+                ("khrt_B1", _c_dbl), ("khrt_C3", _c_dbl), ("rng_seed", ctypes.c_uint64),
+                ("avg_inject_mass", _c_dbl)]
 
 
 _vp = ctypes.c_void_p

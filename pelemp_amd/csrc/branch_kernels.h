@@ -173,6 +173,9 @@ __global__ void __launch_bounds__(256) sb_create_kernel(
       }
       BDrop<NF> c;
       sb_child<NF>(r, k, parent_id, fc, sub_dt, ppp_fact, seed, c);
+      // no valid position (SpraySB.cpp:187-190): a KH-stripped wall film has
+      // no velocity, so its children's directions are 0/0
+      if (!(isfinite(c.pos[0]) && isfinite(c.pos[1]) && isfinite(c.pos[2]))) set_err(err, PELEMP_E_BAD_CHILD);
       // periodic wrap of the new position (the Redistribute stand-in)
 #pragma unroll
       for (int d = 0; d < 3; ++d) {

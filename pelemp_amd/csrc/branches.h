@@ -19,12 +19,20 @@ namespace pelemp {
 
 // splash_breakup (SprayFuelData.H:31-39)
 constexpr int kSBNone = 0;
+constexpr int kSBKH = 1;
 constexpr int kSBTab = 3;
 constexpr int kSBSplashDry = 4;
 constexpr int kSBSplashThermal = 5;
 constexpr int kSBSplashWet = 6;
 // splash_type (AhamedSplash.H:12-19)
 constexpr int kRebound = 0, kDeposit = 1, kSplashDry = 2, kThermalBreakup = 3, kSpread = 4, kSplashWet = 5;
+
+// KH-RT model constants (SpraySetup.cpp:17-19), the mean injected parcel
+// mass N d^3 of the jets (SprayParticles.cpp:230-239) and the model's time
+// step dtmod * flow_dt (SprayParticles.cpp:407-410)
+struct KhrtParams {
+  double B0, B1, C3, avg_inject_mass, dt;
+};
 
 // Liquid properties not needed by the drag/evaporation path
 // (SprayFuelData.H:155-173).
@@ -359,6 +367,126 @@ PM_HDI void tab_split(BDrop<NF>& d, double max_num_ppp, double Utan_total, SBRec
     for (int spf = 0; spf < NF; ++spf) rf.Y0[spf] = d.Y[spf];
     d.alive = false;
   }
+}
+
+// ---------------------------------------------------------------------------
+// KH-RT wave breakup (ReitzKHRT.H:7-178): BM1 is the shed mass (N0 r^3
+// units), BM2 the RT clock (negative: RT off until the first KH event)
+template <int NS, int NF>
+PM_HDI void khrt_update(double Reyn_d, const double* cBoilT, const GasAt<double, NS>& g,
+                        const FuelConst<double, NF>& fd, const LiquidExtra& lx, const KhrtParams& kp,
+                        bool make_new_drops, BDrop<NF>& d, SBRec<NF>& rf) {
+  const double mthresh_jet = 0.03;
+  const double mthresh_drop = 0.9;
+  const double We_crit = 6.;
+  const double A1 = 0.188;
+  const double dt = kp.dt;
+  double num_dens = d.numDens;
+  double rho_part = 0., mu_part = 0.;
+#pragma unroll
+  for (int spf = 0; spf < NF; ++spf) {
+    const double minT = rmin(d.T, cBoilT[spf]);
+    rho_part += d.Y[spf] / rhoL<double, NF>(fd, minT, spf);
+    mu_part += d.Y[spf] * muL(lx, minT, spf);
+  }
+  rho_part = 1. / rho_part;
+  const double min_rad = 4. * rm::cbrt(kMinMass * 3. / (4. * kPIc * rho_part));
+  double shed_mass = d.bm1;
+  double N0 = d.N0;
+  double rt_time = d.bm2;
+  const double sigma = lx.sigma;
+  double rad_part = 0.5 * d.dia;
+  if (rad_part < min_rad) return;
+  const double rad_part3 = rm::pow(rad_part, 3.);
+  double dv[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) dv[q] = g.vel[q] - d.vel[q];
+  const double vel_mag = v3len(dv);
+  const double vel_sq = vel_mag * vel_mag;
+  const double C_D = tab_cd(Reyn_d);
+  const double We_g = g.rho * vel_sq * rad_part / sigma;
+  const double Z = mu_part / std::sqrt(rho_part * rad_part * sigma);
+  const double T = Z * std::sqrt(We_g);
+  bool breakupRT = false;
+  if (rt_time >= 0.) {
+    const double a = 3. / 8. * C_D * g.rho * vel_sq / (rho_part * rad_part);
+    const double momdiff = a * (rho_part - g.rho);
+    const double omega_RT =
+        std::sqrt(2. / (3. * std::sqrt(3. * sigma)) * momdiff * std::sqrt(momdiff) / (rho_part + g.rho));
+    const double KRT = std::sqrt(momdiff / (3. * sigma));
+    const double lam_RT = kp.C3 * kPIc / KRT;
+    if (rad_part < lam_RT) {
+      rt_time = 0.;
+    } else {
+      rt_time += dt;
+    }
+    const double tau_RT = 1. / omega_RT;
+    if (rt_time > tau_RT && lam_RT < rad_part) {
+      rt_time = 0.;
+      num_dens *= rm::pow(rad_part / lam_RT, 3.);
+      rad_part = lam_RT;
+      breakupRT = true;
+    }
+  }
+  if (!breakupRT) {
+    const double lam_KH = rad_part * 9.02 * (1. + 0.45 * std::sqrt(Z)) * (1. + 0.4 * rm::pow(T, 0.7)) /
+                          rm::pow(1. + 0.865 * rm::pow(We_g, 1.67), 0.6);
+    const double omega_KH = (0.34 + 0.385 * We_g * std::sqrt(We_g)) / ((1. + Z) * (1. + 1.4 * rm::pow(T, 0.6))) *
+                            std::sqrt(sigma / (rho_part * rad_part3));
+    const double tau_KH = 3.726 * kp.B1 * rad_part / (lam_KH * omega_KH);
+    double rs = kp.B0 * lam_KH;
+    if (rs < rad_part && We_g > We_crit) {
+      const double rp = (dt / tau_KH * rs + rad_part) / (1. + dt / tau_KH);
+      const double rp3 = rp * rp * rp;
+      double rs3 = rs * rs * rs;
+      double np = num_dens * rad_part3 / rp3;
+      rt_time = 0.;
+      shed_mass += N0 * (rad_part3 - rp3);
+      double ns = shed_mass / rs3;
+      bool create_child_drops = false;
+      if (make_new_drops) {
+        const double rel_shed_drop = shed_mass / (num_dens * rad_part3);
+        const double rel_shed_jet = 8. * shed_mass / kp.avg_inject_mass;
+        if (rel_shed_drop > mthresh_drop) {
+          create_child_drops = true;
+          shed_mass = num_dens * rad_part3;
+          rs = rmax(rs, min_rad);
+          rs3 = rs * rs * rs;
+          ns = shed_mass / rs3;
+          d.alive = false;
+        } else if (rel_shed_jet > mthresh_jet && rs > min_rad && ns > num_dens) {
+          create_child_drops = true;
+          np = (np * rp3 - ns * rs3) / rp3;
+          N0 = np;
+          shed_mass = 0.;
+        }
+      }
+      if (create_child_drops) {
+        rf.type = kSBKH;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          rf.loc[q] = d.pos[q];
+          rf.vel[q] = d.vel[q];
+          rf.norm[q] = d.vel[q] / vel_mag;  // droplet velocity over the relative speed, as written
+        }
+        rf.num_dens = ns;
+        rf.ref_dia = 2. * rs;
+        rf.T0 = d.T;
+        rf.phi1 = A1 * omega_KH * lam_KH;
+        rf.phi2 = 0.;
+        rf.phi3 = 0.;
+#pragma unroll
+        for (int spf = 0; spf < NF; ++spf) rf.Y0[spf] = d.Y[spf];
+      }
+      rad_part = rp;
+      num_dens = np;
+    }
+  }
+  d.dia = 2. * rad_part;
+  d.numDens = num_dens;
+  d.N0 = N0;
+  d.bm1 = shed_mass;
+  d.bm2 = rt_time;
 }
 
 // ---------------------------------------------------------------------------

@@ -565,8 +565,8 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
       return PELEMP_E_BAD_CONFIG;
     }
   }
-  if (sd->do_breakup == 2) {
-    set_last_error("KH-RT breakup (do_breakup = 2) is not enabled in this build of the update");
+  if (sd->do_breakup < 0 || sd->do_breakup > 2) {
+    set_last_error("do_breakup must be 0 (none), 1 (TAB) or 2 (KH-RT)");  // SpraySetup.cpp:138-155
     return PELEMP_E_BAD_CONFIG;
   }
   if ((sd->do_breakup != 0 || sd->do_splash != 0) && form != PELEMP_FORM_FP64) {
@@ -577,7 +577,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
     set_last_error("splash needs fuel_sigma, wall_temp and contact_angle");  // SpraySetup.cpp:157-183
     return PELEMP_E_BAD_CONFIG;
   }
-  if (sd->do_breakup == 1 && !(sd->sigma > 0.)) {
+  if (sd->do_breakup != 0 && !(sd->sigma > 0.)) {
     set_last_error("breakup needs fuel_sigma");  // SpraySetup.cpp:157-165
     return PELEMP_E_BAD_CONFIG;
   }
@@ -614,7 +614,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   // state is then carried in the hot fields
   bool reflective = false;
   for (int d = 0; d < 3; ++d) reflective = reflective || geom->bndry_lo[d] == 1 || geom->bndry_hi[d] == 1;
-  c.branches = (form == PELEMP_FORM_FP64) && (sd->do_breakup == 1 || sd->do_splash != 0 || reflective);
+  c.branches = (form == PELEMP_FORM_FP64) && (sd->do_breakup != 0 || sd->do_splash != 0 || reflective);
   c.hot_brk = c.branches;
   c.slot_cap = c.branches ? 4 * c.cap : c.cap;
   c.params.max_parcel_size = 100.;
@@ -623,6 +623,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.params.khrt_B1 = 7.;
   c.params.khrt_C3 = 1.;
   c.params.rng_seed = 0;
+  c.params.avg_inject_mass = 0.;
   if (form == PELEMP_FORM_FP64) {
     c.nhot = c.hot_brk ? (13 + c.nf) : (10 + c.nf);
     c.wsize = 8;

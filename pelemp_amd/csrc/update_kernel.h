@@ -104,10 +104,11 @@ struct UpdateArgs {
   const double* __restrict__ wf;  // film thickness per cell (splash)
   int32_t* __restrict__ sb_type;  // record type per output position
   double* __restrict__ sb_data;   // record fields [kSBFields + NF][cap]
-  int do_tab;                     // do_breakup == 1
+  int do_breakup;                 // 0 none, 1 TAB, 2 KH-RT
   int make_new_drops;
   int do_splash_box;
   double max_ppp;
+  KhrtParams kh;  // KH-RT constants, mean injected parcel mass, dtmod * flow_dt
 };
 
 // fields of a secondary-droplet record (SBData.H:7-19): norm(3) vel(3)
@@ -530,9 +531,16 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
     b.bm1 = d.bm1;
     b.alive = d.alive;
   }
-  if (b.alive && a.do_tab) {
-    const double Utan = tab_update<M::NS, NF>(Reyn_d, a.sub_dt, L.cBoilT, L.g, a.fc, a.lx, b);
-    if (a.make_new_drops) tab_split<NF>(b, a.max_ppp, Utan, rec);
+  // breakup (SprayParticles.cpp:398-412): the KH-RT update also runs for the
+  // TAB model when no new droplets can be made
+  if (b.alive && a.do_breakup > 0) {
+    double Utan = 0.;
+    if (a.do_breakup == 1) Utan = tab_update<M::NS, NF>(Reyn_d, a.sub_dt, L.cBoilT, L.g, a.fc, a.lx, b);
+    if (a.do_breakup == 1 && a.make_new_drops) {
+      tab_split<NF>(b, a.max_ppp, Utan, rec);
+    } else {
+      khrt_update<M::NS, NF>(Reyn_d, L.cBoilT, L.g, a.fc, a.lx, a.kh, a.make_new_drops != 0, b, rec);
+    }
   }
   // contributions cur_coef * source (SprayParticles.cpp:419-441)
   double* cb = &contrib[t * NSRC];

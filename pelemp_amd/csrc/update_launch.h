@@ -98,7 +98,12 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
       a.wf = c.wf;
       a.sb_type = c.sb_type;
       a.sb_data = c.sb_data;
-      a.do_tab = (c.sd.do_breakup == 1) ? 1 : 0;
+      a.do_breakup = c.sd.do_breakup;
+      a.kh.B0 = c.params.khrt_B0;
+      a.kh.B1 = c.params.khrt_B1;
+      a.kh.C3 = c.params.khrt_C3;
+      a.kh.avg_inject_mass = c.params.avg_inject_mass;
+      a.kh.dt = c.sd.dtmod * p.flow_dt;
       a.make_new_drops = p.make_new_drops;
       a.do_splash_box = p.do_splash_box;
       a.max_ppp = c.params.max_parcel_size;

@@ -115,14 +115,21 @@ class SprayParticleContainer:
         return dict(self.comps)
 
     def set_params(self, max_parcel_size: float = 100.0, breakup_parcel_factor: float = 0.5,
-                   rng_seed: int = 0) -> None:
+                   rng_seed: int = 0, khrt_B0: float = 0.61, khrt_B1: float = 7.0, khrt_C3: float = 1.0,
+                   avg_inject_mass: float = 0.0) -> None:
         """Container parameters of the breakup / splash models (max_parcel_size,
-        breakup_parcel_factor; SpraySetup.cpp:15-19,144-160)."""
+        breakup_parcel_factor, KHRT_B0/B1/C3; SpraySetup.cpp:15-19,138-160) and
+        the mean injected parcel mass N d^3 the KH-RT model compares the shed
+        mass with (SprayParticles.cpp:230-239)."""
         p = C.SprayParamsC()
         C.check(self._lib.pelemp_spray_get_params(self._h, ctypes.byref(p)), "pelemp_spray_get_params")
         p.max_parcel_size = float(max_parcel_size)
         p.breakup_parcel_factor = float(breakup_parcel_factor)
         p.rng_seed = int(rng_seed)
+        p.khrt_B0 = float(khrt_B0)
+        p.khrt_B1 = float(khrt_B1)
+        p.khrt_C3 = float(khrt_C3)
+        p.avg_inject_mass = float(avg_inject_mass)
         C.check(self._lib.pelemp_spray_set_params(self._h, ctypes.byref(p)), "pelemp_spray_set_params")
 
     def set_particles(self, pos: np.ndarray, rdata: np.ndarray, ids: np.ndarray) -> None:

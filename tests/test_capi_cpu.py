@@ -36,6 +36,47 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(C.SprayDataC) == 6 * 4 + n_dbl * 8 + 2 * 8 * 4
 
 
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    # every field offset and struct size of the ctypes mirrors equals what
+    # gcc makes of include/pelemp_spray.h
+    import shutil
+    import subprocess
+
+    import pytest
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no host C compiler")
+    pairs = {
+        "pelemp_spray_data": C.SprayDataC,
+        "pelemp_spray_comps": C.SprayCompsC,
+        "pelemp_fab_view": C.FabViewC,
+        "pelemp_geom": C.GeomC,
+        "pelemp_spray_params": C.SprayParamsC,
+        "pelemp_surrogate_norm": C.SurrogateNormC,
+        "pelemp_mlp_train_cfg": C.MlpTrainCfgC,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            cname, key, val = line.split()
+            got[(cname, key)] = int(val)
+    for cname, py in pairs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, f"{cname}.{fname}"
+
+
 def test_device_count_without_gpu_is_clean():
     # on a host without a GPU this returns 0 devices rather than failing
     assert C.device_count() >= 0

@@ -20,26 +20,37 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def _run_both(prob, sdata, nsteps, seed=11, cap_factor=20):
+def _run_both(prob, sdata, nsteps, seed=11, cap_factor=20, do_move=True, avg_inject_mass=0.0, error_at=None):
     from oracle import oracle_py as O
     from pelemp_amd.spray import DeviceFab, SprayParticleContainer
 
     cap = cap_factor * prob.n
     sd_o = O.latent_shifted(sdata, prob.mech, len(prob.species))
-    branch = dict(max_ppp=100.0, ppp_fact=0.5, seed=seed, next_id=0, cap=cap)
+    branch = dict(max_ppp=100.0, ppp_fact=0.5, seed=seed, next_id=0, cap=cap, avg_inject_mass=avg_inject_mass)
     pc = SprayParticleContainer(prob.mech, sdata, prob.comps, prob.geom, cap, form=0)
-    pc.set_params(100.0, 0.5, seed)
+    pc.set_params(100.0, 0.5, seed, avg_inject_mass=avg_inject_mass)
     pc.set_particles(prob.pos, prob.rdata, prob.ids)
     state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
     pos, rd, ids = prob.pos, prob.rdata, prob.ids
     for step in range(nsteps):
         src_o = prob.new_source()
         st, pos, rd, ids = O.update(prob.mech, sd_o, prob.comps, prob.geom, pos, rd, ids, prob.state, prob.state_lo,
-                                    prob.state_hi, src_o, prob.src_lo, prob.src_hi, prob.flow_dt, branch=branch)
-        assert st == 0, f"oracle status {st} at step {step}"
+                                    prob.state_hi, src_o, prob.src_lo, prob.src_hi, prob.flow_dt, do_move=do_move,
+                                    branch=branch)
         src_g = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+        if step == error_at:
+            # the same failure on both sides, at the same step
+            from pelemp_amd import _capi as C
+
+            assert st != 0, f"oracle status 0 at step {step}"
+            with pytest.raises(C.PelempError) as e:
+                pc.updateParticles(0, state, src_g, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False,
+                                   False, do_move, None, -1.0)
+            assert e.value.code == st
+            return None, st
+        assert st == 0, f"oracle status {st} at step {step}"
         pc.updateParticles(0, state, src_g, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
-                           True, None, -1.0)
+                           do_move, None, -1.0)
         gs = src_g.numpy()
         scale = np.max(np.abs(src_o), axis=(1, 2, 3), keepdims=True)
         err = np.max(np.abs(gs - src_o) / np.maximum(scale, 1e-300))
@@ -82,6 +93,43 @@ def test_config3_small_branch_parity(case):
         assert o[2].max() > n0, "no secondary droplets created"
     bitwise = _compare(g, o, prob.nf)
     assert bitwise, "agreement within 1e-10 but not bitwise"
+
+
+@pytest.mark.parametrize("case", ["khrt", "khrt_splash", "tab_no_move"])
+def test_config3_small_khrt_parity(case):
+    # KH-RT stripping / RT breakup with its children (relative speeds up to
+    # 30 m/s, mean injected parcel 50 x (40 um)^3), alone and with splash; and
+    # the TAB model without a move, where the reference runs the KH-RT update
+    # as well (SprayParticles.cpp:401-411)
+    prob = _problem(du_max=30.0)
+    sd = dict(prob.sdata)
+    sd["do_breakup"] = 1 if case == "tab_no_move" else 2
+    if case == "khrt":
+        sd["do_splash"] = 0
+    # with splash, the run stops at step 3 (see the next test)
+    nsteps = 3 if case == "khrt_splash" else 5
+    g, o = _run_both(prob, sd, nsteps, do_move=(case != "tab_no_move"), avg_inject_mass=50 * 40e-6 ** 3)
+    if case != "tab_no_move":
+        assert o[2].max() > prob.n, "no secondary droplets created"
+    else:
+        # stripped parcels: diameters changed with no droplet moved
+        assert np.any(o[1][:, 4] != prob.rdata[np.argsort(prob.ids)][:, 4])
+    assert _compare(g, o, prob.nf), "agreement within 1e-10 but not bitwise"
+
+
+def test_khrt_stripped_wall_film_is_an_error():
+    # the KH-RT update also runs on wall films (SprayParticles.cpp:398-411
+    # has no film check); a film has no velocity, so the direction of its
+    # children is 0/0 and the reference aborts ("Bad breakup particle",
+    # SpraySB.cpp:187-190).  Both sides report PELEMP_E_BAD_CHILD at the step
+    # where the first film is stripped.
+    from pelemp_amd import _capi as C
+
+    prob = _problem(du_max=30.0)
+    sd = dict(prob.sdata)
+    sd["do_breakup"] = 2
+    _, st = _run_both(prob, sd, 4, avg_inject_mass=50 * 40e-6 ** 3, error_at=3)
+    assert st == C.E_BAD_CHILD
 
 
 def test_config3_children_ids_and_run_to_run():

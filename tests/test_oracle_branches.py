@@ -162,7 +162,7 @@ def _run_oracle(prob, nsteps, branch, sdata=None):
     return out, evap
 
 
-@pytest.mark.parametrize("branches", ["splash_film", "tab"])
+@pytest.mark.parametrize("branches", ["splash_film", "tab", "khrt"])
 def test_wall_splash_film_breakup_run_conserves_mass(branches):
     # Isothermal case (gas, droplets and wall at 300 K): the film model keeps
     # the film height, not its mass, across a temperature change
@@ -170,15 +170,21 @@ def test_wall_splash_film_breakup_run_conserves_mass(branches):
     # heating film would change its mass through the density alone.  The two
     # branch sets are run apart: the reference also applies the TAB update to
     # wall-film particles (SprayParticles.cpp:400-405 has no film check),
-    # which changes a film's diameter at constant height.
-    prob = P.config3(ncell=(16, 16, 32), drops_per_cell=3.0, z_frac=0.25, num_dens=50.0, T0=300.0, wall_T=300.0)
+    # which changes a film's diameter at constant height.  The KH-RT case
+    # uses slower droplets (relative speed up to 30 m/s): at 150 m/s and
+    # 5 atm it strips most parcels into ~100 children per step.
+    du_max = 30.0 if branches == "khrt" else 150.0
+    prob = P.config3(ncell=(16, 16, 32), drops_per_cell=3.0, z_frac=0.25, num_dens=50.0, T0=300.0, wall_T=300.0,
+                     du_max=du_max)
     sd = O.latent_shifted(prob.sdata, prob.mech, len(prob.species))
-    if branches == "tab":
+    if branches in ("tab", "khrt"):
         sd["do_splash"] = 0
+        sd["do_breakup"] = 1 if branches == "tab" else 2
     else:
         sd["do_breakup"] = 0
     m0 = _liquid_mass(sd, prob.rdata)
-    branch = dict(max_ppp=100.0, ppp_fact=0.5, seed=5, next_id=0, cap=20 * prob.n)
+    # mean injected parcel N d^3 for KH-RT: 50 droplets of the 40 um median
+    branch = dict(max_ppp=100.0, ppp_fact=0.5, seed=5, next_id=0, cap=20 * prob.n, avg_inject_mass=50 * 40e-6 ** 3)
     hist, evap = _run_oracle(prob, 5, branch, sd)
     pos, rd, ids = hist[-1]
     assert len(np.unique(ids)) == ids.size
@@ -210,3 +216,122 @@ def test_specular_reflection_closed_form():
     np.testing.assert_allclose(pos[o, 2][crossed], -z_pred[crossed], rtol=0, atol=1e-18)
     assert np.all(rd1[o, 2][crossed] == 20.0)
     assert np.all(rd1[o, 2][~crossed] == -20.0)
+
+
+# ---------------------------------------------------------------------------
+# KH-RT wave breakup (ReitzKHRT.H:7-178).  rdata (nf = 1): vel 0-2, T 3, dia 4,
+# Y 5, numDens 6, N0 7, BM1 = shed mass 8, BM2 = RT clock 9 (negative: RT
+# off), film 10.
+
+
+def _khrt_case(u=100.0, dia=200e-6, p=5 * P.PATM, rt=-1.0, num=1.0):
+    sd, species = _dodecane()
+    gas, rho_g = _gas(species, T=600.0, p=p, u=(u, 0.0, 0.0))
+    return sd, gas, rho_g, _drop(1, dia, num=num, y=0.0, ydot=rt)
+
+
+def test_khrt_below_critical_weber_leaves_the_droplet():
+    # We_g = rho_g u^2 r / sigma <= 6: no KH stripping; RT off (clock < 0)
+    sd, gas, rho_g, rd = _khrt_case(u=5.0, dia=20e-6, p=P.PATM)
+    assert rho_g * 25.0 * 10e-6 / 0.015 < 6.0
+    pid, out, rec, _ = O.khrt_update(sd, 50.0, 1e-6, gas, rd)
+    assert pid == 1 and rec[0] == 0
+    np.testing.assert_array_equal(out, rd)
+
+
+def test_khrt_stripping_is_a_backward_euler_relaxation():
+    # r_new = (dt/tau rs + r) / (1 + dt/tau) solves (r_new - r)/dt = -(r_new - rs)/tau:
+    # the same tau (3.726 B1 r / (Lambda Omega), independent of dt) must come
+    # out of two step sizes, and tau must scale with B1, rs with B0
+    sd, gas, _, rd = _khrt_case()
+    r0 = 0.5 * rd[4]
+
+    def step(dt, B0=0.61, B1=7.0):
+        pid, out, rec, _ = O.khrt_update(sd, 50.0, dt, gas, rd, B0=B0, B1=B1, avg_inject_mass=1e-30)
+        assert pid == 1 and rec[0] == 1  # KH record made at the jet threshold
+        rs = 0.5 * rec[11]
+        rp = 0.5 * out[4]
+        assert rs < rp < r0
+        return dt * (rp - rs) / (r0 - rp), rs
+
+    tau1, rs1 = step(1e-7)
+    tau2, rs2 = step(1e-6)
+    assert tau2 == pytest.approx(tau1, rel=1e-9)
+    assert rs2 == rs1
+    tau3, _ = step(1e-7, B1=14.0)
+    assert tau3 == pytest.approx(2.0 * tau1, rel=1e-9)
+    _, rs4 = step(1e-7, B0=1.22)
+    assert rs4 == pytest.approx(2.0 * rs1, rel=1e-14)
+
+
+@pytest.mark.parametrize("dt,deleted", [(1e-7, False), (1e-3, True)])
+def test_khrt_children_and_parent_share_the_parcel_mass(dt, deleted):
+    # jet threshold: the parent keeps N r^3 minus what the record takes;
+    # drop threshold (shed > 90 %): the parent is removed and the record
+    # carries the whole parcel
+    sd, gas, _, rd = _khrt_case(num=3.0)
+    m0 = rd[6] * rd[4] ** 3
+    pid, out, rec, _ = O.khrt_update(sd, 50.0, dt, gas, rd, avg_inject_mass=1e-30)
+    assert rec[0] == 1
+    assert (pid == -1) == deleted
+    m_rec = rec[10] * rec[11] ** 3
+    m_par = 0.0 if deleted else out[6] * out[4] ** 3
+    assert m_par + m_rec == pytest.approx(m0, rel=1e-12)
+    if not deleted:
+        assert out[8] == 0.0 and out[7] == out[6]  # shed mass reset, N0 = new parcel count
+    # the record moves with the droplet velocity; its Utan is A1 Omega Lambda > 0
+    np.testing.assert_array_equal(rec[4:7], rd[0:3])
+    assert rec[13] > 0.0
+
+
+def test_khrt_without_new_droplets_accumulates_shed_mass():
+    # make_new_drops false (the update without a move): the parcel shrinks at
+    # constant N r^3 and the shed mass N0 (r^3 - r_new^3) accumulates in BM1
+    sd, gas, _, rd = _khrt_case(num=2.0)
+    pid, out, rec, _ = O.khrt_update(sd, 50.0, 1e-6, gas, rd, make_new_drops=False)
+    assert pid == 1 and rec[0] == 0
+    r0, r1 = 0.5 * rd[4], 0.5 * out[4]
+    assert r1 < r0
+    assert out[6] * r1 ** 3 == pytest.approx(rd[6] * r0 ** 3, rel=1e-12)
+    assert out[8] == pytest.approx(rd[7] * (r0 ** 3 - r1 ** 3), rel=1e-10)
+    assert out[9] == 0.0  # the KH event starts the RT clock (it was off, -1)
+
+
+def test_khrt_rayleigh_taylor_breakup():
+    # RT clock past 1/omega_RT and r above the fastest-growing RT wavelength
+    # lambda = C3 pi / K, K = sqrt(a (rho_l - rho_g) / (3 sigma)), with the
+    # drag deceleration a = 3/8 C_D rho_g u^2 / (rho_l r): the parcel breaks
+    # into droplets of radius lambda at constant N r^3
+    import ctypes
+
+    u, r, Re = 100.0, 200e-6, 500.0
+    sd, gas, rho_g, rd = _khrt_case(u=u, dia=2 * r, rt=1.0)
+    pid, out, rec, cB = O.khrt_update(sd, Re, 1e-7, gas, rd)
+    rho_l = O.lib().oracle_rhoL(ctypes.byref(C.spray_data_c(sd)), min(300.0, cB), 0)
+    C_D = 24.0 / Re * (1.0 + Re ** (2.0 / 3.0) / 6.0)
+    a = 3.0 / 8.0 * C_D * rho_g * u ** 2 / (rho_l * r)
+    lam = math.pi / math.sqrt(a * (rho_l - rho_g) / (3.0 * 0.015))
+    assert lam < r
+    assert pid == 1 and rec[0] == 0
+    assert 0.5 * out[4] == pytest.approx(lam, rel=1e-12)
+    assert out[6] * out[4] ** 3 == pytest.approx(rd[6] * rd[4] ** 3, rel=1e-12)
+    assert out[9] == 0.0
+
+
+def test_khrt_stripped_wall_film_stops_the_run():
+    # KH-RT also runs on wall films; a film has no velocity, so its children
+    # have no direction (0/0) and the update reports PELEMP_E_BAD_CHILD, where
+    # the reference aborts (SpraySB.cpp:187-190)
+    prob = P.config3(ncell=(16, 16, 32), drops_per_cell=3.0, z_frac=0.25, du_max=30.0)
+    sd = O.latent_shifted(dict(prob.sdata, do_breakup=2), prob.mech, len(prob.species))
+    branch = dict(seed=11, cap=20 * prob.n, avg_inject_mass=50 * 40e-6 ** 3)
+    pos, rd, ids = prob.pos, prob.rdata, prob.ids
+    sts = []
+    for _ in range(4):
+        src = prob.new_source()
+        st, pos, rd, ids = O.update(prob.mech, sd, prob.comps, prob.geom, pos, rd, ids, prob.state, prob.state_lo,
+                                    prob.state_hi, src, prob.src_lo, prob.src_hi, prob.flow_dt, branch=branch)
+        sts.append(st)
+        if st:
+            break
+    assert sts == [0, 0, 0, C.E_BAD_CHILD]
