@@ -109,6 +109,66 @@ def run_form(prob, form: int, steps: int, warmup: int, stream, acc_steps: int = 
     return res
 
 
+def run_branches(steps: int, warmup: int, stream, cap_factor: float = 5.0):
+    """Config 3 (FP64 with walls, TAB breakup, splash and wall films): the
+    update kernel with its branches inline, plus the passes around it -- the
+    film-thickness pass before the update and the secondary-droplet pass
+    (child counts, scan, creation) after it.  Their time is the
+    branch-compaction cost.  The parcel count grows with the breakup cascade
+    (about 3.6x over the first 6 steps), so the leg runs 1 + 5 steps with
+    room for 5x the initial droplets."""
+    from pelemp_amd import _capi as C
+    from pelemp_amd import problem as P
+    from pelemp_amd.spray import DeviceFab, SprayParticleContainer
+
+    t0 = time.perf_counter()
+    prob = P.config3()
+    print(f"# config3 built: {prob.n} droplets in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    cap = int(prob.n * cap_factor)
+    pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, cap, form=C.FORM_FP64)
+    pc.set_params(100.0, 0.5, 1)
+    pc.set_particles(prob.pos, prob.rdata, prob.ids)
+    state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
+    src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+    h = stream.handle
+    n0 = prob.n
+
+    def step():
+        C.check(C.lib().pelemp_memset_device(src.buf.ptr, 0, src.buf.nbytes), "memset source")
+        pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
+                           True, None, -1.0, stream=h, sync=False)
+
+    for _ in range(warmup):
+        step()
+    pc.sync(h)
+    n_start = pc.num_particles()
+    pc.pass_counts()
+    pc.set_timing(True)
+    timer = C.Timer()
+    C.device_synchronize()
+    timer.start(h)
+    for _ in range(steps):
+        step()
+    timer.stop(h)
+    stream.synchronize()
+    pc.sync(h)
+    ms_total = timer.elapsed_ms()
+    counts = pc.pass_counts()
+    kms, kn = pc.get_timing()
+    n_end = pc.num_particles()
+    pc.close()
+    ds = int(sum(counts))
+    per = [kms[i] / max(kn[i], 1) for i in range(4)]
+    branch_ms = per[2] + per[3]
+    return dict(workload=f"config3: 128x128x256 slab dx=1e-4 m, wall at z=0 (reflective, T_wall=450 K), "
+                         f"outflow at z_hi, gas 600 K / 5 atm moving at -20 m/s, {n0} n-dodecane droplets "
+                         "(median 40 um, sigma_ln 0.6, We 1-300), TAB + splash + wall films, FP64, dt=5e-7 s",
+                droplets_start=n0, droplets_timed_start=n_start, droplets_end=n_end, droplet_steps=ds,
+                value=ds / (ms_total * 1e-3), ms_per_step=ms_total / steps, update_kernel_ms=per[0],
+                sort_ms=per[1], film_pass_ms=per[2], children_pass_ms=per[3], branch_compaction_ms=branch_ms,
+                branch_compaction_frac=branch_ms * steps / ms_total)
+
+
 def cpu_baseline(prob, target_s: float = 12.0):
     """Time the CPU oracle on a bounded sample of the workload."""
     from oracle import oracle_py as O
@@ -187,6 +247,7 @@ def main(argv=None):
     ap.add_argument("--ncell", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-branches", action="store_true", help="skip the config-3 (walls/breakup/splash) leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the 1 GiB copy kernels used to calibrate PMC byte counters")
     args = ap.parse_args(argv)
@@ -257,6 +318,17 @@ def main(argv=None):
                      "kernel": "spray_update_kernel", "bytes_per_droplet_step": mf["bytes_per_droplet_step"]},
         "forms": forms,
     }
+    if not args.no_branches:
+        try:
+            br = run_branches(min(args.steps, 5), min(args.warmup, 1), stream)
+        except C.PelempError as e:
+            br = dict(error=str(e))
+            print(f"# branches (config3) failed: {e}", file=sys.stderr, flush=True)
+        out["branches"] = br
+    if "error" not in out.get("branches", {"error": 1}):
+        print(f"# branches (config3): {br['value']:.4g} droplet-steps/s, {br['ms_per_step']:.3f} ms/step, "
+              f"film pass {br['film_pass_ms']:.3f} ms, children pass {br['children_pass_ms']:.3f} ms, "
+              f"droplets {br['droplets_start']} -> {br['droplets_end']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(prob, args.cpu_seconds)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

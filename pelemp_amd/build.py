@@ -4,8 +4,11 @@
 
 Compiles the three translation units with hipcc and links them into
 pelemp_amd/lib/libpelemp_spray.so.  The FP64 units use -ffp-contract=off
-(bitwise agreement with the CPU oracle); the FP32 unit allows contraction.
-Packed-FP32 instructions are disabled for every unit.
+(bitwise agreement with the CPU oracle); the FP32 unit allows contraction,
+reciprocal multiplication for division and the hardware approximations of
+the elementary functions (its results are held to stated tolerances against
+the FP64 oracle, not to bitwise agreement).  Packed-FP32 instructions are
+disabled for every unit.
 """
 from __future__ import annotations
 
@@ -31,7 +34,7 @@ COMMON = [
 UNITS = {
     "spray_host.hip": ["-ffp-contract=off"],
     "spray_fp64.hip": ["-ffp-contract=off"],
-    "spray_fp32.hip": ["-ffp-contract=fast"],
+    "spray_fp32.hip": ["-ffp-contract=fast", "-freciprocal-math", "-fapprox-func"],
     "spray_surrogate.hip": ["-ffp-contract=off"],
 }
 

@@ -421,7 +421,15 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
         if (xi >= R(1.)) xi = R(0.);  // rounding of a tiny negative xi
         int ic = c[q] + static_cast<int>(sh);
         const int n = a.ncell[q];
-        ic = ((ic % n) + n) % n;  // periodic
+        // periodic: ((ic % n) + n) % n, with the integer division only for a
+        // move of more than one period
+        if (ic < 0) {
+          ic += n;
+          if (ic < 0) ic = ((ic % n) + n) % n;
+        } else if (ic >= n) {
+          ic -= n;
+          if (ic >= n) ic %= n;
+        }
         c[q] = ic;
         pos[q] = xi;
       }
