@@ -89,9 +89,12 @@ __global__ void __launch_bounds__(kScanBlock) sb_scan_block_kernel(const int32_t
   if (threadIdx.x == kScanBlock - 1) block_sum[blockIdx.x] = sh[threadIdx.x];
 }
 
-// exclusive scan of the block sums (one workgroup, loops over chunks)
-__global__ void __launch_bounds__(kScanBlock) sb_scan_sums_kernel(int64_t* __restrict__ block_sum, int32_t nblocks) {
+// exclusive scan of the block sums of the n_ptr entries (one workgroup,
+// loops over chunks)
+__global__ void __launch_bounds__(kScanBlock) sb_scan_sums_kernel(int64_t* __restrict__ block_sum,
+                                                                  const int32_t* __restrict__ n_ptr) {
   __shared__ int64_t sh[kScanBlock];
+  const int32_t nblocks = (*n_ptr + kScanBlock - 1) / kScanBlock;
   int64_t carry = 0;
   for (int32_t base = 0; base < nblocks; base += kScanBlock) {
     const int32_t i = base + threadIdx.x;
@@ -125,61 +128,158 @@ struct SBGeom {
   int bndry_lo[3], ncell[3], ntile[3];
 };
 
-// children of every record; ids[0] next id, ids[1] next slot, ids[2] the
-// number of children (set by sb_total_kernel)
+// Children pass, one thread per child.  ids[0] next id, ids[1] next slot,
+// ids[2] children of this pass.  sb_total_kernel sets ids[2] and the entry
+// count; the parent of child r is the last record position s with
+// offset[s] <= r (a record with no children has the offset of its
+// successor), found by marking each record's first child and an inclusive
+// max-scan over the children.
+__global__ void sb_total_kernel(const int32_t* __restrict__ n_ptr, const int64_t* __restrict__ offset,
+                                const int32_t* __restrict__ count, int64_t* __restrict__ ids,
+                                int32_t* __restrict__ n_new, long long cap) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int32_t n = *n_ptr;
+  const long long total = (n > 0) ? offset[n - 1] + count[n - 1] : 0;
+  const long long m = static_cast<long long>(n) + total;
+  ids[2] = total;
+  *n_new = static_cast<int32_t>(m < cap ? m : cap);
+}
+
+__global__ void __launch_bounds__(256) sb_parent_init_kernel(const int64_t* __restrict__ ids, long long cap,
+                                                             int32_t* __restrict__ parent) {
+  const long long total = ids[2] < cap ? ids[2] : cap;
+  for (long long r = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; r < total;
+       r += static_cast<long long>(gridDim.x) * blockDim.x)
+    parent[r] = -1;
+}
+
+__global__ void __launch_bounds__(256) sb_parent_mark_kernel(const int32_t* __restrict__ n_ptr,
+                                                             const int32_t* __restrict__ count,
+                                                             const int64_t* __restrict__ offset, long long cap,
+                                                             int32_t* __restrict__ parent) {
+  const int32_t n = *n_ptr;
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+    if (count[s] > 0 && offset[s] < cap) parent[offset[s]] = s;
+}
+
+// inclusive max-scan of parent[0, total) in chunks of kScanBlock
+__global__ void __launch_bounds__(kScanBlock) sb_maxscan_block_kernel(int32_t* __restrict__ v,
+                                                                      const int64_t* __restrict__ ids,
+                                                                      long long cap,
+                                                                      int32_t* __restrict__ block_max) {
+  __shared__ int32_t sh[kScanBlock];
+  const long long total = ids[2] < cap ? ids[2] : cap;
+  const long long i = static_cast<long long>(blockIdx.x) * kScanBlock + threadIdx.x;
+  if (static_cast<long long>(blockIdx.x) * kScanBlock >= total) {
+    if (threadIdx.x == 0) block_max[blockIdx.x] = -1;
+    return;
+  }
+  sh[threadIdx.x] = (i < total) ? v[i] : -1;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const int32_t t = (threadIdx.x >= static_cast<unsigned>(off)) ? sh[threadIdx.x - off] : -1;
+    __syncthreads();
+    sh[threadIdx.x] = sh[threadIdx.x] > t ? sh[threadIdx.x] : t;
+    __syncthreads();
+  }
+  if (i < total) v[i] = sh[threadIdx.x];
+  if (threadIdx.x == kScanBlock - 1) block_max[blockIdx.x] = sh[threadIdx.x];
+}
+
+// exclusive max-scan of the block maxima of the children (one workgroup)
+__global__ void __launch_bounds__(kScanBlock) sb_maxscan_sums_kernel(int32_t* __restrict__ block_max,
+                                                                     const int64_t* __restrict__ ids, long long cap) {
+  __shared__ int32_t sh[kScanBlock];
+  const long long total = ids[2] < cap ? ids[2] : cap;
+  const int32_t nblocks = static_cast<int32_t>((total + kScanBlock - 1) / kScanBlock);
+  int32_t carry = -1;
+  for (int32_t base = 0; base < nblocks; base += kScanBlock) {
+    const int32_t i = base + threadIdx.x;
+    const int32_t x = (i < nblocks) ? block_max[i] : -1;
+    sh[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+      const int32_t t = (threadIdx.x >= static_cast<unsigned>(off)) ? sh[threadIdx.x - off] : -1;
+      __syncthreads();
+      sh[threadIdx.x] = sh[threadIdx.x] > t ? sh[threadIdx.x] : t;
+      __syncthreads();
+    }
+    const int32_t excl = (threadIdx.x == 0) ? -1 : sh[threadIdx.x - 1];
+    if (i < nblocks) block_max[i] = carry > excl ? carry : excl;
+    const int32_t tot = sh[kScanBlock - 1];
+    __syncthreads();
+    carry = carry > tot ? carry : tot;
+  }
+}
+
+__global__ void __launch_bounds__(kScanBlock) sb_maxscan_add_kernel(int32_t* __restrict__ v,
+                                                                    const int64_t* __restrict__ ids, long long cap,
+                                                                    const int32_t* __restrict__ block_max) {
+  const long long total = ids[2] < cap ? ids[2] : cap;
+  const long long i = static_cast<long long>(blockIdx.x) * kScanBlock + threadIdx.x;
+  if (i < total) {
+    const int32_t b = block_max[blockIdx.x];
+    v[i] = v[i] > b ? v[i] : b;
+  }
+}
+
+// child r of the pass: record s = parent[r], k = r - offset[s].  The waves
+// step through r together (lane = r mod 64) so that the arrival counters of
+// a cell are advanced once per run of consecutive children in that cell
+// (the children of one record mostly share a cell): one atomic per run.
 template <int NF>
 __global__ void __launch_bounds__(256) sb_create_kernel(
     const int32_t* __restrict__ n_ptr, const int32_t* __restrict__ sb_type, const double* __restrict__ sb_data,
-    const int64_t* __restrict__ offset, const int64_t* __restrict__ ids, FuelConst<double, NF> fc, double sub_dt,
-    double ppp_fact, uint64_t seed, double* __restrict__ f, long long cap, int32_t* __restrict__ slot,
-    int64_t* __restrict__ cold_id, long long slot_cap, uint32_t* __restrict__ key,
+    const int64_t* __restrict__ offset, const int32_t* __restrict__ parent, const int64_t* __restrict__ ids,
+    FuelConst<double, NF> fc, double sub_dt, double ppp_fact, uint64_t seed, double* __restrict__ f, long long cap,
+    int32_t* __restrict__ slot, int64_t* __restrict__ cold_id, long long slot_cap, uint32_t* __restrict__ key,
     int32_t* __restrict__ arrival, int32_t* __restrict__ mov_hi, SBGeom gm, int* err) {
   using F = Fields<NF>;
-  const double* plo = gm.plo;
-  const double* phi = gm.phi;
-  const double* dxi = gm.dxi;
-  const int* bndry_lo = gm.bndry_lo;
-  const int* ncell = gm.ncell;
-  const int* ntile = gm.ntile;
   const int32_t n = *n_ptr;
-  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const int ty = sb_type[s];
-    if (ty == kSBNone) continue;
-    SBRec<NF> r;
-    r.type = ty;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      r.norm[q] = sb_data[(0 + q) * cap + s];
-      r.vel[q] = sb_data[(3 + q) * cap + s];
-      r.loc[q] = sb_data[(6 + q) * cap + s];
+  const long long total = ids[2];
+  const long long id0 = ids[0], slot0 = ids[1];
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  for (long long base = blockIdx.x * static_cast<long long>(blockDim.x) + (threadIdx.x & ~63u); base < total;
+       base += stride) {
+    const long long r = base + lane;
+    const long long q = static_cast<long long>(n) + r;
+    const long long sl = slot0 + r;
+    bool act = r < total;
+    if (act && (q >= cap || sl >= slot_cap)) {
+      set_err(err, PELEMP_E_NOMEM);
+      act = false;
     }
-    r.T0 = sb_data[9 * cap + s];
-    r.ref_dia = sb_data[10 * cap + s];
-    r.num_dens = sb_data[11 * cap + s];
-    r.phi1 = sb_data[12 * cap + s];
-    r.phi2 = sb_data[13 * cap + s];
-    r.phi3 = sb_data[14 * cap + s];
+    uint32_t kk = 0xffffffffu;
+    if (act) {
+      const int32_t s = parent[r];
+      const int k = static_cast<int>(r - offset[s]);
+      SBRec<NF> rc;
+      rc.type = sb_type[s];
 #pragma unroll
-    for (int spf = 0; spf < NF; ++spf) r.Y0[spf] = sb_data[(kSBFields + spf) * cap + s];
-    const int nc = sb_child_count<NF>(r, ppp_fact);
-    const int64_t parent_id = cold_id[slot[s]];
-    for (int k = 0; k < nc; ++k) {
-      const long long rank = offset[s] + k;
-      const long long q = static_cast<long long>(n) + rank;
-      const long long sl = ids[1] + rank;
-      if (q >= cap || sl >= slot_cap) {
-        set_err(err, PELEMP_E_NOMEM);
-        break;
+      for (int d = 0; d < 3; ++d) {
+        rc.norm[d] = sb_data[(0 + d) * cap + s];
+        rc.vel[d] = sb_data[(3 + d) * cap + s];
+        rc.loc[d] = sb_data[(6 + d) * cap + s];
       }
+      rc.T0 = sb_data[9 * cap + s];
+      rc.ref_dia = sb_data[10 * cap + s];
+      rc.num_dens = sb_data[11 * cap + s];
+      rc.phi1 = sb_data[12 * cap + s];
+      rc.phi2 = sb_data[13 * cap + s];
+      rc.phi3 = sb_data[14 * cap + s];
+#pragma unroll
+      for (int spf = 0; spf < NF; ++spf) rc.Y0[spf] = sb_data[(kSBFields + spf) * cap + s];
+      const int64_t parent_id = cold_id[slot[s]];
       BDrop<NF> c;
-      sb_child<NF>(r, k, parent_id, fc, sub_dt, ppp_fact, seed, c);
+      sb_child<NF>(rc, k, parent_id, fc, sub_dt, ppp_fact, seed, c);
       // no valid position (SpraySB.cpp:187-190): a KH-stripped wall film has
       // no velocity, so its children's directions are 0/0
       if (!(isfinite(c.pos[0]) && isfinite(c.pos[1]) && isfinite(c.pos[2]))) set_err(err, PELEMP_E_BAD_CHILD);
       // periodic wrap of the new position (the Redistribute stand-in)
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        if (bndry_lo[d] == 0) c.pos[d] = wrap_periodic(c.pos[d], plo[d], phi[d]);
+        if (gm.bndry_lo[d] == 0) c.pos[d] = wrap_periodic(c.pos[d], gm.plo[d], gm.phi[d]);
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -196,33 +296,42 @@ __global__ void __launch_bounds__(256) sb_create_kernel(
       f[F::BM2 * cap + q] = c.bm2;
       f[F::N0 * cap + q] = c.N0;
       slot[q] = static_cast<int32_t>(sl);
-      cold_id[sl] = ids[0] + rank;
+      cold_id[sl] = id0 + r;
       int cc[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        int ic = static_cast<int>(floor((c.pos[d] - plo[d]) * dxi[d]));
-        ic = ic < 0 ? 0 : (ic >= ncell[d] ? ncell[d] - 1 : ic);
+        int ic = static_cast<int>(floor((c.pos[d] - gm.plo[d]) * gm.dxi[d]));
+        ic = ic < 0 ? 0 : (ic >= gm.ncell[d] ? gm.ncell[d] - 1 : ic);
         cc[d] = ic;
       }
-      const uint32_t kk = tile_key(cc[0], cc[1], cc[2], ntile);
+      kk = tile_key(cc[0], cc[1], cc[2], gm.ntile);
       key[q] = kk;
-      arrival[q] = atomicAdd(&mov_hi[kk], 1) | (2 << 30);
     }
+    // runs of equal keys in consecutive lanes: the first lane of a run takes
+    // the run's arrival indices with one atomic and hands them out
+    const uint32_t kprev = __shfl_up(kk, 1);
+    const bool same = act && lane > 0 && kprev == kk;
+    const unsigned long long same_mask = __ballot(same);
+    const bool lead = act && !same;
+    int32_t first = 0;
+    if (lead) {
+      const int run = 1 + ((lane == 63) ? 0 : __builtin_ctzll(~(same_mask >> (lane + 1))));
+      first = atomicAdd(&mov_hi[kk], run);
+    }
+    const unsigned long long lead_mask = __ballot(lead);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long mine = lead_mask & upto;
+    const int leader = mine ? 63 - __builtin_clzll(mine) : lane;
+    const int32_t b = __shfl(first, leader);
+    if (act) arrival[q] = (b + (lane - leader)) | (2 << 30);
   }
 }
 
-// after sb_create: advance the id / slot counters and the entry count
-__global__ void sb_total_kernel(const int32_t* __restrict__ n_ptr, const int64_t* __restrict__ offset,
-                                const int32_t* __restrict__ count, int64_t* __restrict__ ids,
-                                int32_t* __restrict__ n_new, long long cap) {
+// after sb_create: advance the id and slot counters by the children made
+__global__ void sb_advance_kernel(int64_t* __restrict__ ids) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int32_t n = *n_ptr;
-  const long long total = (n > 0) ? offset[n - 1] + count[n - 1] : 0;
-  const long long m = static_cast<long long>(n) + total;
-  ids[0] += total;
-  ids[1] += total;
-  ids[2] = total;
-  *n_new = static_cast<int32_t>(m < cap ? m : cap);
+  ids[0] += ids[2];
+  ids[1] += ids[2];
 }
 
 }  // namespace pelemp

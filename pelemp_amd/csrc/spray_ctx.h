@@ -102,6 +102,8 @@ struct Ctx {
   int32_t* sb_count = nullptr;   // children per position
   int64_t* sb_offset = nullptr;  // their exclusive scan
   int64_t* sb_block = nullptr;   // scan block sums over positions
+  int32_t* sb_parent = nullptr;  // record position of each child of the pass
+  int32_t* sb_pblock = nullptr;  // its max-scan block maxima
   int64_t* sb_ids = nullptr;    // [next id, next slot, children of the last pass]
   int32_t* n_new = nullptr;     // entries after the children pass
   int64_t slot_cap = 0;         // capacity of the cold (per-slot) arrays

@@ -410,13 +410,23 @@ static int run_children(Ctx& c, const PassParams& p, hipStream_t s) {
     hipLaunchKernelGGL((sb_count_kernel<NF>), dim3(2048), dim3(256), 0, s, n_ptr, c.sb_type, c.sb_data, c.cap,
                        c.params.breakup_parcel_factor, count);
     hipLaunchKernelGGL(sb_scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, count, offset, n_ptr, c.sb_block);
-    hipLaunchKernelGGL(sb_scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.sb_block, static_cast<int32_t>(nb));
+    hipLaunchKernelGGL(sb_scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.sb_block, n_ptr);
     hipLaunchKernelGGL(sb_scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, offset, n_ptr, c.sb_block);
-    hipLaunchKernelGGL((sb_create_kernel<NF>), dim3(2048), dim3(256), 0, s, n_ptr, c.sb_type, c.sb_data, offset,
-                       c.sb_ids, make_fuel_const<double, NF>(c.sd, c.mw.data()), p.sub_dt,
+    hipLaunchKernelGGL(sb_total_kernel, dim3(1), dim3(64), 0, s, n_ptr, offset, count, c.sb_ids, c.n_new, c.cap);
+    // parent record of every child: mark first children, inclusive max-scan
+    hipLaunchKernelGGL(sb_parent_init_kernel, dim3(2048), dim3(256), 0, s, c.sb_ids, c.cap, c.sb_parent);
+    hipLaunchKernelGGL(sb_parent_mark_kernel, dim3(2048), dim3(256), 0, s, n_ptr, count, offset, c.cap,
+                       c.sb_parent);
+    hipLaunchKernelGGL(sb_maxscan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.sb_parent, c.sb_ids, c.cap,
+                       c.sb_pblock);
+    hipLaunchKernelGGL(sb_maxscan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.sb_pblock, c.sb_ids, c.cap);
+    hipLaunchKernelGGL(sb_maxscan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.sb_parent, c.sb_ids, c.cap,
+                       c.sb_pblock);
+    hipLaunchKernelGGL((sb_create_kernel<NF>), dim3(4096), dim3(256), 0, s, n_ptr, c.sb_type, c.sb_data, offset,
+                       c.sb_parent, c.sb_ids, make_fuel_const<double, NF>(c.sd, c.mw.data()), p.sub_dt,
                        c.params.breakup_parcel_factor, c.params.rng_seed, static_cast<double*>(c.fbuf[1 - c.cur]),
                        c.cap, c.slot[1 - c.cur], c.cold_id, c.slot_cap, c.key, c.arrival, c.mov_hi, gm, c.err);
-    hipLaunchKernelGGL(sb_total_kernel, dim3(1), dim3(64), 0, s, n_ptr, offset, count, c.sb_ids, c.n_new, c.cap);
+    hipLaunchKernelGGL(sb_advance_kernel, dim3(1), dim3(64), 0, s, c.sb_ids);
     (void)sizeof(M);
   };
   if (c.mech == PELEMP_MECH_DODECANE_AIR && c.nf == 1) {
@@ -471,7 +481,7 @@ static void free_ctx(Ctx& c) {
                   c.arrival, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
                   c.count_log, c.surr_w, c.surr_cnt, c.wf, c.sb_type, c.sb_data, c.sb_count, c.sb_offset,
-                  c.sb_block, c.sb_ids, c.n_new};
+                  c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -660,6 +670,8 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
     st |= alloc(c, reinterpret_cast<void**>(&c.sb_count), 4 * static_cast<size_t>(c.cap));
     st |= alloc(c, reinterpret_cast<void**>(&c.sb_offset), 8 * static_cast<size_t>(c.cap));
     st |= alloc(c, reinterpret_cast<void**>(&c.sb_block), 8 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_parent), 4 * static_cast<size_t>(c.cap));
+    st |= alloc(c, reinterpret_cast<void**>(&c.sb_pblock), 4 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
     st |= alloc(c, reinterpret_cast<void**>(&c.sb_ids), 8 * 4);
     st |= alloc(c, reinterpret_cast<void**>(&c.n_new), 64);
   }
