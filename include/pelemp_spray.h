@@ -187,10 +187,27 @@ int pelemp_spray_move_kick_drift(pelemp_spray_ctx* ctx,
                                  void* stream);
 
 /* Particle-CFL time step, SprayParticleContainer::estTimestep
- * (SprayParticles.cpp:89-140, without the jet-velocity bound).  Returns -1
- * in *dt when fixed_parts is set or there are no particles. */
-int pelemp_spray_est_timestep(pelemp_spray_ctx* ctx, double cfl, double* dt,
-                              void* stream);
+ * (SprayParticles.cpp:89-140): the minimum over the droplets of
+ * cfl / max_d(|u_d| / dx_d) (1e50 for a droplet at rest), then, when
+ * max_jet_vel > 0, the minimum with cfl dx_0 / max_jet_vel (the bound the
+ * reference takes from its spray jets, :125-131; max_jet_vel is the largest
+ * jet velocity).  *dt is -1 when fixed_parts is set and the largest double
+ * when there are no droplets and no jet bound.  Waits for `stream`. */
+int pelemp_spray_est_timestep(pelemp_spray_ctx* ctx, double cfl,
+                              double max_jet_vel, double* dt, void* stream);
+
+/* SprayParticleContainer::transferSource (SprayParticles.H:250-295) for one
+ * box covering the domain: the ghost cells of tmp_source are added to the
+ * valid cells they are periodic images of (SumBoundary), tmp_source is added
+ * to act_source over the domain grown by min(act_source ghosts,
+ * source_ghosts) -- every component when both have the same number of
+ * components (MultiFab::Add), otherwise through the spray component map of
+ * addSpraySrc (SprayParticles.H:299-319: momentum, density, energy, fuel
+ * species at specIndx + dep_indx) -- and tmp_source is zeroed.  Device
+ * FABs owned by the caller; asynchronous on `stream`. */
+int pelemp_spray_transfer_source(pelemp_spray_ctx* ctx, int32_t source_ghosts,
+                                 pelemp_fab_view* tmp_source,
+                                 pelemp_fab_view* act_source, void* stream);
 
 /* Active droplets at the start of each update pass since the previous call
  * (at most 4096 passes are logged); waits for the device.  The sum is the

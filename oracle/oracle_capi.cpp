@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "spray_oracle.h"
+#include "spray_oracle_transfer.h"
 
 // Breakup / splash parameters of one call (in: container parameters, the id
 // of the next new droplet and the capacity of the output arrays; out: next id).
@@ -296,6 +297,18 @@ int oracle_khrt_update(const pelemp_spray_data* sd, double Reyn, double dt, cons
   out[13] = rf.phi1;
   cboil_out[0] = cB[0];
   return static_cast<int>(p.id);
+}
+
+// transferSource / addSpraySrc on host FABs (see spray_oracle_transfer.h).
+int oracle_transfer_source(const pelemp_geom* g, const pelemp_spray_comps* comps, const int* dep_indx, int nfuel,
+                           int source_ghosts, pelemp_fab_view* tmp, pelemp_fab_view* act) {
+  return oracle::transfer_source(*g, *comps, dep_indx, nfuel, source_ghosts, *tmp, *act);
+}
+
+// estTimestep over vel[3n].
+double oracle_est_timestep(const pelemp_geom* g, int fixed_parts, int64_t n, const double* vel, double cfl,
+                           double max_jet_vel) {
+  return oracle::est_timestep(*g, fixed_parts != 0, n, vel, cfl, max_jet_vel);
 }
 
 // splash_criteria (AhamedSplash.H:125-197): returns the splash_type;

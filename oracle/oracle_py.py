@@ -69,6 +69,12 @@ def lib() -> ctypes.CDLL:
         L.oracle_khrt_update.restype = ctypes.c_int
         L.oracle_khrt_update.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_double, vp, vp,
                                          ctypes.c_int, vp, vp, vp, vp]
+        L.oracle_transfer_source.restype = ctypes.c_int
+        L.oracle_transfer_source.argtypes = [ctypes.POINTER(C.GeomC), ctypes.POINTER(C.SprayCompsC), vp, ctypes.c_int,
+                                             ctypes.c_int, ctypes.POINTER(C.FabViewC), ctypes.POINTER(C.FabViewC)]
+        L.oracle_est_timestep.restype = ctypes.c_double
+        L.oracle_est_timestep.argtypes = [ctypes.POINTER(C.GeomC), ctypes.c_int, ctypes.c_int64, vp, ctypes.c_double,
+                                          ctypes.c_double]
         L.oracle_splash_criteria.restype = ctypes.c_int
         L.oracle_splash_criteria.argtypes = [ctypes.c_double] * 5 + [vp]
         L.oracle_num_threads.restype = ctypes.c_int
@@ -196,6 +202,27 @@ def khrt_update(sdata: dict, Reyn: float, dt: float, gas, rdata, pos=(0.0, 0.0, 
     pid = lib().oracle_khrt_update(ctypes.byref(sd), Reyn, dt, gas.ctypes.data, kp.ctypes.data, int(make_new_drops),
                                    ps.ctypes.data, rd.ctypes.data, out.ctypes.data, cb.ctypes.data)
     return pid, rd, out, float(cb[0])
+
+
+def transfer_source(geom: dict, comps: dict, dep_indx, source_ghosts: int, tmp: np.ndarray, tmp_lo, tmp_hi,
+                    act: np.ndarray, act_lo, act_hi) -> int:
+    """transferSource on host FABs [ncomp, nz, ny, nx] (updated in place):
+    periodic SumBoundary of tmp, add into act (all components when the counts
+    match, else the addSpraySrc map), zero tmp.  Returns the status."""
+    assert tmp.flags["C_CONTIGUOUS"] and act.flags["C_CONTIGUOUS"]
+    dep = np.ascontiguousarray(dep_indx, dtype=np.int32)
+    tv = C.fab_view(tmp.ctypes.data, tmp_lo, tmp_hi, tmp.shape[0])
+    av = C.fab_view(act.ctypes.data, act_lo, act_hi, act.shape[0])
+    return lib().oracle_transfer_source(ctypes.byref(C.geom_c(geom)), ctypes.byref(C.comps_c(comps)),
+                                        dep.ctypes.data, int(dep.size), int(source_ghosts), ctypes.byref(tv),
+                                        ctypes.byref(av))
+
+
+def est_timestep(geom: dict, vel, cfl: float = 0.5, max_jet_vel: float = 0.0, fixed_parts: bool = False) -> float:
+    """estTimestep over the droplet velocities vel[n, 3]."""
+    v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+    return lib().oracle_est_timestep(ctypes.byref(C.geom_c(geom)), int(fixed_parts), v.shape[0], v.ctypes.data,
+                                     float(cfl), float(max_jet_vel))
 
 
 def splash_criteria(Kv: float, Tstar: float, alpha: float, Kcrit: float, del_film: float):

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -19,6 +20,7 @@
 #include "spray_ctx.h"
 #include "update_kernel.h"
 #include "branch_kernels.h"
+#include "transfer_kernels.h"
 
 struct pelemp_spray_ctx {
   pelemp::Ctx c;
@@ -1056,7 +1058,7 @@ int pelemp_spray_sync(pelemp_spray_ctx* h, void* stream) {
   return e;
 }
 
-int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double* dt, void* stream) {
+int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double max_jet_vel, double* dt, void* stream) {
   if (!h || !dt) return PELEMP_E_ARG;
   Ctx& c = h->c;
   if (c.sd.fixed_parts) {
@@ -1081,9 +1083,95 @@ int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double* dt, void*
   PM_HIP(hipStreamSynchronize(s));
   double v;
   memcpy(&v, &bits, 8);
+  // no droplets: the reduction's identity, the largest double (the
+  // reference starts from std::numeric_limits<Real>::max())
   int32_t t = 0;
   PM_HIP(hipMemcpy(&t, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
-  *dt = (t == 0) ? -1. : v;
+  double d = (t == 0) ? std::numeric_limits<double>::max() : v;
+  // injection bound of the jets (SprayParticles.cpp:125-131)
+  if (max_jet_vel > 0.) d = std::min(d, cfl * c.geom.dx[0] / max_jet_vel);
+  *dt = d;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_transfer_source(pelemp_spray_ctx* h, int32_t source_ghosts, pelemp_fab_view* tmp,
+                                 pelemp_fab_view* act, void* stream) {
+  if (!h || !tmp || !act || !tmp->data || !act->data || tmp->data == act->data || source_ghosts < 0) {
+    set_last_error("transfer_source: invalid argument");
+    return PELEMP_E_ARG;
+  }
+  Ctx& c = h->c;
+  const pelemp_geom& g = c.geom;
+  int ng_tmp = 1 << 30, ng_act = 1 << 30;
+  for (int d = 0; d < 3; ++d) {
+    ng_tmp = std::min(ng_tmp, std::min(g.domain_lo[d] - tmp->lo[d], tmp->hi[d] - g.domain_hi[d]));
+    ng_act = std::min(ng_act, std::min(g.domain_lo[d] - act->lo[d], act->hi[d] - g.domain_hi[d]));
+  }
+  if (ng_tmp < 0 || ng_act < 0) {
+    set_last_error("transfer_source: the FABs must cover the domain");
+    return PELEMP_E_ARG;
+  }
+  XferMap map{};
+  if (tmp->ncomp == act->ncomp) {  // MultiFab::Add of every component
+    if (tmp->ncomp > kXferMaxPairs) {
+      set_last_error("transfer_source: too many components");
+      return PELEMP_E_ARG;
+    }
+    for (int q = 0; q < tmp->ncomp; ++q) {
+      map.from[q] = q;
+      map.to[q] = q;
+    }
+    map.npair = tmp->ncomp;
+  } else {  // addSpraySrc (SprayParticles.H:299-319)
+    const pelemp_spray_comps& S = c.comps;
+    int n = 0;
+    for (int dir = 0; dir < 3; ++dir) {
+      map.from[n] = S.momSrcIndx + dir;
+      map.to[n++] = S.momIndx + dir;
+    }
+    map.from[n] = S.rhoSrcIndx;
+    map.to[n++] = S.rhoIndx;
+    map.from[n] = S.engSrcIndx;
+    map.to[n++] = S.engIndx;
+    for (int sp = 0; sp < c.nf; ++sp) {
+      map.from[n] = S.specSrcIndx + sp;
+      map.to[n++] = S.specIndx + c.sd.dep_indx[sp];
+    }
+    map.npair = n;
+    for (int q = 0; q < n; ++q) {
+      if (map.from[q] < 0 || map.from[q] >= tmp->ncomp || map.to[q] < 0 || map.to[q] >= act->ncomp) {
+        set_last_error("transfer_source: spray component outside the FABs");
+        return PELEMP_E_ARG;
+      }
+    }
+  }
+  auto fab = [](const pelemp_fab_view& v) {
+    XferFab f;
+    f.data = v.data;
+    for (int d = 0; d < 3; ++d) {
+      f.lo[d] = v.lo[d];
+      f.n[d] = v.hi[d] - v.lo[d] + 1;
+    }
+    f.cstride = static_cast<long long>(f.n[0]) * f.n[1] * f.n[2];
+    return f;
+  };
+  XferGeom xg;
+  xg.nghost = std::min(std::min(ng_act, static_cast<int>(source_ghosts)), ng_tmp);
+  for (int d = 0; d < 3; ++d) {
+    xg.dlo[d] = g.domain_lo[d];
+    xg.N[d] = g.domain_hi[d] - g.domain_lo[d] + 1;
+    xg.periodic[d] = (g.bndry_lo[d] == 0) ? 1 : 0;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const XferFab ft = fab(*tmp), fa = fab(*act);
+  long long ncell = 1;
+  for (int d = 0; d < 3; ++d) ncell *= xg.N[d] + 2 * xg.nghost;
+  const unsigned grid = static_cast<unsigned>(std::min<long long>((ncell + 255) / 256, 16384));
+  hipLaunchKernelGGL(transfer_add_kernel, dim3(grid), dim3(256), 0, s, ft, fa, map, xg);
+  const long long ntmp = ft.cstride * tmp->ncomp;
+  const unsigned gz = static_cast<unsigned>(std::min<long long>((ntmp + 255) / 256, 16384));
+  hipLaunchKernelGGL(zero_fill_kernel, dim3(gz), dim3(256), 0, s, tmp->data, ntmp);
+  PM_HIP(hipGetLastError());
   return PELEMP_OK;
 }
 

@@ -11,6 +11,8 @@ Method names and argument meanings follow the reference
                   spray_cfl_lev)                         SprayParticles.H:212-226
     moveKick(...)                                        SprayParticles.H:196-209
     estTimestep(level)                                   SprayParticles.H:110
+    transferSource(source_ghosts, level, tmpSource,
+                   actSource)                            SprayParticles.H:250-295
     getSprayData() / getSprayComps()                     SprayParticles.H:378-384
 
 Every call goes through the C ABI of libpelemp_spray.so (HIP kernels for
@@ -196,14 +198,28 @@ class SprayParticleContainer:
     def sync(self, stream=None) -> None:
         C.check(self._lib.pelemp_spray_sync(self._h, _stream_handle(stream)), "device check")
 
-    def estTimestep(self, level: int = 0, cfl: float = 0.5, stream=None) -> float:
-        """SprayParticleContainer::estTimestep (SprayParticles.cpp:89-140),
-        without the spray-jet velocity bound."""
+    def estTimestep(self, level: int = 0, cfl: float = 0.5, max_jet_vel: float = 0.0, stream=None) -> float:
+        """SprayParticleContainer::estTimestep (SprayParticles.cpp:89-140):
+        cfl = spray_cfl; max_jet_vel is the largest velocity of the spray jets
+        (the reference reads it from its jets; 0 for none)."""
         del level
         dt = ctypes.c_double()
-        C.check(self._lib.pelemp_spray_est_timestep(self._h, float(cfl), ctypes.byref(dt), _stream_handle(stream)),
-                "pelemp_spray_est_timestep")
+        C.check(self._lib.pelemp_spray_est_timestep(self._h, float(cfl), float(max_jet_vel), ctypes.byref(dt),
+                                                    _stream_handle(stream)), "pelemp_spray_est_timestep")
         return float(dt.value)
+
+    def transferSource(self, source_ghosts: int, level: int, tmpSource: "DeviceFab", actSource: "DeviceFab",
+                       stream=None, sync: bool = True) -> None:
+        """SprayParticleContainer::transferSource (SprayParticles.H:250-295):
+        periodic SumBoundary of tmpSource, add into actSource (every component,
+        or the addSpraySrc map when the component counts differ), then
+        tmpSource.setVal(0)."""
+        del level
+        tv, av = tmpSource.view(), actSource.view()
+        C.check(self._lib.pelemp_spray_transfer_source(self._h, int(source_ghosts), ctypes.byref(tv), ctypes.byref(av),
+                                                       _stream_handle(stream)), "pelemp_spray_transfer_source")
+        if sync:
+            C.device_synchronize()
 
     # -- timing ---------------------------------------------------------------
     def pass_counts(self) -> list:

@@ -1,0 +1,105 @@
+"""Pins of the oracle's transferSource / addSpraySrc and estTimestep
+(oracle/spray_oracle_transfer.h; SprayParticles.H:250-319,
+SprayParticles.cpp:89-140) by brute force and exact integer-valued sums."""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+
+COMPS = dict(rhoIndx=0, momIndx=1, engIndx=4, utempIndx=5, specIndx=6, rhoSrcIndx=3, momSrcIndx=0, engSrcIndx=4,
+             specSrcIndx=5)
+
+
+def _geom(n=(6, 5, 4), bndry_lo=(0, 0, 0), bndry_hi=(0, 0, 0)):
+    dx = [1e-4, 2e-4, 5e-5]
+    return dict(plo=[0.0] * 3, phi=[n[d] * dx[d] for d in range(3)], dx=dx, domain_lo=[0, 0, 0],
+                domain_hi=[n[0] - 1, n[1] - 1, n[2] - 1], bndry_lo=list(bndry_lo), bndry_hi=list(bndry_hi))
+
+
+def _fab(n, ng, ncomp, rng=None):
+    shape = (ncomp, n[2] + 2 * ng, n[1] + 2 * ng, n[0] + 2 * ng)
+    a = np.zeros(shape) if rng is None else rng.integers(-50, 50, size=shape).astype(np.float64)
+    return a, (-ng, -ng, -ng), (n[0] - 1 + ng, n[1] - 1 + ng, n[2] - 1 + ng)
+
+
+def test_est_timestep_brute_force():
+    g = _geom()
+    rng = np.random.default_rng(3)
+    vel = rng.normal(0.0, 20.0, size=(500, 3))
+    dt = O.est_timestep(g, vel, cfl=0.5)
+    ref = min(0.5 / max(abs(v[d]) / g["dx"][d] for d in range(3)) for v in vel)
+    assert dt == ref
+    # a droplet at rest gives 1e50, no droplets the largest double
+    assert O.est_timestep(g, np.zeros((3, 3))) == 1e50
+    assert O.est_timestep(g, np.zeros((0, 3))) == np.finfo(np.float64).max
+    # the jet bound cfl dx_0 / max_jet_vel, and fixed particles
+    assert O.est_timestep(g, np.zeros((1, 3)), cfl=0.5, max_jet_vel=100.0) == 0.5 * 1e-4 / 100.0
+    assert O.est_timestep(g, vel, fixed_parts=True) == -1.0
+
+
+def test_transfer_conserves_the_periodic_source():
+    # all periodic, act without ghosts: every tmp cell (valid or ghost) lands
+    # in exactly one valid cell; integer values make the sums exact
+    n = (6, 5, 4)
+    g = _geom(n)
+    rng = np.random.default_rng(5)
+    tmp, tlo, thi = _fab(n, 2, 7, rng)
+    act, alo, ahi = _fab(n, 0, 7)
+    tot = tmp.sum(axis=(1, 2, 3))
+    assert O.transfer_source(g, COMPS, [3], 2, tmp, tlo, thi, act, alo, ahi) == 0
+    np.testing.assert_array_equal(act.sum(axis=(1, 2, 3)), tot)
+    assert not tmp.any()  # setVal(0), ghost cells included
+
+
+def test_transfer_ghost_images_and_walls():
+    n = (6, 5, 4)
+    g = _geom(n, bndry_lo=(0, 0, 1), bndry_hi=(0, 0, 1))  # z not periodic
+    tmp, tlo, thi = _fab(n, 2, 1)
+    act, alo, ahi = _fab(n, 1, 1)
+
+    def at(a, lo, i, j, k):
+        return a[0, k - lo[2], j - lo[1], i - lo[0]]
+
+    tmp[0, 0 - tlo[2], 1 - tlo[1], -1 - tlo[0]] = 7.0      # x ghost -> valid (5, 1, 0)
+    tmp[0, 2 - tlo[2], -2 - tlo[1], 6 - tlo[0]] = 11.0     # x and y ghost -> valid (0, 3, 2)
+    tmp[0, -1 - tlo[2], 2 - tlo[1], 2 - tlo[0]] = 13.0     # below the z wall: no image
+    tmp[0, 1 - tlo[2], 1 - tlo[1], 1 - tlo[0]] = 3.0       # valid cell itself
+    assert O.transfer_source(g, COMPS, [3], 1, tmp, tlo, thi, act, alo, ahi) == 0
+    assert at(act, alo, 5, 1, 0) == 7.0
+    assert at(act, alo, 0, 3, 2) == 11.0
+    assert at(act, alo, 1, 1, 1) == 3.0
+    # ghost cells of act within min(act ghosts, source_ghosts) = 1 receive
+    # tmp's ghost values as they are (MultiFab::Add over the grown box)
+    assert at(act, alo, -1, 1, 0) == 7.0
+    assert at(act, alo, 2, 2, -1) == 13.0
+    assert act.sum() == 7.0 * 2 + 11.0 + 3.0 + 13.0
+
+
+def test_add_spray_src_component_map():
+    # tmp holds the spray components (mom 0-2, rho 3, eng 4, species 5..),
+    # act the full state: momentum, density, energy and species dep_indx
+    n = (4, 4, 4)
+    g = _geom(n)
+    nf = 2
+    tmp, tlo, thi = _fab(n, 0, 5 + nf)
+    for c in range(5 + nf):
+        tmp[c] = 10.0 * (c + 1)
+    act, alo, ahi = _fab(n, 0, 6 + 6)
+    dep = [2, 0]
+    assert O.transfer_source(g, COMPS, dep, 0, tmp, tlo, thi, act, alo, ahi) == 0
+    want = np.zeros(12)
+    want[COMPS["momIndx"]:COMPS["momIndx"] + 3] = [10.0, 20.0, 30.0]
+    want[COMPS["rhoIndx"]] = 40.0
+    want[COMPS["engIndx"]] = 50.0
+    want[COMPS["specIndx"] + 2] = 60.0
+    want[COMPS["specIndx"] + 0] = 70.0
+    np.testing.assert_array_equal(act[:, 1, 1, 1], want)
+    np.testing.assert_array_equal(act[:, 0, 3, 2], want)  # no ghosts, no images added twice
+
+
+def test_transfer_needs_fabs_covering_the_domain():
+    n = (4, 4, 4)
+    g = _geom(n)
+    tmp, tlo, thi = _fab(n, 0, 7)
+    act, alo, ahi = _fab((3, 4, 4), 0, 7)
+    assert O.transfer_source(g, COMPS, [3], 0, tmp, tlo, thi, act, alo, ahi) == 6
