@@ -209,6 +209,23 @@ int pelemp_spray_transfer_source(pelemp_spray_ctx* ctx, int32_t source_ghosts,
                                  pelemp_fab_view* tmp_source,
                                  pelemp_fab_view* act_source, void* stream);
 
+/* SprayParticleContainer::computeDerivedVars (SprayDerive.cpp:9-155): adds
+ * the per-cell spray diagnostics to the FAB mf_var (covering the domain)
+ * from component start_indx on, in the order of SpraySetup.cpp:213-239:
+ * spray_mass, spray_density, spray_num, spray_vol, spray_surf_area,
+ * spray_vol_frac, d10, d32, wall_film_hght, wall_film_mass, spray_temp,
+ * num_parcels, spray_x/y/z_vel, and with derive_plot_species == 1 and
+ * several fuels spray_mass_<fuel> per fuel; then the protected divides
+ * (d10, d32, temperature, velocity).  Each cell's droplets are summed in
+ * cell-sorted order.  pelemp_spray_derive_count gives the number of
+ * variables (NumDeriveVars, SprayParticles.H:366-369).  Asynchronous on
+ * `stream`. */
+int pelemp_spray_derive_count(const pelemp_spray_ctx* ctx,
+                              int32_t derive_plot_species, int32_t* n);
+int pelemp_spray_compute_derived(pelemp_spray_ctx* ctx, pelemp_fab_view* mf_var,
+                                 int32_t start_indx, int32_t derive_plot_species,
+                                 void* stream);
+
 /* Active droplets at the start of each update pass since the previous call
  * (at most 4096 passes are logged); waits for the device.  The sum is the
  * droplet-step count of those passes. */

@@ -305,6 +305,17 @@ int oracle_transfer_source(const pelemp_geom* g, const pelemp_spray_comps* comps
   return oracle::transfer_source(*g, *comps, dep_indx, nfuel, source_ghosts, *tmp, *act);
 }
 
+// computeDerivedVars for host particles (pos[3n], rdata[n*NSR]).
+int oracle_compute_derived(int nf, const pelemp_geom* g, const pelemp_spray_data* sd, int64_t n, const double* pos,
+                           const double* rdata, pelemp_fab_view* mf, int start_indx, int with_species) {
+  switch (nf) {
+    case 1: return oracle::compute_derived<1>(*g, *sd, n, pos, rdata, *mf, start_indx, with_species != 0);
+    case 2: return oracle::compute_derived<2>(*g, *sd, n, pos, rdata, *mf, start_indx, with_species != 0);
+    case 6: return oracle::compute_derived<6>(*g, *sd, n, pos, rdata, *mf, start_indx, with_species != 0);
+    default: return PELEMP_E_ARG;
+  }
+}
+
 // estTimestep over vel[3n].
 double oracle_est_timestep(const pelemp_geom* g, int fixed_parts, int64_t n, const double* vel, double cfl,
                            double max_jet_vel) {

@@ -72,6 +72,10 @@ def lib() -> ctypes.CDLL:
         L.oracle_transfer_source.restype = ctypes.c_int
         L.oracle_transfer_source.argtypes = [ctypes.POINTER(C.GeomC), ctypes.POINTER(C.SprayCompsC), vp, ctypes.c_int,
                                              ctypes.c_int, ctypes.POINTER(C.FabViewC), ctypes.POINTER(C.FabViewC)]
+        L.oracle_compute_derived.restype = ctypes.c_int
+        L.oracle_compute_derived.argtypes = [ctypes.c_int, ctypes.POINTER(C.GeomC), ctypes.POINTER(C.SprayDataC),
+                                             ctypes.c_int64, vp, vp, ctypes.POINTER(C.FabViewC), ctypes.c_int,
+                                             ctypes.c_int]
         L.oracle_est_timestep.restype = ctypes.c_double
         L.oracle_est_timestep.argtypes = [ctypes.POINTER(C.GeomC), ctypes.c_int, ctypes.c_int64, vp, ctypes.c_double,
                                           ctypes.c_double]
@@ -216,6 +220,20 @@ def transfer_source(geom: dict, comps: dict, dep_indx, source_ghosts: int, tmp: 
     return lib().oracle_transfer_source(ctypes.byref(C.geom_c(geom)), ctypes.byref(C.comps_c(comps)),
                                         dep.ctypes.data, int(dep.size), int(source_ghosts), ctypes.byref(tv),
                                         ctypes.byref(av))
+
+
+def compute_derived(sdata: dict, geom: dict, pos, rdata, mf: np.ndarray, mf_lo, mf_hi, start_indx: int = 0,
+                    with_species: bool = True) -> int:
+    """computeDerivedVars into the host FAB mf[ncomp, nz, ny, nx] (added to,
+    then the protected divides).  Returns the status."""
+    nf = int(sdata["nfuel"])
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    rdata = np.ascontiguousarray(rdata, dtype=np.float64)
+    assert mf.flags["C_CONTIGUOUS"]
+    mv = C.fab_view(mf.ctypes.data, mf_lo, mf_hi, mf.shape[0])
+    return lib().oracle_compute_derived(nf, ctypes.byref(C.geom_c(geom)), ctypes.byref(C.spray_data_c(sdata)),
+                                        pos.shape[0], pos.ctypes.data, rdata.ctypes.data, ctypes.byref(mv),
+                                        int(start_indx), int(with_species))
 
 
 def est_timestep(geom: dict, vel, cfl: float = 0.5, max_jet_vel: float = 0.0, fixed_parts: bool = False) -> float:

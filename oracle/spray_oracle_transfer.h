@@ -9,10 +9,15 @@
 //      tmpSource.setVal(0))
 //   Source/PP_Spray/SprayParticles.H:299-319  addSpraySrc
 //   Source/PP_Spray/SprayParticles.cpp:89-140 estTimestep
+//   Source/PP_Spray/SprayDerive.cpp:9-155     computeDerivedVars (list of
+//     SpraySetup.cpp:213-239)
 // for one level held in one box that covers the domain (the layout of this
 // build's container).
 //
-// Deviation (documented): SumBoundary adds the ghost cells of tmpSource to
+// Deviations (documented): computeDerivedVars sums each cell's droplets in
+// cell-sorted order (the reference adds with atomics) and uses the droplet's
+// containing cell clamped into the domain (the reference indexes the
+// unclamped cell).  SumBoundary adds the ghost cells of tmpSource to
 // the valid cells they are periodic images of; a valid cell near a corner has
 // several images and AMReX leaves their order to its communication tags.
 // Here they are added in a fixed order: shifts (sz, sy, sx) in {-1,0,1}^3,
@@ -23,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <vector>
 
 #include "../include/pelemp_spray.h"
 
@@ -132,6 +138,101 @@ inline double est_timestep(const pelemp_geom& g, bool fixed_parts, int64_t n, co
   }
   if (max_jet_vel > 0.) dt = std::min(dt, cfl * g.dx[0] / max_jet_vel);
   return dt;
+}
+
+// computeDerivedVars (SprayDerive.cpp:9-155) for particles in any order:
+// pos[3n], rdata[n*NSR] (SprayComps layout), sorted here by containing cell
+// like the update.  mf_var[ncomp][nz][ny][nx] on box [lo, hi] (covering the
+// domain); the spray variables start at start_indx: mass, density, num,
+// volume, surface, volume fraction, d10, d32, wall-film height, wall-film
+// mass, temperature, parcels, velocity(3), then, when with_species, the
+// liquid mass of each fuel.
+template <int NF>
+inline int compute_derived(const pelemp_geom& g, const pelemp_spray_data& fdat, int64_t n, const double* pos,
+                           const double* rdata, pelemp_fab_view& mf, int start_indx, bool with_species) {
+  using C = Comps<NF>;
+  constexpr int NSR = C::pstateNum;
+  // the per-fuel masses are listed only for several fuels (SpraySetup.cpp:236-240)
+  const int nvar = 15 + ((with_species && NF > 1) ? NF : 0);
+  if (start_indx < 0 || start_indx + nvar > mf.ncomp) return PELEMP_E_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (mf.lo[d] > g.domain_lo[d] || mf.hi[d] < g.domain_hi[d]) return PELEMP_E_ARG;
+  const double dxi[3] = {1. / g.dx[0], 1. / g.dx[1], 1. / g.dx[2]};
+  const double cell_vol = g.dx[0] * g.dx[1] * g.dx[2];
+  const double face_area = 1. * g.dx[0] * g.dx[0];
+  std::vector<int64_t> key(n), order(n);
+  for (int64_t i = 0; i < n; ++i) {
+    int c[3];
+    containing_cell(&pos[3 * i], g, dxi, c);
+    key[i] = cell_key(c, g);
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  const int mass_i = 0, dens_i = 1, num_i = 2, vol_i = 3, surf_i = 4, volf_i = 5, d10_i = 6, d32_i = 7,
+            wfh_i = 8, wfm_i = 9, temp_i = 10, nump_i = 11, vel_i = 12, spec_i = 15;
+  int64_t a = 0;
+  while (a < n) {
+    int64_t b = a;
+    while (b < n && key[order[b]] == key[order[a]]) ++b;
+    double acc[16 + NF];
+    for (int q = 0; q < 16 + NF; ++q) acc[q] = 0.;
+    for (int64_t r = a; r < b; ++r) {
+      const double* rd = &rdata[order[r] * NSR];
+      const double T_part = rd[C::pstateT];
+      const double dia_part = rd[C::pstateDia];
+      double rho_part = 0.;
+      for (int spf = 0; spf < NF; ++spf) rho_part += rd[C::pstateY + spf] / rhoL(fdat, T_part, spf);
+      rho_part = 1. / rho_part;
+      const double surf = kPI * dia_part * dia_part;
+      const double vol = kPI / 6. * pm::pow(dia_part, 3.);
+      const double pmass = vol * rho_part;
+      const double num_ppp = rd[C::pstateNumDens];
+      const double film_hght = rd[C::pstateFilmHght];
+      if (film_hght == 0.) {
+        acc[mass_i] += num_ppp * pmass;
+        acc[dens_i] += num_ppp * pmass / cell_vol;
+        acc[num_i] += num_ppp;
+        acc[vol_i] += num_ppp * vol;
+        acc[surf_i] += num_ppp * surf;
+        acc[volf_i] += num_ppp * vol / cell_vol;
+        acc[d10_i] += num_ppp * dia_part;
+        acc[d32_i] += num_ppp * vol * 6.;
+        acc[temp_i] += num_ppp * pmass * T_part;
+        acc[nump_i] += 1.;
+        for (int dir = 0; dir < 3; ++dir) acc[vel_i + dir] += num_ppp * pmass * rd[C::pstateVel + dir];
+        // the fuel masses carry no num_ppp factor (SprayDerive.cpp:117-123)
+        for (int spf = 0; spf < NF; ++spf) acc[spec_i + spf] += rd[C::pstateY + spf] * pmass;
+      } else {
+        const double rad2 = pm::pow(0.5 * dia_part, 2.);
+        const double cur_vol = kPI / 6. * film_hght * (3. * rad2 + film_hght * film_hght);
+        acc[wfh_i] += cur_vol / face_area;
+        acc[wfm_i] += rho_part * cur_vol;
+      }
+    }
+    int c[3];
+    containing_cell(&pos[3 * order[a]], g, dxi, c);
+    for (int q = 0; q < nvar; ++q) {
+      double* v = &mf.data[fab_off(mf, c[0], c[1], c[2], start_indx + q)];
+      *v = *v + acc[q];
+    }
+    a = b;
+  }
+  if (n == 0) return PELEMP_OK;  // the reference visits no particle tile
+  // protected_divide over the whole FAB: only where the denominator is non-zero
+  for (int k = mf.lo[2]; k <= mf.hi[2]; ++k) {
+    for (int j = mf.lo[1]; j <= mf.hi[1]; ++j) {
+      for (int i = mf.lo[0]; i <= mf.hi[0]; ++i) {
+        auto at = [&](int q) -> double& { return mf.data[fab_off(mf, i, j, k, start_indx + q)]; };
+        if (at(num_i) != 0.) at(d10_i) = at(d10_i) / at(num_i);
+        if (at(surf_i) != 0.) at(d32_i) = at(d32_i) / at(surf_i);
+        if (at(mass_i) != 0.) {
+          at(temp_i) = at(temp_i) / at(mass_i);
+          for (int dir = 0; dir < 3; ++dir) at(vel_i + dir) = at(vel_i + dir) / at(mass_i);
+        }
+      }
+    }
+  }
+  return PELEMP_OK;
 }
 
 }  // namespace oracle

@@ -127,6 +127,8 @@ _PROTOS = {
     "pelemp_spray_move_kick_drift": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl,
                                               _c_dbl, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
     "pelemp_spray_est_timestep": (_c_i32, [_vp, _c_dbl, _c_dbl, ctypes.POINTER(_c_dbl), _vp]),
+    "pelemp_spray_derive_count": (_c_i32, [_vp, _c_i32, ctypes.POINTER(_c_i32)]),
+    "pelemp_spray_compute_derived": (_c_i32, [_vp, ctypes.POINTER(FabViewC), _c_i32, _c_i32, _vp]),
     "pelemp_spray_transfer_source": (_c_i32, [_vp, _c_i32, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _vp]),
     "pelemp_spray_pass_counts": (_c_i32, [_vp, ctypes.POINTER(_c_i64), _c_i32, ctypes.POINTER(_c_i32)]),
     "pelemp_spray_set_timing": (_c_i32, [_vp, _c_i32]),

@@ -20,6 +20,7 @@
 #include "spray_ctx.h"
 #include "update_kernel.h"
 #include "branch_kernels.h"
+#include "derive_kernels.h"
 #include "transfer_kernels.h"
 
 struct pelemp_spray_ctx {
@@ -1091,6 +1092,69 @@ int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double max_jet_ve
   // injection bound of the jets (SprayParticles.cpp:125-131)
   if (max_jet_vel > 0.) d = std::min(d, cfl * c.geom.dx[0] / max_jet_vel);
   *dt = d;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_derive_count(const pelemp_spray_ctx* h, int32_t derive_plot_species, int32_t* n) {
+  if (!h || !n) return PELEMP_E_ARG;
+  *n = kDeriveBase + ((derive_plot_species == 1 && h->c.nf > 1) ? h->c.nf : 0);
+  return PELEMP_OK;
+}
+
+int pelemp_spray_compute_derived(pelemp_spray_ctx* h, pelemp_fab_view* mf_var, int32_t start_indx,
+                                 int32_t derive_plot_species, void* stream) {
+  if (!h || !mf_var || !mf_var->data) {
+    set_last_error("compute_derived: invalid argument");
+    return PELEMP_E_ARG;
+  }
+  Ctx& c = h->c;
+  const int nvar = kDeriveBase + ((derive_plot_species == 1 && c.nf > 1) ? c.nf : 0);
+  if (start_indx < 0 || start_indx + nvar > mf_var->ncomp) {
+    set_last_error("compute_derived: the FAB has fewer components than start_indx + the spray variables");
+    return PELEMP_E_ARG;
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (mf_var->lo[d] > c.geom.domain_lo[d] || mf_var->hi[d] < c.geom.domain_hi[d]) {
+      set_last_error("compute_derived: the FAB must cover the domain");
+      return PELEMP_E_ARG;
+    }
+  }
+  XferFab mf;
+  mf.data = mf_var->data;
+  for (int d = 0; d < 3; ++d) {
+    mf.lo[d] = mf_var->lo[d];
+    mf.n[d] = mf_var->hi[d] - mf_var->lo[d] + 1;
+  }
+  mf.cstride = static_cast<long long>(mf.n[0]) * mf.n[1] * mf.n[2];
+  DeriveGeom g;
+  for (int d = 0; d < 3; ++d) {
+    g.dlo[d] = c.geom.domain_lo[d];
+    g.ncell[d] = c.ncell[d];
+    g.ntile[d] = c.ntile[d];
+  }
+  g.cell_vol = c.geom.dx[0] * c.geom.dx[1] * c.geom.dx[2];
+  g.face_area = 1. * c.geom.dx[0] * c.geom.dx[0];  // SprayDerive.cpp:85
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned grid = static_cast<unsigned>(std::min<long long>((mf.cstride + 255) / 256, 16384));
+  auto run = [&](auto nft) {
+    constexpr int NF = decltype(nft)::value;
+    const FuelConst<double, NF> fc = make_fuel_const<double, NF>(c.sd, c.mw.data());
+    if (c.form == PELEMP_FORM_FP64) {
+      hipLaunchKernelGGL((derive_kernel<NF, double>), dim3(grid), dim3(256), 0, s, c.cstart, c.perm,
+                         static_cast<const double*>(c.fbuf[c.cur]), nullptr, c.cap, fc, g, mf, start_indx, nvar);
+    } else {
+      hipLaunchKernelGGL((derive_kernel<NF, float>), dim3(grid), dim3(256), 0, s, c.cstart, c.perm,
+                         static_cast<const float*>(c.fbuf[c.cur]), c.mbuf[c.cur], c.cap, fc, g, mf, start_indx,
+                         nvar);
+    }
+  };
+  switch (c.nf) {
+    case 1: run(std::integral_constant<int, 1>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 6: run(std::integral_constant<int, 6>{}); break;
+    default: set_last_error("unsupported fuel count"); return PELEMP_E_BAD_CONFIG;
+  }
+  PM_HIP(hipGetLastError());
   return PELEMP_OK;
 }
 

@@ -79,6 +79,10 @@ class SprayParticleContainer:
         self.mech = mech
         self.form = form
         self.nf = int(spray_data["nfuel"])
+        from .problem import mech_info
+
+        species = mech_info(mech)["species"]
+        self.fuel_names = [species[int(i)] for i in list(spray_data["indx"])[:self.nf]]
         self.geom = dict(geom)
         self.comps = dict(comps)
         self._sd = C.spray_data_c(spray_data)
@@ -207,6 +211,35 @@ class SprayParticleContainer:
         C.check(self._lib.pelemp_spray_est_timestep(self._h, float(cfl), float(max_jet_vel), ctypes.byref(dt),
                                                     _stream_handle(stream)), "pelemp_spray_est_timestep")
         return float(dt.value)
+
+    DERIVE_NAMES = ("spray_mass", "spray_density", "spray_num", "spray_vol", "spray_surf_area", "spray_vol_frac",
+                    "d10", "d32", "wall_film_hght", "wall_film_mass", "spray_temp", "num_parcels", "spray_x_vel",
+                    "spray_y_vel", "spray_z_vel")
+
+    def DeriveVarNames(self, derive_plot_species: int = 1) -> list:
+        """Names of the derived spray variables (SpraySetup.cpp:213-239)."""
+        names = list(self.DERIVE_NAMES)
+        if derive_plot_species == 1 and len(self.fuel_names) > 1:
+            names += ["spray_mass_" + f for f in self.fuel_names]
+        return names
+
+    def NumDeriveVars(self, derive_plot_species: int = 1) -> int:
+        n = ctypes.c_int32()
+        C.check(self._lib.pelemp_spray_derive_count(self._h, int(derive_plot_species), ctypes.byref(n)),
+                "pelemp_spray_derive_count")
+        return int(n.value)
+
+    def computeDerivedVars(self, mf_var: "DeviceFab", level: int = 0, start_indx: int = 0,
+                           derive_plot_species: int = 1, stream=None, sync: bool = True) -> None:
+        """SprayParticleContainer::computeDerivedVars (SprayDerive.cpp:9-155):
+        adds the per-cell spray diagnostics to mf_var from start_indx on."""
+        del level
+        v = mf_var.view()
+        C.check(self._lib.pelemp_spray_compute_derived(self._h, ctypes.byref(v), int(start_indx),
+                                                       int(derive_plot_species), _stream_handle(stream)),
+                "pelemp_spray_compute_derived")
+        if sync:
+            C.device_synchronize()
 
     def transferSource(self, source_ghosts: int, level: int, tmpSource: "DeviceFab", actSource: "DeviceFab",
                        stream=None, sync: bool = True) -> None:

@@ -1,6 +1,7 @@
-"""Pins of the oracle's transferSource / addSpraySrc and estTimestep
-(oracle/spray_oracle_transfer.h; SprayParticles.H:250-319,
-SprayParticles.cpp:89-140) by brute force and exact integer-valued sums."""
+"""Pins of the oracle's transferSource / addSpraySrc, estTimestep and
+computeDerivedVars (oracle/spray_oracle_transfer.h; SprayParticles.H:250-319,
+SprayParticles.cpp:89-140, SprayDerive.cpp:9-155) by brute force, exact
+integer-valued sums and the definitions of the diagnostics."""
 import numpy as np
 import pytest
 
@@ -103,3 +104,73 @@ def test_transfer_needs_fabs_covering_the_domain():
     tmp, tlo, thi = _fab(n, 0, 7)
     act, alo, ahi = _fab((3, 4, 4), 0, 7)
     assert O.transfer_source(g, COMPS, [3], 0, tmp, tlo, thi, act, alo, ahi) == 6
+
+
+# ---------------------------------------------------------------------------
+# computeDerivedVars (SprayDerive.cpp:9-155) against the definitions of the
+# diagnostics: d10 = sum N d / sum N, d32 = sum N d^3 / sum N d^2 (Sauter),
+# mass-weighted temperature and velocity, film volume of a spherical cap.
+
+
+def _dodecane2():
+    from pelemp_amd import problem as P
+
+    prob = P.config2(ncell=8)
+    return prob
+
+
+def test_derived_vars_definitions():
+    import math
+
+    prob = _dodecane2()
+    sd = prob.sdata
+    nf = prob.nf
+    g = prob.geom
+    dx = g["dx"]
+    rng = np.random.default_rng(9)
+    n = 5
+    pos = np.tile([[1.5 * dx[0], 2.5 * dx[1], 3.5 * dx[2]]], (n, 1)) + rng.uniform(-0.4, 0.4, (n, 3)) * dx
+    rd = np.zeros((n, 10 + nf))
+    rd[:, 0:3] = rng.normal(0, 3, (n, 3))
+    rd[:, 3] = rng.uniform(300, 400, n)
+    rd[:, 4] = rng.uniform(5e-6, 40e-6, n)
+    rd[:, 5] = 0.4
+    rd[:, 6] = 0.6
+    rd[:, 5 + nf] = rng.uniform(1, 50, n)
+    rd[4, 9 + nf] = 2e-6  # a wall film
+    nvar = 15 + nf
+    mf = np.zeros((nvar, 8, 8, 8))
+    assert O.compute_derived(sd, g, pos, rd, mf, (0, 0, 0), (7, 7, 7), 0, True) == 0
+    v = mf[:, 3, 2, 1]
+    N, d, T = rd[:4, 5 + nf], rd[:4, 4], rd[:4, 3]
+    a = sd["rho_coef"]
+    rho = 1.0 / sum(rd[:4, 5 + s] / (a[4 * s] + T * (a[4 * s + 1] + T * (a[4 * s + 2] + T * a[4 * s + 3])))
+                    for s in range(nf))
+    m = N * rho * math.pi / 6 * d ** 3
+    assert v[0] == pytest.approx(m.sum(), rel=1e-13)
+    assert v[1] == pytest.approx(m.sum() / np.prod(dx), rel=1e-13)
+    assert v[2] == pytest.approx(N.sum(), rel=1e-15)
+    assert v[6] == pytest.approx((N * d).sum() / N.sum(), rel=1e-13)           # d10
+    assert v[7] == pytest.approx((N * d ** 3).sum() / (N * d ** 2).sum(), rel=1e-13)  # d32
+    assert v[10] == pytest.approx((m * T).sum() / m.sum(), rel=1e-13)          # mass-weighted T
+    for k in range(3):
+        assert v[12 + k] == pytest.approx((m * rd[:4, k]).sum() / m.sum(), rel=1e-12)
+    assert v[11] == 4.0  # parcels; the film is not counted
+    # film: spherical-cap volume pi/6 h (3 r^2 + h^2) per face area dx^2
+    h, r = 2e-6, 0.5 * rd[4, 4]
+    capv = math.pi / 6 * h * (3 * r * r + h * h)
+    assert v[8] == pytest.approx(capv / dx[0] ** 2, rel=1e-13)
+    # liquid mass per fuel: Y m per parcel, without the parcel count
+    mp = m / N
+    for s in range(nf):
+        assert v[15 + s] == pytest.approx((rd[:4, 5 + s] * mp).sum(), rel=1e-13)
+    # other cells untouched
+    assert np.count_nonzero(mf.sum(axis=0)) == 1
+
+
+def test_derived_vars_no_particles_leaves_the_fab():
+    prob = _dodecane2()
+    mf = np.full((17, 8, 8, 8), 2.0)
+    assert O.compute_derived(prob.sdata, prob.geom, np.zeros((0, 3)), np.zeros((0, 12)), mf, (0, 0, 0),
+                             (7, 7, 7), 0, True) == 0
+    assert np.all(mf == 2.0)
