@@ -124,6 +124,18 @@ int pelemp_spray_create(pelemp_spray_ctx** ctx, int32_t mech, int32_t form,
                         const pelemp_spray_comps* comps,
                         const pelemp_geom* geom, int64_t max_particles,
                         int32_t apply_latent_shift);
+/* The PeleC flavour of the update (built without PELELM_USE_SPRAY): the
+ * state holds rho, rho*u, rho*E, rho*Y_k and T (T only as the guess of
+ * eos.EY2T, SprayInterpolation.H:126-146), the energy source carries the
+ * drag work and the kinetic energy of the evaporated mass (Drag.H:282-286,
+ * 418-421), and everything is CGS (SprayData with the deck's unshifted
+ * reference latent heat, geometry, droplets, state, sources, derived
+ * variables, avg_inject_mass, jet velocity), as PeleC's SprayUnits factors of
+ * 1 imply (SprayFuelData.H:55-64).  FP64 form only. */
+int pelemp_spray_create_pelec(pelemp_spray_ctx** ctx, int32_t mech, int32_t form,
+                              const pelemp_spray_data* sd_cgs,
+                              const pelemp_spray_comps* comps,
+                              const pelemp_geom* geom_cgs, int64_t max_particles);
 int pelemp_spray_destroy(pelemp_spray_ctx* ctx);
 
 /* The SprayData actually in use (after the latent shift). */

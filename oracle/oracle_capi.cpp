@@ -9,6 +9,7 @@
 
 #include "spray_oracle.h"
 #include "spray_oracle_transfer.h"
+#include "../pelemp_amd/csrc/units.h"
 
 // Breakup / splash parameters of one call (in: container parameters, the id
 // of the next new droplet and the capacity of the output arrays; out: next id).
@@ -22,6 +23,9 @@ extern "C" struct oracle_branch_params {
   double khrt_B1;
   double khrt_C3;
   double avg_inject_mass;
+  // 1: PeleC flavour -- CGS SprayData without the latent shift, CGS geometry,
+  // droplets, state and sources (converted to and from SI as the library does)
+  int64_t pelec;
 };
 
 namespace {
@@ -33,6 +37,19 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
                double spray_cfl, int nthreads, oracle_branch_params* bpp, int64_t* n_out) {
   using P = oracle::Particle<NF>;
   constexpr int NSR = oracle::Comps<NF>::pstateNum;
+  const bool pelec = bpp && bpp->pelec;
+  pelemp_spray_data sd_si = *sd;
+  pelemp_geom g_si = *g;
+  if (pelec) {
+    pelemp::cgs::spray_data_to_si(sd_si);
+    pelemp::cgs::geom_to_si(g_si);
+    // latent = ref_latent - h_gas,fuel(ref_T) (SpraySetup.cpp:284-290), in SI
+    double h[Mech::NS];
+    pelemp::GasProps<Mech>::T2Hi(sd_si.ref_T, h);
+    for (int i = 0; i < NF; ++i) sd_si.latent[i] = sd_si.ref_latent[i] - h[sd_si.indx[i]];
+    sd = &sd_si;
+    g = &g_si;
+  }
   std::vector<P> parts;
   parts.reserve(n);
   int64_t max_id = 0;
@@ -42,6 +59,7 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
     for (int d = 0; d < 3; ++d) p.pos[d] = pos[3 * i + d];
     for (int c = 0; c < NSR; ++c) p.rdata[c] = rdata[NSR * i + c];
     p.id = id[i];
+    if (pelec) pelemp::cgs::droplet_to_si(p.pos, p.rdata, NF, sd->do_breakup);
     max_id = std::max(max_id, p.id);
     parts.push_back(p);
   }
@@ -57,6 +75,11 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
     bp.khrt_C3 = bpp->khrt_C3;
     bp.avg_inject_mass = bpp->avg_inject_mass;
     cap = bpp->cap;
+    if (pelec) {
+      bp.pelec = true;
+      bp.avg_inject_mass *= pelemp::cgs::kVol;
+      pelemp::cgs::source_scale(NF, bp.src_scale);
+    }
   } else {
     bp.next_id = max_id + 1;
   }
@@ -73,6 +96,7 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
     if (p.id <= 0) continue;
     for (int d = 0; d < 3; ++d) pos[3 * m + d] = p.pos[d];
     for (int c = 0; c < NSR; ++c) rdata[NSR * m + c] = p.rdata[c];
+    if (pelec) pelemp::cgs::droplet_to_cgs(&pos[3 * m], &rdata[NSR * m], NF, sd->do_breakup);
     id[m] = p.id;
     ++m;
   }
@@ -149,6 +173,17 @@ int oracle_gas_props(int mech, double T, double rho, const double* Y, double* cp
     case PELEMP_MECH_DECANE_DODECANE_AIR: return run(pelemp::Mech_decane_dodecane_air{});
     case PELEMP_MECH_ALKANE6_AIR: return run(pelemp::Mech_alkane6_air{});
     default: return PELEMP_E_ARG;
+  }
+}
+
+// eos.EY2T stand-in (GasProps<M>::EY2T): T from the specific internal
+// energy e [J/kg] and the mass fractions, from the guess T0.
+double oracle_ey2t(int mech, double e, const double* Y, double T0) {
+  switch (mech) {
+    case PELEMP_MECH_DODECANE_AIR: return pelemp::GasProps<pelemp::Mech_dodecane_air>::EY2T(e, Y, T0);
+    case PELEMP_MECH_DECANE_DODECANE_AIR: return pelemp::GasProps<pelemp::Mech_decane_dodecane_air>::EY2T(e, Y, T0);
+    case PELEMP_MECH_ALKANE6_AIR: return pelemp::GasProps<pelemp::Mech_alkane6_air>::EY2T(e, Y, T0);
+    default: return -1.;
   }
 }
 
@@ -306,12 +341,25 @@ int oracle_transfer_source(const pelemp_geom* g, const pelemp_spray_comps* comps
 }
 
 // computeDerivedVars for host particles (pos[3n], rdata[n*NSR]).
+// pelec: CGS SprayData, geometry, droplets and output (converted to SI and
+// back as the library's PeleC flavour does).
 int oracle_compute_derived(int nf, const pelemp_geom* g, const pelemp_spray_data* sd, int64_t n, const double* pos,
-                           const double* rdata, pelemp_fab_view* mf, int start_indx, int with_species) {
+                           const double* rdata, pelemp_fab_view* mf, int start_indx, int with_species, int pelec) {
+  pelemp_geom gs = *g;
+  pelemp_spray_data ss = *sd;
+  std::vector<double> ps(pos, pos + 3 * n), rs(rdata, rdata + (10 + nf) * n);
+  double scale[15 + PELEMP_MAX_FUEL];
+  if (pelec) {
+    pelemp::cgs::geom_to_si(gs);
+    pelemp::cgs::spray_data_to_si(ss);
+    for (int64_t i = 0; i < n; ++i) pelemp::cgs::droplet_to_si(&ps[3 * i], &rs[(10 + nf) * i], nf, ss.do_breakup);
+    pelemp::cgs::derived_scale(nf, scale);
+  }
+  const double* sc = pelec ? scale : nullptr;
   switch (nf) {
-    case 1: return oracle::compute_derived<1>(*g, *sd, n, pos, rdata, *mf, start_indx, with_species != 0);
-    case 2: return oracle::compute_derived<2>(*g, *sd, n, pos, rdata, *mf, start_indx, with_species != 0);
-    case 6: return oracle::compute_derived<6>(*g, *sd, n, pos, rdata, *mf, start_indx, with_species != 0);
+    case 1: return oracle::compute_derived<1>(gs, ss, n, ps.data(), rs.data(), *mf, start_indx, with_species != 0, sc);
+    case 2: return oracle::compute_derived<2>(gs, ss, n, ps.data(), rs.data(), *mf, start_indx, with_species != 0, sc);
+    case 6: return oracle::compute_derived<6>(gs, ss, n, ps.data(), rs.data(), *mf, start_indx, with_species != 0, sc);
     default: return PELEMP_E_ARG;
   }
 }

@@ -28,7 +28,8 @@ class BranchParamsC(ctypes.Structure):
     """oracle_branch_params (oracle_capi.cpp)."""
     _fields_ = [("max_ppp", ctypes.c_double), ("ppp_fact", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("next_id", ctypes.c_int64), ("cap", ctypes.c_int64), ("khrt_B0", ctypes.c_double),
-                ("khrt_B1", ctypes.c_double), ("khrt_C3", ctypes.c_double), ("avg_inject_mass", ctypes.c_double)]
+                ("khrt_B1", ctypes.c_double), ("khrt_C3", ctypes.c_double), ("avg_inject_mass", ctypes.c_double),
+                ("pelec", ctypes.c_int64)]
 
 
 def lib() -> ctypes.CDLL:
@@ -75,7 +76,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_compute_derived.restype = ctypes.c_int
         L.oracle_compute_derived.argtypes = [ctypes.c_int, ctypes.POINTER(C.GeomC), ctypes.POINTER(C.SprayDataC),
                                              ctypes.c_int64, vp, vp, ctypes.POINTER(C.FabViewC), ctypes.c_int,
-                                             ctypes.c_int]
+                                             ctypes.c_int, ctypes.c_int]
+        L.oracle_ey2t.restype = ctypes.c_double
+        L.oracle_ey2t.argtypes = [ctypes.c_int, ctypes.c_double, vp, ctypes.c_double]
         L.oracle_est_timestep.restype = ctypes.c_double
         L.oracle_est_timestep.argtypes = [ctypes.POINTER(C.GeomC), ctypes.c_int, ctypes.c_int64, vp, ctypes.c_double,
                                           ctypes.c_double]
@@ -121,15 +124,16 @@ def update(mech: int, sdata: dict, comps: dict, geom: dict, pos, rdata, ids, sta
     Returns (status, pos, rdata, ids) with the surviving droplets in cell order
     (then the secondary droplets created by breakup / splash).  `branch`
     (max_ppp, ppp_fact, seed, next_id, cap, khrt_B0, khrt_B1, khrt_C3,
-    avg_inject_mass) sets the breakup / splash parameters; its next_id is
-    updated."""
+    avg_inject_mass, pelec) sets the breakup / splash parameters and the
+    flavour (pelec=1: CGS SprayData without the latent shift, CGS geometry,
+    droplets, state and sources); its next_id is updated."""
     nf = int(sdata["nfuel"])
     n = np.asarray(ids).shape[0]
     bp = BranchParamsC(max_ppp=100.0, ppp_fact=0.5, seed=0, next_id=0, cap=n, khrt_B0=0.61, khrt_B1=7.0,
                        khrt_C3=1.0, avg_inject_mass=0.0)
     if branch is not None:
         for k in ("max_ppp", "ppp_fact", "seed", "next_id", "cap", "khrt_B0", "khrt_B1", "khrt_C3",
-                  "avg_inject_mass"):
+                  "avg_inject_mass", "pelec"):
             if k in branch:
                 setattr(bp, k, branch[k])
     cap = max(int(bp.cap), n)
@@ -223,7 +227,7 @@ def transfer_source(geom: dict, comps: dict, dep_indx, source_ghosts: int, tmp: 
 
 
 def compute_derived(sdata: dict, geom: dict, pos, rdata, mf: np.ndarray, mf_lo, mf_hi, start_indx: int = 0,
-                    with_species: bool = True) -> int:
+                    with_species: bool = True, pelec: bool = False) -> int:
     """computeDerivedVars into the host FAB mf[ncomp, nz, ny, nx] (added to,
     then the protected divides).  Returns the status."""
     nf = int(sdata["nfuel"])
@@ -233,7 +237,13 @@ def compute_derived(sdata: dict, geom: dict, pos, rdata, mf: np.ndarray, mf_lo, 
     mv = C.fab_view(mf.ctypes.data, mf_lo, mf_hi, mf.shape[0])
     return lib().oracle_compute_derived(nf, ctypes.byref(C.geom_c(geom)), ctypes.byref(C.spray_data_c(sdata)),
                                         pos.shape[0], pos.ctypes.data, rdata.ctypes.data, ctypes.byref(mv),
-                                        int(start_indx), int(with_species))
+                                        int(start_indx), int(with_species), int(pelec))
+
+
+def ey2t(mech: int, e: float, Y, T0: float) -> float:
+    """Temperature from the specific internal energy e [J/kg] (EY2T)."""
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    return lib().oracle_ey2t(mech, float(e), Y.ctypes.data, float(T0))
 
 
 def est_timestep(geom: dict, vel, cfl: float = 0.5, max_jet_vel: float = 0.0, fixed_parts: bool = False) -> float:

@@ -253,7 +253,8 @@ void calcVaporState(const pelemp_spray_data& fdat, const GasPhaseVals<NS, NF>& g
 // calculateSpraySource (Drag.H:137-441), PELELM_USE_SPRAY flavour.
 template <class Mech, int NF>
 double calculateSpraySource(double flow_dt, GasPhaseVals<Mech::NS, NF>& gpv,
-                            const pelemp_spray_data& fdat, Particle<NF>& p, const double* cBoilT) {
+                            const pelemp_spray_data& fdat, Particle<NF>& p, const double* cBoilT,
+                            bool pelec = false) {
   constexpr int NS = Mech::NS;
   using C = Comps<NF>;
   const double rule = 1. / 3.;
@@ -337,6 +338,11 @@ double calculateSpraySource(double flow_dt, GasPhaseVals<Mech::NS, NF>& gpv,
     if (fdat.mom_trans) {
       const double fm = num_ppp * drag_force;
       for (int d = 0; d < 3; ++d) gpv.fluid_mom_src[d] += fm * diff_vel[d];
+      if (pelec) {
+        // drag work s_d,mu . u_d (Drag.H:282-286, PeleC flavour)
+        const double S_dmu_dot_u = diff_vel[0] * vel_part[0] + diff_vel[1] * vel_part[1] + diff_vel[2] * vel_part[2];
+        gpv.fluid_eng_src += num_ppp * drag_force * S_dmu_dot_u;
+      }
       if (isub == 1) {
         const double inv_tau_var = drag_force_p;
         nsub = std::min(std::max(nsub, sat_int(flow_dt * inv_tau_var) + 1), nSubMax);
@@ -435,6 +441,11 @@ double calculateSpraySource(double flow_dt, GasPhaseVals<Mech::NS, NF>& gpv,
   }
   const double mdot_total = (pmass - startmass) / (fdat.dtmod * flow_dt);
   gpv.fluid_mass_src = num_ppp * mdot_total;
+  if (pelec) {
+    // kinetic energy of the evaporated mass (Drag.H:418-421, PeleC flavour)
+    const double part_ke = 0.5 * (vel_part[0] * vel_part[0] + vel_part[1] * vel_part[1] + vel_part[2] * vel_part[2]);
+    gpv.fluid_eng_src += num_ppp * part_ke * mdot_total;
+  }
   if (fdat.mom_trans) {
     for (int d = 0; d < 3; ++d) gpv.fluid_mom_src[d] += num_ppp * vel_part[d] * mdot_total;
   }
@@ -536,15 +547,51 @@ struct Fab {
   double operator()(const int c[3], int comp) const { return data[off(c, comp)]; }
 };
 
-// InterpolateGasPhase, PELELM_USE_SPRAY flavour (SprayInterpolation.H:99-150)
-template <int NS, int NF>
-bool InterpolateGasPhase(GasPhaseVals<NS, NF>& gpv, const Fab& st, const pelemp_spray_comps& SPI,
-                         const int idx[8][3], const double w[8]) {
+// PeleC flavour of one state cell (SprayInterpolation.H:126-146 without
+// PELELM_USE_SPRAY): the state holds rho, rho*u, rho*E, rho*Y and T in CGS;
+// Y, u, the kinetic energy and the internal energy are formed in CGS as the
+// reference does, then rho, u and e go to SI (x1e3, x1e-2, x1e-4) and T comes
+// from eos.EY2T with the state's T as the guess.
+template <class Mech>
+inline void pelec_cell(const double* S, int64_t cstride, const pelemp_spray_comps& SPI, double& rho, double* Y,
+                       double* vel, double& T) {
+  constexpr int NS = Mech::NS;
+  const double cur_rho = S[SPI.rhoIndx * cstride];
+  const double inv_rho = 1. / cur_rho;
+  for (int n = 0; n < NS; ++n) Y[n] = S[(SPI.specIndx + n) * cstride] * inv_rho;
+  double ke = 0.;
+  double vc[3];
+  for (int dir = 0; dir < 3; ++dir) {
+    vc[dir] = S[(SPI.momIndx + dir) * cstride] * inv_rho;
+    ke += vc[dir] * vc[dir] / 2.;
+  }
+  const double intEng = S[SPI.engIndx * cstride] * inv_rho - ke;
+  rho = cur_rho * 1.e3;
+  for (int dir = 0; dir < 3; ++dir) vel[dir] = vc[dir] * 1.e-2;
+  T = pelemp::GasProps<Mech>::EY2T(intEng * 1.e-4, Y, S[SPI.utempIndx * cstride]);
+}
+
+// InterpolateGasPhase (SprayInterpolation.H:99-150): PELELM_USE_SPRAY flavour,
+// or the PeleC flavour (pelec) through pelec_cell
+template <class Mech, int NF>
+bool InterpolateGasPhase(GasPhaseVals<Mech::NS, NF>& gpv, const Fab& st, const pelemp_spray_comps& SPI,
+                         const int idx[8][3], const double w[8], bool pelec = false) {
+  constexpr int NS = Mech::NS;
   for (int a = 0; a < 8; ++a) {
     const double cw = w[a];
     if (cw > 0.) {
       const int* c = idx[a];
       if (!st.contains(c)) return false;
+      if (pelec) {
+        double rho, Y[NS], vel[3], T;
+        const int64_t cstride = static_cast<int64_t>(st.n[0]) * st.n[1] * st.n[2];
+        pelec_cell<Mech>(&st.data[st.off(c, 0)], cstride, SPI, rho, Y, vel, T);
+        gpv.rho_fluid += cw * rho;
+        for (int n = 0; n < NS; ++n) gpv.Y_fluid[n] += cw * Y[n];
+        for (int dir = 0; dir < 3; ++dir) gpv.vel_fluid[dir] += cw * vel[dir];
+        gpv.T_fluid += cw * T;
+        continue;
+      }
       const double cur_rho = st(c, SPI.rhoIndx);
       gpv.rho_fluid += cw * cur_rho;
       const double inv_rho = 1. / cur_rho;
@@ -626,6 +673,11 @@ struct BranchParams {
   double khrt_B1 = 7.;
   double khrt_C3 = 1.;
   double avg_inject_mass = 0.;
+  // PeleC flavour: CGS state (rho, rho*u, rho*E, rho*Y, T) and CGS sources;
+  // the droplet data are SI, src_scale converts each SI source component
+  // (mom 3, mass, species nf, energy) to CGS at the add into the FAB
+  bool pelec = false;
+  double src_scale[5 + PELEMP_MAX_FUEL] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
 };
 
 // per-droplet values the reference keeps in registers across sub-cycles
@@ -756,7 +808,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
           for (int d = 0; d < 3; ++d) idx[a][d] += g.domain_lo[d];
       }
       gpv.reset();
-      if (!InterpolateGasPhase<NS, NF>(gpv, st, SPI, idx, w)) {
+      if (!InterpolateGasPhase<Mech, NF>(gpv, st, SPI, idx, w, bp.pelec)) {
         err_local = std::max(err_local, PELEMP_E_STATE_BOX);
         continue;
       }
@@ -765,7 +817,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       if (is_film) {
         calculateFilmSource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT);
       } else {
-        cy.Reyn_d = calculateSpraySource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT);
+        cy.Reyn_d = calculateSpraySource<Mech, NF>(sub_dt, gpv, fdat, p, cBoilT, bp.pelec);
       }
       if (p.id > 0 && do_breakup) {
         if (fdat.do_breakup == 1) cy.Utan_total += updateBreakupTAB<NS, NF>(cy.Reyn_d, sub_dt, cBoilT, gpv, fdat, p);
@@ -858,7 +910,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         double acc = 0.;
         for (int64_t i = a; i < b; ++i) acc += contrib[static_cast<size_t>(i) * NSRC + q];
         double* dst = &source_v.data[dep_off[a] + static_cast<int64_t>(comps[q]) * cstride];
-        *dst = *dst + acc;
+        *dst = bp.pelec ? *dst + acc * bp.src_scale[q] : *dst + acc;
       }
     }
   }

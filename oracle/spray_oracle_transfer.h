@@ -149,7 +149,8 @@ inline double est_timestep(const pelemp_geom& g, bool fixed_parts, int64_t n, co
 // liquid mass of each fuel.
 template <int NF>
 inline int compute_derived(const pelemp_geom& g, const pelemp_spray_data& fdat, int64_t n, const double* pos,
-                           const double* rdata, pelemp_fab_view& mf, int start_indx, bool with_species) {
+                           const double* rdata, pelemp_fab_view& mf, int start_indx, bool with_species,
+                           const double* scale = nullptr) {
   using C = Comps<NF>;
   constexpr int NSR = C::pstateNum;
   // the per-fuel masses are listed only for several fuels (SpraySetup.cpp:236-240)
@@ -213,7 +214,7 @@ inline int compute_derived(const pelemp_geom& g, const pelemp_spray_data& fdat, 
     containing_cell(&pos[3 * order[a]], g, dxi, c);
     for (int q = 0; q < nvar; ++q) {
       double* v = &mf.data[fab_off(mf, c[0], c[1], c[2], start_indx + q)];
-      *v = *v + acc[q];
+      *v = scale ? *v + acc[q] * scale[q] : *v + acc[q];
     }
     a = b;
   }

@@ -116,6 +116,8 @@ _PROTOS = {
     "pelemp_spray_get_params": (_c_i32, [_vp, ctypes.POINTER(SprayParamsC)]),
     "pelemp_spray_create": (_c_i32, [ctypes.POINTER(_vp), _c_i32, _c_i32, ctypes.POINTER(SprayDataC),
                                      ctypes.POINTER(SprayCompsC), ctypes.POINTER(GeomC), _c_i64, _c_i32]),
+    "pelemp_spray_create_pelec": (_c_i32, [ctypes.POINTER(_vp), _c_i32, _c_i32, ctypes.POINTER(SprayDataC),
+                                           ctypes.POINTER(SprayCompsC), ctypes.POINTER(GeomC), _c_i64]),
     "pelemp_spray_destroy": (_c_i32, [_vp]),
     "pelemp_spray_get_data": (_c_i32, [_vp, ctypes.POINTER(SprayDataC)]),
     "pelemp_spray_set_particles": (_c_i32, [_vp, _c_i64, _vp, _vp, _vp]),

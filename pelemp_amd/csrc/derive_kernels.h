@@ -24,6 +24,8 @@ constexpr int kDeriveBase = 15;
 struct DeriveGeom {
   int dlo[3], ncell[3], ntile[3];
   double cell_vol, face_area;
+  int pelec;                                    // CGS output (PeleC flavour)
+  double scale[kDeriveBase + PELEMP_MAX_FUEL];  // SI -> CGS per variable
 };
 
 // R: storage type of the droplet fields (double: FP64 form, float: FP32
@@ -111,7 +113,7 @@ __global__ void __launch_bounds__(256) derive_kernel(const int32_t* __restrict__
         }
 #pragma unroll
         for (int q = 0; q < kDeriveBase + NF; ++q)
-          if (q < nvar) V[q * mf.cstride] = V[q * mf.cstride] + acc[q];
+          if (q < nvar) V[q * mf.cstride] = V[q * mf.cstride] + (g.pelec ? acc[q] * g.scale[q] : acc[q]);
       }
     }
     if (ntot > 0) {

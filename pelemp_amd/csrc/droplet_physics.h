@@ -79,6 +79,7 @@ struct FuelConst {
   R y_tab_coef;  // unused placeholder for alignment
   bool mass_trans, mom_trans, fixed_parts;
   int do_breakup;
+  bool pelec;  // PeleC flavour: drag work and kinetic energy in the energy source
 };
 
 template <class R, int NF>
@@ -523,6 +524,12 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
       const R fm = num_ppp * o.drag_force;
 #pragma unroll
       for (int q = 0; q < 3; ++q) s.mom[q] += fm * o.diff_vel[q];
+      if (fd.pelec) {
+        // drag work s_d,mu . u_d (Drag.H:282-286, PeleC flavour)
+        const R S_dmu_dot_u =
+            o.diff_vel[0] * vel_part[0] + o.diff_vel[1] * vel_part[1] + o.diff_vel[2] * vel_part[2];
+        s.eng += num_ppp * o.drag_force * S_dmu_dot_u;
+      }
       if (isub == 1) {
         const R inv_tau_var = o.drag_force_p;
         const int c = sat_int(flow_dt * inv_tau_var) + 1;
@@ -624,6 +631,11 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
   }
   const R mdot_total = (pmass - startmass) / (fd.dtmod * flow_dt);
   s.mass = num_ppp * mdot_total;
+  if (fd.pelec) {
+    // kinetic energy of the evaporated mass (Drag.H:418-421, PeleC flavour)
+    const R part_ke = R(0.5) * (vel_part[0] * vel_part[0] + vel_part[1] * vel_part[1] + vel_part[2] * vel_part[2]);
+    s.eng += num_ppp * part_ke * mdot_total;
+  }
   if (fd.mom_trans) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) s.mom[q] += num_ppp * vel_part[q] * mdot_total;

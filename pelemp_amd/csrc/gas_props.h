@@ -116,6 +116,30 @@ struct GasProps {
     }
   }
 
+  // Temperature from the mixture's specific internal energy e [J/kg] and
+  // mass fractions (the eos.EY2T the PeleC flavour calls in the gather,
+  // SprayInterpolation.H:143-146): Newton on e(T) = sum_k Y_k (h_k - R T / W_k)
+  // with c_v = sum_k Y_k (c_p,k - R / W_k), from the guess T (the state's
+  // temperature), at most 50 steps, until |dT| <= 1e-10 T.
+  static PM_HDI double EY2T(double e, const double* Y, double T) {
+    double h[NS], cp[NS];
+    for (int it = 0; it < 50; ++it) {
+      T2Hi<double>(T, h);
+      T2Cpi<double>(T, cp);
+      double ev = 0., cv = 0.;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const double rw = kRU / M::W[n];
+        ev += Y[n] * (h[n] - rw * T);
+        cv += Y[n] * (cp[n] - rw);
+      }
+      const double dT = (e - ev) / cv;
+      T = T + dT;
+      if (std::fabs(dT) <= 1e-10 * T) break;
+    }
+    return T;
+  }
+
   // Mixture-averaged transport at (T, rho, Y).  Ddiag[k] = rho D_k,mix.
   template <class R>
   static PM_HDI void transport(bool get_mu, bool get_lam, bool get_D, R T,

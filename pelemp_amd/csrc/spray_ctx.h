@@ -96,6 +96,10 @@ struct Ctx {
   // the counters of new droplets (device scalars: next id, next slot)
   pelemp_spray_params params{};
   bool branches = false;
+  // PeleC flavour: CGS state (rho, rho*u, rho*E, rho*Y, T), sources, droplets
+  // and geometry at the boundary; SI inside (src_scale: SI -> CGS sources)
+  bool pelec = false;
+  double src_scale[5 + PELEMP_MAX_FUEL] = {};
   double* wf = nullptr;
   int32_t* sb_type = nullptr;
   double* sb_data = nullptr;

@@ -22,6 +22,7 @@
 #include "branch_kernels.h"
 #include "derive_kernels.h"
 #include "transfer_kernels.h"
+#include "units.h"
 
 struct pelemp_spray_ctx {
   pelemp::Ctx c;
@@ -708,6 +709,29 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   return PELEMP_OK;
 }
 
+int pelemp_spray_create_pelec(pelemp_spray_ctx** out, int32_t mech, int32_t form, const pelemp_spray_data* sd_cgs,
+                              const pelemp_spray_comps* comps, const pelemp_geom* geom_cgs, int64_t max_particles) {
+  if (!out || !sd_cgs || !geom_cgs) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  if (form != PELEMP_FORM_FP64) {
+    set_last_error("the PeleC flavour is supported by the FP64 form only");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  pelemp_spray_data sd = *sd_cgs;
+  pelemp_geom g = *geom_cgs;
+  cgs::spray_data_to_si(sd);
+  cgs::geom_to_si(g);
+  // the deck's reference latent heat is shifted in SI (SpraySetup.cpp:284-290)
+  const int st = pelemp_spray_create(out, mech, form, &sd, comps, &g, max_particles, 1);
+  if (st != PELEMP_OK) return st;
+  Ctx& c = (*out)->c;
+  c.pelec = true;
+  cgs::source_scale(c.nf, c.src_scale);
+  return PELEMP_OK;
+}
+
 int pelemp_spray_destroy(pelemp_spray_ctx* h) {
   if (!h) return PELEMP_OK;
   (void)hipDeviceSynchronize();
@@ -738,6 +762,7 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
     hp.insert(hp.end(), pos + 3 * i, pos + 3 * i + 3);
     hr.insert(hr.end(), rdata + nsr * i, rdata + nsr * i + nsr);
     hid.push_back(id[i]);
+    if (c.pelec) cgs::droplet_to_si(&hp[hp.size() - 3], &hr[hr.size() - nsr], c.nf, c.sd.do_breakup);
   }
   const int64_t m = static_cast<int64_t>(hid.size());
   if (m > c.cap) {
@@ -869,6 +894,9 @@ int pelemp_spray_get_particles(pelemp_spray_ctx* h, int64_t* n, double* pos, dou
     if (pos) PM_HIP(hipMemcpy(pos, dpos, 8 * 3 * m, hipMemcpyDeviceToHost));
     if (rdata) PM_HIP(hipMemcpy(rdata, drd, 8 * nsr * m, hipMemcpyDeviceToHost));
     if (id) PM_HIP(hipMemcpy(id, did, 8 * m, hipMemcpyDeviceToHost));
+    if (c.pelec && pos && rdata) {
+      for (int64_t i = 0; i < m; ++i) cgs::droplet_to_cgs(pos + 3 * i, rdata + nsr * i, c.nf, c.sd.do_breakup);
+    }
   }
   (void)hipFree(dpos);
   (void)hipFree(drd);
@@ -990,6 +1018,7 @@ int pelemp_spray_set_params(pelemp_spray_ctx* h, const pelemp_spray_params* p) {
     return PELEMP_E_BAD_CONFIG;
   }
   h->c.params = *p;
+  if (h->c.pelec) h->c.params.avg_inject_mass *= cgs::kVol;
   return PELEMP_OK;
 }
 
@@ -999,6 +1028,7 @@ int pelemp_spray_get_params(const pelemp_spray_ctx* h, pelemp_spray_params* out)
     return PELEMP_E_ARG;
   }
   *out = h->c.params;
+  if (h->c.pelec) out->avg_inject_mass /= cgs::kVol;
   return PELEMP_OK;
 }
 
@@ -1090,6 +1120,7 @@ int pelemp_spray_est_timestep(pelemp_spray_ctx* h, double cfl, double max_jet_ve
   PM_HIP(hipMemcpy(&t, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
   double d = (t == 0) ? std::numeric_limits<double>::max() : v;
   // injection bound of the jets (SprayParticles.cpp:125-131)
+  if (c.pelec) max_jet_vel *= cgs::kLen;
   if (max_jet_vel > 0.) d = std::min(d, cfl * c.geom.dx[0] / max_jet_vel);
   *dt = d;
   return PELEMP_OK;
@@ -1134,6 +1165,9 @@ int pelemp_spray_compute_derived(pelemp_spray_ctx* h, pelemp_fab_view* mf_var, i
   }
   g.cell_vol = c.geom.dx[0] * c.geom.dx[1] * c.geom.dx[2];
   g.face_area = 1. * c.geom.dx[0] * c.geom.dx[0];  // SprayDerive.cpp:85
+  g.pelec = c.pelec ? 1 : 0;
+  for (int q = 0; q < kDeriveBase + PELEMP_MAX_FUEL; ++q) g.scale[q] = 1.;
+  if (c.pelec) cgs::derived_scale(c.nf, g.scale);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const unsigned grid = static_cast<unsigned>(std::min<long long>((mf.cstride + 255) / 256, 16384));
   auto run = [&](auto nft) {

@@ -109,6 +109,10 @@ struct UpdateArgs {
   int do_splash_box;
   double max_ppp;
   KhrtParams kh;  // KH-RT constants, mean injected parcel mass, dtmod * flow_dt
+  // PeleC flavour (FP64): CGS state rho, rho*u, rho*E, rho*Y, T and CGS sources
+  int pelec;
+  int c_eng;
+  double src_scale[5 + PELEMP_MAX_FUEL];  // SI -> CGS factor per source component
 };
 
 // fields of a secondary-droplet record (SBData.H:7-19): norm(3) vel(3)
@@ -736,6 +740,36 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     const int lx = gx - a.st.lo[0], ly = gy - a.st.lo[1], lz = gz - a.st.lo[2];
     const bool ok = lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2];
     gvalid[h] = ok ? 1 : 0;
+    if constexpr (sizeof(R) == 8) {
+      if (ok && a.pelec) {
+        // PeleC flavour (SprayInterpolation.H:126-146): Y, u, kinetic and
+        // internal energy formed in CGS, then rho, u, e to SI and T from EY2T
+        // with the state's T as the guess
+        const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
+        const double* S = a.st.data + c;
+        const long long cs_ = a.st.cstride;
+        const double cur_rho = S[a.c_rho * cs_];
+        const double inv_rho = 1. / cur_rho;
+        double Y[NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) Y[n] = S[(a.c_spec + n) * cs_] * inv_rho;
+        double ke = 0.;
+        double vc[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          vc[q] = S[(a.c_vel + q) * cs_] * inv_rho;
+          ke += vc[q] * vc[q] / 2.;
+        }
+        const double intEng = S[a.c_eng * cs_] * inv_rho - ke;
+        gas[0 * kHaloCells + h] = cur_rho * 1.e3;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) gas[(1 + n) * kHaloCells + h] = Y[n];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gas[(1 + NS + q) * kHaloCells + h] = vc[q] * 1.e-2;
+        gas[(4 + NS) * kHaloCells + h] = GasProps<M>::EY2T(intEng * 1.e-4, Y, S[a.c_T * cs_]);
+        continue;
+      }
+    }
     if (ok) {
       const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
       const double* S = a.st.data;
@@ -831,7 +865,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
             if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
           } else {
             double* dst = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
-            *dst = *dst + acc[r];
+            *dst = a.pelec ? *dst + acc[r] * a.src_scale[q] : *dst + acc[r];
           }
         }
       }

@@ -59,6 +59,10 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
     a.sub_dt_dxi[d] = static_cast<R>(p.sub_dt * a.dxi[d]);
   }
   a.fc = make_fuel_const<R, NF>(c.sd, c.mw.data());
+  a.fc.pelec = c.pelec;
+  a.pelec = c.pelec ? 1 : 0;
+  a.c_eng = c.comps.engIndx;
+  for (int q = 0; q < 5 + PELEMP_MAX_FUEL; ++q) a.src_scale[q] = c.src_scale[q];
   a.flow_dt = static_cast<R>(p.flow_dt);
   a.sub_dt = static_cast<R>(p.sub_dt);
   a.cur_coef = p.cur_coef;

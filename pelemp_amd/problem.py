@@ -274,3 +274,69 @@ def config2(ncell: int = 128, drops_per_cell: float = 1.6e7 / 128 ** 3, seed: in
     return build_problem("config2", ncell=(ncell, ncell, ncell), p=30.0 * PATM, T0=900.0,
                          mech=C.MECH_DECANE_DODECANE_AIR, fuels=("NC10H22", "NC12H26"), n_drops=n, poisson=True,
                          d_median=15e-6, sigma_ln=0.5, Y_range=(0.3, 0.7), flow_dt=5e-7, nsteps=50, seed=seed, **kw)
+
+
+def to_pelec(prob: SprayProblem, T_guess_factor: float = 1.0) -> SprayProblem:
+    """The same problem in the PeleC flavour's CGS form: SprayData in CGS
+    (the deck's reference latent heat, unshifted), geometry and droplets in
+    cm, the state as rho, rho*u, rho*E, rho*Y and T (g/cm^3, g/(cm^2 s),
+    erg/cm^3; E = e + |u|^2/2 with e the mixture's specific internal energy
+    from the NASA-7 enthalpies), sources in CGS.  T_guess_factor scales the
+    stored temperature, which the PeleC gather only uses as the EY2T guess."""
+    import copy
+
+    p = copy.deepcopy(prob)
+    L, RHO, E = 1.0e-2, 1.0e3, 1.0e-4
+    sd = dict(prob.sdata)
+    sd["sigma"] = sd["sigma"] / 1.0e-3
+    sd["body_force"] = [v / L for v in sd["body_force"]]
+    for k, f in (("cp", E), ("ref_latent", E), ("latent", E)):
+        sd[k] = [v / f for v in sd[k]]
+    psat = list(sd["psat_coef"])
+    for i in range(len(psat) // 4):
+        psat[4 * i + 3] = psat[4 * i + 3] / 1.0e-1
+    sd["psat_coef"] = psat
+    sd["rho_coef"] = [v / RHO for v in sd["rho_coef"]]
+    sd["lambda_coef"] = [v / 1.0e-5 for v in sd["lambda_coef"]]
+    sd["mu_coef"] = [v / 1.0e-1 for v in sd["mu_coef"]]
+    p.sdata = sd
+    g = dict(prob.geom)
+    for k in ("plo", "phi", "dx"):
+        g[k] = [v / L for v in g[k]]
+    p.geom = g
+    p.pos = prob.pos / L
+    rd = prob.rdata.copy()
+    rd[:, 0:3] /= L
+    rd[:, 4] /= L
+    rd[:, 9 + prob.nf] /= L
+    if sd["do_breakup"] == 2:
+        rd[:, 7 + prob.nf] /= 1.0e-6
+    p.rdata = rd
+    cm = prob.comps
+    st = prob.state
+    ns = len(prob.species)
+    rho = st[cm["rhoIndx"]]
+    T = st[cm["utempIndx"]]
+    Y = np.stack([st[cm["specIndx"] + k] / rho for k in range(ns)])
+    # e(T) = sum_k Y_k (h_k - R T / W_k), SI, NASA-7 h_k/(R T / W_k)
+    info = mech_info(prob.mech)
+    e = np.zeros_like(T)
+    for k in range(ns):
+        lo, hi = np.array(info["low"][k]), np.array(info["high"][k])
+        a = np.where((T < info["Tmid"][k])[None], lo[:, None, None, None], hi[:, None, None, None])
+        hRT = a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] / T
+        e += Y[k] * (RU * T / prob.W[k]) * (hRT - 1.0)
+    rho_c = rho / RHO
+    u_c = np.stack([st[cm["momIndx"] + d] / L for d in range(3)])
+    new = np.zeros_like(st)
+    new[cm["rhoIndx"]] = rho_c
+    for d in range(3):
+        new[cm["momIndx"] + d] = rho_c * u_c[d]
+    new[cm["engIndx"]] = rho_c * (e / E + 0.5 * np.sum(u_c * u_c, axis=0))
+    for k in range(ns):
+        new[cm["specIndx"] + k] = rho_c * Y[k]
+    new[cm["utempIndx"]] = T * T_guess_factor
+    p.state = new
+    p.name = prob.name + "_pelec"
+    p.meta = dict(prob.meta, flavour="pelec")
+    return p

@@ -74,7 +74,10 @@ class SprayParticleContainer:
     """Droplets of one AMR level, resident on the GPU."""
 
     def __init__(self, mech: int, spray_data: dict, comps: dict, geom: dict, max_particles: int,
-                 form: int = C.FORM_FP64, apply_latent_shift: bool = True):
+                 form: int = C.FORM_FP64, apply_latent_shift: bool = True, flavour: str = "pelelmex"):
+        """flavour "pelelmex" (SI, velocity and T in the state) or "pelec" (CGS
+        throughout, rho*u and rho*E in the state, the SprayData latent heat
+        unshifted)."""
         self._lib = C.lib()
         self.mech = mech
         self.form = form
@@ -89,10 +92,18 @@ class SprayParticleContainer:
         self._comps = C.comps_c(comps)
         self._geom = C.geom_c(geom)
         h = ctypes.c_void_p()
-        C.check(self._lib.pelemp_spray_create(ctypes.byref(h), mech, form, ctypes.byref(self._sd),
-                                              ctypes.byref(self._comps), ctypes.byref(self._geom),
-                                              int(max_particles), 1 if apply_latent_shift else 0),
-                "pelemp_spray_create")
+        self.flavour = flavour
+        if flavour == "pelec":
+            C.check(self._lib.pelemp_spray_create_pelec(ctypes.byref(h), mech, form, ctypes.byref(self._sd),
+                                                        ctypes.byref(self._comps), ctypes.byref(self._geom),
+                                                        int(max_particles)), "pelemp_spray_create_pelec")
+        elif flavour == "pelelmex":
+            C.check(self._lib.pelemp_spray_create(ctypes.byref(h), mech, form, ctypes.byref(self._sd),
+                                                  ctypes.byref(self._comps), ctypes.byref(self._geom),
+                                                  int(max_particles), 1 if apply_latent_shift else 0),
+                    "pelemp_spray_create")
+        else:
+            raise ValueError(f"unknown flavour {flavour!r}")
         self._h = h
 
     # -- lifetime ----------------------------------------------------------

@@ -1,0 +1,90 @@
+"""Pins of the oracle's PeleC flavour (built without PELELM_USE_SPRAY):
+ * EY2T (the eos call of the gather, SprayInterpolation.H:143-146) inverts
+   e(T) = sum_k Y_k (h_k - R T / W_k) from a wrong guess;
+ * the flavour's CGS boundary against the PeleLMeX flavour on the same
+   physical problem: droplets and the mass, species and momentum sources
+   agree up to the unit-conversion rounding;
+ * its energy source differs from the PeleLMeX one by exactly the kinetic
+   energy of the evaporated mass, N |u_d|^2/2 dm/dt (Drag.H:418-421), when
+   momentum coupling is off (the drag-work term, :282-286, lives inside it)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from pelemp_amd import problem as P
+from pelemp_amd.problem import RU, mech_info
+
+
+def _e_of_T(mech, T, Y):
+    info = mech_info(mech)
+    W = np.array(info["W"])
+    e = 0.0
+    for k in range(len(W)):
+        a = info["low"][k] if T < info["Tmid"][k] else info["high"][k]
+        hRT = a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] / T
+        e += Y[k] * RU * T / W[k] * (hRT - 1.0)
+    return e
+
+
+@pytest.mark.parametrize("T", [350.0, 900.0, 1000.5, 2200.0])
+def test_ey2t_inverts_the_internal_energy(T):
+    mech = 1
+    ns = len(mech_info(mech)["W"])
+    Y = np.array([0.2, 0.7, 0.06, 0.04])[:ns]
+    e = _e_of_T(mech, T, Y)
+    for guess in (0.7 * T, T, 1.3 * T):
+        assert O.ey2t(mech, e, Y, guess) == pytest.approx(T, rel=1e-9)
+
+
+def _run(prob, steps, pelec):
+    sd = dict(prob.sdata) if pelec else O.latent_shifted(prob.sdata, prob.mech, len(prob.species))
+    pos, rd, ids = prob.pos, prob.rdata, prob.ids
+    srcs = []
+    for _ in range(steps):
+        src = prob.new_source()
+        st, pos, rd, ids = O.update(prob.mech, sd, prob.comps, prob.geom, pos, rd, ids, prob.state, prob.state_lo,
+                                    prob.state_hi, src, prob.src_lo, prob.src_hi, prob.flow_dt,
+                                    branch=dict(pelec=int(pelec), cap=prob.n))
+        assert st == 0
+        srcs.append(src)
+    o = np.argsort(ids)
+    return pos[o], rd[o], ids[o], srcs
+
+
+def test_pelec_flavour_matches_pelelmex_up_to_units():
+    prob = P.config1(scale=0.02)
+    prob.sdata = dict(prob.sdata, mom_trans=0)
+    cgs = P.to_pelec(prob, T_guess_factor=1.03)
+    p1, r1, i1, s1 = _run(prob, 1, False)
+    p2, r2, i2, s2 = _run(cgs, 1, True)
+    assert np.array_equal(i1, i2)
+    nf = prob.nf
+    np.testing.assert_allclose(p2 * 1e-2, p1, rtol=1e-12, atol=1e-16)
+    np.testing.assert_allclose(r2[:, 4] * 1e-2, r1[:, 4], rtol=1e-9)      # diameter
+    np.testing.assert_allclose(r2[:, 3], r1[:, 3], rtol=1e-9)             # temperature
+    np.testing.assert_allclose(r2[:, 0:3] * 1e-2, r1[:, 0:3], rtol=1e-9, atol=1e-12)
+    cm = prob.comps
+    ms, ml = s2[0][cm["rhoSrcIndx"]] / 1e-3, s1[0][cm["rhoSrcIndx"]]
+    scale = np.abs(ml).max()
+    assert scale > 0
+    assert np.abs(ms - ml).max() <= 1e-8 * scale
+    # energy: PeleC - PeleLMeX = -(1/V) N |u_d|^2 / 2 dm/dt per droplet (one
+    # droplet per cell is not guaranteed, so compare the sums over the box)
+    V = np.prod(prob.geom["dx"])
+    d0, d1 = prob.rdata[np.argsort(prob.ids)][:, 4], r1[:, 4]
+    a = prob.sdata["rho_coef"]
+
+    def rhoL(T):
+        return a[0] + T * (a[1] + T * (a[2] + T * a[3]))
+
+    T0 = prob.rdata[np.argsort(prob.ids)][:, 3]
+    m0 = math.pi / 6 * rhoL(T0) * d0 ** 3
+    m1 = math.pi / 6 * rhoL(r1[:, 3]) * d1 ** 3
+    N = r1[:, 5 + nf]
+    ke = 0.5 * np.sum(r1[:, 0:3] ** 2, axis=1)
+    de = -(1.0 / V) * np.sum(N * ke * (m1 - m0) / prob.flow_dt)
+    diff = np.sum(s2[0][cm["engSrcIndx"]]) * 1e-1 - np.sum(s1[0][cm["engSrcIndx"]])
+    assert de != 0.0
+    assert diff == pytest.approx(de, rel=1e-6)
