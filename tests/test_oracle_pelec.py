@@ -88,3 +88,43 @@ def test_pelec_flavour_matches_pelelmex_up_to_units():
     diff = np.sum(s2[0][cm["engSrcIndx"]]) * 1e-1 - np.sum(s1[0][cm["engSrcIndx"]])
     assert de != 0.0
     assert diff == pytest.approx(de, rel=1e-6)
+
+
+def test_pelec_drag_work_from_the_momentum_source():
+    # With momentum coupling on, PeleC also adds the drag work
+    # N F_drag (du . u_d) (Drag.H:282-286).  For a droplet alone in its cell
+    # taking one ODE substep (config 1: flow_dt / tau_drag ~ 2e-3), the LMeX
+    # sources give N F_drag du = S_mom - S_mass u_end and N dm/dt = S_mass
+    # (both times -1/V), so the PeleC - LMeX energy source of the cell is
+    # (S_mom - S_mass u_end) . u_start + S_mass |u_end|^2 / 2.
+    prob = P.config1(scale=0.02)
+    cgs = P.to_pelec(prob, T_guess_factor=1.03)
+    p1, r1, i1, s1 = _run(prob, 1, False)
+    _, _, _, s2 = _run(cgs, 1, True)
+    cm = prob.comps
+    g = prob.geom
+    o = np.argsort(prob.ids)
+    pos0, rd0 = prob.pos[o], prob.rdata[o]
+    cell = np.floor((pos0 - np.array(g["plo"])) / np.array(g["dx"])).astype(int)
+    lo = np.array(prob.src_lo)
+    keys, counts = np.unique(cell, axis=0, return_counts=True)
+    alone = {tuple(k) for k, c in zip(keys, counts) if c == 1}
+    assert len(alone) > 100
+    checked = 0
+    work_total = 0.0
+    for n in range(len(pos0)):
+        c = tuple(cell[n])
+        if c not in alone:
+            continue
+        i, j, k = np.array(c) - lo
+        s_mom = np.array([s1[0][cm["momSrcIndx"] + d][k, j, i] for d in range(3)])
+        s_mass = s1[0][cm["rhoSrcIndx"]][k, j, i]
+        u0, u1 = rd0[n, 0:3], r1[n, 0:3]
+        want = (s_mom - s_mass * u1) @ u0 + s_mass * 0.5 * (u1 @ u1)
+        got = s2[0][cm["engSrcIndx"]][k, j, i] * 1e-1 - s1[0][cm["engSrcIndx"]][k, j, i]
+        scale = abs(s1[0][cm["engSrcIndx"]][k, j, i])
+        assert abs(got - want) <= 1e-9 * scale + 1e-6 * abs(want), (n, got, want)
+        work_total += abs((s_mom - s_mass * u1) @ u0)
+        checked += 1
+    assert checked == len(alone)
+    assert work_total > 0.0
