@@ -46,6 +46,11 @@ extern "C" {
 #define PELEMP_E_NOMEM 7       /* device allocation failed */
 #define PELEMP_E_BAD_CHILD 8   /* SpraySB.cpp:187-190: a secondary droplet
                                   without a valid position */
+#define PELEMP_E_BAD_INJECTION 9 /* SprayInjection.H:39-43 (mean droplet too
+                                    small), :149-152 (injected droplet
+                                    outside the domain), :286-289 (lattice
+                                    droplet outside), and the jet checks of
+                                    SprayJet.cpp:28-133, SprayJet.H:71-138 */
 
 /* Mirror of SprayData (SprayFuelData.H:115-212), fixed at PELEMP_MAX_FUEL
  * fuel slots; the first nfuel are used. */
@@ -169,6 +174,139 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* ctx, int64_t n,
 int pelemp_spray_get_particles(pelemp_spray_ctx* ctx, int64_t* n, double* pos,
                                double* rdata, int64_t* id);
 int pelemp_spray_num_particles(pelemp_spray_ctx* ctx, int64_t* n);
+/* Appends particles (same AoS meaning) to those already in the context, as
+ * the reference's injection routines append host particles to the tiles
+ * (SprayInjection.H:164-179, :297-310); for injectors that run on the host,
+ * e.g. a SprayJet subclass with its own get_new_particle.  Inside each cell
+ * the new droplets follow the existing ones, in input order.  Ids must be
+ * > 0; later secondary droplets take ids past the largest id in the
+ * context. */
+int pelemp_spray_add_particles(pelemp_spray_ctx* ctx, int64_t n,
+                               const double* pos, const double* rdata,
+                               const int64_t* id);
+
+/* ---- Injection (SprayInjection.H, SprayJet.{H,cpp}, Distributions.cpp) ---- */
+
+/* Droplet-diameter distributions (Distribution/Distributions.cpp:1-174);
+ * dist_param meaning per type. */
+#define PELEMP_DIST_UNIFORM 0    /* [diameter] */
+#define PELEMP_DIST_NORMAL 1     /* [mean_dia, std_dev] */
+#define PELEMP_DIST_LOGNORMAL 2  /* [mean_dia, std_dev] of the diameter */
+#define PELEMP_DIST_WEIBULL 3    /* [mean_dia, k] */
+#define PELEMP_DIST_CHISQUARED 4 /* [d32] */
+
+/* The members of SprayJet (SprayJet.H:262-293) that the injection reads and
+ * updates; angles in radians, as stored by the SprayJet constructors
+ * (SprayJet.cpp:30-37,116-121).  Units are the context's (SI, or CGS for a
+ * PeleC context).  The rate-of-injection tables of readROI are caller-owned
+ * arrays of n_roi points (n_roi = 0: constant jet_vel and mass_flow).
+ * amrex::Random() draws are replaced by counter-based numbers keyed by
+ * (seed, n_injections, candidate droplet, draw); n_injections counts the
+ * calls that reached the droplet loop. */
+typedef struct pelemp_spray_jet {
+  double cent[3];
+  double norm[3];       /* normalised by pelemp_spray_jet_check */
+  double spread_angle;  /* m_spreadAngle: full cone angle */
+  double swirl_angle;   /* m_swirlAngle */
+  double hollow_spread; /* m_hollowSpread */
+  double jet_dia;
+  double jet_vel;       /* m_jetVel (constant jet) */
+  double max_jet_vel;   /* m_maxJetVel (estTimestep's jet bound) */
+  double mass_flow;     /* m_massFlow (constant jet) */
+  double jet_T;
+  double jet_Y[PELEMP_MAX_FUEL];
+  double start_time, end_time;
+  int32_t hollow_spray;
+  int32_t dist_type;
+  double dist_param[2];
+  int32_t n_roi;
+  int32_t reserved;
+  const double* roi_time;
+  const double* roi_mdot;
+  const double* roi_vel;
+  /* injection state, public in the reference (SprayJet.H:258-266) */
+  double num_ppp;        /* m_numPPP; < 0: set by the first injection */
+  double sum_inj_mass;   /* m_sumInjMass: mass held over, -1 = none */
+  double sum_inj_time;   /* m_sumInjTime */
+  double min_parcel;     /* m_minParcel, initially 1 */
+  double total_inj_mass; /* m_totalInjMass */
+  double total_inj_time; /* m_totalInjTime */
+  uint64_t seed;
+  int64_t n_injections;
+} pelemp_spray_jet;
+
+/* The constructor checks of SprayJet (SprayJet.cpp:28-36,55-57,76-85,
+ * 123-134; SprayJet.H:71-79,121-139): angles in range, a jet centre inside
+ * the domain, mass fractions summing to 1, non-negative rates; normalises
+ * norm.  *avg_dia receives get_avg_dia() of the distribution. */
+int pelemp_spray_jet_check(const pelemp_spray_ctx* ctx, pelemp_spray_jet* jet,
+                           double* avg_dia);
+
+/* SprayJet::readROI (SprayJet.cpp:194-256): parses a rate-of-injection file
+ * (a header line, then "time[ms];mdot" lines) into n points of time [s],
+ * mdot and velocity mdot / (rho_part * area * cd); mass_conv is 1e-3 for
+ * the PeleLMeX flavour (g/s -> kg/s) and 1 for PeleC.  *n is in: capacity,
+ * out: points.  Also returns the start / end time and the largest jet
+ * velocity. */
+int pelemp_spray_read_roi(const char* path, double rho_part, double cd,
+                          double jet_dia, double mass_conv, int32_t* n,
+                          double* time, double* mdot, double* vel,
+                          double* start_time, double* end_time,
+                          double* max_jet_vel);
+
+/* SprayParticleContainer::sprayInjection (SprayInjection.H:10-180): injects
+ * the droplets of one jet for a step of sim_dt at `time`, on the device,
+ * and appends them to the context (ids continue past the largest id).  The
+ * jet's state (num_ppp, held-over mass and time, min_parcel, totals) is
+ * updated as the reference updates its SprayJet.  The droplet loop (draw a
+ * candidate, keep it if its diameter exceeds the minimum, until the
+ * injected mass reaches the target) runs with the candidates drawn in
+ * parallel and the mass summed in candidate order.  *n_injected: droplets
+ * added.  Waits for `stream`. */
+int pelemp_spray_inject(pelemp_spray_ctx* ctx, pelemp_spray_jet* jet,
+                        double time, double sim_dt, int64_t* n_injected,
+                        void* stream);
+
+/* SprayParticleContainer::uniformSprayInit (SprayInjection.H:198-348), one
+ * rank: num_part[0]*num_part[1]*num_part[2] droplets on a lattice of the
+ * domain, (i + 0.5) * ProbLength / num_part, with the given velocity,
+ * diameter, temperature, mass fractions and parcel size, generated on the
+ * device and appended.  Ids continue past the largest id in lattice order
+ * (x fastest); the reference appends them in reverse order, and so does
+ * this call. */
+int pelemp_spray_uniform_init(pelemp_spray_ctx* ctx, const int32_t num_part[3],
+                              const double vel[3], double dia, double T,
+                              const double* Y, double num_ppp);
+
+/* ---- On-disk formats (SprayIO.cpp) ---- */
+
+/* injection_data.log of a checkpoint (SprayIO.cpp:49-97): the number of
+ * jets, then per jet its name, num_ppp, held-over mass and time,
+ * min_parcel, total mass and time (15 significant digits). */
+int pelemp_spray_write_injection_log(const char* path, int32_t njets,
+                                     const char* const* names,
+                                     const pelemp_spray_jet* jets);
+/* PostInitRestart (SprayIO.cpp:100-160): reads such a file and sets the
+ * state of every jet whose name matches; jets not in the file keep their
+ * state.  Duplicate or empty names are an error. */
+int pelemp_spray_read_injection_log(const char* path, int32_t njets,
+                                    const char* const* names,
+                                    pelemp_spray_jet* jets);
+
+/* The ASCII particle file written with plot files (SprayIO.cpp:35-47,
+ * AMReX ParticleContainer::WriteAsciiFile): the particle count, then one
+ * line per particle: position, id, cpu (0), the real components in
+ * SprayComps order; 15 significant digits. */
+int pelemp_spray_write_ascii(pelemp_spray_ctx* ctx, const char* path);
+
+/* Checkpoint and restart of the droplets in the context (SprayIO.cpp:6-34
+ * writes them through AMReX's Checkpoint): a text header (format tag,
+ * count, fuel count, the real-component names of SprayIO.cpp:9-33) and the
+ * particles in the AoS meaning as little-endian binary (positions, ids,
+ * rdata), so a restart is bit for bit. */
+int pelemp_spray_checkpoint(pelemp_spray_ctx* ctx, const char* dir,
+                            int32_t nfuel_names, const char* const* fuel_names);
+int pelemp_spray_restart(pelemp_spray_ctx* ctx, const char* dir);
 
 /* The droplet update for one level, SprayParticleContainer::updateParticles
  * (SprayParticles.cpp:142-502) with the same argument meanings; `state` and

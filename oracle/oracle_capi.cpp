@@ -9,6 +9,7 @@
 
 #include "spray_oracle.h"
 #include "spray_oracle_transfer.h"
+#include "spray_oracle_inject.h"
 #include "../pelemp_amd/csrc/units.h"
 
 // Breakup / splash parameters of one call (in: container parameters, the id
@@ -368,6 +369,80 @@ int oracle_compute_derived(int nf, const pelemp_geom* g, const pelemp_spray_data
 double oracle_est_timestep(const pelemp_geom* g, int fixed_parts, int64_t n, const double* vel, double cfl,
                            double max_jet_vel) {
   return oracle::est_timestep(*g, fixed_parts != 0, n, vel, cfl, max_jet_vel);
+}
+
+// sprayInjection of one jet (SprayInjection.H:10-180).  sd / g in the
+// flavour's units (CGS when pelec, converted here as the library converts);
+// droplets out in the same units.  *n: in capacity, out droplets (on
+// PELEMP_E_ARG the needed count).
+int oracle_inject(const pelemp_spray_data* sd_in, const pelemp_geom* g_in, pelemp_spray_jet* jet, double time,
+                  double sim_dt, int64_t next_id, int pelec, int64_t* n, double* pos, double* rd, int64_t* ids) {
+  pelemp_spray_data sd = *sd_in;
+  pelemp_geom g = *g_in;
+  if (pelec) {
+    pelemp::cgs::spray_data_to_si(sd);
+    pelemp::cgs::geom_to_si(g);
+  }
+  std::vector<double> p, r;
+  std::vector<int64_t> id;
+  const int st = oracle::spray_injection(sd, g, *jet, time, sim_dt, next_id, pelec != 0, p, r, id);
+  if (st) return st;
+  const int64_t m = static_cast<int64_t>(id.size());
+  const int nsr = 10 + sd.nfuel;
+  if (m > *n) {
+    *n = m;
+    return PELEMP_E_ARG;
+  }
+  for (int64_t i = 0; i < m; ++i) {
+    if (pelec) pelemp::cgs::droplet_to_cgs(&p[3 * i], &r[nsr * i], sd.nfuel, sd.do_breakup);
+    std::memcpy(pos + 3 * i, &p[3 * i], 24);
+    std::memcpy(rd + nsr * i, &r[nsr * i], 8 * nsr);
+    ids[i] = id[i];
+  }
+  *n = m;
+  return PELEMP_OK;
+}
+
+// uniformSprayInit, one rank (SprayInjection.H:198-348)
+int oracle_uniform_init(const pelemp_spray_data* sd_in, const pelemp_geom* g_in, const int* num_part,
+                        const double* vel, double dia, double T, const double* Y, double num_ppp, int64_t next_id,
+                        int pelec, int64_t* n, double* pos, double* rd, int64_t* ids) {
+  pelemp_spray_data sd = *sd_in;
+  pelemp_geom g = *g_in;
+  if (pelec) {
+    pelemp::cgs::spray_data_to_si(sd);
+    pelemp::cgs::geom_to_si(g);
+  }
+  std::vector<double> p, r;
+  std::vector<int64_t> id;
+  const int st = oracle::uniform_spray_init(sd, g, num_part, vel, dia, T, Y, num_ppp, next_id, pelec != 0, p, r, id);
+  if (st) return st;
+  const int64_t m = static_cast<int64_t>(id.size());
+  const int nsr = 10 + sd.nfuel;
+  if (m > *n) {
+    *n = m;
+    return PELEMP_E_ARG;
+  }
+  for (int64_t i = 0; i < m; ++i) {
+    if (pelec) pelemp::cgs::droplet_to_cgs(&p[3 * i], &r[nsr * i], sd.nfuel, sd.do_breakup);
+    std::memcpy(pos + 3 * i, &p[3 * i], 24);
+    std::memcpy(rd + nsr * i, &r[nsr * i], 8 * nsr);
+    ids[i] = id[i];
+  }
+  *n = m;
+  return PELEMP_OK;
+}
+
+// one diameter draw of a distribution and get_avg_dia (Distributions.cpp)
+double oracle_dist_sample(int type, const double* params, double u1, double u2, double* avg_dia) {
+  oracle::inj::Dist d;
+  d.init(type, params);
+  if (avg_dia) *avg_dia = d.get_avg_dia();
+  return d.get_dia(u1, u2);
+}
+
+double oracle_inj_random(uint64_t seed, int64_t call, int64_t cand, int slot) {
+  return oracle::inj::random01(seed, call, cand, slot);
 }
 
 // splash_criteria (AhamedSplash.H:125-197): returns the splash_type;

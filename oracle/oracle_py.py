@@ -85,6 +85,19 @@ def lib() -> ctypes.CDLL:
         L.oracle_splash_criteria.restype = ctypes.c_int
         L.oracle_splash_criteria.argtypes = [ctypes.c_double] * 5 + [vp]
         L.oracle_num_threads.restype = ctypes.c_int
+        for nm in ("oracle_inject", "oracle_uniform_init"):
+            getattr(L, nm).restype = ctypes.c_int
+        L.oracle_inject.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.POINTER(C.GeomC), ctypes.POINTER(C.SprayJetC),
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int64), vp, vp, vp]
+        L.oracle_uniform_init.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.POINTER(C.GeomC), vp, vp,
+                                          ctypes.c_double, ctypes.c_double, vp, ctypes.c_double, ctypes.c_int64,
+                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int64), vp, vp, vp]
+        L.oracle_dist_sample.restype = ctypes.c_double
+        L.oracle_dist_sample.argtypes = [ctypes.c_int, vp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_double)]
+        L.oracle_inj_random.restype = ctypes.c_double
+        L.oracle_inj_random.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -258,3 +271,54 @@ def splash_criteria(Kv: float, Tstar: float, alpha: float, Kcrit: float, del_fil
     out = np.zeros(3)
     t = lib().oracle_splash_criteria(Kv, Tstar, alpha, Kcrit, del_film, out.ctypes.data)
     return t, int(out[0]), float(out[1]), float(out[2])
+
+
+def _particles_out(nf: int, run):
+    """Call run(n_ptr, pos, rd, ids) with growing output arrays."""
+    cap = 1024
+    while True:
+        n = ctypes.c_int64(cap)
+        pos = np.zeros((cap, 3))
+        rd = np.zeros((cap, 10 + nf))
+        ids = np.zeros(cap, dtype=np.int64)
+        st = run(ctypes.byref(n), pos.ctypes.data, rd.ctypes.data, ids.ctypes.data)
+        if st == C.E_ARG and n.value > cap:
+            cap = n.value
+            continue
+        m = n.value if st == 0 else 0
+        return st, pos[:m].copy(), rd[:m].copy(), ids[:m].copy()
+
+
+def inject(sdata: dict, geom: dict, jet: "C.SprayJetC", time: float, sim_dt: float, next_id: int,
+           pelec: bool = False):
+    """sprayInjection of one jet (the jet's state is updated in place):
+    (status, pos, rdata, ids) of the new droplets."""
+    sd, g = C.spray_data_c(sdata), C.geom_c(geom)
+    return _particles_out(int(sdata["nfuel"]), lambda n, p, r, i: lib().oracle_inject(
+        ctypes.byref(sd), ctypes.byref(g), ctypes.byref(jet), float(time), float(sim_dt), int(next_id),
+        int(pelec), n, p, r, i))
+
+
+def uniform_init(sdata: dict, geom: dict, num_part, vel, dia: float, T: float, Y, num_ppp: float, next_id: int,
+                 pelec: bool = False):
+    """uniformSprayInit on one rank: (status, pos, rdata, ids)."""
+    sd, g = C.spray_data_c(sdata), C.geom_c(geom)
+    npart = np.ascontiguousarray(num_part, dtype=np.int32)
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    Yv = np.ascontiguousarray(np.atleast_1d(Y), dtype=np.float64)
+    return _particles_out(int(sdata["nfuel"]), lambda n, p, r, i: lib().oracle_uniform_init(
+        ctypes.byref(sd), ctypes.byref(g), npart.ctypes.data, v.ctypes.data, float(dia), float(T), Yv.ctypes.data,
+        float(num_ppp), int(next_id), int(pelec), n, p, r, i))
+
+
+def dist_sample(dist_type: int, params, u1: float, u2: float = 0.5):
+    """One diameter of a distribution from the uniforms (u1, u2), and its
+    get_avg_dia."""
+    p = np.ascontiguousarray(list(params) + [0.0] * (2 - len(params)), dtype=np.float64)
+    avg = ctypes.c_double()
+    d = lib().oracle_dist_sample(int(dist_type), p.ctypes.data, float(u1), float(u2), ctypes.byref(avg))
+    return d, avg.value
+
+
+def inj_random(seed: int, call: int, cand: int, slot: int) -> float:
+    return lib().oracle_inj_random(int(seed), int(call), int(cand), int(slot))

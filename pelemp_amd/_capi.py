@@ -21,6 +21,7 @@ E_HIP = 5
 E_ARG = 6
 E_NOMEM = 7
 E_BAD_CHILD = 8
+E_BAD_INJECTION = 9
 ERROR_NAMES = {
     E_STATE_BOX: "state box too small",
     E_SRC_BOX: "source box too small",
@@ -30,7 +31,16 @@ ERROR_NAMES = {
     E_ARG: "invalid argument",
     E_NOMEM: "out of device memory",
     E_BAD_CHILD: "secondary droplet without a valid position",
+    E_BAD_INJECTION: "bad injection",
 }
+
+DIST_UNIFORM = 0
+DIST_NORMAL = 1
+DIST_LOGNORMAL = 2
+DIST_WEIBULL = 3
+DIST_CHISQUARED = 4
+DIST_TYPES = {"Uniform": DIST_UNIFORM, "Normal": DIST_NORMAL, "LogNormal": DIST_LOGNORMAL,
+              "Weibull": DIST_WEIBULL, "ChiSquared": DIST_CHISQUARED}
 
 MECH_DODECANE_AIR = 0
 MECH_DECANE_DODECANE_AIR = 1
@@ -109,6 +119,18 @@ class SprayParamsC(ctypes.Structure):
                 ("avg_inject_mass", _c_dbl)]
 
 
+class SprayJetC(ctypes.Structure):
+    _fields_ = [("cent", _c_dbl * 3), ("norm", _c_dbl * 3), ("spread_angle", _c_dbl), ("swirl_angle", _c_dbl),
+                ("hollow_spread", _c_dbl), ("jet_dia", _c_dbl), ("jet_vel", _c_dbl), ("max_jet_vel", _c_dbl),
+                ("mass_flow", _c_dbl), ("jet_T", _c_dbl), ("jet_Y", _c_dbl * MAX_FUEL), ("start_time", _c_dbl),
+                ("end_time", _c_dbl), ("hollow_spray", _c_i32), ("dist_type", _c_i32), ("dist_param", _c_dbl * 2),
+                ("n_roi", _c_i32), ("reserved", _c_i32), ("roi_time", ctypes.c_void_p),
+                ("roi_mdot", ctypes.c_void_p), ("roi_vel", ctypes.c_void_p), ("num_ppp", _c_dbl),
+                ("sum_inj_mass", _c_dbl), ("sum_inj_time", _c_dbl), ("min_parcel", _c_dbl),
+                ("total_inj_mass", _c_dbl), ("total_inj_time", _c_dbl), ("seed", ctypes.c_uint64),
+                ("n_injections", _c_i64)]
+
+
 _vp = ctypes.c_void_p
 _PNORM = ctypes.POINTER(SurrogateNormC)
 _PROTOS = {
@@ -123,6 +145,18 @@ _PROTOS = {
     "pelemp_spray_set_particles": (_c_i32, [_vp, _c_i64, _vp, _vp, _vp]),
     "pelemp_spray_get_particles": (_c_i32, [_vp, ctypes.POINTER(_c_i64), _vp, _vp, _vp]),
     "pelemp_spray_num_particles": (_c_i32, [_vp, ctypes.POINTER(_c_i64)]),
+    "pelemp_spray_add_particles": (_c_i32, [_vp, _c_i64, _vp, _vp, _vp]),
+    "pelemp_spray_jet_check": (_c_i32, [_vp, ctypes.POINTER(SprayJetC), ctypes.POINTER(_c_dbl)]),
+    "pelemp_spray_read_roi": (_c_i32, [ctypes.c_char_p, _c_dbl, _c_dbl, _c_dbl, _c_dbl, ctypes.POINTER(_c_i32),
+                                       _vp, _vp, _vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_dbl),
+                                       ctypes.POINTER(_c_dbl)]),
+    "pelemp_spray_inject": (_c_i32, [_vp, ctypes.POINTER(SprayJetC), _c_dbl, _c_dbl, ctypes.POINTER(_c_i64), _vp]),
+    "pelemp_spray_uniform_init": (_c_i32, [_vp, _vp, _vp, _c_dbl, _c_dbl, _vp, _c_dbl]),
+    "pelemp_spray_write_injection_log": (_c_i32, [ctypes.c_char_p, _c_i32, _vp, ctypes.POINTER(SprayJetC)]),
+    "pelemp_spray_read_injection_log": (_c_i32, [ctypes.c_char_p, _c_i32, _vp, ctypes.POINTER(SprayJetC)]),
+    "pelemp_spray_write_ascii": (_c_i32, [_vp, ctypes.c_char_p]),
+    "pelemp_spray_checkpoint": (_c_i32, [_vp, ctypes.c_char_p, _c_i32, _vp]),
+    "pelemp_spray_restart": (_c_i32, [_vp, ctypes.c_char_p]),
     "pelemp_spray_update": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl, _c_dbl,
                                      _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
     "pelemp_spray_sync": (_c_i32, [_vp, _vp]),
@@ -229,6 +263,43 @@ def spray_data_dict(s: SprayDataC) -> dict:
         v = getattr(s, name)
         out[name] = list(v) if hasattr(v, "__len__") else v
     return out
+
+
+def jet_c(jet_cent, jet_norm, spread_angle: float, jet_dia: float, jet_vel: float, mass_flow: float,
+          jet_temp: float, jet_Y=None, dist_type: str = "Uniform", dist_params=(0.0, 0.0), start_time: float = 0.0,
+          end_time: float = 1.0e8, phi_swirl: float = 0.0, hollow_spray: bool = False, hollow_spread: float = 0.0,
+          inject_ppp: float = -1.0, seed: int = 0) -> SprayJetC:
+    """A SprayJetC as the reference's direct constructor fills SprayJet
+    (SprayJet.cpp:95-141): angles given in degrees, stored in radians; the
+    normal as given (pelemp_spray_jet_check normalises it)."""
+    if dist_type not in DIST_TYPES:
+        raise ValueError(f"unknown distribution {dist_type!r}")
+    j = SprayJetC()
+    for d in range(3):
+        j.cent[d] = float(jet_cent[d])
+        j.norm[d] = float(jet_norm[d])
+    j.spread_angle = float(spread_angle) * np.pi / 180.0
+    j.swirl_angle = float(phi_swirl) * np.pi / 180.0
+    j.hollow_spread = float(hollow_spread) * np.pi / 180.0
+    j.jet_dia = float(jet_dia)
+    j.jet_vel = float(jet_vel)
+    j.max_jet_vel = float(jet_vel)
+    j.mass_flow = float(mass_flow)
+    j.jet_T = float(jet_temp)
+    for i, y in enumerate([1.0] if jet_Y is None else list(jet_Y)):
+        j.jet_Y[i] = float(y)
+    j.start_time = float(start_time)
+    j.end_time = float(end_time)
+    j.hollow_spray = 1 if hollow_spray else 0
+    j.dist_type = DIST_TYPES[dist_type]
+    j.dist_param[0] = float(dist_params[0])
+    j.dist_param[1] = float(dist_params[1]) if len(dist_params) > 1 else 0.0
+    j.num_ppp = float(inject_ppp)
+    j.sum_inj_mass = -1.0
+    j.sum_inj_time = -1.0
+    j.min_parcel = 1.0
+    j.seed = int(seed)
+    return j
 
 
 def fab_view(ptr: int, lo, hi, ncomp: int) -> FabViewC:

@@ -85,6 +85,8 @@ struct Ctx {
   double* cold_bm1 = nullptr;
   double* cold_bm2 = nullptr;
   int64_t nslots = 0;  // slots handed out
+  int64_t next_id = 1; // id of the next new droplet (without the branches;
+                       // with them the device counter sb_ids[0] holds it)
   int64_t host_count = 0;  // active droplets after the last sort (host copy, when known)
   // active droplets at the start of every pass since the last read
   static constexpr int kCountLog = 4096;

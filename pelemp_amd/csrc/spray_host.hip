@@ -7,8 +7,12 @@
 // counting sort; no host synchronisation inside the pass chain.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <fstream>
 #include <limits>
+#include <sstream>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -23,6 +27,7 @@
 #include "derive_kernels.h"
 #include "transfer_kernels.h"
 #include "units.h"
+#include "inject_kernels.h"
 
 struct pelemp_spray_ctx {
   pelemp::Ctx c;
@@ -351,11 +356,14 @@ __global__ void __launch_bounds__(256) math_eval_kernel(int fn, int64_t n, const
 
 // ---------------------------------------------------------------------------
 
-static int run_sort(Ctx& c, hipStream_t s) {
+// Full stable sort of buffer cur by cell key.  keep_entries: the device
+// count n_entries of storage entries is already set (droplets appended);
+// otherwise it is the active count of the previous sort.
+static int run_sort(Ctx& c, hipStream_t s, bool keep_entries = false) {
   hipEvent_t ev0;
   ctx_time_begin(c, 1, s, &ev0);
-  // entries in the buffer written last = active count of the previous sort
-  PM_HIP(hipMemcpyAsync(c.n_entries, c.cstart + c.ncells, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  if (!keep_entries)
+    PM_HIP(hipMemcpyAsync(c.n_entries, c.cstart + c.ncells, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
   PM_HIP(hipMemsetAsync(c.cnt, 0, sizeof(int32_t) * c.ncells, s));
   const int grid_s = 2048;
   hipLaunchKernelGGL(count_keys_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.cnt, c.arrival);
@@ -531,6 +539,425 @@ void mech_h(int mech, double T, double* h) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Appending droplets (add_particles, injection, lattice initialisation)
+
+struct RhoCoef {
+  double a[4 * PELEMP_MAX_FUEL];
+};
+
+// FP32 form: droplet mass pi/6 rho_l(T) d^3 with the mixture density
+// 1/sum(Y/rhoL(T)), the same expression the host uses for set_particles
+__global__ void __launch_bounds__(256) drop_mass_kernel(int64_t n, int nf, int nsr, const double* __restrict__ rdata,
+                                                        RhoCoef rc, double* __restrict__ mass) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* r = rdata + static_cast<int64_t>(nsr) * i;
+  const double T = r[3];
+  double inv = 0.;
+  for (int s = 0; s < nf; ++s) {
+    const double* a = &rc.a[4 * s];
+    inv += r[5 + s] / (a[0] + T * (a[1] + T * (a[2] + T * a[3])));
+  }
+  mass[i] = kPIc / 6. * (1. / inv) * (r[4] * r[4] * r[4]);
+}
+
+// next id and next free slot
+static int read_counters(Ctx& c, int64_t* next_id, int64_t* next_slot) {
+  if (c.branches) {
+    int64_t ids[4];
+    PM_HIP(hipMemcpy(ids, c.sb_ids, sizeof(ids), hipMemcpyDeviceToHost));
+    *next_id = ids[0];
+    *next_slot = ids[1];
+  } else {
+    *next_id = c.next_id;
+    *next_slot = c.nslots;
+  }
+  return PELEMP_OK;
+}
+
+static int write_counters(Ctx& c, int64_t next_id, int64_t next_slot) {
+  c.next_id = next_id;
+  c.nslots = next_slot;
+  if (c.branches) {
+    const int64_t ids[2] = {next_id, next_slot};
+    PM_HIP(hipMemcpy(c.sb_ids, ids, sizeof(ids), hipMemcpyHostToDevice));
+  }
+  return PELEMP_OK;
+}
+
+// Give the live droplets the slots 0..total-1 (their sorted positions).
+static int compact_slots(Ctx& c, int64_t* next_slot) {
+  int32_t total = 0;
+  PM_HIP(hipMemcpy(&total, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
+  const bool with_bm = !c.hot_brk;
+  const size_t n = std::max<int32_t>(total, 1);
+  int64_t* tid = nullptr;
+  double* tbm = nullptr;
+  PM_HIP(hipMalloc(&tid, 8 * n));
+  if (with_bm && hipMalloc(&tbm, 3 * 8 * n) != hipSuccess) {
+    (void)hipFree(tid);
+    set_last_error("hipMalloc failed (slot compaction)");
+    return PELEMP_E_NOMEM;
+  }
+  hipLaunchKernelGGL(slot_compact_kernel, dim3(1024), dim3(256), 0, 0, c.cstart + c.ncells, c.perm, c.slot[c.cur],
+                     c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2, with_bm ? 1 : 0, tid, tbm, tbm ? tbm + n : nullptr,
+                     tbm ? tbm + 2 * n : nullptr);
+  PM_HIP(hipGetLastError());
+  if (total > 0) {
+    PM_HIP(hipMemcpy(c.cold_id, tid, 8 * static_cast<size_t>(total), hipMemcpyDeviceToDevice));
+    if (with_bm) {
+      PM_HIP(hipMemcpy(c.cold_n0, tbm, 8 * static_cast<size_t>(total), hipMemcpyDeviceToDevice));
+      PM_HIP(hipMemcpy(c.cold_bm1, tbm + n, 8 * static_cast<size_t>(total), hipMemcpyDeviceToDevice));
+      PM_HIP(hipMemcpy(c.cold_bm2, tbm + 2 * n, 8 * static_cast<size_t>(total), hipMemcpyDeviceToDevice));
+    }
+  }
+  PM_HIP(hipDeviceSynchronize());
+  (void)hipFree(tid);
+  if (tbm) (void)hipFree(tbm);
+  *next_slot = total;
+  return PELEMP_OK;
+}
+
+// Move the live droplets of buffer cur, in sorted order, to the front of the
+// other buffer (dead entries of the last pass are dropped); the cell
+// offsets do not change and the permutation becomes the identity.
+static int compact_storage(Ctx& c) {
+  const int32_t* total = c.cstart + c.ncells;
+  const int nxt = 1 - c.cur;
+  const dim3 grid(1024), block(256);
+  if (c.wsize == 8) {
+    hipLaunchKernelGGL((gather_rows_kernel<double>), grid, block, 0, 0, total, c.perm,
+                       static_cast<const double*>(c.fbuf[c.cur]), static_cast<double*>(c.fbuf[nxt]), c.nhot, c.cap);
+  } else {
+    hipLaunchKernelGGL((gather_rows_kernel<float>), grid, block, 0, 0, total, c.perm,
+                       static_cast<const float*>(c.fbuf[c.cur]), static_cast<float*>(c.fbuf[nxt]), c.nhot, c.cap);
+    hipLaunchKernelGGL((gather_rows_kernel<double>), grid, block, 0, 0, total, c.perm, c.mbuf[c.cur], c.mbuf[nxt],
+                       c.nf, c.cap);
+  }
+  hipLaunchKernelGGL((gather_rows_kernel<int32_t>), grid, block, 0, 0, total, c.perm, c.slot[c.cur], c.slot[nxt], 1,
+                     c.cap);
+  // keys through the arrival scratch (the key array is not double-buffered)
+  hipLaunchKernelGGL((gather_rows_kernel<uint32_t>), grid, block, 0, 0, total, c.perm, c.key,
+                     reinterpret_cast<uint32_t*>(c.arrival), 1, c.cap);
+  hipLaunchKernelGGL(identity_perm_kernel, grid, block, 0, 0, total, c.perm, c.n_entries);
+  PM_HIP(hipGetLastError());
+  int32_t n = 0;
+  PM_HIP(hipMemcpy(&n, total, 4, hipMemcpyDeviceToHost));
+  if (n > 0) PM_HIP(hipMemcpy(c.key, c.arrival, 4 * static_cast<size_t>(n), hipMemcpyDeviceToDevice));
+  PM_HIP(hipDeviceSynchronize());
+  c.cur = nxt;
+  return PELEMP_OK;
+}
+
+// Append m droplets staged on the device in the AoS meaning (SI) after the
+// storage entries of buffer cur, then sort.  new_next_id: the id counter
+// after these droplets (at least the current one).
+static int append_staged(Ctx& c, int64_t m, const double* dpos, const double* drd, const int64_t* did,
+                         int64_t new_next_id) {
+  if (m <= 0) return PELEMP_OK;
+  PM_HIP(hipDeviceSynchronize());
+  int32_t n_old = 0;
+  PM_HIP(hipMemcpy(&n_old, c.n_entries, 4, hipMemcpyDeviceToHost));
+  if (static_cast<int64_t>(n_old) + m > c.cap) {
+    // entries deleted by the last pass still occupy storage: drop them
+    int32_t total = 0;
+    PM_HIP(hipMemcpy(&total, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
+    if (static_cast<int64_t>(total) + m > c.cap) {
+      set_last_error("context capacity too small for the new droplets");
+      return PELEMP_E_NOMEM;
+    }
+    const int st0 = compact_storage(c);
+    if (st0) return st0;
+    n_old = total;
+  }
+  int64_t next_id = 0, next_slot = 0;
+  int st = read_counters(c, &next_id, &next_slot);
+  if (st) return st;
+  if (next_slot + m > c.slot_cap) {
+    st = compact_slots(c, &next_slot);
+    if (st) return st;
+    if (next_slot + m > c.slot_cap) {
+      set_last_error("context capacity too small for the new droplets");
+      return PELEMP_E_NOMEM;
+    }
+  }
+  const int nsr = 10 + c.nf;
+  const double dxi[3] = {1. / c.geom.dx[0], 1. / c.geom.dx[1], 1. / c.geom.dx[2]};
+  const dim3 grid(static_cast<unsigned>((m + 255) / 256)), block(256);
+  if (c.form == PELEMP_FORM_FP64) {
+    double* f = static_cast<double*>(c.fbuf[c.cur]) + n_old;
+    hipLaunchKernelGGL((ingest_kernel<double, true>), grid, block, 0, 0, m, c.nf, nsr, dpos, drd, did, f, c.cap,
+                       c.hot_brk ? 1 : 0, c.slot[c.cur] + n_old, next_slot, c.cold_id, c.cold_n0, c.cold_bm1,
+                       c.cold_bm2, c.key + n_old, c.geom.plo[0], c.geom.plo[1], c.geom.plo[2], dxi[0], dxi[1],
+                       dxi[2], c.ncell[0], c.ncell[1], c.ncell[2], c.ntile[0], c.ntile[1], c.ntile[2]);
+  } else {
+    double* dm = nullptr;
+    PM_HIP(hipMalloc(&dm, 8 * m));
+    RhoCoef rc{};
+    for (int q = 0; q < 4 * PELEMP_MAX_FUEL; ++q) rc.a[q] = c.sd.rho_coef[q];
+    hipLaunchKernelGGL(drop_mass_kernel, grid, block, 0, 0, m, c.nf, nsr, drd, rc, dm);
+    float* f = static_cast<float*>(c.fbuf[c.cur]) + n_old;
+    hipLaunchKernelGGL(ingest32_kernel, grid, block, 0, 0, m, c.nf, nsr, dpos, drd, dm, did, f,
+                       c.mbuf[c.cur] + n_old, c.cap, c.slot[c.cur] + n_old, next_slot, c.cold_id, c.cold_n0,
+                       c.cold_bm1, c.cold_bm2, c.key + n_old, c.geom.plo[0], c.geom.plo[1], c.geom.plo[2], dxi[0],
+                       dxi[1], dxi[2], c.ncell[0], c.ncell[1], c.ncell[2], c.ntile[0], c.ntile[1], c.ntile[2]);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipDeviceSynchronize());
+    (void)hipFree(dm);
+  }
+  PM_HIP(hipGetLastError());
+  const int32_t n_new = n_old + static_cast<int32_t>(m);
+  PM_HIP(hipMemcpy(c.n_entries, &n_new, 4, hipMemcpyHostToDevice));
+  st = run_sort(c, 0, true);
+  if (st) return st;
+  PM_HIP(hipDeviceSynchronize());
+  st = write_counters(c, std::max(next_id, new_next_id), next_slot + m);
+  if (st) return st;
+  int32_t total = 0;
+  PM_HIP(hipMemcpy(&total, c.cstart + c.ncells, 4, hipMemcpyDeviceToHost));
+  c.host_count = total;
+  return PELEMP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Spray jets (SprayJet.{H,cpp})
+
+// interpolateROI (SprayJet.H:223-239); the bracketing index is kept inside
+// the table (the reference reads before its start at time == time[0])
+static double roi_interp(const pelemp_spray_jet& j, double time, const double* vals) {
+  int i = 0;
+  double ctime = j.roi_time[0];
+  while (ctime < time && i < j.n_roi - 1) ctime = j.roi_time[++i];
+  if (i < 1) i = 1;
+  const double time1 = j.roi_time[i - 1];
+  const double time2 = j.roi_time[i];
+  const double invt = (time - time1) / (time2 - time1);
+  const double val1 = vals[i - 1];
+  const double val2 = vals[i];
+  return val1 + (val2 - val1) * invt;
+}
+
+// jet_vel / mass_flow_rate (SprayJet.H:38-57), in the context's units
+static double jet_velocity(const pelemp_spray_jet& j, double time) {
+  return j.n_roi > 0 ? roi_interp(j, time, j.roi_vel) : j.jet_vel;
+}
+static double jet_mdot(const pelemp_spray_jet& j, double time) {
+  return j.n_roi > 0 ? roi_interp(j, time, j.roi_mdot) : j.mass_flow;
+}
+
+// ChiSquared::init (Distributions.cpp:134-149)
+static void chi_table(double* rvals) {
+  const double xiend = 12.;
+  const double dxi = xiend / 100.;
+  const double rend = 1. - pm::exp(-xiend) * (1. + xiend * (1. + xiend * (0.5 + xiend / 6.)));
+  for (int i = 0; i < kChiBins; i++) {
+    const double xi = dxi * (i + 1);
+    const double rval = 1. - pm::exp(-xi) * (1. + xi * (1. + xi * (0.5 + xi / 6.)));
+    rvals[i] = rval / rend;
+  }
+}
+
+// the distribution as sampled, and get_avg_dia (Distributions.cpp)
+static InjDist make_dist(const pelemp_spray_jet& j, double* avg_dia) {
+  InjDist d{};
+  d.type = j.dist_type;
+  d.p0 = j.dist_param[0];
+  d.p1 = j.dist_param[1];
+  *avg_dia = j.dist_param[0];
+  if (j.dist_type == PELEMP_DIST_LOGNORMAL) {
+    // LogNormal::init (Distributions.cpp:67-75)
+    const double mean = j.dist_param[0], sd = j.dist_param[1];
+    d.p0 = 2. * pm::log(mean) - 0.5 * pm::log(sd * sd + mean * mean);
+    d.p1 = std::sqrt(std::max(-2. * pm::log(mean) + pm::log(sd * sd + mean * mean), 0.));
+  } else if (j.dist_type == PELEMP_DIST_CHISQUARED) {
+    chi_table(d.chi);
+    // ChiSquared::get_avg_dia (Distributions.cpp:159-174)
+    const double dmean = j.dist_param[0] / 3.;
+    const double dxi = 12. / 100.;
+    const double fact = 0.5;
+    int curn = 0;
+    double curr = d.chi[0];
+    double curxi = 0.;
+    while (fact > curr) {
+      curn++;
+      curr = d.chi[curn];
+      curxi += dxi;
+    }
+    *avg_dia = curxi * dmean;
+  }
+  return d;
+}
+
+static int check_jet(const Ctx& c, pelemp_spray_jet& j, double* avg_dia) {
+  if (j.dist_type < PELEMP_DIST_UNIFORM || j.dist_type > PELEMP_DIST_CHISQUARED) {
+    set_last_error("unknown droplet distribution");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (!(j.spread_angle >= 0. && j.spread_angle <= kInjPI)) {
+    set_last_error("Spread angle must be between 0 and 180");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (!(j.swirl_angle >= 0. && j.swirl_angle <= 0.5 * kInjPI)) {
+    set_last_error("Swirl angle must be between 0 and 90");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (!(j.hollow_spread >= 0. && j.hollow_spread <= kInjPI)) {
+    set_last_error("Hollow spread must be between 0 and 180");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (j.n_roi == 0 && (j.jet_vel < 0. || j.mass_flow < 0.)) {
+    set_last_error("Jet u, T, and mdot must be greater than 0");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (j.n_roi < 0 || (j.n_roi > 0 && (j.n_roi < 2 || !j.roi_time || !j.roi_mdot || !j.roi_vel))) {
+    set_last_error("rate-of-injection table needs at least two points");
+    return PELEMP_E_ARG;
+  }
+  const double mag = std::sqrt(j.norm[0] * j.norm[0] + j.norm[1] * j.norm[1] + j.norm[2] * j.norm[2]);
+  if (!(mag > 0.)) {
+    set_last_error("jet normal is zero");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  for (int d = 0; d < 3; ++d) j.norm[d] /= mag;
+  for (int d = 0; d < 3; ++d) {
+    const double x = c.pelec ? j.cent[d] * cgs::kLen : j.cent[d];
+    if (x > c.geom.phi[d] || x < c.geom.plo[d]) {
+      set_last_error("Jet center outside of problem domain");
+      return PELEMP_E_BAD_INJECTION;
+    }
+  }
+  if (c.nf > 1) {
+    double sumtest = 0.;
+    for (int s = 0; s < c.nf; ++s) sumtest += j.jet_Y[s];
+    if (std::fabs(1. - sumtest) > 1.E-8) {
+      set_last_error("Jet mass fractions must sum to 1");
+      return PELEMP_E_BAD_INJECTION;
+    }
+  }
+  make_dist(j, avg_dia);
+  return PELEMP_OK;
+}
+
+template <int NF>
+static int launch_inject(const InjJet<NF>& jj, double inject_mass, long long max_cand, long long out_cap,
+                         long long next_id, double* dpos, double* drd, long long* did, InjResult* dres,
+                         hipStream_t s) {
+  const InjResult zero{};
+  PM_HIP(hipMemcpyAsync(dres, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((inject_kernel<NF>), dim3(1), dim3(kInjBlock), 0, s, jj, inject_mass, max_cand, out_cap, next_id,
+                     dpos, drd, did, dres);
+  PM_HIP(hipGetLastError());
+  return PELEMP_OK;
+}
+
+// The droplet loop of sprayInjection on the device; droplets staged and
+// appended.  jet: SI.
+template <int NF>
+static int inject_droplets(Ctx& c, const pelemp_spray_jet& j, double time, double dt, double inject_mass,
+                           double num_ppp, double min_dia, double expected, double* cur_mass, int64_t* n_added,
+                           hipStream_t s) {
+  InjJet<NF> jj{};
+  double avg_dia = 0.;
+  jj.dist = make_dist(j, &avg_dia);
+  for (int d = 0; d < 3; ++d) jj.cent[d] = j.cent[d];
+  // transform_loc_vel's jet angles (SprayJet.H:175-181)
+  const double norm_mag = std::sqrt(j.norm[0] * j.norm[0] + j.norm[1] * j.norm[1] + j.norm[2] * j.norm[2]);
+  const double theta_jet = pm::acos(j.norm[2] / norm_mag);
+  const double phi_jet = pm::atan2(j.norm[1] / norm_mag, j.norm[0] / norm_mag);
+  jj.sp1 = pm::sin(phi_jet);
+  jj.cp1 = pm::cos(phi_jet);
+  jj.st1 = pm::sin(theta_jet);
+  jj.ct1 = pm::cos(theta_jet);
+  jj.spread = j.spread_angle;
+  jj.swirl = j.swirl_angle;
+  jj.hollow_spread = j.hollow_spread;
+  jj.jet_dia = j.jet_dia;
+  jj.hollow = j.hollow_spray;
+  jj.umag = jet_velocity(j, time);
+  jj.T = j.jet_T;
+  // rho_part of an injected droplet (SprayInjection.H:129-139)
+  double rho_part = 0.;
+  if (NF > 1) {
+    for (int s2 = 0; s2 < NF; ++s2) {
+      jj.Y[s2] = j.jet_Y[s2];
+      const double* a = &c.sd.rho_coef[4 * s2];
+      rho_part += j.jet_Y[s2] / (a[0] + j.jet_T * (a[1] + j.jet_T * (a[2] + j.jet_T * a[3])));
+    }
+    rho_part = 1. / rho_part;
+  } else {
+    jj.Y[0] = 1.;
+    const double* a = &c.sd.rho_coef[0];
+    rho_part = a[0] + j.jet_T * (a[1] + j.jet_T * (a[2] + j.jet_T * a[3]));
+  }
+  jj.rho_part = rho_part;
+  jj.min_dia = min_dia;
+  jj.num_ppp = num_ppp;
+  jj.dt = dt;
+  jj.initial_bm2 = c.sd.do_breakup == 2 ? -1. : 0.;
+  jj.seed = j.seed;
+  jj.call = j.n_injections;
+  for (int d = 0; d < 3; ++d) {
+    jj.plo[d] = c.geom.plo[d];
+    jj.dxi[d] = 1. / c.geom.dx[d];
+    jj.ncell[d] = c.ncell[d];
+  }
+  // candidate limit: the reference loops until the mass is reached; a
+  // distribution that keeps almost no candidates would never finish
+  const double want = std::min(std::max(16. * expected + 65536., 1048576.), 6.7108864e7);
+  const long long max_cand = static_cast<long long>(want);
+  int64_t next_id = 0, next_slot = 0;
+  int st = read_counters(c, &next_id, &next_slot);
+  if (st) return st;
+  long long out_cap = static_cast<long long>(std::min(2. * expected + 1024., static_cast<double>(c.cap) + 1.));
+  const int nsr = 10 + NF;
+  InjResult res{};
+  double *dpos = nullptr, *drd = nullptr;
+  long long* did = nullptr;
+  InjResult* dres = nullptr;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    PM_HIP(hipMalloc(&dpos, 8 * 3 * static_cast<size_t>(out_cap)));
+    PM_HIP(hipMalloc(&drd, 8 * nsr * static_cast<size_t>(out_cap)));
+    PM_HIP(hipMalloc(&did, 8 * static_cast<size_t>(out_cap)));
+    PM_HIP(hipMalloc(&dres, sizeof(InjResult)));
+    st = launch_inject<NF>(jj, inject_mass, max_cand, out_cap, next_id, dpos, drd, did, dres, s);
+    if (st) return st;
+    PM_HIP(hipMemcpyAsync(&res, dres, sizeof(res), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    if (res.n_out <= out_cap) break;
+    // more droplets than staged: the same draws again with room for all
+    (void)hipFree(dpos);
+    (void)hipFree(drd);
+    (void)hipFree(did);
+    (void)hipFree(dres);
+    dpos = drd = nullptr;
+    did = nullptr;
+    dres = nullptr;
+    out_cap = res.n_out;
+  }
+  auto release = [&]() {
+    (void)hipFree(dpos);
+    (void)hipFree(drd);
+    (void)hipFree(did);
+    (void)hipFree(dres);
+  };
+  if (res.unfinished) {
+    release();
+    set_last_error("injection: candidate limit reached before the injected mass");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (res.outside) {
+    release();
+    set_last_error("Bad injection particle");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  *cur_mass = res.cur_mass;
+  *n_added = res.n_out;
+  st = append_staged(c, res.n_out, dpos, drd, reinterpret_cast<const int64_t*>(did), next_id + res.n_out);
+  release();
+  return st;
+}
+
 }  // namespace pelemp
 
 using namespace pelemp;
@@ -698,6 +1125,11 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.total = c.scalars + 1;
   c.err = c.scalars + 2;
   if (c.surr_cnt && hipMemset(c.surr_cnt, 0, 2 * sizeof(unsigned long long)) != hipSuccess) st = PELEMP_E_HIP;
+  if (c.sb_ids) {
+    // counters of new droplets: next id, next slot, children of the last pass
+    const int64_t ids[4] = {1, 0, 0, 0};
+    if (hipMemcpy(c.sb_ids, ids, sizeof(ids), hipMemcpyHostToDevice) != hipSuccess) st = PELEMP_E_HIP;
+  }
   if (st != PELEMP_OK || hipMemset(c.scalars, 0, 64) != hipSuccess ||
       hipMemset(c.cstart, 0, 4 * (static_cast<size_t>(c.ncells) + 1)) != hipSuccess) {
     free_ctx(c);
@@ -826,11 +1258,12 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
     PM_HIP(hipGetLastError());
   }
   c.nslots = m;
+  int64_t max_id = 0;
+  for (int64_t i = 0; i < m; ++i) max_id = std::max(max_id, hid[i]);
+  c.next_id = max_id + 1;
   if (c.branches) {
     // counters of new droplets: next id (past the largest id handed in),
     // next slot, children of the last pass
-    int64_t max_id = 0;
-    for (int64_t i = 0; i < m; ++i) max_id = std::max(max_id, hid[i]);
     const int64_t ids[4] = {max_id + 1, m, 0, 0};
     PM_HIP(hipMemcpy(c.sb_ids, ids, sizeof(ids), hipMemcpyHostToDevice));
   }
@@ -903,6 +1336,498 @@ int pelemp_spray_get_particles(pelemp_spray_ctx* h, int64_t* n, double* pos, dou
   (void)hipFree(did);
   *n = m;
   return PELEMP_OK;
+}
+
+int pelemp_spray_add_particles(pelemp_spray_ctx* h, int64_t n, const double* pos, const double* rdata,
+                               const int64_t* id) {
+  if (!h || n < 0 || (n > 0 && (!pos || !rdata || !id))) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  const int nsr = 10 + c.nf;
+  std::vector<double> hp, hr;
+  std::vector<int64_t> hid;
+  int64_t max_id = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (id[i] <= 0) continue;
+    hp.insert(hp.end(), pos + 3 * i, pos + 3 * i + 3);
+    hr.insert(hr.end(), rdata + nsr * i, rdata + nsr * i + nsr);
+    hid.push_back(id[i]);
+    max_id = std::max(max_id, id[i]);
+    if (c.pelec) cgs::droplet_to_si(&hp[hp.size() - 3], &hr[hr.size() - nsr], c.nf, c.sd.do_breakup);
+  }
+  const int64_t m = static_cast<int64_t>(hid.size());
+  if (m == 0) return PELEMP_OK;
+  for (int64_t i = 0; i < m; ++i)
+    for (int d = 0; d < 3; ++d)
+      if (!(hp[3 * i + d] >= c.geom.plo[d] && hp[3 * i + d] <= c.geom.phi[d])) {
+        set_last_error("particle outside the domain");
+        return PELEMP_E_LEFT_DOMAIN;
+      }
+  double *dpos = nullptr, *drd = nullptr;
+  int64_t* did = nullptr;
+  PM_HIP(hipMalloc(&dpos, 8 * 3 * m));
+  PM_HIP(hipMalloc(&drd, 8 * nsr * m));
+  PM_HIP(hipMalloc(&did, 8 * m));
+  PM_HIP(hipMemcpy(dpos, hp.data(), 8 * 3 * m, hipMemcpyHostToDevice));
+  PM_HIP(hipMemcpy(drd, hr.data(), 8 * nsr * m, hipMemcpyHostToDevice));
+  PM_HIP(hipMemcpy(did, hid.data(), 8 * m, hipMemcpyHostToDevice));
+  const int st = append_staged(c, m, dpos, drd, did, max_id + 1);
+  (void)hipFree(dpos);
+  (void)hipFree(drd);
+  (void)hipFree(did);
+  return st;
+}
+
+int pelemp_spray_jet_check(const pelemp_spray_ctx* h, pelemp_spray_jet* jet, double* avg_dia) {
+  if (!h || !jet) return PELEMP_E_ARG;
+  double a = 0.;
+  const int st = check_jet(h->c, *jet, &a);
+  if (avg_dia) *avg_dia = a;
+  return st;
+}
+
+int pelemp_spray_read_roi(const char* path, double rho_part, double cd, double jet_dia, double mass_conv,
+                          int32_t* n, double* time, double* mdot, double* vel, double* start_time,
+                          double* end_time, double* max_jet_vel) {
+  if (!path || !n || *n < 0) return PELEMP_E_ARG;
+  std::ifstream infile(path);
+  if (!infile.good()) {
+    set_last_error("ROI file does not exists");
+    return PELEMP_E_ARG;
+  }
+  std::stringstream buf;
+  buf << infile.rdbuf();
+  std::istringstream iss(buf.str());
+  std::string line;
+  std::getline(iss, line);  // header
+  const double jet_area = 0.25 * kInjPI * pm::pow(jet_dia, 2.);
+  const double tconv = 1.E-3;  // ms -> s
+  double maxv = 0., t_start = 0., t_end = 1.E8, prev_time = 0.;
+  bool started = false;
+  std::vector<double> tt, mm, vv;
+  while (std::getline(iss, line)) {
+    for (char& ch : line)
+      if (ch == ';') ch = ' ';
+    std::istringstream sin(line);
+    double t = 0., md = 0.;
+    sin >> t;
+    sin >> md;
+    t *= tconv;
+    md *= mass_conv;
+    if (md > 0.) {
+      if (!started) {
+        started = true;
+        t_start = t;
+        tt.push_back(prev_time);
+        mm.push_back(0.);
+        vv.push_back(0.);
+      }
+      const double v = md / (rho_part * jet_area * cd);
+      maxv = std::max(maxv, v);
+      tt.push_back(t);
+      mm.push_back(md);
+      vv.push_back(v);
+      t_end = t;
+    } else if (started) {
+      t_end = t;
+      tt.push_back(t);
+      mm.push_back(0.);
+      vv.push_back(0.);
+      break;
+    }
+    prev_time = t;
+  }
+  const int32_t np = static_cast<int32_t>(tt.size());
+  if (np > *n) {
+    *n = np;
+    set_last_error("ROI table capacity too small");
+    return PELEMP_E_ARG;
+  }
+  for (int32_t i = 0; i < np; ++i) {
+    if (time) time[i] = tt[i];
+    if (mdot) mdot[i] = mm[i];
+    if (vel) vel[i] = vv[i];
+  }
+  *n = np;
+  if (start_time) *start_time = t_start;
+  if (end_time) *end_time = t_end;
+  if (max_jet_vel) *max_jet_vel = maxv;
+  return PELEMP_OK;
+}
+
+int pelemp_spray_inject(pelemp_spray_ctx* h, pelemp_spray_jet* jet, double time, double sim_dt, int64_t* n_injected,
+                        void* stream) {
+  if (!h || !jet) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  if (n_injected) *n_injected = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // jet_active (SprayJet.H:71-86); the checks on a copy, so the caller's
+  // jet (normalised once by pelemp_spray_jet_check) is not changed
+  double avg_dia = 0.;
+  {
+    pelemp_spray_jet chk = *jet;
+    const int st0 = check_jet(c, chk, &avg_dia);
+    if (st0) return st0;
+  }
+  int st = PELEMP_OK;
+  if (jet->jet_dia < 0.) {
+    set_last_error("Jet diameter is not set");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (avg_dia < 0.) {
+    set_last_error("Average droplet diameter is not set");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  if (jet->jet_T < 0.) {
+    set_last_error("Droplet temperature is not set");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  const bool active = !(time < jet->start_time || time > jet->end_time);
+  // SprayInjection.H:17-22
+  if (!active || sim_dt <= 0. || jet_velocity(*jet, time) < 0.) return PELEMP_OK;
+  // the jet in SI (a PeleC jet is CGS); rates evaluated in the jet's units
+  pelemp_spray_jet j = *jet;
+  double mdot = jet_mdot(*jet, time);
+  double umag = jet_velocity(*jet, time);
+  if (c.pelec) {
+    cgs::jet_to_si(j);
+    mdot *= cgs::kMass;
+    umag *= cgs::kLen;
+    make_dist(j, &avg_dia);  // the SI mean diameter
+  }
+  j.n_roi = 0;  // rates below are the evaluated ones
+  j.jet_vel = umag;
+  // SprayInjection.H:28-66
+  double rho_avg = 0.;
+  for (int spf = 0; spf < c.nf; ++spf) {
+    const double* a = &c.sd.rho_coef[4 * spf];
+    const double Y = c.nf > 1 ? j.jet_Y[spf] : 1.;
+    rho_avg += Y / (a[0] + j.jet_T * (a[1] + j.jet_T * (a[2] + j.jet_T * a[3])));
+  }
+  rho_avg = 1. / rho_avg;
+  const double Pi_six = kInjPI / 6.;
+  const double avg_mass = Pi_six * rho_avg * pm::pow(avg_dia, 3.);
+  const double min_dia = pm::cbrt(6. * kMinMass / (kInjPI * rho_avg));
+  if (avg_dia < min_dia || avg_mass < kMinMass) {
+    set_last_error("Average droplet size too small, floating point issues expected");
+    return PELEMP_E_BAD_INJECTION;
+  }
+  double dt = sim_dt;
+  double inject_mass = mdot * sim_dt;
+  if (j.sum_inj_mass > 0.) {
+    dt += j.sum_inj_time;
+    inject_mass += j.sum_inj_mass;
+  }
+  double num_ppp = j.num_ppp;
+  if (num_ppp < 0.) {
+    num_ppp = inject_mass / avg_mass;
+    j.num_ppp = num_ppp;
+  }
+  if (inject_mass / (num_ppp * avg_mass) < j.min_parcel) {
+    j.sum_inj_mass = inject_mass;
+    j.sum_inj_time = dt;
+  } else {
+    const double expected = inject_mass / (num_ppp * avg_mass);
+    double cur_mass = 0.;
+    int64_t added = 0;
+    switch (c.nf) {
+      case 1: st = inject_droplets<1>(c, j, time, dt, inject_mass, num_ppp, min_dia, expected, &cur_mass, &added, s); break;
+      case 2: st = inject_droplets<2>(c, j, time, dt, inject_mass, num_ppp, min_dia, expected, &cur_mass, &added, s); break;
+      case 6: st = inject_droplets<6>(c, j, time, dt, inject_mass, num_ppp, min_dia, expected, &cur_mass, &added, s); break;
+      default: set_last_error("unsupported fuel count"); return PELEMP_E_BAD_CONFIG;
+    }
+    if (st) return st;
+    j.n_injections += 1;
+    // SprayInjection.H:160-180
+    const double est_mdot = cur_mass / dt;
+    if (est_mdot - mdot > 0.5 * mdot) j.min_parcel += 1.;
+    j.total_inj_mass += cur_mass;
+    j.total_inj_time += dt;
+    j.sum_inj_mass = -1.;
+    j.sum_inj_time = -1.;
+    if (n_injected) *n_injected = added;
+  }
+  // the state back into the caller's jet
+  if (c.pelec) {
+    cgs::jet_state_to_cgs(j, *jet);
+  } else {
+    jet->num_ppp = j.num_ppp;
+    jet->sum_inj_mass = j.sum_inj_mass;
+    jet->sum_inj_time = j.sum_inj_time;
+    jet->min_parcel = j.min_parcel;
+    jet->total_inj_mass = j.total_inj_mass;
+    jet->total_inj_time = j.total_inj_time;
+    jet->n_injections = j.n_injections;
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_spray_uniform_init(pelemp_spray_ctx* h, const int32_t num_part[3], const double vel[3], double dia,
+                              double T, const double* Y, double num_ppp) {
+  if (!h || !num_part || !vel || !Y) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  if (num_part[0] <= 0 || num_part[1] <= 0 || num_part[2] <= 0) return PELEMP_E_ARG;
+  const long long n = static_cast<long long>(num_part[0]) * num_part[1] * num_part[2];
+  int64_t n_old = 0;
+  int st = pelemp_spray_num_particles(h, &n_old);
+  if (st) return st;
+  if (n > c.cap) {
+    set_last_error("context capacity too small for the new droplets");
+    return PELEMP_E_NOMEM;
+  }
+  const double f = c.pelec ? cgs::kLen : 1.;
+  double dxp[3];
+  for (int d = 0; d < 3; ++d) dxp[d] = (c.geom.phi[d] - c.geom.plo[d]) / static_cast<double>(num_part[d]);
+  const double v[3] = {vel[0] * f, vel[1] * f, vel[2] * f};
+  const double dia_si = dia * f;
+  const int nsr = 10 + c.nf;
+  int64_t next_id = 0, next_slot = 0;
+  st = read_counters(c, &next_id, &next_slot);
+  if (st) return st;
+  double *dpos = nullptr, *drd = nullptr, *dY = nullptr;
+  long long* did = nullptr;
+  int* dout = nullptr;
+  PM_HIP(hipMalloc(&dpos, 8 * 3 * static_cast<size_t>(n)));
+  PM_HIP(hipMalloc(&drd, 8 * nsr * static_cast<size_t>(n)));
+  PM_HIP(hipMalloc(&did, 8 * static_cast<size_t>(n)));
+  PM_HIP(hipMalloc(&dY, 8 * PELEMP_MAX_FUEL));
+  PM_HIP(hipMalloc(&dout, 4));
+  PM_HIP(hipMemcpy(dY, Y, 8 * c.nf, hipMemcpyHostToDevice));
+  PM_HIP(hipMemset(dout, 0, 4));
+  const double ib2 = c.sd.do_breakup == 2 ? -1. : 0.;
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+  auto run = [&](auto nft) {
+    constexpr int NF = decltype(nft)::value;
+    hipLaunchKernelGGL((lattice_kernel<NF>), grid, block, 0, 0, n, num_part[0], num_part[1], dxp[0], dxp[1], dxp[2],
+                       v[0], v[1], v[2], dia_si, T, dY, num_ppp, ib2, next_id, c.geom.plo[0], c.geom.plo[1],
+                       c.geom.plo[2], 1. / c.geom.dx[0], 1. / c.geom.dx[1], 1. / c.geom.dx[2], c.ncell[0],
+                       c.ncell[1], c.ncell[2], dpos, drd, did, dout);
+  };
+  switch (c.nf) {
+    case 1: run(std::integral_constant<int, 1>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 6: run(std::integral_constant<int, 6>{}); break;
+    default: set_last_error("unsupported fuel count"); st = PELEMP_E_BAD_CONFIG;
+  }
+  int outside = 0;
+  if (!st) {
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpy(&outside, dout, 4, hipMemcpyDeviceToHost));
+    if (outside) {
+      set_last_error("Bad particle");
+      st = PELEMP_E_BAD_INJECTION;
+    } else {
+      st = append_staged(c, n, dpos, drd, reinterpret_cast<const int64_t*>(did), next_id + n);
+    }
+  }
+  (void)hipFree(dpos);
+  (void)hipFree(drd);
+  (void)hipFree(did);
+  (void)hipFree(dY);
+  (void)hipFree(dout);
+  return st;
+}
+
+int pelemp_spray_write_injection_log(const char* path, int32_t njets, const char* const* names,
+                                     const pelemp_spray_jet* jets) {
+  if (!path || njets < 0 || (njets > 0 && (!names || !jets))) return PELEMP_E_ARG;
+  std::ofstream file(path, std::ios::out | std::ios::trunc);
+  if (!file.good()) {
+    set_last_error(std::string("cannot open ") + path);
+    return PELEMP_E_ARG;
+  }
+  file << njets << "\n";
+  file.precision(15);
+  for (int32_t i = 0; i < njets; ++i) {
+    const pelemp_spray_jet& j = jets[i];
+    file << names[i] << " " << j.num_ppp << " " << j.sum_inj_mass << " " << j.sum_inj_time << " " << j.min_parcel
+         << " " << j.total_inj_mass << " " << j.total_inj_time << "\n";
+  }
+  file.close();
+  if (!file.good()) {
+    set_last_error("Problem writing injection file");
+    return PELEMP_E_ARG;
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_spray_read_injection_log(const char* path, int32_t njets, const char* const* names,
+                                    pelemp_spray_jet* jets) {
+  if (!path || njets < 0 || (njets > 0 && (!names || !jets))) return PELEMP_E_ARG;
+  if (njets == 0) return PELEMP_OK;
+  // PostInitRestart's name checks (SprayIO.cpp:105-118): names must be set;
+  // repeated names next to each other are an error (std::unique)
+  for (int32_t i = 0; i < njets; ++i) {
+    if (!names[i] || names[i][0] == '\0') {
+      set_last_error("Jet not provided name");
+      return PELEMP_E_ARG;
+    }
+    if (i > 0 && std::strcmp(names[i], names[i - 1]) == 0) {
+      set_last_error("Duplicate jet names detected");
+      return PELEMP_E_ARG;
+    }
+  }
+  std::ifstream in(path);
+  if (!in.good()) return PELEMP_OK;  // no file: jets keep their state
+  int in_numjets = 0;
+  in >> in_numjets;
+  for (int i = 0; i < in_numjets; ++i) {
+    std::string name;
+    double v[6];
+    in >> name >> v[0] >> v[1] >> v[2] >> v[3] >> v[4] >> v[5];
+    if (in.fail()) {
+      set_last_error("malformed injection file");
+      return PELEMP_E_ARG;
+    }
+    for (int32_t mj = 0; mj < njets; ++mj) {
+      if (name == names[mj]) {
+        jets[mj].num_ppp = v[0];
+        jets[mj].sum_inj_mass = v[1];
+        jets[mj].sum_inj_time = v[2];
+        jets[mj].min_parcel = v[3];
+        jets[mj].total_inj_mass = v[4];
+        jets[mj].total_inj_time = v[5];
+      }
+    }
+  }
+  return PELEMP_OK;
+}
+
+// all particles in the caller's units (AoS meaning)
+static int fetch_all(pelemp_spray_ctx* h, std::vector<double>& pos, std::vector<double>& rd,
+                     std::vector<int64_t>& ids) {
+  int64_t n = 0;
+  int st = pelemp_spray_num_particles(h, &n);
+  if (st) return st;
+  const int nsr = 10 + h->c.nf;
+  pos.resize(3 * std::max<int64_t>(n, 1));
+  rd.resize(nsr * std::max<int64_t>(n, 1));
+  ids.resize(std::max<int64_t>(n, 1));
+  st = pelemp_spray_get_particles(h, &n, pos.data(), rd.data(), ids.data());
+  pos.resize(3 * n);
+  rd.resize(nsr * n);
+  ids.resize(n);
+  return st;
+}
+
+int pelemp_spray_write_ascii(pelemp_spray_ctx* h, const char* path) {
+  if (!h || !path) return PELEMP_E_ARG;
+  std::vector<double> pos, rd;
+  std::vector<int64_t> ids;
+  int st = fetch_all(h, pos, rd, ids);
+  if (st) return st;
+  const int nsr = 10 + h->c.nf;
+  std::ofstream file(path, std::ios::out | std::ios::trunc);
+  if (!file.good()) {
+    set_last_error(std::string("cannot open ") + path);
+    return PELEMP_E_ARG;
+  }
+  file << ids.size() << '\n';
+  file.precision(15);
+  for (size_t i = 0; i < ids.size(); ++i) {
+    for (int d = 0; d < 3; ++d) file << pos[3 * i + d] << ' ';
+    file << ids[i] << ' ' << 0 << ' ';
+    for (int q = 0; q < nsr; ++q) file << rd[nsr * i + q] << ' ';
+    file << '\n';
+  }
+  file.close();
+  if (!file.good()) {
+    set_last_error("Problem writing particle file");
+    return PELEMP_E_ARG;
+  }
+  return PELEMP_OK;
+}
+
+static const char kCkptTag[] = "pelemp_spray_particles_v1";
+
+int pelemp_spray_checkpoint(pelemp_spray_ctx* h, const char* dir, int32_t nfuel_names,
+                            const char* const* fuel_names) {
+  if (!h || !dir) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  std::vector<double> pos, rd;
+  std::vector<int64_t> ids;
+  int st = fetch_all(h, pos, rd, ids);
+  if (st) return st;
+  int64_t next_id = 0, next_slot = 0;
+  st = read_counters(c, &next_id, &next_slot);
+  if (st) return st;
+  const std::string pdir = std::string(dir) + "/particles";
+  mkdir(dir, 0755);
+  mkdir(pdir.c_str(), 0755);
+  // component names of SprayParticleIO (SprayIO.cpp:9-33)
+  std::vector<std::string> names = {"xvel", "yvel", "zvel", "temperature", "diam"};
+  for (int sp = 0; sp < c.nf; ++sp)
+    names.push_back("spray_mf_" + ((fuel_names && sp < nfuel_names && fuel_names[sp]) ? std::string(fuel_names[sp])
+                                                                                       : std::to_string(sp)));
+  names.push_back("number_density");
+  names.push_back("num_dens0");
+  if (c.sd.do_breakup == 1) {
+    names.push_back("Y_TAB");
+    names.push_back("Ydot_TAB");
+  } else if (c.sd.do_breakup == 2) {
+    names.push_back("shed_mass");
+    names.push_back("rt_time");
+  } else {
+    names.push_back("unused1");
+    names.push_back("unused2");
+  }
+  names.push_back("wall_film_height");
+  {
+    std::ofstream hd(pdir + "/Header.pelemp", std::ios::out | std::ios::trunc);
+    hd << kCkptTag << '\n' << ids.size() << '\n' << next_id << '\n' << c.nf << '\n' << names.size() << '\n';
+    for (const auto& nm : names) hd << nm << '\n';
+    hd.close();
+    if (!hd.good()) {
+      set_last_error("Problem writing checkpoint header");
+      return PELEMP_E_ARG;
+    }
+  }
+  std::ofstream df(pdir + "/DATA.pelemp", std::ios::out | std::ios::trunc | std::ios::binary);
+  df.write(reinterpret_cast<const char*>(pos.data()), static_cast<std::streamsize>(8 * pos.size()));
+  df.write(reinterpret_cast<const char*>(ids.data()), static_cast<std::streamsize>(8 * ids.size()));
+  df.write(reinterpret_cast<const char*>(rd.data()), static_cast<std::streamsize>(8 * rd.size()));
+  df.close();
+  if (!df.good()) {
+    set_last_error("Problem writing checkpoint data");
+    return PELEMP_E_ARG;
+  }
+  return PELEMP_OK;
+}
+
+int pelemp_spray_restart(pelemp_spray_ctx* h, const char* dir) {
+  if (!h || !dir) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  const std::string pdir = std::string(dir) + "/particles";
+  std::ifstream hd(pdir + "/Header.pelemp");
+  std::string tag;
+  int64_t n = 0, next_id = 1;
+  int nf = 0;
+  size_t ncomp = 0;
+  hd >> tag >> n >> next_id >> nf >> ncomp;
+  if (!hd.good() || tag != kCkptTag || n < 0) {
+    set_last_error("not a particle checkpoint: " + pdir);
+    return PELEMP_E_ARG;
+  }
+  if (nf != c.nf || ncomp != static_cast<size_t>(10 + c.nf)) {
+    set_last_error("checkpoint fuel count differs from the context's");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  const int nsr = 10 + c.nf;
+  std::vector<double> pos(3 * n), rd(nsr * n);
+  std::vector<int64_t> ids(n);
+  std::ifstream df(pdir + "/DATA.pelemp", std::ios::binary);
+  df.read(reinterpret_cast<char*>(pos.data()), static_cast<std::streamsize>(8 * pos.size()));
+  df.read(reinterpret_cast<char*>(ids.data()), static_cast<std::streamsize>(8 * ids.size()));
+  df.read(reinterpret_cast<char*>(rd.data()), static_cast<std::streamsize>(8 * rd.size()));
+  if (!df.good()) {
+    set_last_error("truncated checkpoint data");
+    return PELEMP_E_ARG;
+  }
+  int st = pelemp_spray_set_particles(h, n, pos.data(), rd.data(), ids.data());
+  if (st) return st;
+  int64_t cur_id = 0, next_slot = 0;
+  st = read_counters(c, &cur_id, &next_slot);
+  if (st) return st;
+  return write_counters(c, std::max(cur_id, next_id), next_slot);
 }
 
 int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,

@@ -76,6 +76,33 @@ inline void droplet_to_cgs(double* pos, double* rd, int nf, int do_breakup) {
   if (do_breakup == 2) rd[7 + nf] /= kVol;
 }
 
+constexpr double kMass = 1.e-3;    // g -> kg (also g/s -> kg/s)
+
+// a spray jet's lengths, velocities, diameters and masses to SI; the
+// held-over mass keeps its "none" value (-1)
+inline void jet_to_si(pelemp_spray_jet& j) {
+  for (int d = 0; d < 3; ++d) j.cent[d] *= kLen;
+  j.jet_dia *= kLen;
+  j.jet_vel *= kLen;
+  j.max_jet_vel *= kLen;
+  j.mass_flow *= kMass;
+  j.dist_param[0] *= kLen;
+  if (j.dist_type == PELEMP_DIST_NORMAL || j.dist_type == PELEMP_DIST_LOGNORMAL) j.dist_param[1] *= kLen;
+  if (j.sum_inj_mass > 0.) j.sum_inj_mass *= kMass;
+  j.total_inj_mass *= kMass;
+}
+
+// the injection state of an SI jet back into the CGS jet it came from
+inline void jet_state_to_cgs(const pelemp_spray_jet& si, pelemp_spray_jet& j) {
+  j.num_ppp = si.num_ppp;
+  j.sum_inj_mass = si.sum_inj_mass > 0. ? si.sum_inj_mass / kMass : si.sum_inj_mass;
+  j.sum_inj_time = si.sum_inj_time;
+  j.min_parcel = si.min_parcel;
+  j.total_inj_mass = si.total_inj_mass / kMass;
+  j.total_inj_time = si.total_inj_time;
+  j.n_injections = si.n_injections;
+}
+
 // SI -> CGS factor of each source component in deposition order:
 // momentum(3), mass, species(nf), energy
 inline void source_scale(int nf, double* s) {

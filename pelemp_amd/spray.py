@@ -14,6 +14,14 @@ Method names and argument meanings follow the reference
     transferSource(source_ghosts, level, tmpSource,
                    actSource)                            SprayParticles.H:250-295
     getSprayData() / getSprayComps()                     SprayParticles.H:378-384
+    sprayInjection(time, spray_jet, sim_dt, level)       SprayInjection.H:10-180
+    uniformSprayInit(num_part, vel_part, dia_part, T_part, Y_part, level,
+                     num_redist, num_ppp)                SprayInjection.H:198-348
+    SprayParticleIO(level, is_checkpoint, dir)           SprayIO.cpp:6-98
+    PostInitRestart(dir)                                 SprayIO.cpp:100-160
+
+SprayJet mirrors Source/PP_Spray/SprayJet.{H,cpp} (constant or rate-shaped
+jets with the five droplet-size distributions of Distribution/).
 
 Every call goes through the C ABI of libpelemp_spray.so (HIP kernels for
 gfx950).  `state` and `source` are device FABs (DeviceFab), the MultiFab of
@@ -62,6 +70,81 @@ class DeviceFab:
             self.buf.zero()
 
 
+class SprayJet:
+    """A spray jet (SprayJet.{H,cpp}), built like the reference's direct
+    constructor (SprayJet.cpp:95-141): angles in degrees, the distribution by
+    name with its parameters (Uniform: diameter; Normal / LogNormal:
+    mean_dia, std_dev; Weibull: mean_dia, k; ChiSquared: d32).  Units are
+    the container's flavour units.  `seed` keys the counter-based random
+    numbers of its injections."""
+
+    def __init__(self, jet_name: str, spc: "SprayParticleContainer", jet_cent, jet_norm, spread_angle: float,
+                 jet_dia: float, jet_vel: float, mass_flow: float, jet_temp: float, jet_Y=None,
+                 dist_type: str = "Uniform", dist_params=(0.0, 0.0), start_time: float = 0.0,
+                 end_time: float = 1.0e8, phi_swirl: float = 0.0, hollow_spray: bool = False,
+                 hollow_spread: float = 0.0, inject_ppp: float = -1.0, seed: int = 0):
+        self.m_jetName = jet_name
+        self._spc = spc
+        j = C.jet_c(jet_cent, jet_norm, spread_angle, jet_dia, jet_vel, mass_flow, jet_temp, jet_Y, dist_type,
+                    dist_params, start_time, end_time, phi_swirl, hollow_spray, hollow_spread, inject_ppp, seed)
+        self._j = j
+        self._roi = None
+        avg = ctypes.c_double()
+        C.check(spc._lib.pelemp_spray_jet_check(spc._h, ctypes.byref(j), ctypes.byref(avg)), "SprayJet")
+        self.m_avgDia = float(avg.value)
+
+    def readROI(self, roi_file: str, rho_part: float, cd: float) -> None:
+        """SprayJet::readROI (SprayJet.cpp:194-256): rate of injection from a
+        file of 'time [ms];mdot' lines; sets start/end time and max_jet_vel."""
+        mconv = 1.0e-3 if self._spc.flavour == "pelelmex" else 1.0
+        n = ctypes.c_int32(0)
+        lib = self._spc._lib
+        st = lib.pelemp_spray_read_roi(roi_file.encode(), float(rho_part), float(cd), self._j.jet_dia, mconv,
+                                       ctypes.byref(n), None, None, None, None, None, None)
+        if st not in (C.OK, C.E_ARG) or (st == C.E_ARG and n.value == 0):
+            C.check(st, "SprayJet.readROI")
+        t, m, v = (np.zeros(max(n.value, 1)) for _ in range(3))
+        t0, t1, vmax = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        C.check(lib.pelemp_spray_read_roi(roi_file.encode(), float(rho_part), float(cd), self._j.jet_dia, mconv,
+                                          ctypes.byref(n), t.ctypes.data, m.ctypes.data, v.ctypes.data,
+                                          ctypes.byref(t0), ctypes.byref(t1), ctypes.byref(vmax)), "SprayJet.readROI")
+        self._roi = (t[:n.value].copy(), m[:n.value].copy(), v[:n.value].copy())
+        j = self._j
+        j.n_roi = n.value
+        j.roi_time, j.roi_mdot, j.roi_vel = (a.ctypes.data for a in self._roi)
+        j.start_time, j.end_time, j.max_jet_vel = t0.value, t1.value, vmax.value
+
+    # accessors of SprayJet.H
+    def jet_name(self) -> str:
+        return self.m_jetName
+
+    def num_ppp(self) -> float:
+        return float(self._j.num_ppp)
+
+    def set_num_ppp(self, v: float) -> None:
+        self._j.num_ppp = float(v)
+
+    def max_jet_vel(self) -> float:
+        return float(self._j.max_jet_vel)
+
+    def get_avg_dia(self) -> float:
+        return self.m_avgDia
+
+    def jet_active(self, time: float) -> bool:
+        return not (time < self._j.start_time or time > self._j.end_time)
+
+    def state(self) -> dict:
+        """The public injection members (SprayJet.H:258-266)."""
+        j = self._j
+        return dict(num_ppp=j.num_ppp, m_sumInjMass=j.sum_inj_mass, m_sumInjTime=j.sum_inj_time,
+                    m_minParcel=j.min_parcel, m_totalInjMass=j.total_inj_mass, m_totalInjTime=j.total_inj_time,
+                    n_injections=j.n_injections)
+
+    def reset_sum(self) -> None:
+        self._j.sum_inj_mass = -1.0
+        self._j.sum_inj_time = -1.0
+
+
 def _stream_handle(stream) -> Optional[int]:
     if stream is None:
         return None
@@ -105,6 +188,7 @@ class SprayParticleContainer:
         else:
             raise ValueError(f"unknown flavour {flavour!r}")
         self._h = h
+        self.m_sprayJets: list = []
 
     # -- lifetime ----------------------------------------------------------
     def close(self) -> None:
@@ -186,6 +270,12 @@ class SprayParticleContainer:
         as the kernels are queued; `sync()` then reports device-side errors.
         """
         del level, ltransparm
+        if self.m_sprayJets and int(self._sd.do_breakup) == 2:
+            # the KH-RT parcel-mass threshold from the jets (SprayParticles.cpp:230-239)
+            p = C.SprayParamsC()
+            C.check(self._lib.pelemp_spray_get_params(self._h, ctypes.byref(p)), "pelemp_spray_get_params")
+            p.avg_inject_mass = self.avg_inject_mass()
+            C.check(self._lib.pelemp_spray_set_params(self._h, ctypes.byref(p)), "pelemp_spray_set_params")
         sv, dv = state.view(), source.view()
         h = _stream_handle(stream)
         C.check(self._lib.pelemp_spray_update(self._h, ctypes.byref(sv), ctypes.byref(dv), float(flow_dt),
@@ -264,6 +354,100 @@ class SprayParticleContainer:
                                                        _stream_handle(stream)), "pelemp_spray_transfer_source")
         if sync:
             C.device_synchronize()
+
+    # -- injection and initialisation ----------------------------------------
+    def add_particles(self, pos: np.ndarray, rdata: np.ndarray, ids: np.ndarray) -> None:
+        """Append droplets (the tiles' host-particle copy of SprayInjection.H:
+        164-179), e.g. from a host-side injector."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rdata = np.ascontiguousarray(rdata, dtype=np.float64)
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        n = ids.shape[0]
+        if pos.shape != (n, 3) or rdata.shape != (n, self.NSR):
+            raise ValueError("particle arrays have the wrong shape")
+        C.check(self._lib.pelemp_spray_add_particles(self._h, n, pos.ctypes.data, rdata.ctypes.data,
+                                                     ids.ctypes.data), "pelemp_spray_add_particles")
+
+    def sprayInjection(self, time: float, spray_jet: SprayJet, sim_dt: float, level: int = 0,
+                       stream=None) -> int:
+        """SprayParticleContainer::sprayInjection (SprayInjection.H:10-180) on
+        the device; returns the number of droplets injected."""
+        del level
+        n = ctypes.c_int64()
+        C.check(self._lib.pelemp_spray_inject(self._h, ctypes.byref(spray_jet._j), float(time), float(sim_dt),
+                                              ctypes.byref(n), _stream_handle(stream)), "sprayInjection")
+        if spray_jet not in self.m_sprayJets:
+            self.m_sprayJets.append(spray_jet)
+        return int(n.value)
+
+    def uniformSprayInit(self, num_part, vel_part, dia_part: float, T_part: float, Y_part, level: int = 0,
+                         num_redist: int = -1, num_ppp: float = 1.0) -> None:
+        """SprayParticleContainer::uniformSprayInit (SprayInjection.H:198-348),
+        one rank; num_redist only schedules the reference's rank-by-rank copy."""
+        del level, num_redist
+        npart = (ctypes.c_int32 * 3)(*[int(x) for x in num_part])
+        vel = (ctypes.c_double * 3)(*[float(x) for x in vel_part])
+        Y = np.ascontiguousarray(np.atleast_1d(np.asarray(Y_part, dtype=np.float64)))
+        if Y.shape[0] != self.nf:
+            raise ValueError("Y_part needs one value per fuel")
+        C.check(self._lib.pelemp_spray_uniform_init(self._h, npart, vel, float(dia_part), float(T_part),
+                                                    Y.ctypes.data, float(num_ppp)), "uniformSprayInit")
+
+    def avg_inject_mass(self) -> float:
+        """sum over jets of N d_avg^3 / n_jets (SprayParticles.cpp:230-239)."""
+        if not self.m_sprayJets:
+            return 0.0
+        nj = float(len(self.m_sprayJets))
+        return sum(js.num_ppp() * js.get_avg_dia() ** 3 / nj for js in self.m_sprayJets)
+
+    # -- on-disk formats ------------------------------------------------------
+    write_ascii_files = False
+
+    def SprayParticleIO(self, level: int, is_checkpoint: bool, dir: str, fuel_names=None) -> None:
+        """SprayParticleContainer::SprayParticleIO (SprayIO.cpp:6-98): the
+        particle checkpoint under dir/particles, the spray<N>.p3d ASCII file
+        next to dir when write_ascii_files, and injection_data.log of the jets
+        in a checkpoint."""
+        import os
+        import re
+
+        names = list(fuel_names) if fuel_names is not None else list(self.fuel_names)
+        arr = (ctypes.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+        C.check(self._lib.pelemp_spray_checkpoint(self._h, dir.encode(), len(names), arr), "SprayParticleIO")
+        if level == 0 and self.write_ascii_files:
+            m = re.search(r"(\d+)[^\d]*$", dir)
+            num = m.group(1) if m else ""
+            fname = os.path.join(os.path.dirname(dir), "spray" + num + ".p3d")
+            C.check(self._lib.pelemp_spray_write_ascii(self._h, fname.encode()), "WriteAsciiFile")
+        if is_checkpoint and self.m_sprayJets:
+            os.makedirs(os.path.join(dir, "particles"), exist_ok=True)
+            self.write_injection_log(os.path.join(dir, "particles", "injection_data.log"))
+
+    def write_injection_log(self, path: str) -> None:
+        jets = (C.SprayJetC * len(self.m_sprayJets))(*[js._j for js in self.m_sprayJets])
+        nm = (ctypes.c_char_p * len(self.m_sprayJets))(*[js.m_jetName.encode() for js in self.m_sprayJets])
+        C.check(self._lib.pelemp_spray_write_injection_log(path.encode(), len(self.m_sprayJets), nm, jets),
+                "write injection log")
+
+    def Restart(self, dir: str) -> None:
+        """Reads the particles of a checkpoint written by SprayParticleIO."""
+        C.check(self._lib.pelemp_spray_restart(self._h, dir.encode()), "Restart")
+
+    def PostInitRestart(self, dir: str = "") -> None:
+        """SprayParticleContainer::PostInitRestart (SprayIO.cpp:100-160): the
+        jets' injection state from dir/particles/injection_data.log."""
+        import os
+
+        if not self.m_sprayJets:
+            return
+        n = len(self.m_sprayJets)
+        jets = (C.SprayJetC * n)(*[js._j for js in self.m_sprayJets])
+        nm = (ctypes.c_char_p * n)(*[js.m_jetName.encode() for js in self.m_sprayJets])
+        path = os.path.join(dir, "particles", "injection_data.log")
+        C.check(self._lib.pelemp_spray_read_injection_log(path.encode(), n, nm, jets), "PostInitRestart")
+        for js, jc in zip(self.m_sprayJets, jets):
+            for f in ("num_ppp", "sum_inj_mass", "sum_inj_time", "min_parcel", "total_inj_mass", "total_inj_time"):
+                setattr(js._j, f, getattr(jc, f))
 
     # -- timing ---------------------------------------------------------------
     def pass_counts(self) -> list:

@@ -1,0 +1,328 @@
+"""Pins of the oracle's injection and initialisation (oracle/spray_oracle_inject.h;
+SprayInjection.H:10-348, SprayJet.{H,cpp}, Distribution/Distributions.cpp) and
+of the library's host-side file formats (readROI, injection_data.log):
+
+ * distributions against their closed forms (normal moments, the log-normal
+   parametrisation by the diameter's own mean and standard deviation, the
+   Weibull quantile and mean lambda Gamma(1 + 1/k), the ChiSquared table as
+   the Gamma(4) CDF on [0, 12] and its walk);
+ * injected droplets against the jet geometry: every droplet goes back
+   along its velocity to a point of the jet disk, at the spread angle its
+   radius implies, turned by the swirl angle about the jet axis;
+ * the mass loop: the droplets' own masses sum to the injected mass, which
+   reaches the target with the last droplet and not before; the held-over
+   mass and the min_parcel rule;
+ * readROI against the SprayA rate-of-injection file (a reference test
+   input, tests/golden/roi_spraya_wbreakup.dat)."""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import special, stats
+
+from oracle import oracle_py as O
+from pelemp_amd import _capi as C
+from pelemp_amd import problem as P
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _u(n, seed=1):
+    return [O.inj_random(seed, 0, k, 3) for k in range(n)], [O.inj_random(seed, 0, k, 4) for k in range(n)]
+
+
+def test_random_numbers_uniform_and_keyed():
+    u = np.array([O.inj_random(7, 3, k, s) for k in range(4000) for s in range(6)])
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 4 * math.sqrt(1 / 12 / u.size)
+    # every key gives its own number
+    assert O.inj_random(7, 3, 5, 1) != O.inj_random(7, 3, 5, 2)
+    assert O.inj_random(7, 3, 5, 1) != O.inj_random(7, 4, 5, 1)
+    assert O.inj_random(7, 3, 5, 1) != O.inj_random(8, 3, 5, 1)
+    assert O.inj_random(7, 3, 5, 1) == O.inj_random(7, 3, 5, 1)
+
+
+def test_normal_and_lognormal_moments():
+    u1, u2 = _u(40000)
+    d = np.array([O.dist_sample(C.DIST_NORMAL, (20e-6, 4e-6), a, b)[0] for a, b in zip(u1, u2)])
+    n = d.size
+    assert abs(d.mean() - 20e-6) < 4 * 4e-6 / math.sqrt(n)
+    assert abs(d.std() - 4e-6) < 4 * 4e-6 / math.sqrt(2 * n)
+    # LogNormal is parametrised by the diameter's mean and standard deviation
+    # (Distributions.cpp:67-75): E[d] = mean, std[d] = std
+    mean, sd = 30e-6, 12e-6
+    d = np.array([O.dist_sample(C.DIST_LOGNORMAL, (mean, sd), a, b)[0] for a, b in zip(u1, u2)])
+    assert np.all(d > 0)
+    assert abs(d.mean() - mean) < 4 * sd / math.sqrt(n)
+    sig2 = math.log(1 + (sd / mean) ** 2)
+    # the log of the diameter is normal with variance log(1 + (sd/mean)^2)
+    assert abs(np.log(d).var() - sig2) < 5 * sig2 * math.sqrt(2 / n)
+    assert O.dist_sample(C.DIST_LOGNORMAL, (mean, sd), 0.3)[1] == mean
+
+
+def test_weibull_quantile_and_mean():
+    lam, k = 25e-6, 3.0
+    for u in (0.01, 0.3, 0.77, 0.999):
+        d, avg = O.dist_sample(C.DIST_WEIBULL, (lam, k), u)
+        assert d == pytest.approx(lam * (-math.log(1 - u)) ** (1 / k), rel=1e-14)
+        assert avg == lam
+    u1, _ = _u(40000, seed=5)
+    d = np.array([O.dist_sample(C.DIST_WEIBULL, (lam, k), a)[0] for a in u1])
+    m = lam * special.gamma(1 + 1 / k)
+    s = lam * math.sqrt(special.gamma(1 + 2 / k) - special.gamma(1 + 1 / k) ** 2)
+    assert abs(d.mean() - m) < 4 * s / math.sqrt(d.size)
+
+
+def test_chisquared_table_and_walk():
+    # rvals[i] = CDF of Gamma(4, 1) at 0.12 (i + 1), normalised on [0, 12]; a
+    # uniform u in (rvals[n-1], rvals[n]] gives the bin's left edge
+    # 0.12 n * d32 / 3
+    d32 = 60e-6
+    g = stats.gamma(4)
+    rv = g.cdf(0.12 * np.arange(1, 101)) / g.cdf(12.0)
+    for nbin in (0, 1, 7, 30, 64, 98):
+        lo = rv[nbin - 1] if nbin > 0 else 0.0
+        u = lo + 0.5 * (rv[nbin] - lo)
+        d, avg = O.dist_sample(C.DIST_CHISQUARED, (d32,), u)
+        assert d == pytest.approx(0.12 * nbin * d32 / 3, rel=1e-12, abs=1e-30)
+    # get_avg_dia: the walk at 0.5 (the bin of the Gamma(4) median)
+    nmed = int(np.searchsorted(rv, 0.5))
+    assert avg == pytest.approx(0.12 * nmed * d32 / 3, rel=1e-12)
+    assert O.dist_sample(C.DIST_UNIFORM, (17e-6,), 0.4) == (17e-6, 17e-6)
+
+
+def _prob():
+    return P.config2(ncell=16)
+
+
+def _jet(prob, **kw):
+    g = prob.geom
+    cent = kw.pop("cent", [0.5 * (a + b) for a, b in zip(g["plo"], g["phi"])])
+    norm = np.array(kw.pop("norm", [0.3, -0.2, 1.0]))
+    norm = norm / np.linalg.norm(norm)
+    args = dict(spread_angle=20.0, jet_dia=1e-4, jet_vel=50.0, mass_flow=1e-3, jet_temp=300.0, jet_Y=[0.4, 0.6],
+                dist_type="Normal", dist_params=(20e-6, 5e-6), inject_ppp=5.0, seed=3)
+    args.update(kw)
+    return C.jet_c(cent, norm, **args)
+
+
+def _rhoL(sd, T, s):
+    a = sd["rho_coef"][4 * s:4 * s + 4]
+    return a[0] + T * (a[1] + T * (a[2] + T * a[3]))
+
+
+def _masses(sd, rd, nf):
+    T, d = rd[:, 3], rd[:, 4]
+    inv = sum(rd[:, 5 + s] / _rhoL(sd, T, s) for s in range(nf))
+    return math.pi / 6 / inv * d ** 3 * rd[:, 5 + nf]
+
+
+@pytest.mark.parametrize("swirl", [0.0, 25.0])
+def test_injected_droplets_follow_the_jet_geometry(swirl):
+    prob = _prob()
+    jet = _jet(prob, phi_swirl=swirl)
+    dt = 1e-6
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, dt, 1000)
+    assert st == 0 and len(ids) > 20
+    np.testing.assert_array_equal(ids, 1000 + np.arange(len(ids)))
+    n = np.array([jet.norm[d] for d in range(3)])
+    c = np.array([jet.cent[d] for d in range(3)])
+    vel = rd[:, 0:3]
+    umag = np.linalg.norm(vel, axis=1)
+    np.testing.assert_allclose(umag, 50.0, rtol=1e-13)
+    # back along the velocity to the disk through the centre, normal n
+    pmov = ((pos - c) @ n) / (dt * (vel @ n))
+    assert np.all(pmov >= 0) and np.all(pmov < 1)
+    loc = pos - (pmov * dt)[:, None] * vel
+    r = np.linalg.norm(loc - c, axis=1)
+    assert np.all(r <= 0.5e-4 * (1 + 1e-12))
+    # spread: the angle to the axis is (r / (D/2)) * spread / 2
+    theta = np.arccos(np.clip((vel @ n) / umag, -1, 1))
+    np.testing.assert_allclose(theta, r / 0.5e-4 * math.radians(20.0) / 2, atol=1e-9)
+    # swirl: the velocity across the axis is the radial direction turned by
+    # the swirl angle about n
+    er = (loc - c) / r[:, None]
+    vp = vel - (vel @ n)[:, None] * n
+    vp /= np.linalg.norm(vp, axis=1)[:, None]
+    ok = r > 1e-3 * 0.5e-4
+    cosang = np.sum(er * vp, axis=1)[ok]
+    sinang = (np.cross(er, vp) @ n)[ok]
+    np.testing.assert_allclose(cosang, math.cos(math.radians(swirl)), atol=1e-8)
+    np.testing.assert_allclose(sinang, math.sin(math.radians(swirl)), atol=1e-8)
+    # the droplets' composition, temperature, parcel size, breakup state
+    np.testing.assert_array_equal(rd[:, 3], 300.0)
+    np.testing.assert_array_equal(rd[:, 5], 0.4)
+    np.testing.assert_array_equal(rd[:, 7], 5.0)
+    np.testing.assert_array_equal(rd[:, 8], 5.0)
+    np.testing.assert_array_equal(rd[:, 9:12], 0.0)
+
+
+def test_mass_loop_reaches_the_target_with_the_last_droplet():
+    prob = _prob()
+    jet = _jet(prob)
+    dt = 1e-6
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, dt, 1)
+    assert st == 0
+    m = _masses(prob.sdata, rd, prob.nf)
+    target = 1e-3 * dt
+    assert m.sum() == pytest.approx(jet.total_inj_mass, rel=1e-12)
+    assert m.sum() >= target * (1 - 1e-12)
+    assert m[:-1].sum() < target
+    assert jet.total_inj_time == dt and jet.sum_inj_mass == -1.0 and jet.n_injections == 1
+    # min_parcel rule: over-injection by more than half the rate adds one
+    over = m.sum() / dt - 1e-3 > 0.5 * 1e-3
+    assert jet.min_parcel == (2.0 if over else 1.0)
+
+
+def test_held_over_mass_and_first_parcel_size():
+    prob = _prob()
+    # inject_ppp < 0: the first injection sets N so that its mass is one mean
+    # parcel; min_parcel 3 holds the mass over until three parcels' worth
+    jet = _jet(prob, inject_ppp=-1.0, dist_type="Uniform", dist_params=(20e-6,))
+    jet.min_parcel = 3.0
+    dt = 1e-6
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, dt, 1)
+    assert st == 0 and len(ids) == 0
+    rho = 1 / (0.4 / _rhoL(prob.sdata, 300.0, 0) + 0.6 / _rhoL(prob.sdata, 300.0, 1))
+    avg_mass = math.pi / 6 * rho * 20e-6 ** 3
+    assert jet.num_ppp == pytest.approx(1e-3 * dt / avg_mass, rel=1e-13)
+    assert jet.sum_inj_mass == pytest.approx(1e-3 * dt, rel=1e-15) and jet.sum_inj_time == dt
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, dt, dt, 1)
+    assert len(ids) == 0 and jet.sum_inj_time == 2 * dt
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 2 * dt, dt, 1)
+    # three steps of mass, each parcel one step's worth: three parcels
+    assert st == 0 and len(ids) == 3
+    assert jet.total_inj_time == pytest.approx(3 * dt, rel=1e-15)
+    assert jet.sum_inj_mass == -1.0
+
+
+def test_rejected_candidates_and_bad_injection():
+    prob = _prob()
+    # a normal distribution reaching below the minimum droplet: no droplet
+    # at or below min_dia is kept
+    jet = _jet(prob, dist_params=(3e-6, 4e-6))
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, 1e-6, 1)
+    assert st == 0 and len(ids) > 0
+    rho = 1 / (0.4 / _rhoL(prob.sdata, 300.0, 0) + 0.6 / _rhoL(prob.sdata, 300.0, 1))
+    min_dia = (6 * 1e-16 / (math.pi * rho)) ** (1 / 3)
+    assert rd[:, 4].min() > min_dia
+    # mean droplet below the minimum mass (SprayInjection.H:39-43)
+    jet = _jet(prob, dist_type="Uniform", dist_params=(1e-7,))
+    assert O.inject(prob.sdata, prob.geom, jet, 0.0, 1e-6, 1)[0] == C.E_BAD_INJECTION
+    # a jet on the domain face pointing out: droplets outside (Where fails)
+    g = prob.geom
+    jet = _jet(prob, cent=[0.5 * g["phi"][0], 0.5 * g["phi"][1], 0.0], norm=[0, 0, -1])
+    assert O.inject(prob.sdata, prob.geom, jet, 0.0, 1e-6, 1)[0] == C.E_BAD_INJECTION
+    # inactive jets and non-positive steps inject nothing
+    jet = _jet(prob, start_time=1.0)
+    assert O.inject(prob.sdata, prob.geom, jet, 0.0, 1e-6, 1)[0] == 0 and jet.n_injections == 0
+    jet = _jet(prob)
+    st, _, _, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, 0.0, 1)
+    assert st == 0 and len(ids) == 0
+
+
+def test_hollow_cone():
+    prob = _prob()
+    hs = 10.0
+    jet = _jet(prob, hollow_spray=True, hollow_spread=hs, norm=[0, 0, 1])
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, 1e-6, 1)
+    assert st == 0
+    vel = rd[:, 0:3]
+    theta = np.arccos(vel[:, 2] / np.linalg.norm(vel, axis=1))
+    half = math.radians(20.0) / 2
+    # every droplet leaves from the rim at spread/2 +- hollow_spread/2
+    assert np.all(np.abs(theta - half) <= math.radians(hs) / 2 + 1e-12)
+
+
+def test_uniform_spray_init_lattice():
+    prob = _prob()
+    g = prob.geom
+    npart = (3, 4, 5)
+    st, pos, rd, ids = O.uniform_init(prob.sdata, g, npart, (1.0, 2.0, 3.0), 25e-6, 320.0, (0.3, 0.7), 7.0, 11)
+    assert st == 0 and len(ids) == 60
+    # ids in lattice order (x fastest), stored from the back of the list
+    np.testing.assert_array_equal(ids, 11 + np.arange(59, -1, -1))
+    idx = ids - 11
+    i0, i1, i2 = idx % 3, (idx // 3) % 4, idx // 12
+    L = np.array(g["phi"]) - np.array(g["plo"])
+    want = (np.stack([i0, i1, i2], axis=1) + 0.5) * (L / np.array(npart))
+    np.testing.assert_array_equal(pos, want)
+    np.testing.assert_array_equal(rd[:, 0:5], np.tile([1.0, 2.0, 3.0, 320.0, 25e-6], (60, 1)))
+    np.testing.assert_array_equal(rd[:, 5:7], np.tile([0.3, 0.7], (60, 1)))
+    np.testing.assert_array_equal(rd[:, 7:9], 7.0)
+
+
+def test_read_roi_sprayA():
+    # SprayJet::readROI: time in ms, mdot in g/s; the table starts at the
+    # last zero-rate time before the jet starts and stops after the first
+    # zero rate after it; velocity mdot / (rho A cd)
+    L = C.lib()
+    path = os.path.join(GOLDEN, "roi_spraya_wbreakup.dat")
+    raw = np.loadtxt(path, delimiter=";", skiprows=1)
+    rho, cd, D = 0.7, 0.89, 9e-3
+    n = ctypes.c_int32(len(raw) + 2)
+    t, m, v = (np.zeros(len(raw) + 2) for _ in range(3))
+    t0, t1, vmax = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    assert L.pelemp_spray_read_roi(path.encode(), rho, cd, D, 1.0, ctypes.byref(n), t.ctypes.data, m.ctypes.data,
+                                   v.ctypes.data, ctypes.byref(t0), ctypes.byref(t1), ctypes.byref(vmax)) == 0
+    pos = np.flatnonzero(raw[:, 1] > 0)
+    first, last = pos[0], pos[-1]
+    ends_with_zero = last + 1 < len(raw)
+    assert n.value == (last - first + 1) + 1 + (1 if ends_with_zero else 0)
+    assert t0.value == raw[first, 0] * 1e-3
+    assert t[0] == raw[first - 1, 0] * 1e-3 and m[0] == 0.0
+    np.testing.assert_array_equal(m[1:last - first + 2], raw[first:last + 1, 1])
+    area = 0.25 * math.pi * D ** 2
+    np.testing.assert_allclose(v[1:last - first + 2], raw[first:last + 1, 1] / (rho * area * cd), rtol=1e-14)
+    assert vmax.value == pytest.approx(raw[:, 1].max() / (rho * area * cd), rel=1e-14)
+    assert t1.value == raw[last + 1 if ends_with_zero else last, 0] * 1e-3
+    # too small a table: the needed size comes back
+    n2 = ctypes.c_int32(3)
+    assert L.pelemp_spray_read_roi(path.encode(), rho, cd, D, 1.0, ctypes.byref(n2), None, None, None, None, None,
+                                   None) == C.E_ARG
+    assert n2.value == n.value
+
+
+def test_roi_interpolation_in_the_oracle():
+    prob = _prob()
+    tt = np.array([0.0, 1e-6, 3e-6, 4e-6])
+    mm = np.array([0.0, 2e-3, 1e-3, 0.0])
+    vv = np.array([0.0, 40.0, 20.0, 0.0])
+    jet = _jet(prob, start_time=1e-6, end_time=4e-6)
+    jet.n_roi = 4
+    jet.roi_time, jet.roi_mdot, jet.roi_vel = tt.ctypes.data, mm.ctypes.data, vv.ctypes.data
+    dt = 1e-7
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 2e-6, dt, 1)
+    assert st == 0 and len(ids) > 0
+    # the rate and the speed at t = 2e-6 are halfway along the second segment
+    np.testing.assert_allclose(np.linalg.norm(rd[:, 0:3], axis=1), 30.0, rtol=1e-13)
+    m = _masses(prob.sdata, rd, prob.nf)
+    assert m[:-1].sum() < 1.5e-3 * dt <= m.sum() * (1 + 1e-12)
+
+
+def test_injection_log_round_trip(tmp_path):
+    L = C.lib()
+    prob = _prob()
+    jets = (C.SprayJetC * 2)(_jet(prob), _jet(prob))
+    jets[0].num_ppp, jets[0].sum_inj_mass, jets[0].sum_inj_time = 3.25, 1.2345678901234e-9, 4e-7
+    jets[0].min_parcel, jets[0].total_inj_mass, jets[0].total_inj_time = 2.0, 7.5e-6, 1.5e-3
+    jets[1].num_ppp = 9.0
+    names = (ctypes.c_char_p * 2)(b"jet1", b"jet2")
+    path = str(tmp_path / "injection_data.log")
+    assert L.pelemp_spray_write_injection_log(path.encode(), 2, names, jets) == 0
+    lines = open(path).read().split("\n")
+    assert lines[0] == "2"
+    f = lines[1].split()
+    assert f[0] == "jet1" and float(f[1]) == 3.25 and f[2] == "1.2345678901234e-09"
+    # read into fresh jets: names decide, order does not
+    back = (C.SprayJetC * 2)(_jet(prob), _jet(prob))
+    rn = (ctypes.c_char_p * 2)(b"jet2", b"jet1")
+    assert L.pelemp_spray_read_injection_log(path.encode(), 2, rn, back) == 0
+    assert back[1].num_ppp == 3.25 and back[1].sum_inj_mass == 1.2345678901234e-9
+    assert back[1].min_parcel == 2.0 and back[1].total_inj_time == 1.5e-3
+    assert back[0].num_ppp == 9.0
+    # adjacent duplicate names are rejected (std::unique, SprayIO.cpp:114-118)
+    dn = (ctypes.c_char_p * 2)(b"jet1", b"jet1")
+    assert L.pelemp_spray_read_injection_log(path.encode(), 2, dn, back) == C.E_ARG
