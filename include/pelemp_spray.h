@@ -51,6 +51,10 @@ extern "C" {
                                     outside the domain), :286-289 (lattice
                                     droplet outside), and the jet checks of
                                     SprayJet.cpp:28-133, SprayJet.H:71-138 */
+#define PELEMP_E_EB_STENCIL 10 /* SprayInterpolation.H:562: no uncovered
+                                  cell of large enough volume fraction in the
+                                  EB stencil (the reference then indexes
+                                  indx_array[-1]) */
 
 /* Mirror of SprayData (SprayFuelData.H:115-212), fixed at PELEMP_MAX_FUEL
  * fuel slots; the first nfuel are used. */
@@ -184,6 +188,36 @@ int pelemp_spray_num_particles(pelemp_spray_ctx* ctx, int64_t* n);
 int pelemp_spray_add_particles(pelemp_spray_ctx* ctx, int64_t n,
                                const double* pos, const double* rdata,
                                const int64_t* id);
+
+/* ---- Embedded boundaries (SprayInterpolation.H:201-634,
+ *      WallFunctions.H:124-157, SprayParticles.cpp:192-199,271-291,413-418) */
+
+/* The EB data of the level for the state box, as AMReX's
+ * EBFArrayBoxFactory holds it (getEBCellFlagFab, getVolFrac, getCentroid,
+ * getBndryCent, getBndryNormal), device pointers owned by the caller, each a
+ * FAB over the box [lo, hi] (component-major, x fastest).  flags: AMReX
+ * EBCellFlag words, cell type in bits 0-1 (0 regular, 1 single-valued,
+ * 2 multi-valued, 3 covered), neighbour (i, j, k) in {-1,0,1}^3 connected
+ * when bit PELEMP_EB_NGBR_BIT(i, j, k) is set.  Centroids are relative to
+ * the cell centre in cell units; the boundary normal points out of the
+ * fluid. */
+#define PELEMP_EB_NGBR_BIT(i, j, k) (5 + ((i) + 1) + 3 * ((j) + 1) + 9 * ((k) + 1))
+typedef struct pelemp_eb_view {
+  const uint32_t* flags;
+  const double* vfrac;
+  const double* ccent; /* 3 components */
+  const double* bcent; /* 3 components */
+  const double* bnorm; /* 3 components */
+  int32_t lo[3];
+  int32_t hi[3];
+} pelemp_eb_view;
+/* Attach (eb != NULL) or detach the EB of the level.  With an EB that has
+ * cut or covered cells in the box, the update interpolates with
+ * eb_interp, scales deposits in cut cells by 1/vfrac, reflects droplets
+ * (or splashes them) off the EB, and deposits a droplet into the closest
+ * acceptable cell where eb_interp moves it; FP64 form.  The box must cover
+ * the state box used by the updates. */
+int pelemp_spray_set_eb(pelemp_spray_ctx* ctx, const pelemp_eb_view* eb);
 
 /* ---- Injection (SprayInjection.H, SprayJet.{H,cpp}, Distributions.cpp) ---- */
 

@@ -31,6 +31,10 @@ extern "C" struct oracle_branch_params {
 
 namespace {
 
+// EB of the level for the next updates (oracle_set_eb)
+oracle::EB g_eb;
+int64_t g_last_redirects = 0;
+
 template <class Mech, int NF>
 int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, const pelemp_geom* g,
                int64_t n, double* pos, double* rdata, int64_t* id, const pelemp_fab_view* state,
@@ -87,7 +91,7 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
   pelemp_fab_view src = *source;
   const int st = oracle::updateParticles<Mech, NF>(parts, *sd, *comps, *g, *state, src, flow_dt,
                                                    isVirt != 0, isGhost != 0, do_move != 0,
-                                                   spray_cfl, nthreads, bp);
+                                                   spray_cfl, nthreads, bp, g_eb);
   int64_t m = 0;
   for (const P& p : parts)
     if (p.id > 0) ++m;
@@ -103,6 +107,7 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
   }
   *n_out = m;
   if (bpp) bpp->next_id = bp.next_id;
+  g_last_redirects = bp.n_redirect;
   return st;
 }
 
@@ -132,6 +137,49 @@ int oracle_update(int mech, int nf, const pelemp_spray_data* sd, const pelemp_sp
                                                    flow_dt, isVirt, isGhost, do_move, spray_cfl,
                                                    nthreads, bp, n_out);
   return PELEMP_E_ARG;
+}
+
+// The EB used by the following oracle_update calls (host arrays in the
+// layout of pelemp_eb_view); NULL detaches it.
+void oracle_set_eb(const pelemp_eb_view* eb) {
+  g_eb = oracle::EB();
+  if (!eb) return;
+  g_eb.flags = eb->flags;
+  g_eb.vfrac = eb->vfrac;
+  g_eb.ccent = eb->ccent;
+  g_eb.bcent = eb->bcent;
+  g_eb.bnorm = eb->bnorm;
+  for (int d = 0; d < 3; ++d) {
+    g_eb.lo[d] = eb->lo[d];
+    g_eb.hi[d] = eb->hi[d];
+  }
+  g_eb.on = true;
+}
+
+// deposits sent to another cell by eb_interp in the last oracle_update
+int64_t oracle_last_redirects(void) { return g_last_redirects; }
+
+// eb_interp for one droplet (SprayInterpolation.H:395-632), domain-relative
+// cell indices: ijkc in/out, pos in/out; idx[8][3], w[8] out.  Returns
+// do_fe_interp (0/1) or -PELEMP_E_EB_STENCIL.
+int oracle_eb_interp(const pelemp_geom* g, double min_eb_vfrac, double* pos, int* ijkc, int* idx, double* w) {
+  const double dxi[3] = {1. / g->dx[0], 1. / g->dx[1], 1. / g->dx[2]};
+  double lx[3];
+  int ijk[3];
+  for (int d = 0; d < 3; ++d) {
+    lx[d] = (pos[d] - g->plo[d]) * dxi[d] + 0.5;
+    ijk[d] = static_cast<int>(std::floor(lx[d]));
+  }
+  int bflags[3] = {0, 0, 0};
+  int st = PELEMP_OK;
+  int ix[8][3];
+  const bool fe = oracle::eb_interp(pos, ijkc, ijk, *g, dxi, g_eb, min_eb_vfrac, ix, w,
+                                    [&](int (*i8)[3], double* w8) { oracle::trilinear_interp(ijk, lx, i8, w8, bflags); },
+                                    st);
+  if (st != PELEMP_OK) return -st;
+  for (int a = 0; a < 8; ++a)
+    for (int d = 0; d < 3; ++d) idx[3 * a + d] = ix[a][d];
+  return fe ? 1 : 0;
 }
 
 // Host evaluation of the pm:: elementary functions (fn codes as in

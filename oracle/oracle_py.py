@@ -96,6 +96,11 @@ def lib() -> ctypes.CDLL:
         L.oracle_dist_sample.restype = ctypes.c_double
         L.oracle_dist_sample.argtypes = [ctypes.c_int, vp, ctypes.c_double, ctypes.c_double,
                                          ctypes.POINTER(ctypes.c_double)]
+        L.oracle_set_eb.restype = None
+        L.oracle_set_eb.argtypes = [ctypes.POINTER(C.EBViewC)]
+        L.oracle_eb_interp.restype = ctypes.c_int
+        L.oracle_eb_interp.argtypes = [ctypes.POINTER(C.GeomC), ctypes.c_double, vp, vp, vp, vp]
+        L.oracle_last_redirects.restype = ctypes.c_int64
         L.oracle_inj_random.restype = ctypes.c_double
         L.oracle_inj_random.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
         _lib = L
@@ -322,3 +327,43 @@ def dist_sample(dist_type: int, params, u1: float, u2: float = 0.5):
 
 def inj_random(seed: int, call: int, cand: int, slot: int) -> float:
     return lib().oracle_inj_random(int(seed), int(call), int(cand), int(slot))
+
+
+_eb_keep = None
+
+
+def set_eb(eb) -> None:
+    """The EB (dict of host arrays flags, vfrac, ccent, bcent, bnorm, lo, hi)
+    for the following update calls; None detaches it."""
+    global _eb_keep
+    if eb is None:
+        lib().oracle_set_eb(None)
+        _eb_keep = None
+        return
+    arrs = {k: np.ascontiguousarray(eb[k], dtype=(np.uint32 if k == "flags" else np.float64))
+            for k in ("flags", "vfrac", "ccent", "bcent", "bnorm")}
+    v = C.EBViewC()
+    v.flags, v.vfrac, v.ccent = arrs["flags"].ctypes.data, arrs["vfrac"].ctypes.data, arrs["ccent"].ctypes.data
+    v.bcent, v.bnorm = arrs["bcent"].ctypes.data, arrs["bnorm"].ctypes.data
+    for d in range(3):
+        v.lo[d] = int(eb["lo"][d])
+        v.hi[d] = int(eb["hi"][d])
+    _eb_keep = (arrs, v)
+    lib().oracle_set_eb(ctypes.byref(v))
+
+
+def eb_interp(geom: dict, pos, min_eb_vfrac: float = 0.0):
+    """eb_interp for one droplet with the current EB: (do_fe_interp,
+    cells[8, 3] (domain-relative), weights[8], deposit cell, position)."""
+    g = C.geom_c(geom)
+    p = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    ijkc = np.floor((p - np.array(geom["plo"])) / np.array(geom["dx"])).astype(np.int32)
+    idx = np.zeros((8, 3), dtype=np.int32)
+    w = np.zeros(8)
+    r = lib().oracle_eb_interp(ctypes.byref(g), float(min_eb_vfrac), p.ctypes.data, ijkc.ctypes.data,
+                               idx.ctypes.data, w.ctypes.data)
+    return r, idx, w, ijkc, p
+
+
+def last_redirects() -> int:
+    return int(lib().oracle_last_redirects())

@@ -47,6 +47,7 @@
 #include "../include/pelemp_spray.h"
 #include "../pelemp_amd/csrc/gas_props.h"
 #include "../pelemp_amd/csrc/pm_math.h"
+#include "spray_oracle_eb.h"
 
 namespace oracle {
 
@@ -678,6 +679,7 @@ struct BranchParams {
   // (mom 3, mass, species nf, energy) to CGS at the add into the FAB
   bool pelec = false;
   double src_scale[5 + PELEMP_MAX_FUEL] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
+  int64_t n_redirect = 0;  // out: deposits eb_interp sent to another cell
 };
 
 // per-droplet values the reference keeps in registers across sub-cycles
@@ -695,7 +697,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
                     const pelemp_spray_comps& SPI, const pelemp_geom& g,
                     const pelemp_fab_view& state_v, pelemp_fab_view& source_v, double flow_dt,
                     bool isVirt, bool isGhost, bool do_move, double spray_cfl_lev, int nthreads,
-                    BranchParams& bp) {
+                    BranchParams& bp, const EB& eb = EB()) {
   constexpr int NS = Mech::NS;
   using C = Comps<NF>;
   const bool isActive = !(isVirt || isGhost);
@@ -706,7 +708,13 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
   const double inv_vol = 1. / vol;
   bool at_bounds = false;
   for (int d = 0; d < 3; ++d) at_bounds = at_bounds || g.bndry_lo[d] != 0 || g.bndry_hi[d] != 0;
-  const bool do_splash_box = do_splash && at_bounds;
+  // EB in the box: any cut or covered cell (SprayParticles.cpp:271-291)
+  bool eb_in_box = false;
+  if (eb.on) {
+    const int64_t ncb = eb.cstride();
+    for (int64_t i = 0; i < ncb && !eb_in_box; ++i) eb_in_box = (eb.flags[i] & 3u) != 0u;
+  }
+  const bool do_splash_box = do_splash && (at_bounds || eb_in_box);
   const bool make_new_drops = (do_breakup || do_splash_box) && isActive && do_move;
   const double sub_cfl = 0.5;
   double sub_dt = flow_dt;
@@ -771,6 +779,10 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     const int64_t n = static_cast<int64_t>(parts.size());
     std::vector<double> contrib(static_cast<size_t>(n) * NSRC, 0.);
     std::vector<int64_t> dep_off(n, -1);
+    // own: offset of the containing cell (the sort key), -1 when the droplet
+    // deposits nothing; redirected droplets deposit into dep_off != own
+    std::vector<int64_t> own_off(n, -1);
+    std::vector<unsigned char> redir(n, 0);
     int err_local = PELEMP_OK;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads) reduction(max : err_local)
     for (int64_t pid = 0; pid < n; ++pid) {
@@ -800,10 +812,31 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       const bool is_film = p.rdata[C::pstateFilmHght] > 0.;
       int idx[8][3];
       double w[8];
+      // the reference's ijkc: the unclamped containing cell, domain-relative
+      int ijkc_rel[3];
+      for (int d = 0; d < 3; ++d) ijkc_rel[d] = static_cast<int>(std::floor((p.pos[d] - g.plo[d]) * dxi[d]));
+      const int own_cell[3] = {ijkc[0], ijkc[1], ijkc[2]};
+      bool do_fe_interp = false;
       {
         // trilinear weights use lx - ijk in domain-relative index space
         int ijk_rel[3] = {ijk[0] - g.domain_lo[0], ijk[1] - g.domain_lo[1], ijk[2] - g.domain_lo[2]};
-        trilinear_interp(ijk_rel, lx, idx, w, bflags);
+        if (eb_in_box) {
+          int st_eb = PELEMP_OK;
+          do_fe_interp = eb_interp(p.pos, ijkc_rel, ijk_rel, g, dxi, eb, fdat.min_eb_vfrac, idx, w,
+                                   [&](int (*ix)[3], double* wx) { trilinear_interp(ijk_rel, lx, ix, wx, bflags); },
+                                   st_eb);
+          if (st_eb != PELEMP_OK) {
+            err_local = std::max(err_local, st_eb);
+            continue;
+          }
+          // with EB weights the deposit cell is eb_interp's: the droplet's
+          // cell or the closest acceptable cell of the stencil, possibly a
+          // ghost cell; other droplets keep their (clamped) cell
+          if (do_fe_interp)
+            for (int d = 0; d < 3; ++d) ijkc[d] = ijkc_rel[d] + g.domain_lo[d];
+        } else {
+          trilinear_interp(ijk_rel, lx, idx, w, bflags);
+        }
         for (int a = 0; a < 8; ++a)
           for (int d = 0; d < 3; ++d) idx[a][d] += g.domain_lo[d];
       }
@@ -832,7 +865,10 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
           }
         }
       }
-      const double cur_coef = -inv_vol * sub_dt / flow_dt;
+      // SprayParticles.cpp:413-419: 1/vfrac in a single-valued deposit cell
+      double cvol = inv_vol;
+      if (eb_in_box && eb.single(ijkc[0], ijkc[1], ijkc[2])) cvol *= 1. / (eb.vf(ijkc[0], ijkc[1], ijkc[2]));
+      const double cur_coef = -cvol * sub_dt / flow_dt;
       if (!srcv.contains(ijkc)) {
         if (!isGhost) {
           err_local = std::max(err_local, PELEMP_E_SRC_BOX);
@@ -849,6 +885,8 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         }
         cb[4 + NF] = cur_coef * gpv.fluid_eng_src;
         dep_off[pid] = srcv.off(ijkc, 0);
+        redir[pid] = (ijkc[0] != own_cell[0] || ijkc[1] != own_cell[1] || ijkc[2] != own_cell[2]) ? 1 : 0;
+        own_off[pid] = srcv.contains(own_cell) ? srcv.off(own_cell, 0) : -1;
       }
       if (do_move && !fdat.fixed_parts && p.id > 0 && !is_film) {
         const double pos0[3] = {p.pos[0], p.pos[1], p.pos[2]};
@@ -856,20 +894,24 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
           const double cvel = p.rdata[C::pstateVel + dir];
           p.pos[dir] += sub_dt * cvel;
         }
-        if (at_bounds) {
+        if (at_bounds || do_fe_interp) {
           if (check_bounds(p.pos, g, bflags)) {
             p.id = -1;
           } else {
-            // the film thickness array covers the domain cells; a droplet
+            // film thickness of the gather's cell (the EB's closest cell when
+            // moved there); the array covers the domain cells, and a droplet
             // that started behind a wall sees none (the reference reads its
             // ghost cell, which holds no film)
+            (void)pos0;
             bool inside = true;
+            int fc[3];
             for (int d = 0; d < 3; ++d) {
-              const int ic = static_cast<int>(std::floor((pos0[d] - g.plo[d]) * dxi[d])) + g.domain_lo[d];
-              inside = inside && ic >= g.domain_lo[d] && ic <= g.domain_hi[d];
+              fc[d] = ijkc_rel[d] + g.domain_lo[d];
+              inside = inside && fc[d] >= g.domain_lo[d] && fc[d] <= g.domain_hi[d];
             }
-            const double film_h = (do_splash_box && inside) ? wf[wf_index(ijkc)] : 0.;
-            impose_wall<NF>(do_splash_box, p, fdat, g, bflags, cBoilT, gpv.p_fluid, cy.rec, film_h);
+            const double film_h = (do_splash_box && inside) ? wf[wf_index(fc)] : 0.;
+            impose_wall<NF>(do_splash_box, p, fdat, g, bflags, cBoilT, gpv.p_fluid, cy.rec, film_h,
+                            eb_in_box ? &eb : nullptr, ijkc_rel);
           }
         }
       }
@@ -892,25 +934,42 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     for (int q = 0; q < 4; ++q) comps[q] = comp_of[q];
     for (int spf = 0; spf < NF; ++spf) comps[4 + spf] = SPI.specSrcIndx + spf;
     comps[4 + NF] = SPI.engSrcIndx;
-    // droplets of one cell are adjacent after the sort and share dep_off
+    // droplets of one cell are adjacent after the sort and share own_off;
+    // a cell's sum leaves out the droplets that deposit elsewhere (EB)
     std::vector<int64_t> seg_start;
     for (int64_t i = 0; i < n; ++i) {
-      if (i == 0 || dep_off[i] != dep_off[i - 1]) seg_start.push_back(i);
+      if (i == 0 || own_off[i] != own_off[i - 1]) seg_start.push_back(i);
     }
     seg_start.push_back(n);
     const int64_t nseg = static_cast<int64_t>(seg_start.size()) - 1;
+    auto on_comp = [&](int q) {
+      // disabled couplings leave their source components untouched
+      if (q < 3 && !fdat.mom_trans) return false;
+      if (q >= 3 && q < 4 + NF && !fdat.mass_trans) return false;
+      return true;
+    };
 #pragma omp parallel for schedule(static) num_threads(nthreads)
     for (int64_t s = 0; s < nseg; ++s) {
       const int64_t a = seg_start[s], b = seg_start[s + 1];
-      if (dep_off[a] < 0) continue;
+      if (own_off[a] < 0) continue;
       for (int q = 0; q < NSRC; ++q) {
-        // disabled couplings leave their source components untouched
-        if (q < 3 && !fdat.mom_trans) continue;
-        if (q >= 3 && q < 4 + NF && !fdat.mass_trans) continue;
+        if (!on_comp(q)) continue;
         double acc = 0.;
-        for (int64_t i = a; i < b; ++i) acc += contrib[static_cast<size_t>(i) * NSRC + q];
-        double* dst = &source_v.data[dep_off[a] + static_cast<int64_t>(comps[q]) * cstride];
+        for (int64_t i = a; i < b; ++i)
+          if (!redir[i]) acc += contrib[static_cast<size_t>(i) * NSRC + q];
+        double* dst = &source_v.data[own_off[a] + static_cast<int64_t>(comps[q]) * cstride];
         *dst = bp.pelec ? *dst + acc * bp.src_scale[q] : *dst + acc;
+      }
+    }
+    // then the redirected contributions, in droplet order
+    for (int64_t i = 0; i < n; ++i) {
+      if (!redir[i] || dep_off[i] < 0) continue;
+      ++bp.n_redirect;
+      for (int q = 0; q < NSRC; ++q) {
+        if (!on_comp(q)) continue;
+        const double v = contrib[static_cast<size_t>(i) * NSRC + q];
+        double* dst = &source_v.data[dep_off[i] + static_cast<int64_t>(comps[q]) * cstride];
+        *dst = bp.pelec ? *dst + v * bp.src_scale[q] : *dst + v;
       }
     }
   }

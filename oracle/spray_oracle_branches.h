@@ -702,7 +702,8 @@ void droplet_splashing(Particle<NF>& p, const pelemp_spray_data& fdat, double p_
 // check_wall (WallFunctions.H:89-171), Cartesian faces: pos normalised by
 // dx relative to plo; returns whether the particle is behind the wall.
 inline bool check_wall(const double pos[3], const int bflags[3], const double dx[3], const int ijkc[3],
-                       double& par_dot, double normal[3]) {
+                       double& par_dot, double normal[3], const EB* eb = nullptr, const pelemp_geom* g = nullptr,
+                       const int* ijkc_prev = nullptr) {
   bool wall_check = false;
   double bcentv[3];
   for (int dir = 0; dir < 3; ++dir) {
@@ -715,6 +716,8 @@ inline bool check_wall(const double pos[3], const int bflags[3], const double dx
       bcentv[dir] = 0.;
     }
   }
+  // EB walls (WallFunctions.H:124-157)
+  if (eb && !wall_check) wall_check = eb_wall(*eb, *g, ijkc, ijkc_prev, normal, bcentv);
   bool refl_check = false;
   if (wall_check) {
     const double tolerance = 2.220446049250313e-16;
@@ -728,7 +731,8 @@ inline bool check_wall(const double pos[3], const int bflags[3], const double dx
 // impose_wall (WallFunctions.H:173-234), Cartesian walls
 template <int NF>
 void impose_wall(bool do_splash, Particle<NF>& p, const pelemp_spray_data& fdat, const pelemp_geom& g,
-                 const int bflags[3], const double* cBoilT, double p_fluid, SBRecord<NF>& rf, double film_h) {
+                 const int bflags[3], const double* cBoilT, double p_fluid, SBRecord<NF>& rf, double film_h,
+                 const EB* eb = nullptr, const int* ijkc_prev = nullptr) {
   using C = Comps<NF>;
   double par_dot = 2.;
   double normal[3];
@@ -739,7 +743,7 @@ void impose_wall(bool do_splash, Particle<NF>& p, const pelemp_spray_data& fdat,
     normpos[d] = (p.pos[d] - g.plo[d]) / g.dx[d];
     ijkc[d] = static_cast<int>(std::floor(normpos[d]));
   }
-  const bool do_refl = check_wall(normpos, bflags, g.dx, ijkc, par_dot, normal);
+  const bool do_refl = check_wall(normpos, bflags, g.dx, ijkc, par_dot, normal, eb, &g, ijkc_prev);
   if (do_refl) {
     const double Nw_Vp = vdot(normal, vel_part);
     if (do_splash) {

@@ -22,6 +22,7 @@ E_ARG = 6
 E_NOMEM = 7
 E_BAD_CHILD = 8
 E_BAD_INJECTION = 9
+E_EB_STENCIL = 10
 ERROR_NAMES = {
     E_STATE_BOX: "state box too small",
     E_SRC_BOX: "source box too small",
@@ -32,6 +33,7 @@ ERROR_NAMES = {
     E_NOMEM: "out of device memory",
     E_BAD_CHILD: "secondary droplet without a valid position",
     E_BAD_INJECTION: "bad injection",
+    E_EB_STENCIL: "EB stencil without an acceptable cell",
 }
 
 DIST_UNIFORM = 0
@@ -119,6 +121,11 @@ class SprayParamsC(ctypes.Structure):
                 ("avg_inject_mass", _c_dbl)]
 
 
+class EBViewC(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_void_p), ("vfrac", ctypes.c_void_p), ("ccent", ctypes.c_void_p),
+                ("bcent", ctypes.c_void_p), ("bnorm", ctypes.c_void_p), ("lo", _c_i32 * 3), ("hi", _c_i32 * 3)]
+
+
 class SprayJetC(ctypes.Structure):
     _fields_ = [("cent", _c_dbl * 3), ("norm", _c_dbl * 3), ("spread_angle", _c_dbl), ("swirl_angle", _c_dbl),
                 ("hollow_spread", _c_dbl), ("jet_dia", _c_dbl), ("jet_vel", _c_dbl), ("max_jet_vel", _c_dbl),
@@ -146,6 +153,7 @@ _PROTOS = {
     "pelemp_spray_get_particles": (_c_i32, [_vp, ctypes.POINTER(_c_i64), _vp, _vp, _vp]),
     "pelemp_spray_num_particles": (_c_i32, [_vp, ctypes.POINTER(_c_i64)]),
     "pelemp_spray_add_particles": (_c_i32, [_vp, _c_i64, _vp, _vp, _vp]),
+    "pelemp_spray_set_eb": (_c_i32, [_vp, ctypes.POINTER(EBViewC)]),
     "pelemp_spray_jet_check": (_c_i32, [_vp, ctypes.POINTER(SprayJetC), ctypes.POINTER(_c_dbl)]),
     "pelemp_spray_read_roi": (_c_i32, [ctypes.c_char_p, _c_dbl, _c_dbl, _c_dbl, _c_dbl, ctypes.POINTER(_c_i32),
                                        _vp, _vp, _vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_dbl),

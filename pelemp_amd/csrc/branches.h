@@ -14,6 +14,7 @@
 #pragma once
 
 #include "droplet_physics.h"
+#include "eb.h"
 
 namespace pelemp {
 
@@ -664,7 +665,7 @@ PM_HDI void droplet_splashing(BDrop<NF>& d, const FuelConst<double, NF>& fd, con
 
 // check_wall (WallFunctions.H:89-171), Cartesian faces; pos normalised by dx
 PM_HDI bool check_wall(const double* pos, const int* bflags, const double* dx, const int* ijkc, double& par_dot,
-                       double* normal) {
+                       double* normal, const EbDev* eb = nullptr, const int* ijkc_prev = nullptr) {
   bool wall_check = false;
   double bcentv[3];
 #pragma unroll
@@ -678,6 +679,8 @@ PM_HDI bool check_wall(const double* pos, const int* bflags, const double* dx, c
       bcentv[dir] = 0.;
     }
   }
+  // EB walls (WallFunctions.H:124-157)
+  if (eb && !wall_check) wall_check = eb_wall(*eb, ijkc, ijkc_prev, normal, bcentv);
   bool refl_check = false;
   if (wall_check) {
     const double tolerance = 2.220446049250313e-16;
@@ -692,7 +695,7 @@ PM_HDI bool check_wall(const double* pos, const int* bflags, const double* dx, c
 template <int NF>
 PM_HDI void impose_wall(bool do_splash, BDrop<NF>& d, const FuelConst<double, NF>& fd, const LiquidExtra& lx,
                         const double* plo, const double* dx, const int* bflags, const double* cBoilT, double p_fluid,
-                        SBRec<NF>& rf, double film_h) {
+                        SBRec<NF>& rf, double film_h, const EbDev* eb = nullptr, const int* ijkc_prev = nullptr) {
   double par_dot = 2.;
   double normal[3];
   const double vel_part[3] = {d.vel[0], d.vel[1], d.vel[2]};
@@ -703,7 +706,7 @@ PM_HDI void impose_wall(bool do_splash, BDrop<NF>& d, const FuelConst<double, NF
     normpos[q] = (d.pos[q] - plo[q]) / dx[q];
     ijkc[q] = static_cast<int>(floor(normpos[q]));
   }
-  if (check_wall(normpos, bflags, dx, ijkc, par_dot, normal)) {
+  if (check_wall(normpos, bflags, dx, ijkc, par_dot, normal, eb, ijkc_prev)) {
     const double Nw_Vp = v3dot(normal, vel_part);
     if (do_splash) {
       droplet_splashing<NF>(d, fd, lx, p_fluid, vel_part, par_dot, normal, cBoilT, rf, film_h);

@@ -114,6 +114,17 @@ struct Ctx {
   int32_t* n_new = nullptr;     // entries after the children pass
   int64_t slot_cap = 0;         // capacity of the cold (per-slot) arrays
 
+  // embedded boundary (caller-owned device arrays): the view, whether the
+  // box has cut or covered cells, the domain cells next to them (linear,
+  // x fastest) and the per-droplet redirected deposits of a pass
+  bool eb_set = false;
+  bool eb_on = false;
+  pelemp_eb_view eb{};
+  int32_t* eb_cells = nullptr;
+  int32_t n_eb_cells = 0;
+  uint32_t* eb_rkey = nullptr;
+  double* eb_rval = nullptr;
+
   // surrogate form: packed network (SurrWeights), its normalisation, and the
   // droplet-step counters [surrogate, fallback]
   void* surr_w = nullptr;

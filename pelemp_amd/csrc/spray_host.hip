@@ -455,6 +455,48 @@ static int run_children(Ctx& c, const PassParams& p, hipStream_t s) {
   return PELEMP_OK;
 }
 
+// Deposits sent by eb_interp to a neighbouring cell, added by the cells next
+// to the EB after the update pass (before the re-sort rewrites cstart).
+static int run_eb_redirect(Ctx& c, const PassParams& p, hipStream_t s) {
+  EbRedirectArgs r{};
+  r.cells = c.eb_cells;
+  r.ncells_list = c.n_eb_cells;
+  r.cstart = c.cstart;
+  r.rkey = c.eb_rkey;
+  r.rval = c.eb_rval;
+  r.cap = c.cap;
+  const pelemp_fab_view& sv = *p.source;
+  for (int d = 0; d < 3; ++d) {
+    r.ncell[d] = c.ncell[d];
+    r.ntile[d] = c.ntile[d];
+    r.dom_lo[d] = c.geom.domain_lo[d];
+    r.src_lo[d] = sv.lo[d];
+    r.src_n[d] = sv.hi[d] - sv.lo[d] + 1;
+  }
+  r.src = sv.data;
+  r.src_cstride = static_cast<long long>(r.src_n[0]) * r.src_n[1] * r.src_n[2];
+  const int nsrc = 5 + c.nf;
+  for (int q = 0; q < nsrc; ++q) {
+    r.comp[q] = (q < 3) ? c.comps.momSrcIndx + q
+                        : (q == 3) ? c.comps.rhoSrcIndx
+                                   : (q < 4 + c.nf) ? c.comps.specSrcIndx + (q - 4) : c.comps.engSrcIndx;
+    r.on[q] = (q < 3) ? (c.sd.mom_trans != 0) : (q < 4 + c.nf) ? (c.sd.mass_trans != 0) : 1;
+    r.scale[q] = c.src_scale[q];
+  }
+  r.pelec = c.pelec ? 1 : 0;
+  r.is_ghost = p.is_ghost;
+  r.err = c.err;
+  const dim3 grid(static_cast<unsigned>((c.n_eb_cells + 127) / 128)), block(128);
+  switch (c.nf) {
+    case 1: hipLaunchKernelGGL(eb_redirect_kernel<1>, grid, block, 0, s, r); break;
+    case 2: hipLaunchKernelGGL(eb_redirect_kernel<2>, grid, block, 0, s, r); break;
+    case 6: hipLaunchKernelGGL(eb_redirect_kernel<6>, grid, block, 0, s, r); break;
+    default: set_last_error("unsupported fuel count"); return PELEMP_E_BAD_CONFIG;
+  }
+  PM_HIP(hipGetLastError());
+  return PELEMP_OK;
+}
+
 // Re-sort after an update pass (incremental form, see sort_kernels.h).
 // n_src: device count of entries in the new buffer when it differs from the
 // previous sort's count (secondary droplets appended).
@@ -488,12 +530,29 @@ static int alloc(Ctx& c, void** p, size_t bytes) {
   return PELEMP_OK;
 }
 
+// Buffers of the branch kernel: film thickness per cell, secondary-droplet
+// records and their scans, the new-droplet counters.
+static int alloc_branch_buffers(Ctx& c) {
+  int st = PELEMP_OK;
+  st |= alloc(c, reinterpret_cast<void**>(&c.wf), 8 * static_cast<size_t>(c.ncells));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_type), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_data), 8 * static_cast<size_t>(kSBFields + c.nf) * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_count), 4 * static_cast<size_t>(c.cap));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_offset), 8 * static_cast<size_t>(c.cap));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_block), 8 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_parent), 4 * static_cast<size_t>(c.cap));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_pblock), 4 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sb_ids), 8 * 4);
+  st |= alloc(c, reinterpret_cast<void**>(&c.n_new), 64);
+  return st;
+}
+
 static void free_ctx(Ctx& c) {
   void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.mbuf[0], c.mbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt,
                   c.arrival, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
                   c.count_log, c.surr_w, c.surr_cnt, c.wf, c.sb_type, c.sb_data, c.sb_count, c.sb_offset,
-                  c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new};
+                  c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new, c.eb_cells, c.eb_rkey, c.eb_rval};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -1094,18 +1153,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.scalars), 64);
   st |= alloc(c, reinterpret_cast<void**>(&c.dscratch), 64);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_id), 8 * c.slot_cap);
-  if (c.branches) {
-    st |= alloc(c, reinterpret_cast<void**>(&c.wf), 8 * static_cast<size_t>(c.ncells));
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_type), 4 * c.cap);
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_data), 8 * static_cast<size_t>(kSBFields + c.nf) * c.cap);
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_count), 4 * static_cast<size_t>(c.cap));
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_offset), 8 * static_cast<size_t>(c.cap));
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_block), 8 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_parent), 4 * static_cast<size_t>(c.cap));
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_pblock), 4 * (static_cast<size_t>(c.cap) / kScanBlock + 2));
-    st |= alloc(c, reinterpret_cast<void**>(&c.sb_ids), 8 * 4);
-    st |= alloc(c, reinterpret_cast<void**>(&c.n_new), 64);
-  }
+  if (c.branches) st |= alloc_branch_buffers(c);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_n0), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm1), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm2), 8 * c.cap);
@@ -1830,6 +1878,100 @@ int pelemp_spray_restart(pelemp_spray_ctx* h, const char* dir) {
   return write_counters(c, std::max(cur_id, next_id), next_slot);
 }
 
+int pelemp_spray_set_eb(pelemp_spray_ctx* h, const pelemp_eb_view* eb) {
+  if (!h) return PELEMP_E_ARG;
+  Ctx& c = h->c;
+  if (!eb) {
+    c.eb_set = false;
+    c.eb_on = false;
+    c.n_eb_cells = 0;
+    return PELEMP_OK;
+  }
+  if (c.form != PELEMP_FORM_FP64) {
+    set_last_error("embedded boundaries need the FP64 form");
+    return PELEMP_E_BAD_CONFIG;
+  }
+  if (!eb->flags || !eb->vfrac || !eb->ccent || !eb->bcent || !eb->bnorm) return PELEMP_E_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (eb->lo[d] > c.geom.domain_lo[d] - 1 || eb->hi[d] < c.geom.domain_hi[d] + 1) {
+      set_last_error("the EB box must cover the domain grown by one cell");
+      return PELEMP_E_ARG;
+    }
+  if (!c.branches) {
+    // the EB runs in the branch kernel: switch an empty context to its
+    // layout (breakup state in the hot fields, wider slot arrays)
+    int64_t n = 0;
+    int st = pelemp_spray_num_particles(h, &n);
+    if (st) return st;
+    if (n > 0 || c.nslots > 0) {
+      set_last_error("attach the EB before handing over droplets");
+      return PELEMP_E_BAD_CONFIG;
+    }
+    (void)hipFree(c.fbuf[0]);
+    (void)hipFree(c.fbuf[1]);
+    (void)hipFree(c.cold_id);
+    c.fbuf[0] = c.fbuf[1] = nullptr;
+    c.cold_id = nullptr;
+    c.branches = true;
+    c.hot_brk = true;
+    c.nhot = 13 + c.nf;
+    c.slot_cap = 4 * c.cap;
+    const size_t fb = c.wsize * c.nhot * static_cast<size_t>(c.cap);
+    st = alloc(c, &c.fbuf[0], fb);
+    st |= alloc(c, &c.fbuf[1], fb);
+    st |= alloc(c, reinterpret_cast<void**>(&c.cold_id), 8 * c.slot_cap);
+    st |= alloc_branch_buffers(c);
+    if (st) return PELEMP_E_NOMEM;
+    const int64_t ids[4] = {c.next_id, 0, 0, 0};
+    PM_HIP(hipMemcpy(c.sb_ids, ids, sizeof(ids), hipMemcpyHostToDevice));
+  }
+  // cut or covered cells, and the domain cells next to them
+  const int nb[3] = {eb->hi[0] - eb->lo[0] + 1, eb->hi[1] - eb->lo[1] + 1, eb->hi[2] - eb->lo[2] + 1};
+  const size_t ncb = static_cast<size_t>(nb[0]) * nb[1] * nb[2];
+  std::vector<uint32_t> hf(ncb);
+  PM_HIP(hipMemcpy(hf.data(), eb->flags, 4 * ncb, hipMemcpyDeviceToHost));
+  auto irregular = [&](int i, int j, int k) {  // absolute indices
+    if (i < eb->lo[0] || i > eb->hi[0] || j < eb->lo[1] || j > eb->hi[1] || k < eb->lo[2] || k > eb->hi[2])
+      return false;
+    return (hf[(static_cast<size_t>(k - eb->lo[2]) * nb[1] + (j - eb->lo[1])) * nb[0] + (i - eb->lo[0])] & 3u) != 0u;
+  };
+  bool any = false;
+  for (size_t i = 0; i < ncb && !any; ++i) any = (hf[i] & 3u) != 0u;
+  std::vector<int32_t> cells;
+  if (any) {
+    // deposit targets: cells of the domain grown by one next to the EB
+    const int* dl = c.geom.domain_lo;
+    for (int k = -1; k <= c.ncell[2]; ++k)
+      for (int j = -1; j <= c.ncell[1]; ++j)
+        for (int i = -1; i <= c.ncell[0]; ++i) {
+          bool near = false;
+          for (int dz = -1; dz <= 1 && !near; ++dz)
+            for (int dy = -1; dy <= 1 && !near; ++dy)
+              for (int dx = -1; dx <= 1 && !near; ++dx)
+                near = irregular(dl[0] + i + dx, dl[1] + j + dy, dl[2] + k + dz);
+          if (near) cells.push_back(static_cast<int32_t>(eb_cell_code(i, j, k, c.ncell)));
+        }
+  }
+  if (c.eb_cells) {
+    (void)hipFree(c.eb_cells);
+    c.eb_cells = nullptr;
+  }
+  if (!cells.empty()) {
+    if (alloc(c, reinterpret_cast<void**>(&c.eb_cells), 4 * cells.size())) return PELEMP_E_NOMEM;
+    PM_HIP(hipMemcpy(c.eb_cells, cells.data(), 4 * cells.size(), hipMemcpyHostToDevice));
+  }
+  if (any && !c.eb_rkey) {
+    int st = alloc(c, reinterpret_cast<void**>(&c.eb_rkey), 4 * static_cast<size_t>(c.cap));
+    st |= alloc(c, reinterpret_cast<void**>(&c.eb_rval), 8 * static_cast<size_t>(5 + c.nf) * c.cap);
+    if (st) return PELEMP_E_NOMEM;
+  }
+  c.eb = *eb;
+  c.eb_set = true;
+  c.eb_on = any;
+  c.n_eb_cells = static_cast<int32_t>(cells.size());
+  return PELEMP_OK;
+}
+
 int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,
                         double flow_dt, double time, int32_t state_ghosts, int32_t source_ghosts, int32_t isVirt,
                         int32_t isGhost, int32_t do_move, double spray_cfl_lev, void* stream) {
@@ -1884,7 +2026,7 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
   const bool isActive = !(isVirt || isGhost);
   const bool do_splash = c.sd.do_splash != 0 && isActive && do_move;
   p.branches = c.branches ? 1 : 0;
-  p.do_splash_box = (do_splash && p.at_bounds) ? 1 : 0;
+  p.do_splash_box = (do_splash && (p.at_bounds || c.eb_on)) ? 1 : 0;
   p.make_new_drops = ((c.sd.do_breakup > 0 || p.do_splash_box) && isActive && do_move) ? 1 : 0;
   if (c.branches && num_iter > 1) {
     set_last_error("breakup / splash / films need a single particle sub-cycle (spray_cfl_lev <= 0.5)");
@@ -1911,6 +2053,10 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
                                                   : launch_update_surr(c, p, s);
     if (st != PELEMP_OK) return st;
     ctx_time_end(c, 0, s, ev0);
+    if (c.eb_on && c.n_eb_cells > 0) {
+      const int ste = run_eb_redirect(c, p, s);
+      if (ste != PELEMP_OK) return ste;
+    }
     if (p.make_new_drops) {
       hipEvent_t evc;
       ctx_time_begin(c, 3, s, &evc);

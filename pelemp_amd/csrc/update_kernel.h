@@ -36,6 +36,7 @@
 
 #include "branches.h"
 #include "droplet_physics.h"
+#include "eb.h"
 #include "layout.h"
 #include "surrogate_mlp.h"
 
@@ -113,7 +114,41 @@ struct UpdateArgs {
   int pelec;
   int c_eng;
   double src_scale[5 + PELEMP_MAX_FUEL];  // SI -> CGS factor per source component
+  // embedded boundary (FP64 branch kernel, cut or covered cells in the box)
+  int eb_on;
+  EbDev eb;
+  double min_eb_vfrac;
+  double inv_vol;
+  uint32_t* __restrict__ eb_rkey;  // per sorted position: key of the cell the deposit goes to, or kDeadKey
+  double* __restrict__ eb_rval;    // that deposit [5 + NF][cap]
 };
+
+// One cell of the PeleC state (SprayInterpolation.H:126-146): Y, u, kinetic
+// and internal energy formed in CGS, then rho, u, e to SI and T from EY2T
+// with the state's T as the guess.  S points at the cell's first component.
+template <class M>
+__device__ __attribute__((always_inline)) inline void pelec_state_cell(const double* S, long long cs, int c_rho,
+                                                                      int c_spec, int c_vel, int c_eng, int c_T,
+                                                                      double& rho, double* Y, double* vel,
+                                                                      double& T) {
+  constexpr int NS = M::NS;
+  const double cur_rho = S[c_rho * cs];
+  const double inv_rho = 1. / cur_rho;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) Y[n] = S[(c_spec + n) * cs] * inv_rho;
+  double ke = 0.;
+  double vc[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    vc[q] = S[(c_vel + q) * cs] * inv_rho;
+    ke += vc[q] * vc[q] / 2.;
+  }
+  const double intEng = S[c_eng * cs] * inv_rho - ke;
+  rho = cur_rho * 1.e3;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) vel[q] = vc[q] * 1.e-2;
+  T = GasProps<M>::EY2T(intEng * 1.e-4, Y, S[c_T * cs]);
+}
 
 // fields of a secondary-droplet record (SBData.H:7-19): norm(3) vel(3)
 // loc(3) T0 ref_dia num_dens phi1 phi2 phi3, then Y0[NF]
@@ -176,6 +211,8 @@ struct DropLocal {
   GasAt<R, M::NS> g;
   R cBoilT[NF];
   int bflags[3];
+  int ijkc[3];  // branch kernel: the gather's cell (unclamped, or the EB's closest cell)
+  bool fe;      // branch kernel: EB weights were used
 };
 
 // Load droplet s of the tile and gather its gas state.
@@ -285,7 +322,7 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
       ssv[1][q] = sx_hi[q];
     }
   }
-  // InterpolateGasPhase (SprayInterpolation.H:99-150), LMeX flavour
+  // InterpolateGasPhase (SprayInterpolation.H:99-150)
   GasAt<R, NS>& g = L.g;
   g.rho = R(0);
   g.T = R(0);
@@ -293,52 +330,91 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
   for (int q = 0; q < 3; ++q) g.vel[q] = R(0);
 #pragma unroll
   for (int n = 0; n < NS; ++n) g.Y[n] = R(0);
+  // add cell (rx, ry, rz) (domain-relative) with weight cw
+  auto add_cell = [&](int rx, int ry, int rz, R cw) {
+    int hx = rx - otx + 1, hy = ry - oty + 1, hz = rz - otz + 1;
+    const bool inr = hx >= 0 && hx < kHaloW && hy >= 0 && hy < kHaloW && hz >= 0 && hz < kHaloW;
+    if constexpr (AT_BOUNDS) {
+      // a droplet more than half a cell behind a reflective wall (kept in the
+      // boundary cell) reaches past the halo: read the state's ghost cells
+      // directly, with the staging's arithmetic
+      if (!inr) {
+        const int lx = a.dom_lo[0] + rx - a.st.lo[0];
+        const int ly = a.dom_lo[1] + ry - a.st.lo[1];
+        const int lz = a.dom_lo[2] + rz - a.st.lo[2];
+        if (!(lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2])) {
+          set_err(a.err, PELEMP_E_STATE_BOX);
+          return;
+        }
+        const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
+        const double* S = a.st.data;
+        if constexpr (sizeof(R) == 8) {
+          if (a.pelec) {
+            double rho, Y[NS], vel[3], T;
+            pelec_state_cell<M>(S + c, a.st.cstride, a.c_rho, a.c_spec, a.c_vel, a.c_eng, a.c_T, rho, Y, vel, T);
+            g.rho += cw * rho;
 #pragma unroll
-  for (int kk = -1; kk < 1; ++kk) {
+            for (int n = 0; n < NS; ++n) g.Y[n] += cw * Y[n];
 #pragma unroll
-    for (int jj = -1; jj < 1; ++jj) {
-#pragma unroll
-      for (int ii = -1; ii < 1; ++ii) {
-        const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
-        if (cw > R(0.)) {
-          int hx = ijk[0] + ii - otx + 1, hy = ijk[1] + jj - oty + 1, hz = ijk[2] + kk - otz + 1;
-          const bool inr = hx >= 0 && hx < kHaloW && hy >= 0 && hy < kHaloW && hz >= 0 && hz < kHaloW;
-          if constexpr (AT_BOUNDS) {
-            // a droplet more than half a cell behind a reflective wall (kept
-            // in the boundary cell) reaches past the halo: read the state's
-            // ghost cells directly, with the staging's arithmetic
-            if (!inr) {
-              const int lx = a.dom_lo[0] + ijk[0] + ii - a.st.lo[0];
-              const int ly = a.dom_lo[1] + ijk[1] + jj - a.st.lo[1];
-              const int lz = a.dom_lo[2] + ijk[2] + kk - a.st.lo[2];
-              if (!(lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2])) {
-                set_err(a.err, PELEMP_E_STATE_BOX);
-                continue;
-              }
-              const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
-              const double* S = a.st.data;
-              const double rho = S[a.c_rho * a.st.cstride + c];
-              const double inv_rho = 1. / rho;
-              g.rho += cw * static_cast<R>(rho);
-#pragma unroll
-              for (int n = 0; n < NS; ++n) g.Y[n] += cw * static_cast<R>(S[(a.c_spec + n) * a.st.cstride + c] * inv_rho);
-#pragma unroll
-              for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
-              g.T += cw * static_cast<R>(S[a.c_T * a.st.cstride + c]);
-              continue;
-            }
+            for (int q = 0; q < 3; ++q) g.vel[q] += cw * vel[q];
+            g.T += cw * T;
+            return;
           }
-          hx = inr ? hx : 0;
-          hy = inr ? hy : 0;
-          hz = inr ? hz : 0;
-          const int h = (hz * kHaloW + hy) * kHaloW + hx;
-          if (!inr || !gvalid[h]) set_err(a.err, PELEMP_E_STATE_BOX);
-          g.rho += cw * gas[h];
+        }
+        const double rho = S[a.c_rho * a.st.cstride + c];
+        const double inv_rho = 1. / rho;
+        g.rho += cw * static_cast<R>(rho);
 #pragma unroll
-          for (int n = 0; n < NS; ++n) g.Y[n] += cw * gas[(1 + n) * kHaloCells + h];
+        for (int n = 0; n < NS; ++n) g.Y[n] += cw * static_cast<R>(S[(a.c_spec + n) * a.st.cstride + c] * inv_rho);
 #pragma unroll
-          for (int q = 0; q < 3; ++q) g.vel[q] += cw * gas[(1 + NS + q) * kHaloCells + h];
-          g.T += cw * gas[(4 + NS) * kHaloCells + h];
+        for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
+        g.T += cw * static_cast<R>(S[a.c_T * a.st.cstride + c]);
+        return;
+      }
+    }
+    hx = inr ? hx : 0;
+    hy = inr ? hy : 0;
+    hz = inr ? hz : 0;
+    const int h = (hz * kHaloW + hy) * kHaloW + hx;
+    if (!inr || !gvalid[h]) set_err(a.err, PELEMP_E_STATE_BOX);
+    g.rho += cw * gas[h];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) g.Y[n] += cw * gas[(1 + n) * kHaloCells + h];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g.vel[q] += cw * gas[(1 + NS + q) * kHaloCells + h];
+    g.T += cw * gas[(4 + NS) * kHaloCells + h];
+  };
+  bool eb_weights = false;
+  if constexpr (BR) {
+    // the reference's ijkc: the unclamped containing cell
+    L.fe = false;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) L.ijkc[q] = static_cast<int>(floor((static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q]));
+    if (a.eb_on) {
+      // eb_interp (SprayInterpolation.H:395-632) when the stencil has cut or
+      // covered cells; it may move the droplet and its deposit cell
+      EbStencil es;
+      const int r = eb_interp(a.eb, L.pos, L.ijkc, ijk, a.plo, a.dx, a.dxi, a.min_eb_vfrac, es);
+      if (r < 0) {
+        set_err(a.err, PELEMP_E_EB_STENCIL);
+      } else if (r > 0) {
+        L.fe = true;
+        eb_weights = true;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (es.w[q] > 0.) add_cell(es.idx[q][0], es.idx[q][1], es.idx[q][2], static_cast<R>(es.w[q]));
+      }
+    }
+  }
+  if (!eb_weights) {
+#pragma unroll
+    for (int kk = -1; kk < 1; ++kk) {
+#pragma unroll
+      for (int jj = -1; jj < 1; ++jj) {
+#pragma unroll
+        for (int ii = -1; ii < 1; ++ii) {
+          const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
+          if (cw > R(0.)) add_cell(ijk[0] + ii, ijk[1] + jj, ijk[2] + kk, cw);
         }
       }
     }
@@ -501,7 +577,8 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
 template <class M, int NF, bool AT_BOUNDS>
 __device__ __attribute__((always_inline)) inline void drop_update_br(const UpdateArgs<double, NF>& a, int tile,
                                                                      int t, int32_t s, DropLocal<M, NF, double>& L,
-                                                                     double* contrib, unsigned char* stayed) {
+                                                                     double* contrib, unsigned char* stayed,
+                                                                     unsigned char* redir) {
   constexpr int NSRC = 5 + NF;
   using F = Fields<NF>;
   const long long cap = a.cap;
@@ -554,36 +631,62 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
       khrt_update<M::NS, NF>(Reyn_d, L.cBoilT, L.g, a.fc, a.lx, a.kh, a.make_new_drops != 0, b, rec);
     }
   }
-  // contributions cur_coef * source (SprayParticles.cpp:419-441)
+  // contributions cur_coef * source (SprayParticles.cpp:413-441); with an EB
+  // 1/vfrac in a cut deposit cell, and eb_interp's closest cell may be a
+  // neighbour: that deposit is kept aside for the redirect pass
+  double coef = a.cur_coef;
+  bool to_other = false;
+  int dep[3] = {L.cx, L.cy, L.cz};
+  if (a.eb_on) {
+    // with EB weights, eb_interp's cell: the droplet's own, or the closest
+    // acceptable cell of the stencil (a neighbour, possibly a ghost cell)
+    if (L.fe) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) dep[q] = L.ijkc[q];
+    }
+    if (a.eb.single(dep[0], dep[1], dep[2])) {
+      double cvol = a.inv_vol;
+      cvol *= 1. / (a.eb.vf(dep[0], dep[1], dep[2]));
+      coef = -cvol * a.sub_dt / a.flow_dt;
+    }
+    to_other = dep[0] != L.cx || dep[1] != L.cy || dep[2] != L.cz;
+  }
   double* cb = &contrib[t * NSRC];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) cb[q] = a.cur_coef * sr.mom[q];
-  cb[3] = a.cur_coef * sr.mass;
+  for (int q = 0; q < 3; ++q) cb[q] = coef * sr.mom[q];
+  cb[3] = coef * sr.mass;
 #pragma unroll
-  for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = a.cur_coef * sr.Ydot[spf];
-  cb[4 + NF] = a.cur_coef * sr.eng;
+  for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = coef * sr.Ydot[spf];
+  cb[4 + NF] = coef * sr.eng;
+  redir[t] = to_other ? 1 : 0;
+  if (a.eb_on) {
+    a.eb_rkey[s] = to_other ? eb_cell_code(dep[0], dep[1], dep[2], a.ncell) : kDeadKey;
+    if (to_other) {
+#pragma unroll
+      for (int q = 0; q < NSRC; ++q) a.eb_rval[q * cap + s] = cb[q];
+    }
+  }
 
   // drift, bounds and walls (SprayParticles.cpp:443-479), periodic wrap
   if (a.do_move && !a.fc.fixed_parts && b.alive && !is_film) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) b.pos[q] += a.sub_dt * b.vel[q];
-    if constexpr (AT_BOUNDS) {
+    if (AT_BOUNDS || L.fe) {
       if (dev_check_bounds(b.pos, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, L.bflags)) {
         b.alive = false;
       } else {
-        // film thickness of the cell the droplet started in; a droplet that
-        // started behind a wall sees none (the reference reads its ghost
-        // cell, which holds no film)
+        // film thickness of the gather's cell; a droplet that started behind
+        // a wall sees none (the reference reads its ghost cell, which holds
+        // no film)
         bool inside = true;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const int ic = static_cast<int>(floor((L.pos[q] - a.plo[q]) * a.dxi[q]));
-          inside = inside && ic >= 0 && ic < a.ncell[q];
-        }
-        const double film_h = (a.do_splash_box && inside)
-                                  ? a.wf[(static_cast<long long>(L.cz) * a.ncell[1] + L.cy) * a.ncell[0] + L.cx]
-                                  : 0.;
-        impose_wall<NF>(a.do_splash_box != 0, b, a.fc, a.lx, a.plo, a.dx, L.bflags, L.cBoilT, L.g.p, rec, film_h);
+        for (int q = 0; q < 3; ++q) inside = inside && L.ijkc[q] >= 0 && L.ijkc[q] < a.ncell[q];
+        const double film_h =
+            (a.do_splash_box && inside)
+                ? a.wf[(static_cast<long long>(L.ijkc[2]) * a.ncell[1] + L.ijkc[1]) * a.ncell[0] + L.ijkc[0]]
+                : 0.;
+        impose_wall<NF>(a.do_splash_box != 0, b, a.fc, a.lx, a.plo, a.dx, L.bflags, L.cBoilT, L.g.p, rec, film_h,
+                        a.eb_on ? &a.eb : nullptr, L.ijkc);
       }
     }
   }
@@ -711,7 +814,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
                                                                    unsigned char* gvalid, int32_t* cs,
                                                                    double* contrib, unsigned char* stayed,
                                                                    const SurrWeights* sw, unsigned& n_surr,
-                                                                   unsigned& n_fb) {
+                                                                   unsigned& n_fb, unsigned char* redir = nullptr) {
   constexpr int NS = M::NS;
   constexpr int NSRC = 5 + NF;   // mom[3], mass, Ydot[NF], eng
   constexpr int NITEM = (kTileCells * NSRC + kUpdThreads - 1) / kUpdThreads;
@@ -742,31 +845,16 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     gvalid[h] = ok ? 1 : 0;
     if constexpr (sizeof(R) == 8) {
       if (ok && a.pelec) {
-        // PeleC flavour (SprayInterpolation.H:126-146): Y, u, kinetic and
-        // internal energy formed in CGS, then rho, u, e to SI and T from EY2T
-        // with the state's T as the guess
+        // PeleC flavour: the cell in SI, T from EY2T
         const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
-        const double* S = a.st.data + c;
-        const long long cs_ = a.st.cstride;
-        const double cur_rho = S[a.c_rho * cs_];
-        const double inv_rho = 1. / cur_rho;
-        double Y[NS];
-#pragma unroll
-        for (int n = 0; n < NS; ++n) Y[n] = S[(a.c_spec + n) * cs_] * inv_rho;
-        double ke = 0.;
-        double vc[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          vc[q] = S[(a.c_vel + q) * cs_] * inv_rho;
-          ke += vc[q] * vc[q] / 2.;
-        }
-        const double intEng = S[a.c_eng * cs_] * inv_rho - ke;
-        gas[0 * kHaloCells + h] = cur_rho * 1.e3;
+        double rho, Y[NS], vel[3], T;
+        pelec_state_cell<M>(a.st.data + c, a.st.cstride, a.c_rho, a.c_spec, a.c_vel, a.c_eng, a.c_T, rho, Y, vel, T);
+        gas[0 * kHaloCells + h] = rho;
 #pragma unroll
         for (int n = 0; n < NS; ++n) gas[(1 + n) * kHaloCells + h] = Y[n];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) gas[(1 + NS + q) * kHaloCells + h] = vc[q] * 1.e-2;
-        gas[(4 + NS) * kHaloCells + h] = GasProps<M>::EY2T(intEng * 1.e-4, Y, S[a.c_T * cs_]);
+        for (int q = 0; q < 3; ++q) gas[(1 + NS + q) * kHaloCells + h] = vel[q];
+        gas[(4 + NS) * kHaloCells + h] = T;
         continue;
       }
     }
@@ -798,7 +886,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       if (s < d1) {
         DropLocal<M, NF, R> L;
         drop_gather<M, NF, R, true, AT_BOUNDS, true>(a, gas, gvalid, cs, otx, oty, otz, s, L);
-        drop_update_br<M, NF, AT_BOUNDS>(a, tile, t, s, L, contrib, stayed);
+        drop_update_br<M, NF, AT_BOUNDS>(a, tile, t, s, L, contrib, stayed, redir);
       }
     } else if constexpr (!SURR) {
       if (s < d1) {
@@ -837,7 +925,13 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
         const int32_t lo = cs[c] > base ? cs[c] : base;
         const int32_t hi = cs[c + 1] < chunk_end ? cs[c + 1] : chunk_end;
         double v = acc[r];
-        for (int32_t i = lo; i < hi; ++i) v = v + contrib[(i - base) * NSRC + q];
+        if constexpr (BR) {
+          // deposits sent to another cell by eb_interp are left out here
+          for (int32_t i = lo; i < hi; ++i)
+            if (!redir[i - base]) v = v + contrib[(i - base) * NSRC + q];
+        } else {
+          for (int32_t i = lo; i < hi; ++i) v = v + contrib[(i - base) * NSRC + q];
+        }
         acc[r] = v;
       }
     }
@@ -884,10 +978,11 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ double contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
+  __shared__ unsigned char redir[BR ? kUpdThreads : 1];
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
   process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib,
-                                                      stayed, nullptr, n_surr, n_fb);
+                                                      stayed, nullptr, n_surr, n_fb, redir);
 }
 
 // Surrogate form: gridDim.x workgroups (a multiple of 8) loop over the

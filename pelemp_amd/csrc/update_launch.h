@@ -111,6 +111,23 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
       a.make_new_drops = p.make_new_drops;
       a.do_splash_box = p.do_splash_box;
       a.max_ppp = c.params.max_parcel_size;
+      a.inv_vol = 1. / (c.geom.dx[0] * c.geom.dx[1] * c.geom.dx[2]);
+      a.eb_on = c.eb_on ? 1 : 0;
+      if (c.eb_on) {
+        a.eb.flags = c.eb.flags;
+        a.eb.vfrac = c.eb.vfrac;
+        a.eb.ccent = c.eb.ccent;
+        a.eb.bcent = c.eb.bcent;
+        a.eb.bnorm = c.eb.bnorm;
+        for (int d = 0; d < 3; ++d) {
+          a.eb.lo[d] = c.eb.lo[d] - c.geom.domain_lo[d];
+          a.eb.n[d] = c.eb.hi[d] - c.eb.lo[d] + 1;
+        }
+        a.eb.cstride = static_cast<long long>(a.eb.n[0]) * a.eb.n[1] * a.eb.n[2];
+        a.min_eb_vfrac = c.sd.min_eb_vfrac;
+        a.eb_rkey = c.eb_rkey;
+        a.eb_rval = c.eb_rval;
+      }
       if (p.at_bounds) {
         hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true>), grid, block, 0, s, a);
       } else {

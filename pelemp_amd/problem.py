@@ -340,3 +340,64 @@ def to_pelec(prob: SprayProblem, T_guess_factor: float = 1.0) -> SprayProblem:
     p.name = prob.name + "_pelec"
     p.meta = dict(prob.meta, flavour="pelec")
     return p
+
+
+def eb_plane(prob: SprayProblem, point, normal, nsub: int = 6) -> dict:
+    """A planar embedded boundary over the state box of `prob`, in the layout
+    of AMReX's EB data (EBCellFlag words, volume fraction, cell centroid,
+    boundary centroid and boundary normal, each relative to the cell centre
+    in cell units).  The solid is the side {x : (x - point) . normal > 0};
+    `normal` (made unit) is the boundary normal, pointing out of the fluid.
+    Volume fractions and centroids come from nsub^3 sample points per cell;
+    the boundary centroid is the point of the plane nearest the cell centre;
+    a cell is connected to each neighbour that is not covered."""
+    g = prob.geom
+    lo, hi = np.array(prob.state_lo), np.array(prob.state_hi)
+    n = hi - lo + 1
+    dx = np.array(g["dx"])
+    plo = np.array(g["plo"])
+    dl = np.array(g["domain_lo"])
+    nrm = np.asarray(normal, dtype=np.float64)
+    nrm = nrm / np.linalg.norm(nrm)
+    x0 = np.asarray(point, dtype=np.float64)
+    # cell centres (physical), arrays indexed [k, j, i]
+    ii = [np.arange(n[d]) + lo[d] for d in range(3)]
+    cz, cy, cx = np.meshgrid(ii[2], ii[1], ii[0], indexing="ij")
+    cent = np.stack([(cx - dl[0] + 0.5) * dx[0] + plo[0], (cy - dl[1] + 0.5) * dx[1] + plo[1],
+                     (cz - dl[2] + 0.5) * dx[2] + plo[2]])
+    sub = (np.arange(nsub) + 0.5) / nsub - 0.5
+    vol = np.zeros(n[::-1])
+    csum = np.zeros((3,) + tuple(n[::-1]))
+    for a in sub:
+        for b in sub:
+            for c in sub:
+                off = np.array([a, b, c])
+                pt = cent + (off * dx)[:, None, None, None]
+                fluid = np.tensordot(nrm, pt - x0[:, None, None, None], axes=1) <= 0.0
+                vol += fluid
+                csum += fluid[None] * off[:, None, None, None]
+    vfrac = vol / nsub ** 3
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ccent = np.where(vol[None] > 0, csum / np.maximum(vol, 1)[None], 0.0)
+    covered = vol == 0
+    cut = (vol > 0) & (vol < nsub ** 3)
+    # boundary centroid: nearest point of the plane, in cell units
+    dist = np.tensordot(nrm, cent - x0[:, None, None, None], axes=1)
+    bc = (-dist[None] * nrm[:, None, None, None]) / dx[:, None, None, None]
+    bcent = np.where(cut[None], bc, 0.0)
+    bnorm = np.where(cut[None], np.broadcast_to(nrm[:, None, None, None], bc.shape), 0.0)
+    ccent = np.where(cut[None], ccent, 0.0)
+    vfrac = np.where(cut, vfrac, np.where(covered, 0.0, 1.0))
+    flags = np.zeros(n[::-1], dtype=np.uint32)
+    flags[cut] = 1
+    flags[covered] = 3
+    # connectivity to the 26 neighbours and self for cells that are not covered
+    padc = np.pad(covered, 1, mode="edge")
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                nb = padc[1 + dk:1 + dk + n[2], 1 + dj:1 + dj + n[1], 1 + di:1 + di + n[0]]
+                bit = np.uint32(1) << np.uint32(5 + (di + 1) + 3 * (dj + 1) + 9 * (dk + 1))
+                flags |= np.where(~covered & ~nb, bit, np.uint32(0)).astype(np.uint32)
+    return dict(flags=flags, vfrac=vfrac, ccent=ccent, bcent=bcent, bnorm=bnorm, lo=tuple(int(v) for v in lo),
+                hi=tuple(int(v) for v in hi), point=x0, normal=nrm)

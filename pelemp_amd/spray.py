@@ -355,6 +355,32 @@ class SprayParticleContainer:
         if sync:
             C.device_synchronize()
 
+    # -- embedded boundary -----------------------------------------------------
+    def set_eb(self, eb: Optional[dict]) -> None:
+        """Attach the level's EB (dict of numpy arrays flags (uint32), vfrac,
+        ccent[3], bcent[3], bnorm[3] over the box lo..hi, component-major), the
+        data AMReX's EBFArrayBoxFactory holds; None detaches it.  The arrays
+        are copied to the device and kept with the container."""
+        if eb is None:
+            C.check(self._lib.pelemp_spray_set_eb(self._h, None), "pelemp_spray_set_eb")
+            self._eb_bufs = None
+            return
+        bufs = {}
+        for k, dt in (("flags", np.uint32), ("vfrac", np.float64), ("ccent", np.float64), ("bcent", np.float64),
+                      ("bnorm", np.float64)):
+            a = np.ascontiguousarray(eb[k], dtype=dt)
+            b = C.DeviceBuffer(a.nbytes)
+            b.upload(a)
+            bufs[k] = b
+        v = C.EBViewC()
+        v.flags, v.vfrac, v.ccent = bufs["flags"].ptr, bufs["vfrac"].ptr, bufs["ccent"].ptr
+        v.bcent, v.bnorm = bufs["bcent"].ptr, bufs["bnorm"].ptr
+        for d in range(3):
+            v.lo[d] = int(eb["lo"][d])
+            v.hi[d] = int(eb["hi"][d])
+        C.check(self._lib.pelemp_spray_set_eb(self._h, ctypes.byref(v)), "pelemp_spray_set_eb")
+        self._eb_bufs = bufs
+
     # -- injection and initialisation ----------------------------------------
     def add_particles(self, pos: np.ndarray, rdata: np.ndarray, ids: np.ndarray) -> None:
         """Append droplets (the tiles' host-particle copy of SprayInjection.H:
