@@ -216,7 +216,7 @@ struct DropLocal {
 };
 
 // Load droplet s of the tile and gather its gas state.
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false>
 __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateArgs<R, NF>& a, const R* gas,
                                                                   const unsigned char* gvalid, const int32_t* cs,
                                                                   int otx, int oty, int otz, int32_t s,
@@ -390,7 +390,7 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
     L.fe = false;
 #pragma unroll
     for (int q = 0; q < 3; ++q) L.ijkc[q] = static_cast<int>(floor((static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q]));
-    if (a.eb_on) {
+    if constexpr (EB) {
       // eb_interp (SprayInterpolation.H:395-632) when the stencil has cut or
       // covered cells; it may move the droplet and its deposit cell
       EbStencil es;
@@ -574,7 +574,7 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
 // FP64 form with the branches (SprayParticles.cpp:354-479): film source or
 // droplet source, TAB update and split, deposition, drift, wall reflection /
 // splash, store of the breakup state and of the secondary-droplet record.
-template <class M, int NF, bool AT_BOUNDS>
+template <class M, int NF, bool AT_BOUNDS, bool EB = false>
 __device__ __attribute__((always_inline)) inline void drop_update_br(const UpdateArgs<double, NF>& a, int tile,
                                                                      int t, int32_t s, DropLocal<M, NF, double>& L,
                                                                      double* contrib, unsigned char* stayed,
@@ -637,7 +637,7 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
   double coef = a.cur_coef;
   bool to_other = false;
   int dep[3] = {L.cx, L.cy, L.cz};
-  if (a.eb_on) {
+  if constexpr (EB) {
     // with EB weights, eb_interp's cell: the droplet's own, or the closest
     // acceptable cell of the stencil (a neighbour, possibly a ghost cell)
     if (L.fe) {
@@ -658,8 +658,8 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = coef * sr.Ydot[spf];
   cb[4 + NF] = coef * sr.eng;
-  redir[t] = to_other ? 1 : 0;
-  if (a.eb_on) {
+  if constexpr (EB) {
+    redir[t] = to_other ? 1 : 0;
     a.eb_rkey[s] = to_other ? eb_cell_code(dep[0], dep[1], dep[2], a.ncell) : kDeadKey;
     if (to_other) {
 #pragma unroll
@@ -671,7 +671,7 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
   if (a.do_move && !a.fc.fixed_parts && b.alive && !is_film) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) b.pos[q] += a.sub_dt * b.vel[q];
-    if (AT_BOUNDS || L.fe) {
+    if (AT_BOUNDS || (EB && L.fe)) {
       if (dev_check_bounds(b.pos, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, L.bflags)) {
         b.alive = false;
       } else {
@@ -686,7 +686,7 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
                 ? a.wf[(static_cast<long long>(L.ijkc[2]) * a.ncell[1] + L.ijkc[1]) * a.ncell[0] + L.ijkc[0]]
                 : 0.;
         impose_wall<NF>(a.do_splash_box != 0, b, a.fc, a.lx, a.plo, a.dx, L.bflags, L.cBoilT, L.g.p, rec, film_h,
-                        a.eb_on ? &a.eb : nullptr, L.ijkc);
+                        EB ? &a.eb : nullptr, L.ijkc);
       }
     }
   }
@@ -809,7 +809,7 @@ __device__ __attribute__((always_inline)) inline bool surr_stage_drop(const Upda
   return use;
 }
 
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false, bool EB = false>
 __device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, R* gas,
                                                                    unsigned char* gvalid, int32_t* cs,
                                                                    double* contrib, unsigned char* stayed,
@@ -885,8 +885,8 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     if constexpr (BR) {
       if (s < d1) {
         DropLocal<M, NF, R> L;
-        drop_gather<M, NF, R, true, AT_BOUNDS, true>(a, gas, gvalid, cs, otx, oty, otz, s, L);
-        drop_update_br<M, NF, AT_BOUNDS>(a, tile, t, s, L, contrib, stayed, redir);
+        drop_gather<M, NF, R, true, AT_BOUNDS, true, EB>(a, gas, gvalid, cs, otx, oty, otz, s, L);
+        drop_update_br<M, NF, AT_BOUNDS, EB>(a, tile, t, s, L, contrib, stayed, redir);
       }
     } else if constexpr (!SURR) {
       if (s < d1) {
@@ -925,7 +925,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
         const int32_t lo = cs[c] > base ? cs[c] : base;
         const int32_t hi = cs[c + 1] < chunk_end ? cs[c + 1] : chunk_end;
         double v = acc[r];
-        if constexpr (BR) {
+        if constexpr (EB) {
           // deposits sent to another cell by eb_interp are left out here
           for (int32_t i = lo; i < hi; ++i)
             if (!redir[i - base]) v = v + contrib[(i - base) * NSRC + q];
@@ -968,7 +968,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
   if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
 }
 
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false>
 __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NG = 5 + M::NS;  // staged gas fields: rho, Y[NS], vel[3], T
@@ -978,10 +978,10 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ double contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
-  __shared__ unsigned char redir[BR ? kUpdThreads : 1];
+  __shared__ unsigned char redir[EB ? kUpdThreads : 1];
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
-  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib,
+  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib,
                                                       stayed, nullptr, n_surr, n_fb, redir);
 }
 

@@ -128,7 +128,13 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
         a.eb_rkey = c.eb_rkey;
         a.eb_rval = c.eb_rval;
       }
-      if (p.at_bounds) {
+      if (c.eb_on) {
+        if (p.at_bounds) {
+          hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true, true>), grid, block, 0, s, a);
+        } else {
+          hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true, true>), grid, block, 0, s, a);
+        }
+      } else if (p.at_bounds) {
         hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true>), grid, block, 0, s, a);
       } else {
         hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true>), grid, block, 0, s, a);
