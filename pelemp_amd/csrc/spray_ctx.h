@@ -36,6 +36,9 @@ struct Ctx;
 int launch_update_fp64(Ctx& c, const PassParams& p, hipStream_t s);
 int launch_update_fp32(Ctx& c, const PassParams& p, hipStream_t s);
 int launch_update_surr(Ctx& c, const PassParams& p, hipStream_t s);
+// PeleC flavour: the state box converted to SI (rho, Y, vel, T) into
+// ctx.pelec_conv, once per update call, before the passes.
+int launch_pelec_convert(Ctx& c, const pelemp_fab_view& st, hipStream_t s);
 
 // Mechanism tables (spray_host.hip).
 int mech_ns(int mech);
@@ -102,6 +105,8 @@ struct Ctx {
   // and geometry at the boundary; SI inside (src_scale: SI -> CGS sources)
   bool pelec = false;
   double src_scale[5 + PELEMP_MAX_FUEL] = {};
+  double* pelec_conv = nullptr;  // the state box in SI (5 + NS fields)
+  size_t pelec_conv_bytes = 0;
   double* wf = nullptr;
   int32_t* sb_type = nullptr;
   double* sb_data = nullptr;

@@ -552,7 +552,8 @@ static void free_ctx(Ctx& c) {
                   c.arrival, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
                   c.count_log, c.surr_w, c.surr_cnt, c.wf, c.sb_type, c.sb_data, c.sb_count, c.sb_offset,
-                  c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new, c.eb_cells, c.eb_rkey, c.eb_rval};
+                  c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new, c.eb_cells, c.eb_rkey, c.eb_rval,
+                  c.pelec_conv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -2031,6 +2032,10 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
   if (c.branches && num_iter > 1) {
     set_last_error("breakup / splash / films need a single particle sub-cycle (spray_cfl_lev <= 0.5)");
     return PELEMP_E_BAD_CONFIG;
+  }
+  if (c.pelec) {
+    const int stc = launch_pelec_convert(c, *state, s);
+    if (stc != PELEMP_OK) return stc;
   }
   for (int it = 0; it < num_iter; ++it) {
     if (c.npass < Ctx::kCountLog) {

@@ -113,6 +113,7 @@ struct UpdateArgs {
   // PeleC flavour (FP64): CGS state rho, rho*u, rho*E, rho*Y, T and CGS sources
   int pelec;
   int c_eng;
+  const double* __restrict__ conv;  // PeleC: the state box in SI (pelec_convert_kernel)
   double src_scale[5 + PELEMP_MAX_FUEL];  // SI -> CGS factor per source component
   // embedded boundary (FP64 branch kernel, cut or covered cells in the box)
   int eb_on;
@@ -122,6 +123,7 @@ struct UpdateArgs {
   uint32_t* __restrict__ eb_rkey;  // per sorted position: key of the cell the deposit goes to, or kDeadKey
   double* __restrict__ eb_rval;    // that deposit [5 + NF][cap]
 };
+
 
 // One cell of the PeleC state (SprayInterpolation.H:126-146): Y, u, kinetic
 // and internal energy formed in CGS, then rho, u, e to SI and T from EY2T
@@ -148,6 +150,27 @@ __device__ __attribute__((always_inline)) inline void pelec_state_cell(const dou
 #pragma unroll
   for (int q = 0; q < 3; ++q) vel[q] = vc[q] * 1.e-2;
   T = GasProps<M>::EY2T(intEng * 1.e-4, Y, S[c_T * cs]);
+}
+
+// The PeleC state of the whole state box in SI, once per update (one thread
+// per cell): conv[(q * ncells) + cell] for q = rho, Y[NS], vel[3], T, so the
+// update kernel stages PeleC cells without the EY2T solve.
+template <class M>
+__global__ void __launch_bounds__(256) pelec_convert_kernel(const double* __restrict__ S, long long ncells,
+                                                            int c_rho, int c_spec, int c_vel, int c_eng, int c_T,
+                                                            double* __restrict__ conv) {
+  constexpr int NS = M::NS;
+  for (long long c = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; c < ncells;
+       c += static_cast<long long>(gridDim.x) * blockDim.x) {
+    double rho, Y[NS], vel[3], T;
+    pelec_state_cell<M>(S + c, ncells, c_rho, c_spec, c_vel, c_eng, c_T, rho, Y, vel, T);
+    conv[c] = rho;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) conv[(1 + n) * ncells + c] = Y[n];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) conv[(1 + NS + q) * ncells + c] = vel[q];
+    conv[(4 + NS) * ncells + c] = T;
+  }
 }
 
 // fields of a secondary-droplet record (SBData.H:7-19): norm(3) vel(3)
@@ -350,14 +373,14 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
         const double* S = a.st.data;
         if constexpr (sizeof(R) == 8) {
           if (a.pelec) {
-            double rho, Y[NS], vel[3], T;
-            pelec_state_cell<M>(S + c, a.st.cstride, a.c_rho, a.c_spec, a.c_vel, a.c_eng, a.c_T, rho, Y, vel, T);
-            g.rho += cw * rho;
+            const double* V = a.conv + c;
+            const long long cs = a.st.cstride;
+            g.rho += cw * V[0];
 #pragma unroll
-            for (int n = 0; n < NS; ++n) g.Y[n] += cw * Y[n];
+            for (int n = 0; n < NS; ++n) g.Y[n] += cw * V[(1 + n) * cs];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) g.vel[q] += cw * vel[q];
-            g.T += cw * T;
+            for (int q = 0; q < 3; ++q) g.vel[q] += cw * V[(1 + NS + q) * cs];
+            g.T += cw * V[(4 + NS) * cs];
             return;
           }
         }
@@ -386,11 +409,12 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
   };
   bool eb_weights = false;
   if constexpr (BR) {
-    // the reference's ijkc: the unclamped containing cell
     L.fe = false;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) L.ijkc[q] = static_cast<int>(floor((static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q]));
     if constexpr (EB) {
+      // the reference's ijkc: the unclamped containing cell
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        L.ijkc[q] = static_cast<int>(floor((static_cast<double>(pos[q]) - a.plo[q]) * a.dxi[q]));
       // eb_interp (SprayInterpolation.H:395-632) when the stencil has cut or
       // covered cells; it may move the droplet and its deposit cell
       EbStencil es;
@@ -675,18 +699,27 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
       if (dev_check_bounds(b.pos, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, L.bflags)) {
         b.alive = false;
       } else {
-        // film thickness of the gather's cell; a droplet that started behind
-        // a wall sees none (the reference reads its ghost cell, which holds
-        // no film)
+        // film thickness of the gather's cell (the EB's closest cell when
+        // eb_interp moved it); a droplet that started behind a wall sees
+        // none (the reference reads its ghost cell, which holds no film)
+        int fc[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          if constexpr (EB) {
+            fc[q] = L.ijkc[q];
+          } else {
+            fc[q] = static_cast<int>(floor((L.pos[q] - a.plo[q]) * a.dxi[q]));
+          }
+        }
         bool inside = true;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) inside = inside && L.ijkc[q] >= 0 && L.ijkc[q] < a.ncell[q];
+        for (int q = 0; q < 3; ++q) inside = inside && fc[q] >= 0 && fc[q] < a.ncell[q];
         const double film_h =
             (a.do_splash_box && inside)
-                ? a.wf[(static_cast<long long>(L.ijkc[2]) * a.ncell[1] + L.ijkc[1]) * a.ncell[0] + L.ijkc[0]]
+                ? a.wf[(static_cast<long long>(fc[2]) * a.ncell[1] + fc[1]) * a.ncell[0] + fc[0]]
                 : 0.;
         impose_wall<NF>(a.do_splash_box != 0, b, a.fc, a.lx, a.plo, a.dx, L.bflags, L.cBoilT, L.g.p, rec, film_h,
-                        EB ? &a.eb : nullptr, L.ijkc);
+                        EB ? &a.eb : nullptr, EB ? L.ijkc : nullptr);
       }
     }
   }
@@ -845,16 +878,12 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     gvalid[h] = ok ? 1 : 0;
     if constexpr (sizeof(R) == 8) {
       if (ok && a.pelec) {
-        // PeleC flavour: the cell in SI, T from EY2T
+        // PeleC flavour: the cell in SI, converted before the update
         const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
-        double rho, Y[NS], vel[3], T;
-        pelec_state_cell<M>(a.st.data + c, a.st.cstride, a.c_rho, a.c_spec, a.c_vel, a.c_eng, a.c_T, rho, Y, vel, T);
-        gas[0 * kHaloCells + h] = rho;
+        const double* V = a.conv + c;
+        const long long cs = a.st.cstride;
 #pragma unroll
-        for (int n = 0; n < NS; ++n) gas[(1 + n) * kHaloCells + h] = Y[n];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) gas[(1 + NS + q) * kHaloCells + h] = vel[q];
-        gas[(4 + NS) * kHaloCells + h] = T;
+        for (int f = 0; f < 5 + NS; ++f) gas[f * kHaloCells + h] = V[f * cs];
         continue;
       }
     }

@@ -62,6 +62,7 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.fc.pelec = c.pelec;
   a.pelec = c.pelec ? 1 : 0;
   a.c_eng = c.comps.engIndx;
+  a.conv = c.pelec_conv;
   for (int q = 0; q < 5 + PELEMP_MAX_FUEL; ++q) a.src_scale[q] = c.src_scale[q];
   a.flow_dt = static_cast<R>(p.flow_dt);
   a.sub_dt = static_cast<R>(p.sub_dt);
@@ -156,6 +157,32 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_last_error(std::string("update launch: ") + hipGetErrorString(e));
+    return PELEMP_E_HIP;
+  }
+  return PELEMP_OK;
+}
+
+template <class M>
+int launch_pelec_convert_t(Ctx& c, const pelemp_fab_view& st, hipStream_t s) {
+  const long long ncells = static_cast<long long>(st.hi[0] - st.lo[0] + 1) * (st.hi[1] - st.lo[1] + 1) *
+                           (st.hi[2] - st.lo[2] + 1);
+  const size_t bytes = 8 * static_cast<size_t>(5 + M::NS) * static_cast<size_t>(ncells);
+  if (bytes > c.pelec_conv_bytes) {
+    if (c.pelec_conv) (void)hipFree(c.pelec_conv);
+    c.pelec_conv = nullptr;
+    c.pelec_conv_bytes = 0;
+    if (hipMalloc(&c.pelec_conv, bytes) != hipSuccess) {
+      set_last_error("hipMalloc failed (PeleC state conversion)");
+      return PELEMP_E_NOMEM;
+    }
+    c.pelec_conv_bytes = bytes;
+  }
+  const unsigned nb = static_cast<unsigned>(std::min<long long>((ncells + 255) / 256, 65535));
+  hipLaunchKernelGGL(pelec_convert_kernel<M>, dim3(nb), dim3(256), 0, s, st.data, ncells, c.comps.rhoIndx,
+                     c.comps.specIndx, c.comps.momIndx, c.comps.engIndx, c.comps.utempIndx, c.pelec_conv);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_last_error(std::string("PeleC state conversion: ") + hipGetErrorString(e));
     return PELEMP_E_HIP;
   }
   return PELEMP_OK;
