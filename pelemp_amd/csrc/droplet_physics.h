@@ -192,6 +192,36 @@ PM_HDI void calc_boil_t(const FuelConst<R, NF>& f, R p, R* cBoilT) {
 template <class R>
 PM_HDI R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
   if (B_M <= C_eps) return R(0);
+  if constexpr (sizeof(R) == 4) {
+    // single precision: (1+B_M)^phi = exp(phi ln(1+B_M)) with the logarithm
+    // taken once, and (1+B_T)^0.7 = exp(0.7 log1p(B_T)) sharing log1p(B_T)
+    const int maxIter = 100;
+    const R NU2 = Nu_0 - R(2);
+    const R lnBM1 = rm::log1p(B_M);
+    R phi = ratio / Nu_0;
+    R B_T_old = rm::exp(phi * lnBM1) - R(1);
+    R logB = rm::log1p(B_T_old);
+    R invFT = B_T_old / (logB * rm::exp(R(0.7) * logB));
+    R Nu_num = R(2) + NU2 * invFT;
+    phi = ratio / Nu_num;
+    R B_T = rm::exp(phi * lnBM1) - R(1);
+    R error = rm::fabs(B_T - B_T_old);
+    int k = 0;
+    while (k < maxIter && error > B_eps) {
+      B_T_old = B_T;
+      logB = rm::log1p(B_T);
+      invFT = B_T / (logB * rm::exp(R(0.7) * logB));
+      Nu_num = R(2) + NU2 * invFT;
+      phi = ratio / Nu_num;
+      B_T = rm::exp(phi * lnBM1) - R(1);
+      error = rm::fabs(B_T - B_T_old);
+      k++;
+    }
+    logB = rm::log1p(B_T);
+    invFT = B_T / (logB * rm::exp(R(0.7) * logB));
+    Nu_num = R(2) + NU2 * invFT;
+    return Nu_num * logB / B_T;
+  }
   const int maxIter = 100;
   const R NU2 = Nu_0 - R(2);
   const R BM1 = R(1) + B_M;
@@ -412,7 +442,12 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
       cp_fuel /= sumYSkin;
       const R Sc_skin = mu_skin / rhoDtotal;
       const R logB = rm::log1p(B_M);
-      const R invFM = B_M / (logB * rm::pow(R(1.) + B_M, R(0.7)));
+      R invFM;
+      if constexpr (sizeof(R) == 4) {
+        invFM = B_M / (logB * rm::exp(R(0.7) * logB));  // (1+B_M)^0.7 from log1p(B_M)
+      } else {
+        invFM = B_M / (logB * rm::pow(R(1.) + B_M, R(0.7)));
+      }
       const R Sh_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Sc_skin);
       const R Sh_num = R(2.) + (Sh_0 - R(2.)) * invFM;
       const R mdotcoeff = static_cast<R>(kPIc) * dia_part * Sh_num * logB;

@@ -56,7 +56,9 @@ __device__ inline float log1p(float x) { return ::log1pf(x); }
 __device__ inline float pow(float x, float y) {
   return (x == 0.0f) ? ((y > 0.0f) ? 0.0f : __builtin_huge_valf()) : __expf(y * __logf(x));
 }
-__device__ inline float cbrt(float x) { return ::cbrtf(x); }
+// x >= 0 on every call site (masses, Reynolds and Peclet groups, mixture
+// sums): the hardware exp2/log2 pair instead of the software cbrtf
+__device__ inline float cbrt(float x) { return __expf(__logf(x) * (1.f / 3.f)); }
 __device__ inline float sqrt(float x) { return __builtin_sqrtf(x); }
 __device__ inline float fabs(float x) { return __builtin_fabsf(x); }
 #else
@@ -165,7 +167,11 @@ struct GasProps {
         const R m3 = m * m * m;
         s += X[n] * (m3 * m3);
       }
-      mu = rm::cbrt(rm::sqrt(s));
+      if constexpr (sizeof(R) == 4) {
+        mu = rm::exp(rm::log(s) * R(1. / 6.));  // s^(1/6)
+      } else {
+        mu = rm::cbrt(rm::sqrt(s));
+      }
     }
     if (get_lam) {
       R s = R(0);

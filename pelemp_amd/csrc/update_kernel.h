@@ -218,7 +218,7 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // registers) without spills; with more species it keeps 1.
 template <class R, int NF>
 struct UpdOccupancy {
-  static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : (NF <= 2 ? 3 : 1);
+  static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1);
 };
 
 // Per-droplet values carried from the gather to the update.
