@@ -236,6 +236,119 @@ def measured_traffic(form: str):
     return None
 
 
+def _timed(fn, reps: int, stream):
+    """Mean milliseconds of fn() over reps calls, HIP events on `stream`."""
+    from pelemp_amd import _capi as C
+
+    fn()
+    C.device_synchronize()
+    timer = C.Timer()
+    timer.start(stream.handle)
+    for _ in range(reps):
+        fn()
+    timer.stop(stream.handle)
+    stream.synchronize()
+    return timer.elapsed_ms() / reps
+
+
+def run_rows(prob, stream, reps: int = 5):
+    """The steps around the update on the config-2 problem, FP64: transferSource
+    (tmp FAB with 2 ghosts into a state-layout FAB), estTimestep,
+    computeDerivedVars (17 variables), the PeleC flavour's update step (same
+    droplets and gas in CGS), the update step with a planar EB cutting the
+    box, device injection (one call of a jet injecting ~2e5 parcels, and the
+    oracle's sequential CPU loop for the same call) and uniformSprayInit of
+    the same number of droplets."""
+    from pelemp_amd import _capi as C
+    from pelemp_amd import problem as P
+    from pelemp_amd.spray import DeviceFab, SprayJet, SprayParticleContainer
+
+    h = stream.handle
+    out = {}
+    cap = int(prob.n * 1.05) + 1024
+    g = prob.geom
+    n = [g["domain_hi"][d] - g["domain_lo"][d] + 1 for d in range(3)]
+    pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, g, cap)
+    pc.set_particles(prob.pos, prob.rdata, prob.ids)
+    state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
+    src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+    nsrc = prob.src_shape[0]
+    tmp = DeviceFab((-2, -2, -2), (n[0] + 1, n[1] + 1, n[2] + 1), nsrc)
+    act = DeviceFab((-1, -1, -1), (n[0], n[1], n[2]), 6 + len(prob.species))
+    out["transfer_source_ms"] = _timed(lambda: pc.transferSource(2, 0, tmp, act, stream=h, sync=False), reps, stream)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pc.estTimestep(0, 0.5, 0.0, stream=h)
+    out["est_timestep_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+    nvar = pc.NumDeriveVars(1)
+    mf = DeviceFab((0, 0, 0), (n[0] - 1, n[1] - 1, n[2] - 1), nvar)
+    out["compute_derived_ms"] = _timed(lambda: pc.computeDerivedVars(mf, 0, 0, 1, stream=h, sync=False), reps, stream)
+    out["compute_derived_vars"] = nvar
+
+    def upd(c, st):
+        c.updateParticles(0, st, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False, True,
+                          None, -1.0, stream=h, sync=False)
+
+    out["lmex_step_ms"] = _timed(lambda: upd(pc, state), reps, stream)
+    pc.close()
+    # PeleC flavour: the same problem in CGS (state rho, rho u, rho E, rho Y, T)
+    pcg = P.to_pelec(prob, T_guess_factor=1.02)
+    pce = SprayParticleContainer(pcg.mech, pcg.sdata, pcg.comps, pcg.geom, cap, flavour="pelec")
+    pce.set_particles(pcg.pos, pcg.rdata, pcg.ids)
+    state_c = DeviceFab(pcg.state_lo, pcg.state_hi, pcg.state.shape[0], pcg.state)
+    out["pelec_step_ms"] = _timed(lambda: upd(pce, state_c), reps, stream)
+    pce.close()
+    del pcg, state_c
+    # a plane EB (solid below 0.3 of the box height, cutting a layer of cells
+    # at 0.37 of a cell); the droplets of the fluid side, short of the top
+    # face (whose periodic image is the solid), and of a thin layer behind
+    # the EB
+    L, dz = g["phi"][2], g["dx"][2]
+    eb = P.eb_plane(prob, [0.0, 0.0, 0.3 * L + 0.37 * dz], [0.0, 0.0, -1.0], nsub=4)
+    d = (prob.pos - eb["point"]) @ eb["normal"]
+    keep = (d < 0.3 * dz) & (prob.pos[:, 2] < L - 3.0 * dz)
+    pce = SprayParticleContainer(prob.mech, dict(prob.sdata, min_eb_vfrac=0.05), prob.comps, g, cap)
+    pce.set_eb(eb)
+    pce.set_particles(prob.pos[keep], prob.rdata[keep], prob.ids[keep])
+    out["eb_step_ms"] = _timed(lambda: upd(pce, state), reps, stream)
+    out["eb_droplets"] = int(np.count_nonzero(keep))
+    out["eb_cut_cells"] = int(np.count_nonzero((eb["flags"] & 3) == 1))
+    pce.close()
+    del eb
+    # injection: a jet in an empty container, ~2e5 parcels per call
+    pci = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, g, 4 * 200000 + 1024)
+    cent = [0.5 * (a + b) for a, b in zip(g["plo"], g["phi"])]
+    mk = dict(jet_cent=cent, jet_norm=[0.0, 0.0, 1.0], spread_angle=20.0, jet_dia=2e-4, jet_vel=50.0, mass_flow=2.0,
+              jet_temp=300.0, jet_Y=[1.0 / prob.nf] * prob.nf if prob.nf > 1 else None, dist_type="Normal",
+              dist_params=(20e-6, 5e-6), phi_swirl=10.0, inject_ppp=3.0, seed=5)
+    jet = SprayJet("jet1", pci, **mk)
+    oj = C.SprayJetC.from_buffer_copy(jet._j)  # the checked jet before its first injection
+    t0 = time.perf_counter()
+    n_inj = pci.sprayInjection(0.0, jet, 1e-6)
+    out["inject_ms"] = (time.perf_counter() - t0) * 1e3
+    out["inject_droplets"] = n_inj
+    try:
+        from oracle import oracle_py as O
+
+        t0 = time.perf_counter()
+        O.inject(prob.sdata, g, oj, 0.0, 1e-6, 1)
+        out["inject_cpu_oracle_ms"] = (time.perf_counter() - t0) * 1e3
+    except Exception as e:  # the oracle is optional on the GPU box
+        out["inject_cpu_oracle_ms"] = None
+        print(f"# rows: oracle injection skipped: {e}", file=sys.stderr, flush=True)
+    pci.close()
+    # uniformSprayInit of ~1.6e7 droplets
+    npart = (256, 256, 244)
+    pcu = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, g, int(np.prod(npart)) + 1024)
+    t0 = time.perf_counter()
+    pcu.uniformSprayInit(npart, (1.0, 0.0, 0.0), 20e-6, 300.0, [1.0 / prob.nf] * prob.nf, num_ppp=2.0)
+    C.device_synchronize()
+    out["uniform_init_ms"] = (time.perf_counter() - t0) * 1e3
+    out["uniform_init_droplets"] = int(np.prod(npart))
+    pcu.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +361,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-branches", action="store_true", help="skip the config-3 (walls/breakup/splash) leg")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the leg timing transferSource, estTimestep, derived variables, PeleC, EB, injection")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the 1 GiB copy kernels used to calibrate PMC byte counters")
     args = ap.parse_args(argv)
@@ -318,6 +433,13 @@ def main(argv=None):
                      "kernel": "spray_update_kernel", "bytes_per_droplet_step": mf["bytes_per_droplet_step"]},
         "forms": forms,
     }
+    if not args.no_rows and "fp64" in forms:
+        try:
+            out["rows"] = run_rows(prob, stream)
+            print(f"# rows: {json.dumps(out['rows'])}", file=sys.stderr, flush=True)
+        except C.PelempError as e:
+            out["rows"] = dict(error=str(e))
+            print(f"# rows failed: {e}", file=sys.stderr, flush=True)
     if not args.no_branches:
         try:
             br = run_branches(min(args.steps, 5), min(args.warmup, 1), stream)

@@ -103,6 +103,24 @@ struct InjDrop {
   double pos[3], vel[3], dia, mass;  // mass = num_ppp * pmass
 };
 
+// diameter of candidate k (draw slots 3-4), its acceptance and parcel mass:
+// the part of a candidate the injected-mass sum needs
+template <int NF>
+PM_HDI double inj_dia(const InjJet<NF>& j, int64_t k) {
+  return inj_sample_dia(j.dist, inj_uniform(j.seed, j.call, k, 3), inj_uniform(j.seed, j.call, k, 4));
+}
+
+template <int NF>
+PM_HDI bool inj_accept(const InjJet<NF>& j, double dia) {
+  return dia > j.min_dia && j.umag > 0.;
+}
+
+template <int NF>
+PM_HDI double inj_parcel_mass(const InjJet<NF>& j, double dia) {
+  const double pmass = kInjPI / 6. * j.rho_part * pm::pow(dia, 3.);
+  return j.num_ppp * pmass;
+}
+
 template <int NF>
 PM_HDI void inj_candidate(const InjJet<NF>& j, int64_t k, InjDrop& o) {
   double radp = inj_uniform(j.seed, j.call, k, 0);
@@ -117,9 +135,9 @@ PM_HDI void inj_candidate(const InjJet<NF>& j, int64_t k, InjDrop& o) {
     theta_spread += j.hollow_spread * r;
   }
   const double phi_swirl = j.swirl;
-  const double dia = inj_sample_dia(j.dist, inj_uniform(j.seed, j.call, k, 3), inj_uniform(j.seed, j.call, k, 4));
+  const double dia = inj_dia(j, k);
   o.dia = dia;
-  o.accept = dia > j.min_dia && j.umag > 0.;
+  o.accept = inj_accept(j, dia);
   if (!o.accept) {
     o.mass = 0.;
     return;
@@ -143,8 +161,7 @@ PM_HDI void inj_candidate(const InjJet<NF>& j, int64_t k, InjDrop& o) {
   }
   const double pmov = inj_uniform(j.seed, j.call, k, 5);
   for (int d = 0; d < 3; ++d) o.pos[d] = loc[d] + pmov * j.dt * vel[d];
-  const double pmass = kInjPI / 6. * j.rho_part * pm::pow(dia, 3.);
-  o.mass = j.num_ppp * pmass;
+  o.mass = inj_parcel_mass(j, dia);
 }
 
 // AMReX Where() on one box covering the domain: the containing cell

@@ -1,16 +1,25 @@
 // inject_kernels.h -- device side of the injection and initialisation
 // routines (SprayInjection.H:10-348) and of appending droplets.
 //
-// inject_kernel runs the droplet loop of sprayInjection (SprayInjection.H:
-// 78-158): candidates are drawn until the injected mass reaches the target.
-// One workgroup of kInjBlock threads draws kInjBlock candidates at a time in
-// parallel (each candidate has its own counter-based random numbers); one
-// thread then adds their masses in candidate order, the order of the
-// reference's loop, so the stopping candidate and the injected mass are the
-// reference loop's own; the kept droplets of the chunk get their output
-// index from a block prefix count.  Injection counts are small (tens to
-// thousands of droplets per step), so one workgroup is the right size: the
-// launch is latency-bound either way.
+// The droplet loop of sprayInjection (SprayInjection.H:78-158) draws
+// candidates until the injected mass reaches the target; the mass is summed
+// candidate by candidate, so the stopping candidate and the injected mass
+// are the reference loop's own only if that FP64 sum keeps its order.  The
+// loop runs in batches of candidates, three launches each:
+//   inj_mass_kernel   every candidate's parcel mass (0 when rejected) and
+//                     the accepted count of each workgroup, chip-wide;
+//   inj_scan_kernel   one wavefront: the ordered sum (uniform loads, every
+//                     lane adding the same values), the stopping candidate,
+//                     and the exclusive scan of the workgroup counts;
+//   inj_write_kernel  the accepted candidates up to the stop recomputed in
+//                     full and written at their output index (workgroup
+//                     offset + rank from a block ballot), chip-wide.
+// Adding 0 for a rejected candidate leaves the sum unchanged and cannot
+// stop the loop, which matches the reference skipping it.  Parcel masses are
+// not negative (the host rejects a non-positive liquid density), so the
+// running sum does not decrease: a group of kInjSumGroup candidates whose
+// sum stays below the target holds no stop, and only the group that reaches
+// it (or turns the sum NaN) is summed again candidate by candidate.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,112 +29,134 @@
 namespace pelemp {
 
 constexpr int kInjBlock = 256;
+constexpr int kInjSumGroup = 32;
 
-// result words of one injection launch
+// result words of one injection
 struct InjResult {
-  double cur_mass;   // injected mass
-  long long n_out;   // droplets kept (may exceed the staging capacity)
-  long long n_cand;  // candidates drawn
-  int outside;       // a kept droplet outside the domain (Where() failed)
-  int unfinished;    // the candidate limit was reached before the target
+  double cur_mass;       // injected mass (carried from batch to batch)
+  long long stop;        // batch index of the candidate that reached the mass, -1 if none
+  long long n_out;       // droplets kept up to the stop (with a stop)
+  long long batch_kept;  // accepted candidates of the batch
+  int outside;           // a kept droplet outside the domain (Where() failed)
 };
+
+template <int NF>
+__global__ void __launch_bounds__(kInjBlock) inj_mass_kernel(InjJet<NF> j, long long k0,
+                                                             double* __restrict__ mass,
+                                                             int* __restrict__ blk_cnt) {
+  __shared__ int s_wave[kInjBlock / 64];
+  const long long i = static_cast<long long>(blockIdx.x) * kInjBlock + threadIdx.x;
+  const double dia = inj_dia(j, k0 + i);
+  const bool acc = inj_accept(j, dia);
+  mass[i] = acc ? inj_parcel_mass(j, dia) : 0.;
+  const unsigned long long bal = __ballot(acc);
+  if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kInjBlock / 64; ++w) t += s_wave[w];
+    blk_cnt[blockIdx.x] = t;
+  }
+}
+
+// One wavefront of 64 lanes; nb is a multiple of kInjSumGroup.  blk_cnt becomes the
+// exclusive prefix of the workgroup counts.
+__global__ void __launch_bounds__(64) inj_scan_kernel(const double* __restrict__ mass, long long nb,
+                                                      int* __restrict__ blk_cnt, int nblk, double inject_mass,
+                                                      InjResult* __restrict__ res) {
+  constexpr int G = kInjSumGroup;
+  const int lane = threadIdx.x;
+  double cur = res->cur_mass;
+  long long stop = -1;
+  double m[G], mn[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) m[q] = mass[q];
+  for (long long base = 0; base < nb; base += G) {
+    // the next group's loads in flight while this group is summed
+    const long long nx = (base + G < nb) ? base + G : base;
+#pragma unroll
+    for (int q = 0; q < G; ++q) mn[q] = mass[nx + q];
+    double c = cur;
+#pragma unroll
+    for (int q = 0; q < G; ++q) c = c + m[q];
+    if (c < inject_mass) {
+      cur = c;
+#pragma unroll
+      for (int q = 0; q < G; ++q) m[q] = mn[q];
+      continue;
+    }
+    for (int q = 0; q < G; ++q) {
+      cur = cur + m[q];
+      if (!(cur < inject_mass)) {
+        stop = base + q;
+        break;
+      }
+    }
+    break;
+  }
+  int carry = 0;
+  for (int b0 = 0; b0 < nblk; b0 += 64) {
+    const int b = b0 + lane;
+    const int v = (b < nblk) ? blk_cnt[b] : 0;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(x, off);
+      if (lane >= off) x += t;
+    }
+    if (b < nblk) blk_cnt[b] = carry + x - v;
+    carry += __shfl(x, 63);
+  }
+  if (lane == 0) {
+    res->cur_mass = cur;
+    res->stop = stop;
+    res->batch_kept = carry;
+  }
+}
 
 // Staging in the AoS meaning (pos[3], rdata[10 + NF], id).
 template <int NF>
-__global__ void __launch_bounds__(kInjBlock) inject_kernel(InjJet<NF> j, double inject_mass, long long max_cand,
-                                                           long long out_cap, long long next_id,
-                                                           double* __restrict__ pos, double* __restrict__ rdata,
-                                                           long long* __restrict__ id,
-                                                           InjResult* __restrict__ res) {
+__global__ void __launch_bounds__(kInjBlock) inj_write_kernel(InjJet<NF> j, long long k0,
+                                                              const int* __restrict__ blk_off, long long out_base,
+                                                              long long out_cap, long long next_id,
+                                                              double* __restrict__ pos, double* __restrict__ rdata,
+                                                              long long* __restrict__ id,
+                                                              InjResult* __restrict__ res) {
   constexpr int NSR = 10 + NF;
-  __shared__ double s_mass[kInjBlock];
-  __shared__ int s_acc[kInjBlock];
   __shared__ int s_wave[kInjBlock / 64];
-  __shared__ int s_stop;
-  __shared__ double s_cur;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wv = tid >> 6;
-  double cur = 0.;  // thread 0's running mass
-  long long out_base = 0;
-  long long base = 0;
-  int outside = 0;
-  bool done = !(cur < inject_mass);
-  while (!done) {
-    const long long k = base + tid;
-    InjDrop o;
-    inj_candidate<NF>(j, k, o);
-    s_mass[tid] = o.mass;
-    s_acc[tid] = o.accept ? 1 : 0;
-    __syncthreads();
-    if (tid == 0) {
-      // the reference's loop condition, candidate by candidate
-      int stop = -1;
-      for (int q0 = 0; q0 < kInjBlock && stop < 0; q0 += 8) {
-        double m[8];
-        int a[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          m[q] = s_mass[q0 + q];
-          a[q] = s_acc[q0 + q];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          if (stop < 0 && a[q]) {
-            cur = cur + m[q];
-            if (!(cur < inject_mass)) stop = q0 + q;
-          }
-        }
-      }
-      s_stop = stop;
-      s_cur = cur;
+  const long long stop = res->stop;
+  const long long blk0 = static_cast<long long>(blockIdx.x) * kInjBlock;
+  if (stop >= 0 && blk0 > stop) return;  // the whole workgroup is past the stop
+  const long long i = blk0 + threadIdx.x;
+  InjDrop o;
+  inj_candidate<NF>(j, k0 + i, o);
+  const bool keep = o.accept && (stop < 0 || i <= stop);
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(o.accept);
+  const unsigned long long lower = (lane == 0) ? 0ull : (bal & ((~0ull) >> (64 - lane)));
+  if (lane == 0) s_wave[wv] = __popcll(bal);
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < wv; ++w) off += s_wave[w];
+  if (!keep) return;
+  const long long oi = out_base + blk_off[blockIdx.x] + off + __popcll(lower);
+  if (!inj_inside(o.pos, j.plo, j.dxi, j.ncell)) atomicOr(&res->outside, 1);
+  if (i == stop) res->n_out = oi + 1;
+  if (oi < out_cap) {
+    double* r = rdata + oi * NSR;
+    for (int d = 0; d < 3; ++d) {
+      pos[3 * oi + d] = o.pos[d];
+      r[d] = o.vel[d];
     }
-    __syncthreads();
-    const int stop = s_stop;
-    const bool keep = o.accept && (stop < 0 || tid <= stop);
-    // output index: kept candidates before this one in the chunk
-    const unsigned long long bal = __ballot(keep);
-    const unsigned long long lower = (lane == 0) ? 0ull : (bal & ((~0ull) >> (64 - lane)));
-    const int rank_w = __popcll(lower);
-    if (lane == 0) s_wave[wv] = __popcll(bal);
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int w = 0; w < kInjBlock / 64; ++w) {
-      off += (w < wv) ? s_wave[w] : 0;
-      tot += s_wave[w];
-    }
-    if (keep) {
-      const long long oi = out_base + off + rank_w;
-      if (!inj_inside(o.pos, j.plo, j.dxi, j.ncell)) outside = 1;
-      if (oi < out_cap) {
-        double* r = rdata + oi * NSR;
-        for (int d = 0; d < 3; ++d) {
-          pos[3 * oi + d] = o.pos[d];
-          r[d] = o.vel[d];
-        }
-        r[3] = j.T;
-        r[4] = o.dia;
-        for (int s = 0; s < NF; ++s) r[5 + s] = NF > 1 ? j.Y[s] : 1.;
-        r[5 + NF] = j.num_ppp;  // numDens
-        r[6 + NF] = j.num_ppp;  // N0
-        r[7 + NF] = 0.;         // BM1
-        r[8 + NF] = j.initial_bm2;
-        r[9 + NF] = 0.;         // film height
-        id[oi] = next_id + oi;
-      }
-    }
-    out_base += tot;
-    base += kInjBlock;
-    done = (stop >= 0) || (base >= max_cand);
-    __syncthreads();
-  }
-  // every thread that saw a droplet outside reports it
-  if (outside) atomicOr(&res->outside, 1);
-  if (tid == 0) {
-    res->cur_mass = cur;
-    res->n_out = out_base;
-    res->n_cand = base;
-    res->unfinished = (cur < inject_mass) ? 1 : 0;
+    r[3] = j.T;
+    r[4] = o.dia;
+    for (int s = 0; s < NF; ++s) r[5 + s] = NF > 1 ? j.Y[s] : 1.;
+    r[5 + NF] = j.num_ppp;  // numDens
+    r[6 + NF] = j.num_ppp;  // N0
+    r[7 + NF] = 0.;         // BM1
+    r[8 + NF] = j.initial_bm2;
+    r[9 + NF] = 0.;         // film height
+    id[oi] = next_id + oi;
   }
 }
 

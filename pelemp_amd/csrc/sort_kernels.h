@@ -12,8 +12,13 @@
 //               arrival slot;
 //   2. scan:    cstart = exclusive prefix sum of cnt (three-phase block scan);
 //   3. scatter: perm[cstart[key] + slot] = position;
-//   4. order:   each cell's perm segment is sorted ascending (segments hold
-//               ~ppc entries; insertion sort, one thread per cell).
+//   4. order:   each cell's perm segment is sorted ascending: segments of up
+//               to kSeqSortMax entries (a few droplets per cell) by one
+//               thread with an insertion sort; longer ones (droplets
+//               injected into the few cells of a nozzle, breakup children
+//               of one cell) are listed and sorted by a workgroup each
+//               (sort_long_groups_kernel), so no segment costs O(m^2) on
+//               one thread.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -83,12 +88,17 @@ __global__ void __launch_bounds__(kScanBlock) scan_sums_kernel(int32_t* __restri
 }
 
 // Phase 3: add block offsets; cstart[n] = total.
+// (also clears the count of long segments that the order step lists)
 __global__ void __launch_bounds__(kScanBlock) scan_add_kernel(int32_t* __restrict__ out, int32_t n,
                                                                const int32_t* __restrict__ block_sum,
-                                                               const int32_t* __restrict__ total) {
+                                                               const int32_t* __restrict__ total,
+                                                               int32_t* __restrict__ n_long) {
   const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
   if (i < n) out[i] += block_sum[blockIdx.x];
-  if (i == 0) out[n] = *total;
+  if (i == 0) {
+    out[n] = *total;
+    *n_long = 0;
+  }
 }
 
 __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __restrict__ key,
@@ -163,43 +173,129 @@ __device__ inline void insertion_sort_global(int32_t* __restrict__ v, int32_t a,
   }
 }
 
+// Segments longer than this are sorted by a workgroup.  The list of such
+// segments holds at most n / (kSeqSortMax + 1) entries.
+constexpr int32_t kSeqSortMax = 32;
+constexpr int kLongSortThreads = 1024;
+constexpr int kLongSortChunk = 4096;  // entries sorted in LDS at a time
+constexpr int kLongSortGrid = 256;    // workgroups working through the list
+
+__device__ inline void sort_or_defer(int32_t* __restrict__ perm, int32_t a, int32_t len, int2* __restrict__ long_list,
+                                     int32_t* __restrict__ n_long) {
+  if (len <= kSeqSortMax) {
+    insertion_sort_global(perm, a, a + len);
+    return;
+  }
+  const int32_t q = atomicAdd(n_long, 1);
+  long_list[q] = make_int2(a, len);
+}
+
+// Sort each listed segment perm[a, a + m) ascending, one workgroup per
+// segment: chunks of kLongSortChunk entries by a bitonic network in LDS,
+// then merges of sorted runs of doubling width through tmp[a, a + m) (each
+// thread merges one slice of the output, found by a merge-path search).
+// Entries are distinct storage positions below INT32_MAX, which pads the
+// last chunk.
+__global__ void __launch_bounds__(kLongSortThreads) sort_long_groups_kernel(const int2* __restrict__ long_list,
+                                                                             const int32_t* __restrict__ n_long,
+                                                                             int32_t* __restrict__ perm,
+                                                                             int32_t* __restrict__ tmp) {
+  __shared__ int32_t sh[kLongSortChunk];
+  const int32_t nl = *n_long;
+  const int tid = threadIdx.x;
+  for (int32_t g = blockIdx.x; g < nl; g += gridDim.x) {
+    const int32_t a = long_list[g].x;
+    const int32_t m = long_list[g].y;
+    for (int32_t c0 = 0; c0 < m; c0 += kLongSortChunk) {
+      const int32_t len = min(kLongSortChunk, m - c0);
+      for (int e = tid; e < kLongSortChunk; e += kLongSortThreads) sh[e] = (e < len) ? perm[a + c0 + e] : INT32_MAX;
+      __syncthreads();
+      for (int k = 2; k <= kLongSortChunk; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int e = tid; e < kLongSortChunk / 2; e += kLongSortThreads) {
+            const int i = 2 * e - (e & (j - 1));  // e with a zero bit inserted at log2(j)
+            const int l = i + j;
+            const bool up = (i & k) == 0;
+            const int32_t x = sh[i], y = sh[l];
+            if ((x > y) == up) {
+              sh[i] = y;
+              sh[l] = x;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (int e = tid; e < len; e += kLongSortThreads) perm[a + c0 + e] = sh[e];
+      __syncthreads();
+    }
+    int32_t* src = perm + a;
+    int32_t* dst = tmp + a;
+    for (int32_t w = kLongSortChunk; w < m; w <<= 1) {
+      for (int32_t lo = 0; lo < m; lo += 2 * w) {
+        const int32_t mid = min(lo + w, m), hi = min(lo + 2 * w, m);
+        const int32_t la = mid - lo, lb = hi - mid, tot = hi - lo;
+        const int32_t per = (tot + kLongSortThreads - 1) / kLongSortThreads;
+        const int32_t o0 = min(tid * per, tot), o1 = min(o0 + per, tot);
+        if (o0 < o1) {
+          const int32_t* A = src + lo;
+          const int32_t* B = src + mid;
+          // i = entries of A among the first o0 outputs
+          int32_t ilo = max(0, o0 - lb), ihi = min(o0, la);
+          while (ilo < ihi) {
+            const int32_t i = (ilo + ihi) >> 1;
+            if (A[i] < B[o0 - i - 1])
+              ilo = i + 1;
+            else
+              ihi = i;
+          }
+          int32_t i = ilo, j = o0 - ilo;
+          for (int32_t o = o0; o < o1; ++o) {
+            if (j >= lb || (i < la && A[i] < B[j]))
+              dst[lo + o] = A[i++];
+            else
+              dst[lo + o] = B[j++];
+          }
+        }
+      }
+      __syncthreads();
+      int32_t* t = src;
+      src = dst;
+      dst = t;
+    }
+    if (src != perm + a)
+      for (int32_t e = tid; e < m; e += kLongSortThreads) perm[a + e] = src[e];
+    __syncthreads();
+  }
+}
+
 // order the mover groups of each cell by storage position
 __global__ void __launch_bounds__(256) order_movers_kernel(const int32_t* __restrict__ cstart,
                                                             const int32_t* __restrict__ mov_lo,
                                                             const int32_t* __restrict__ stay_cnt,
                                                             const int32_t* __restrict__ mov_hi, int32_t ncells,
-                                                            int32_t* __restrict__ perm) {
+                                                            int32_t* __restrict__ perm, int2* __restrict__ long_list,
+                                                            int32_t* __restrict__ n_long) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int32_t lo = mov_lo[c];
   const int32_t hi = mov_hi[c];
   if (lo < 2 && hi < 2) return;
   const int32_t a = cstart[c];
-  if (lo > 1) insertion_sort_global(perm, a, a + lo);
-  if (hi > 1) {
-    const int32_t h0 = a + lo + stay_cnt[c];
-    insertion_sort_global(perm, h0, h0 + hi);
-  }
+  if (lo > 1) sort_or_defer(perm, a, lo, long_list, n_long);
+  if (hi > 1) sort_or_defer(perm, a + lo + stay_cnt[c], hi, long_list, n_long);
 }
 
-// Sort each cell's segment of perm ascending (insertion sort in place; a
-// segment holds the droplets of one cell, ~ppc entries).
+// Sort each cell's segment of perm ascending (a segment holds the droplets
+// of one cell).
 __global__ void __launch_bounds__(256) order_cells_kernel(const int32_t* __restrict__ cstart,
                                                            int32_t ncells,
-                                                           int32_t* __restrict__ perm) {
+                                                           int32_t* __restrict__ perm, int2* __restrict__ long_list,
+                                                           int32_t* __restrict__ n_long) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int32_t a = cstart[c];
   const int32_t b = cstart[c + 1];
-  for (int32_t i = a + 1; i < b; ++i) {
-    const int32_t x = perm[i];
-    int32_t j = i - 1;
-    while (j >= a && perm[j] > x) {
-      perm[j + 1] = perm[j];
-      --j;
-    }
-    perm[j + 1] = x;
-  }
+  if (b - a > 1) sort_or_defer(perm, a, b - a, long_list, n_long);
 }
 
 }  // namespace pelemp

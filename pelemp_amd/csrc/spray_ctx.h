@@ -72,6 +72,8 @@ struct Ctx {
   int32_t* cstart = nullptr;  // ncells + 1
   int32_t* cnt = nullptr;
   int32_t* arrival = nullptr;
+  int2* long_list = nullptr;  // (start, length) of perm segments too long for one thread
+  int32_t* n_long = nullptr;  // entries of long_list (in scalars)
   int32_t* mov_lo = nullptr;    // per cell: droplets arriving from lower keys
   int32_t* mov_hi = nullptr;    // per cell: droplets arriving from higher keys
   int32_t* stay_cnt = nullptr;  // per cell: droplets that stayed
@@ -80,7 +82,7 @@ struct Ctx {
   int32_t* block_sum = nullptr;
   int32_t* total = nullptr;
   int32_t* err = nullptr;
-  int32_t* scalars = nullptr;  // n_entries, total, err in one allocation
+  int32_t* scalars = nullptr;  // n_entries, total, err, n_long in one allocation
   double* dscratch = nullptr;  // reduction scratch
   // cold per-slot data (indexed by slot)
   int64_t* cold_id = nullptr;

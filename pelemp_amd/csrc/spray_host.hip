@@ -370,11 +370,15 @@ static int run_sort(Ctx& c, hipStream_t s, bool keep_entries = false) {
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
   hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, c.ncells, c.cstart, c.block_sum);
   hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.block_sum, nb, c.total);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total,
+                     c.n_long);
   hipLaunchKernelGGL(scatter_perm_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.perm);
   hipLaunchKernelGGL(order_cells_kernel, dim3((c.ncells + 255) / 256), dim3(256), 0, s, c.cstart, c.ncells,
-                     c.perm);
+                     c.perm, c.long_list, c.n_long);
+  // arrival is free once the scatter has read it: the merge buffer
+  hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
+                     c.n_long, c.perm, c.arrival);
   PM_HIP(hipGetLastError());
   ctx_time_end(c, 1, s, ev0);
   return PELEMP_OK;
@@ -510,11 +514,14 @@ static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src = nu
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
   hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, c.ncells, c.cstart, c.block_sum);
   hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.block_sum, nb, c.total);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total,
+                     c.n_long);
   hipLaunchKernelGGL(scatter_incr_kernel, dim3(2048), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.mov_lo, c.stay_cnt, c.perm);
   hipLaunchKernelGGL(order_movers_kernel, dim3(gc), dim3(256), 0, s, c.cstart, c.mov_lo, c.stay_cnt, c.mov_hi,
-                     c.ncells, c.perm);
+                     c.ncells, c.perm, c.long_list, c.n_long);
+  hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
+                     c.n_long, c.perm, c.arrival);
   PM_HIP(hipGetLastError());
   ctx_time_end(c, 1, s, ev0);
   return PELEMP_OK;
@@ -549,7 +556,7 @@ static int alloc_branch_buffers(Ctx& c) {
 
 static void free_ctx(Ctx& c) {
   void* ptrs[] = {c.fbuf[0], c.fbuf[1], c.mbuf[0], c.mbuf[1], c.slot[0], c.slot[1], c.perm, c.cstart, c.cnt,
-                  c.arrival, c.mov_lo, c.key,
+                  c.arrival, c.long_list, c.mov_lo, c.key,
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
                   c.count_log, c.surr_w, c.surr_cnt, c.wf, c.sb_type, c.sb_data, c.sb_count, c.sb_offset,
                   c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new, c.eb_cells, c.eb_rkey, c.eb_rval,
@@ -899,16 +906,52 @@ static int check_jet(const Ctx& c, pelemp_spray_jet& j, double* avg_dia) {
   return PELEMP_OK;
 }
 
+// Batches of candidates (inject_kernels.h) until the mass is reached or
+// `limit` candidates are drawn.  Returns the droplets kept (also those past
+// out_cap, which are counted but not staged), the injected mass, whether the
+// limit stopped the loop and whether a droplet lies outside the domain.
+struct InjOutcome {
+  long long n_out = 0;
+  double cur_mass = 0.;
+  bool unfinished = false;
+  bool outside = false;
+};
+
 template <int NF>
-static int launch_inject(const InjJet<NF>& jj, double inject_mass, long long max_cand, long long out_cap,
-                         long long next_id, double* dpos, double* drd, long long* did, InjResult* dres,
-                         hipStream_t s) {
-  const InjResult zero{};
-  PM_HIP(hipMemcpyAsync(dres, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL((inject_kernel<NF>), dim3(1), dim3(kInjBlock), 0, s, jj, inject_mass, max_cand, out_cap, next_id,
-                     dpos, drd, did, dres);
-  PM_HIP(hipGetLastError());
-  return PELEMP_OK;
+static int run_inject(const InjJet<NF>& jj, double inject_mass, long long limit, long long batch, long long out_cap,
+                      long long next_id, double* dpos, double* drd, long long* did, double* dmass, int* dblk,
+                      InjResult* dres, hipStream_t s, InjOutcome* out) {
+  *out = InjOutcome{};
+  if (!(0. < inject_mass)) return PELEMP_OK;  // the reference loop does not start
+  InjResult r{};
+  r.stop = -1;
+  PM_HIP(hipMemcpyAsync(dres, &r, sizeof(r), hipMemcpyHostToDevice, s));
+  long long k0 = 0, out_base = 0;
+  while (true) {
+    const long long nb = std::min(batch, limit - k0);
+    const unsigned nblk = static_cast<unsigned>(nb / kInjBlock);
+    hipLaunchKernelGGL((inj_mass_kernel<NF>), dim3(nblk), dim3(kInjBlock), 0, s, jj, k0, dmass, dblk);
+    hipLaunchKernelGGL(inj_scan_kernel, dim3(1), dim3(64), 0, s, dmass, nb, dblk, static_cast<int>(nblk),
+                       inject_mass, dres);
+    hipLaunchKernelGGL((inj_write_kernel<NF>), dim3(nblk), dim3(kInjBlock), 0, s, jj, k0, dblk, out_base, out_cap,
+                       next_id, dpos, drd, did, dres);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpyAsync(&r, dres, sizeof(r), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    out->outside = out->outside || r.outside != 0;
+    out->cur_mass = r.cur_mass;
+    if (r.stop >= 0) {
+      out->n_out = r.n_out;
+      return PELEMP_OK;
+    }
+    out_base += r.batch_kept;
+    k0 += nb;
+    if (k0 >= limit) {
+      out->n_out = out_base;
+      out->unfinished = true;
+      return PELEMP_OK;
+    }
+  }
 }
 
 // The droplet loop of sprayInjection on the device; droplets staged and
@@ -950,6 +993,11 @@ static int inject_droplets(Ctx& c, const pelemp_spray_jet& j, double time, doubl
     const double* a = &c.sd.rho_coef[0];
     rho_part = a[0] + j.jet_T * (a[1] + j.jet_T * (a[2] + j.jet_T * a[3]));
   }
+  if (!(rho_part > 0.)) {
+    // parcel masses would not be positive: the reference loop never ends
+    set_last_error("injection: non-positive liquid density at the jet temperature");
+    return PELEMP_E_BAD_INJECTION;
+  }
   jj.rho_part = rho_part;
   jj.min_dia = min_dia;
   jj.num_ppp = num_ppp;
@@ -971,36 +1019,46 @@ static int inject_droplets(Ctx& c, const pelemp_spray_jet& j, double time, doubl
   if (st) return st;
   long long out_cap = static_cast<long long>(std::min(2. * expected + 1024., static_cast<double>(c.cap) + 1.));
   const int nsr = 10 + NF;
-  InjResult res{};
-  double *dpos = nullptr, *drd = nullptr;
+  // candidates in batches of about the expected count (whole workgroups),
+  // up to max_cand rounded up to whole workgroups
+  const long long limit = (max_cand + kInjBlock - 1) / kInjBlock * kInjBlock;
+  const double bw = std::min(std::max(1.05 * expected + 4096., 4096.), 1.6777216e7);
+  const long long batch = std::min(limit, (static_cast<long long>(bw) + kInjBlock - 1) / kInjBlock * kInjBlock);
+  double *dpos = nullptr, *drd = nullptr, *dmass = nullptr;
   long long* did = nullptr;
+  int* dblk = nullptr;
   InjResult* dres = nullptr;
+  auto release = [&]() {
+    for (void* p : {static_cast<void*>(dpos), static_cast<void*>(drd), static_cast<void*>(did),
+                    static_cast<void*>(dmass), static_cast<void*>(dblk), static_cast<void*>(dres)})
+      if (p) (void)hipFree(p);
+    dpos = drd = dmass = nullptr;
+    did = nullptr;
+    dblk = nullptr;
+    dres = nullptr;
+  };
+  InjOutcome res{};
   for (int attempt = 0; attempt < 2; ++attempt) {
-    PM_HIP(hipMalloc(&dpos, 8 * 3 * static_cast<size_t>(out_cap)));
-    PM_HIP(hipMalloc(&drd, 8 * nsr * static_cast<size_t>(out_cap)));
-    PM_HIP(hipMalloc(&did, 8 * static_cast<size_t>(out_cap)));
-    PM_HIP(hipMalloc(&dres, sizeof(InjResult)));
-    st = launch_inject<NF>(jj, inject_mass, max_cand, out_cap, next_id, dpos, drd, did, dres, s);
-    if (st) return st;
-    PM_HIP(hipMemcpyAsync(&res, dres, sizeof(res), hipMemcpyDeviceToHost, s));
-    PM_HIP(hipStreamSynchronize(s));
+    if (hipMalloc(&dpos, 8 * 3 * static_cast<size_t>(out_cap)) != hipSuccess ||
+        hipMalloc(&drd, 8 * nsr * static_cast<size_t>(out_cap)) != hipSuccess ||
+        hipMalloc(&did, 8 * static_cast<size_t>(out_cap)) != hipSuccess ||
+        hipMalloc(&dmass, 8 * static_cast<size_t>(batch)) != hipSuccess ||
+        hipMalloc(&dblk, 4 * static_cast<size_t>(batch / kInjBlock)) != hipSuccess ||
+        hipMalloc(&dres, sizeof(InjResult)) != hipSuccess) {
+      release();
+      set_last_error("injection: device allocation failed");
+      return PELEMP_E_NOMEM;
+    }
+    st = run_inject<NF>(jj, inject_mass, limit, batch, out_cap, next_id, dpos, drd, did, dmass, dblk, dres, s, &res);
+    if (st) {
+      release();
+      return st;
+    }
     if (res.n_out <= out_cap) break;
     // more droplets than staged: the same draws again with room for all
-    (void)hipFree(dpos);
-    (void)hipFree(drd);
-    (void)hipFree(did);
-    (void)hipFree(dres);
-    dpos = drd = nullptr;
-    did = nullptr;
-    dres = nullptr;
+    release();
     out_cap = res.n_out;
   }
-  auto release = [&]() {
-    (void)hipFree(dpos);
-    (void)hipFree(drd);
-    (void)hipFree(did);
-    (void)hipFree(dres);
-  };
   if (res.unfinished) {
     release();
     set_last_error("injection: candidate limit reached before the injected mass");
@@ -1146,6 +1204,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.cstart), 4 * (static_cast<size_t>(c.ncells) + 1));
   st |= alloc(c, reinterpret_cast<void**>(&c.cnt), 4 * static_cast<size_t>(c.ncells));
   st |= alloc(c, reinterpret_cast<void**>(&c.arrival), 4 * c.cap);
+  st |= alloc(c, reinterpret_cast<void**>(&c.long_list), sizeof(int2) * (c.cap / (kSeqSortMax + 1) + 1));
   // mov_lo, stay_cnt, mov_hi in one allocation so one memset clears them
   st |= alloc(c, reinterpret_cast<void**>(&c.mov_lo), 3 * 4 * static_cast<size_t>(c.ncells));
   st |= alloc(c, reinterpret_cast<void**>(&c.key), 4 * c.cap);
@@ -1173,6 +1232,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.n_entries = c.scalars;
   c.total = c.scalars + 1;
   c.err = c.scalars + 2;
+  c.n_long = c.scalars + 3;
   if (c.surr_cnt && hipMemset(c.surr_cnt, 0, 2 * sizeof(unsigned long long)) != hipSuccess) st = PELEMP_E_HIP;
   if (c.sb_ids) {
     // counters of new droplets: next id, next slot, children of the last pass
