@@ -14,7 +14,8 @@
 //   3. scatter: perm[cstart[key] + slot] = position;
 //   4. order:   each cell's perm segment is sorted ascending: segments of up
 //               to kSeqSortMax entries (a few droplets per cell) by one
-//               thread with an insertion sort; longer ones (droplets
+//               thread (insertion sort, or a sorting network in registers);
+//               longer ones (droplets
 //               injected into the few cells of a nozzle, breakup children
 //               of one cell) are listed and sorted by a workgroup each
 //               (sort_long_groups_kernel), so no segment costs O(m^2) on
@@ -176,26 +177,67 @@ __device__ inline void insertion_sort_global(int32_t* __restrict__ v, int32_t a,
 // Segments longer than this are sorted by a workgroup.  The list of such
 // segments holds at most n / (kSeqSortMax + 1) entries.
 constexpr int32_t kSeqSortMax = 32;
-constexpr int kLongSortThreads = 1024;
-constexpr int kLongSortChunk = 4096;  // entries sorted in LDS at a time
-constexpr int kLongSortGrid = 256;    // workgroups working through the list
+constexpr int kLongSortThreads = 256;
+constexpr int kLongSortChunk = 4096;  // most entries sorted in LDS at a time
+constexpr int kLongSortGrid = 2048;   // workgroups working through the list
 
+// perm[a, a + len) for len <= N sorted in registers: all loads issued at
+// once, a bitonic network with compile-time indices, the stores
+template <int N>
+__device__ inline void sort_short(int32_t* __restrict__ perm, int32_t a, int32_t len) {
+  int32_t v[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = (q < len) ? perm[a + q] : INT32_MAX;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int32_t x = v[i], y = v[l];
+          const bool up = (i & k) == 0;
+          v[i] = up ? min(x, y) : max(x, y);
+          v[l] = up ? max(x, y) : min(x, y);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (q < len) perm[a + q] = v[q];
+}
+
+// Sort a short segment in place, or list a long one: every lane of the
+// wavefront calls this (len 0 for none); one atomic per wavefront reserves
+// the list entries of its long segments.
 __device__ inline void sort_or_defer(int32_t* __restrict__ perm, int32_t a, int32_t len, int2* __restrict__ long_list,
                                      int32_t* __restrict__ n_long) {
-  if (len <= kSeqSortMax) {
-    insertion_sort_global(perm, a, a + len);
-    return;
+  if (len > 1 && len <= 4) insertion_sort_global(perm, a, a + len);
+  else if (len > 4 && len <= 8) sort_short<8>(perm, a, len);
+  else if (len > 8 && len <= 16) sort_short<16>(perm, a, len);
+  else if (len > 16 && len <= kSeqSortMax) sort_short<32>(perm, a, len);
+  const bool is_long = len > kSeqSortMax;
+  const unsigned long long bal = __ballot(is_long);
+  if (bal == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(static_cast<long long>(bal)) - 1;
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(n_long, __popcll(bal));
+  base = __shfl(base, leader);
+  if (is_long) {
+    const unsigned long long lower = (lane == 0) ? 0ull : (bal & ((~0ull) >> (64 - lane)));
+    long_list[base + __popcll(lower)] = make_int2(a, len);
   }
-  const int32_t q = atomicAdd(n_long, 1);
-  long_list[q] = make_int2(a, len);
 }
 
 // Sort each listed segment perm[a, a + m) ascending, one workgroup per
-// segment: chunks of kLongSortChunk entries by a bitonic network in LDS,
-// then merges of sorted runs of doubling width through tmp[a, a + m) (each
-// thread merges one slice of the output, found by a merge-path search).
-// Entries are distinct storage positions below INT32_MAX, which pads the
-// last chunk.
+// segment: chunks of up to kLongSortChunk entries by a bitonic network in
+// LDS sized to the chunk (the next power of two, at least 64), then merges
+// of sorted runs of doubling width through tmp[a, a + m) (each thread merges
+// one slice of the output, found by a merge-path search).  Entries are
+// distinct storage positions below INT32_MAX, which pads a chunk.
 __global__ void __launch_bounds__(kLongSortThreads) sort_long_groups_kernel(const int2* __restrict__ long_list,
                                                                              const int32_t* __restrict__ n_long,
                                                                              int32_t* __restrict__ perm,
@@ -208,11 +250,13 @@ __global__ void __launch_bounds__(kLongSortThreads) sort_long_groups_kernel(cons
     const int32_t m = long_list[g].y;
     for (int32_t c0 = 0; c0 < m; c0 += kLongSortChunk) {
       const int32_t len = min(kLongSortChunk, m - c0);
-      for (int e = tid; e < kLongSortChunk; e += kLongSortThreads) sh[e] = (e < len) ? perm[a + c0 + e] : INT32_MAX;
+      int P = 64;
+      while (P < len) P <<= 1;
+      for (int e = tid; e < P; e += kLongSortThreads) sh[e] = (e < len) ? perm[a + c0 + e] : INT32_MAX;
       __syncthreads();
-      for (int k = 2; k <= kLongSortChunk; k <<= 1) {
+      for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-          for (int e = tid; e < kLongSortChunk / 2; e += kLongSortThreads) {
+          for (int e = tid; e < P / 2; e += kLongSortThreads) {
             const int i = 2 * e - (e & (j - 1));  // e with a zero bit inserted at log2(j)
             const int l = i + j;
             const bool up = (i & k) == 0;
@@ -276,13 +320,17 @@ __global__ void __launch_bounds__(256) order_movers_kernel(const int32_t* __rest
                                                             int32_t* __restrict__ perm, int2* __restrict__ long_list,
                                                             int32_t* __restrict__ n_long) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int32_t lo = mov_lo[c];
-  const int32_t hi = mov_hi[c];
-  if (lo < 2 && hi < 2) return;
-  const int32_t a = cstart[c];
-  if (lo > 1) sort_or_defer(perm, a, lo, long_list, n_long);
-  if (hi > 1) sort_or_defer(perm, a + lo + stay_cnt[c], hi, long_list, n_long);
+  int32_t lo = 0, hi = 0, a = 0, h0 = 0;
+  if (c < ncells) {
+    lo = mov_lo[c];
+    hi = mov_hi[c];
+    if (lo > 1 || hi > 1) {
+      a = cstart[c];
+      h0 = a + lo + stay_cnt[c];
+    }
+  }
+  sort_or_defer(perm, a, lo, long_list, n_long);
+  sort_or_defer(perm, h0, hi, long_list, n_long);
 }
 
 // Sort each cell's segment of perm ascending (a segment holds the droplets
@@ -292,10 +340,12 @@ __global__ void __launch_bounds__(256) order_cells_kernel(const int32_t* __restr
                                                            int32_t* __restrict__ perm, int2* __restrict__ long_list,
                                                            int32_t* __restrict__ n_long) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int32_t a = cstart[c];
-  const int32_t b = cstart[c + 1];
-  if (b - a > 1) sort_or_defer(perm, a, b - a, long_list, n_long);
+  int32_t a = 0, len = 0;
+  if (c < ncells) {
+    a = cstart[c];
+    len = cstart[c + 1] - a;
+  }
+  sort_or_defer(perm, a, len, long_list, n_long);
 }
 
 }  // namespace pelemp
