@@ -174,12 +174,14 @@ __device__ inline void insertion_sort_global(int32_t* __restrict__ v, int32_t a,
   }
 }
 
-// Segments longer than this are sorted by a workgroup.  The list of such
-// segments holds at most n / (kSeqSortMax + 1) entries.
-constexpr int32_t kSeqSortMax = 32;
+// Segments of up to kSeqSortMax entries are sorted by their thread, up to
+// kWaveSortMax by their wavefront; longer ones are listed and sorted by a
+// workgroup each.  The list holds at most n / (kWaveSortMax + 1) entries.
+constexpr int32_t kSeqSortMax = 16;
+constexpr int32_t kWaveSortMax = 64;  // segments up to this length: sorted by the wavefront that finds them
 constexpr int kLongSortThreads = 256;
 constexpr int kLongSortChunk = 4096;  // most entries sorted in LDS at a time
-constexpr int kLongSortGrid = 2048;   // workgroups working through the list
+constexpr int kLongSortGrid = 1024;   // workgroups working through the list
 
 // perm[a, a + len) for len <= N sorted in registers: all loads issued at
 // once, a bitonic network with compile-time indices, the stores
@@ -209,19 +211,42 @@ __device__ inline void sort_short(int32_t* __restrict__ perm, int32_t a, int32_t
     if (q < len) perm[a + q] = v[q];
 }
 
-// Sort a short segment in place, or list a long one: every lane of the
-// wavefront calls this (len 0 for none); one atomic per wavefront reserves
-// the list entries of its long segments.
+// perm[a, a + m) for m <= 64 sorted by the whole wavefront (uniform a, m):
+// one entry per lane, a bitonic network over lanes
+__device__ inline void wave_sort64(int32_t* __restrict__ perm, int32_t a, int32_t m, int lane) {
+  int32_t v = (lane < m) ? perm[a + lane] : INT32_MAX;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int32_t w = __shfl_xor(v, j);
+      const bool lower = (lane & j) == 0;
+      const bool up = (lane & k) == 0;
+      v = (lower == up) ? min(v, w) : max(v, w);
+    }
+  }
+  if (lane < m) perm[a + lane] = v;
+}
+
+// Sort a segment in place, or list a long one: every lane of the wavefront
+// calls this (len 0 for none).  Segments of kSeqSortMax < len <=
+// kWaveSortMax are taken one after the other by the whole wavefront; one
+// atomic per wavefront reserves the list entries of its long segments.
 __device__ inline void sort_or_defer(int32_t* __restrict__ perm, int32_t a, int32_t len, int2* __restrict__ long_list,
                                      int32_t* __restrict__ n_long) {
   if (len > 1 && len <= 4) insertion_sort_global(perm, a, a + len);
   else if (len > 4 && len <= 8) sort_short<8>(perm, a, len);
-  else if (len > 8 && len <= 16) sort_short<16>(perm, a, len);
-  else if (len > 16 && len <= kSeqSortMax) sort_short<32>(perm, a, len);
-  const bool is_long = len > kSeqSortMax;
+  else if (len > 8 && len <= kSeqSortMax) sort_short<16>(perm, a, len);
+  const int lane = threadIdx.x & 63;
+  unsigned long long wb = __ballot(len > kSeqSortMax && len <= kWaveSortMax);
+  while (wb != 0ull) {
+    const int src = __ffsll(static_cast<long long>(wb)) - 1;
+    wave_sort64(perm, __shfl(a, src), __shfl(len, src), lane);
+    wb &= wb - 1ull;
+  }
+  const bool is_long = len > kWaveSortMax;
   const unsigned long long bal = __ballot(is_long);
   if (bal == 0ull) return;
-  const int lane = threadIdx.x & 63;
   const int leader = __ffsll(static_cast<long long>(bal)) - 1;
   int32_t base = 0;
   if (lane == leader) base = atomicAdd(n_long, __popcll(bal));
@@ -232,12 +257,13 @@ __device__ inline void sort_or_defer(int32_t* __restrict__ perm, int32_t a, int3
   }
 }
 
-// Sort each listed segment perm[a, a + m) ascending, one workgroup per
-// segment: chunks of up to kLongSortChunk entries by a bitonic network in
-// LDS sized to the chunk (the next power of two, at least 64), then merges
-// of sorted runs of doubling width through tmp[a, a + m) (each thread merges
-// one slice of the output, found by a merge-path search).  Entries are
-// distinct storage positions below INT32_MAX, which pads a chunk.
+// Sort each listed segment perm[a, a + m) ascending (m > kWaveSortMax), one
+// workgroup each: chunks
+// of up to kLongSortChunk entries by a bitonic network in LDS sized to the
+// chunk (the next power of two), then merges of sorted runs of doubling
+// width through tmp[a, a + m) (each thread merges one slice of the output,
+// found by a merge-path search).  Entries are distinct storage positions
+// below INT32_MAX, which pads a network.
 __global__ void __launch_bounds__(kLongSortThreads) sort_long_groups_kernel(const int2* __restrict__ long_list,
                                                                              const int32_t* __restrict__ n_long,
                                                                              int32_t* __restrict__ perm,
@@ -250,7 +276,7 @@ __global__ void __launch_bounds__(kLongSortThreads) sort_long_groups_kernel(cons
     const int32_t m = long_list[g].y;
     for (int32_t c0 = 0; c0 < m; c0 += kLongSortChunk) {
       const int32_t len = min(kLongSortChunk, m - c0);
-      int P = 64;
+      int P = 128;
       while (P < len) P <<= 1;
       for (int e = tid; e < P; e += kLongSortThreads) sh[e] = (e < len) ? perm[a + c0 + e] : INT32_MAX;
       __syncthreads();

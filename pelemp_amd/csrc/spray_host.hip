@@ -1204,7 +1204,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.cstart), 4 * (static_cast<size_t>(c.ncells) + 1));
   st |= alloc(c, reinterpret_cast<void**>(&c.cnt), 4 * static_cast<size_t>(c.ncells));
   st |= alloc(c, reinterpret_cast<void**>(&c.arrival), 4 * c.cap);
-  st |= alloc(c, reinterpret_cast<void**>(&c.long_list), sizeof(int2) * (c.cap / (kSeqSortMax + 1) + 1));
+  st |= alloc(c, reinterpret_cast<void**>(&c.long_list), sizeof(int2) * (c.cap / (kWaveSortMax + 1) + 1));
   // mov_lo, stay_cnt, mov_hi in one allocation so one memset clears them
   st |= alloc(c, reinterpret_cast<void**>(&c.mov_lo), 3 * 4 * static_cast<size_t>(c.ncells));
   st |= alloc(c, reinterpret_cast<void**>(&c.key), 4 * c.cap);
