@@ -10,7 +10,8 @@
 // the order in which the GPU threads run:
 //   1. count:   cnt[key]++ (integer atomics), remembering each droplet's
 //               arrival slot;
-//   2. scan:    cstart = exclusive prefix sum of cnt (three-phase block scan);
+//   2. scan:    cstart = exclusive prefix sum of cnt (block scans, then each
+//               block adds the sum of the block totals before it);
 //   3. scatter: perm[cstart[key] + slot] = position;
 //   4. order:   each cell's perm segment is sorted ascending: segments of up
 //               to kSeqSortMax entries (a few droplets per cell) by one
@@ -63,43 +64,29 @@ __global__ void __launch_bounds__(kScanBlock) scan_block_kernel(const int32_t* _
   if (threadIdx.x == kScanBlock - 1) block_sum[blockIdx.x] = sh[threadIdx.x];
 }
 
-// Phase 2: exclusive scan of the block totals (single block, loops).
-__global__ void __launch_bounds__(kScanBlock) scan_sums_kernel(int32_t* __restrict__ block_sum,
-                                                                int32_t nblocks,
-                                                                int32_t* __restrict__ total) {
-  __shared__ int32_t sh[kScanBlock];
-  int32_t carry = 0;
-  for (int32_t base = 0; base < nblocks; base += kScanBlock) {
-    const int32_t i = base + threadIdx.x;
-    const int32_t v = (i < nblocks) ? block_sum[i] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kScanBlock; off <<= 1) {
-      const int32_t t = (threadIdx.x >= static_cast<unsigned>(off)) ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nblocks) block_sum[i] = carry + sh[threadIdx.x] - v;
-    const int32_t tot = sh[kScanBlock - 1];
-    __syncthreads();
-    carry += tot;
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
-
-// Phase 3: add block offsets; cstart[n] = total.
-// (also clears the count of long segments that the order step lists)
+// Phase 2: each block adds the sum of the totals of the blocks before it
+// (integer sums: exact in any order); the last block writes out[n] = total.
+// Also clears the count of long segments that the order step lists.
 __global__ void __launch_bounds__(kScanBlock) scan_add_kernel(int32_t* __restrict__ out, int32_t n,
                                                                const int32_t* __restrict__ block_sum,
-                                                               const int32_t* __restrict__ total,
+                                                               int32_t nblocks, int32_t* __restrict__ total,
                                                                int32_t* __restrict__ n_long) {
+  __shared__ int32_t red[kScanBlock / 64];
+  int32_t part = 0;
+  for (int32_t b = threadIdx.x; b < static_cast<int32_t>(blockIdx.x); b += kScanBlock) part += block_sum[b];
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  int32_t base = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w) base += red[w];
   const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-  if (i < n) out[i] += block_sum[blockIdx.x];
-  if (i == 0) {
-    out[n] = *total;
-    *n_long = 0;
+  if (i < n) out[i] += base;
+  if (threadIdx.x == 0 && static_cast<int32_t>(blockIdx.x) == nblocks - 1) {
+    const int32_t tot = base + block_sum[nblocks - 1];
+    out[n] = tot;
+    *total = tot;
   }
+  if (i == 0) *n_long = 0;
 }
 
 __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __restrict__ key,
@@ -128,12 +115,21 @@ __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __res
 // the two mover groups (usually empty or one droplet) need ordering.
 constexpr int32_t kArrRankMask = (1 << 30) - 1;
 
-// cnt = mov_lo + stay_cnt + mov_hi, per cell
+// cnt = mov_lo + stay_cnt + mov_hi, per cell.  Thread 0 also sets the
+// entry count of the new buffer (n_src, or the previous sort's total) and
+// logs the previous total (the droplets at the pass start) into log_slot.
 __global__ void __launch_bounds__(256) sum3_kernel(const int32_t* __restrict__ a, const int32_t* __restrict__ b,
                                                     const int32_t* __restrict__ c, int32_t n,
-                                                    int32_t* __restrict__ out) {
+                                                    int32_t* __restrict__ out, const int32_t* __restrict__ prev_total,
+                                                    const int32_t* __restrict__ n_src,
+                                                    int32_t* __restrict__ n_entries, int32_t* __restrict__ log_slot) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] + b[i] + c[i];
+  if (i == 0) {
+    const int32_t pt = *prev_total;
+    *n_entries = n_src ? *n_src : pt;
+    if (log_slot) *log_slot = pt;
+  }
 }
 
 __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __restrict__ key,
@@ -338,11 +334,12 @@ __global__ void __launch_bounds__(kLongSortThreads) sort_long_groups_kernel(cons
   }
 }
 
-// order the mover groups of each cell by storage position
+// order the mover groups of each cell by storage position; the cell's
+// counters are cleared for the next pass
 __global__ void __launch_bounds__(256) order_movers_kernel(const int32_t* __restrict__ cstart,
-                                                            const int32_t* __restrict__ mov_lo,
-                                                            const int32_t* __restrict__ stay_cnt,
-                                                            const int32_t* __restrict__ mov_hi, int32_t ncells,
+                                                            int32_t* __restrict__ mov_lo,
+                                                            int32_t* __restrict__ stay_cnt,
+                                                            int32_t* __restrict__ mov_hi, int32_t ncells,
                                                             int32_t* __restrict__ perm, int2* __restrict__ long_list,
                                                             int32_t* __restrict__ n_long) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,22 +351,31 @@ __global__ void __launch_bounds__(256) order_movers_kernel(const int32_t* __rest
       a = cstart[c];
       h0 = a + lo + stay_cnt[c];
     }
+    // the counters start the next pass at zero
+    mov_lo[c] = 0;
+    stay_cnt[c] = 0;
+    mov_hi[c] = 0;
   }
   sort_or_defer(perm, a, lo, long_list, n_long);
   sort_or_defer(perm, h0, hi, long_list, n_long);
 }
 
 // Sort each cell's segment of perm ascending (a segment holds the droplets
-// of one cell).
+// of one cell); the cell's pass counters (mov_lo, stay_cnt, mov_hi, each
+// ncells long from `counters`) are cleared for the next pass.
 __global__ void __launch_bounds__(256) order_cells_kernel(const int32_t* __restrict__ cstart,
                                                            int32_t ncells,
                                                            int32_t* __restrict__ perm, int2* __restrict__ long_list,
-                                                           int32_t* __restrict__ n_long) {
+                                                           int32_t* __restrict__ n_long,
+                                                           int32_t* __restrict__ counters) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   int32_t a = 0, len = 0;
   if (c < ncells) {
     a = cstart[c];
     len = cstart[c + 1] - a;
+    counters[c] = 0;
+    counters[ncells + c] = 0;
+    counters[2 * static_cast<size_t>(ncells) + c] = 0;
   }
   sort_or_defer(perm, a, len, long_list, n_long);
 }

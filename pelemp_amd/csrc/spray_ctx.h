@@ -74,6 +74,7 @@ struct Ctx {
   int32_t* arrival = nullptr;
   int2* long_list = nullptr;  // (start, length) of perm segments too long for one thread
   int32_t* n_long = nullptr;  // entries of long_list (in scalars)
+  bool counters_dirty = true;  // mov_lo / stay_cnt / mov_hi not cleared since the last update pass
   int32_t* mov_lo = nullptr;    // per cell: droplets arriving from lower keys
   int32_t* mov_hi = nullptr;    // per cell: droplets arriving from higher keys
   int32_t* stay_cnt = nullptr;  // per cell: droplets that stayed

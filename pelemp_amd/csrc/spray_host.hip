@@ -369,13 +369,13 @@ static int run_sort(Ctx& c, hipStream_t s, bool keep_entries = false) {
   hipLaunchKernelGGL(count_keys_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.cnt, c.arrival);
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
   hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, c.ncells, c.cstart, c.block_sum);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.block_sum, nb, c.total);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total,
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
                      c.n_long);
   hipLaunchKernelGGL(scatter_perm_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.perm);
   hipLaunchKernelGGL(order_cells_kernel, dim3((c.ncells + 255) / 256), dim3(256), 0, s, c.cstart, c.ncells,
-                     c.perm, c.long_list, c.n_long);
+                     c.perm, c.long_list, c.n_long, c.mov_lo);
+  c.counters_dirty = false;
   // arrival is free once the scatter has read it: the merge buffer
   hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
                      c.n_long, c.perm, c.arrival);
@@ -504,22 +504,22 @@ static int run_eb_redirect(Ctx& c, const PassParams& p, hipStream_t s) {
 // Re-sort after an update pass (incremental form, see sort_kernels.h).
 // n_src: device count of entries in the new buffer when it differs from the
 // previous sort's count (secondary droplets appended).
-static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src = nullptr) {
+// log_slot: where the droplet count at the pass start is logged (or null).
+static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src, int32_t* log_slot) {
   hipEvent_t ev0;
   ctx_time_begin(c, 1, s, &ev0);
-  PM_HIP(hipMemcpyAsync(c.n_entries, n_src ? n_src : c.cstart + c.ncells, sizeof(int32_t),
-                        hipMemcpyDeviceToDevice, s));
   const unsigned gc = static_cast<unsigned>((c.ncells + 255) / 256);
-  hipLaunchKernelGGL(sum3_kernel, dim3(gc), dim3(256), 0, s, c.mov_lo, c.stay_cnt, c.mov_hi, c.ncells, c.cnt);
+  hipLaunchKernelGGL(sum3_kernel, dim3(gc), dim3(256), 0, s, c.mov_lo, c.stay_cnt, c.mov_hi, c.ncells, c.cnt,
+                     c.cstart + c.ncells, n_src, c.n_entries, log_slot);
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
   hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, c.ncells, c.cstart, c.block_sum);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, s, c.block_sum, nb, c.total);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, c.total,
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
                      c.n_long);
   hipLaunchKernelGGL(scatter_incr_kernel, dim3(2048), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.mov_lo, c.stay_cnt, c.perm);
   hipLaunchKernelGGL(order_movers_kernel, dim3(gc), dim3(256), 0, s, c.cstart, c.mov_lo, c.stay_cnt, c.mov_hi,
                      c.ncells, c.perm, c.long_list, c.n_long);
+  c.counters_dirty = false;
   hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
                      c.n_long, c.perm, c.arrival);
   PM_HIP(hipGetLastError());
@@ -2098,12 +2098,11 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
     if (stc != PELEMP_OK) return stc;
   }
   for (int it = 0; it < num_iter; ++it) {
-    if (c.npass < Ctx::kCountLog) {
-      PM_HIP(hipMemcpyAsync(c.count_log + c.npass, c.cstart + c.ncells, sizeof(int32_t),
-                            hipMemcpyDeviceToDevice, s));
-    }
+    int32_t* log_slot = (c.npass < Ctx::kCountLog) ? c.count_log + c.npass : nullptr;
     ++c.npass;
-    PM_HIP(hipMemsetAsync(c.mov_lo, 0, 3 * 4 * static_cast<size_t>(c.ncells), s));
+    // the last sort cleared the pass counters, unless a pass stopped early
+    if (c.counters_dirty) PM_HIP(hipMemsetAsync(c.mov_lo, 0, 3 * 4 * static_cast<size_t>(c.ncells), s));
+    c.counters_dirty = true;
     if (p.do_splash_box) {
       hipEvent_t evf;
       ctx_time_begin(c, 2, s, &evf);
@@ -2130,7 +2129,7 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
       ctx_time_end(c, 3, s, evc);
     }
     c.cur = 1 - c.cur;
-    const int st2 = run_sort_incremental(c, s, p.make_new_drops ? c.n_new : nullptr);
+    const int st2 = run_sort_incremental(c, s, p.make_new_drops ? c.n_new : nullptr, log_slot);
     if (st2 != PELEMP_OK) return st2;
   }
   return PELEMP_OK;
