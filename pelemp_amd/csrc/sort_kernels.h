@@ -44,14 +44,28 @@ __global__ void __launch_bounds__(256) count_keys_kernel(const uint32_t* __restr
   }
 }
 
-// Phase 1: per-block inclusive scan of cnt into out (exclusive form written)
-// and block totals.
-__global__ void __launch_bounds__(kScanBlock) scan_block_kernel(const int32_t* __restrict__ cnt,
-                                                                 int32_t n, int32_t* __restrict__ out,
-                                                                 int32_t* __restrict__ block_sum) {
+// Phase 1: per-block inclusive scan of the per-cell counts into out
+// (exclusive form written) and block totals.  The count of a cell is c0, or
+// c0 + c1 + c2 when c1 is given (the three groups of an incremental pass).
+// With n_entries given, thread 0 also sets the entry count of the new buffer
+// (n_src, or the previous sort's total out[n]) and logs the previous total
+// (the droplets at the pass start) into log_slot when that is given.
+__global__ void __launch_bounds__(kScanBlock) scan_block_kernel(const int32_t* __restrict__ c0,
+                                                                 const int32_t* __restrict__ c1,
+                                                                 const int32_t* __restrict__ c2, int32_t n,
+                                                                 int32_t* __restrict__ out,
+                                                                 int32_t* __restrict__ block_sum,
+                                                                 const int32_t* __restrict__ n_src,
+                                                                 int32_t* __restrict__ n_entries,
+                                                                 int32_t* __restrict__ log_slot) {
   __shared__ int32_t sh[kScanBlock];
   const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-  const int32_t v = (i < n) ? cnt[i] : 0;
+  const int32_t v = (i < n) ? (c1 ? c0[i] + c1[i] + c2[i] : c0[i]) : 0;
+  if (i == 0 && n_entries) {
+    const int32_t pt = out[n];
+    *n_entries = n_src ? *n_src : pt;
+    if (log_slot) *log_slot = pt;
+  }
   sh[threadIdx.x] = v;
   __syncthreads();
   for (int off = 1; off < kScanBlock; off <<= 1) {
@@ -114,23 +128,6 @@ __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __res
 // order is [movers from lower keys][stayers][movers from higher keys]; only
 // the two mover groups (usually empty or one droplet) need ordering.
 constexpr int32_t kArrRankMask = (1 << 30) - 1;
-
-// cnt = mov_lo + stay_cnt + mov_hi, per cell.  Thread 0 also sets the
-// entry count of the new buffer (n_src, or the previous sort's total) and
-// logs the previous total (the droplets at the pass start) into log_slot.
-__global__ void __launch_bounds__(256) sum3_kernel(const int32_t* __restrict__ a, const int32_t* __restrict__ b,
-                                                    const int32_t* __restrict__ c, int32_t n,
-                                                    int32_t* __restrict__ out, const int32_t* __restrict__ prev_total,
-                                                    const int32_t* __restrict__ n_src,
-                                                    int32_t* __restrict__ n_entries, int32_t* __restrict__ log_slot) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = a[i] + b[i] + c[i];
-  if (i == 0) {
-    const int32_t pt = *prev_total;
-    *n_entries = n_src ? *n_src : pt;
-    if (log_slot) *log_slot = pt;
-  }
-}
 
 __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __restrict__ key,
                                                             const int32_t* __restrict__ n_ptr,
