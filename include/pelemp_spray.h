@@ -296,7 +296,10 @@ int pelemp_spray_read_roi(const char* path, double rho_part, double cd,
  * candidate, keep it if its diameter exceeds the minimum, until the
  * injected mass reaches the target) runs with the candidates drawn in
  * parallel and the mass summed in candidate order.  *n_injected: droplets
- * added.  Waits for `stream`. */
+ * added.  PELEMP_E_BAD_INJECTION for a droplet outside the domain (the
+ * reference's "Bad injection particle" abort), a candidate limit reached
+ * before the mass, or a liquid density at the jet temperature that is not
+ * positive (the reference loop would not end).  Waits for `stream`. */
 int pelemp_spray_inject(pelemp_spray_ctx* ctx, pelemp_spray_jet* jet,
                         double time, double sim_dt, int64_t* n_injected,
                         void* stream);
