@@ -5,7 +5,9 @@
 // droplets in cell-sorted order (the reference adds with atomics) and adds the
 // sums to the FAB; then every cell of the box gets the protected divides
 // (d10 /= num, d32 /= surface, temperature and velocity /= mass, only where
-// the denominator is non-zero).  Variables from start_indx, in the order of
+// the denominator is non-zero).  The cell's variables are loaded once into
+// registers, updated there and stored once (the FAB pointer with a run-time
+// stride would otherwise serialise every load behind the previous store).  Variables from start_indx, in the order of
 // SpraySetup.cpp:213-239: mass, density, num, volume, surface, volume
 // fraction, d10, d32, wall-film height, wall-film mass, temperature,
 // parcels, velocity(3), and for several fuels the liquid mass of each.
@@ -48,15 +50,28 @@ __global__ void __launch_bounds__(256) derive_kernel(const int32_t* __restrict__
     const int k = mf.lo[2] + static_cast<int>(cell / (static_cast<long long>(mf.n[0]) * mf.n[1]));
     double* V = mf.data + start_indx * mf.cstride + cell;
     const int ci = i - g.dlo[0], cj = j - g.dlo[1], ck = k - g.dlo[2];
+    int32_t s0 = 0, s1 = 0;
     if (ci >= 0 && ci < g.ncell[0] && cj >= 0 && cj < g.ncell[1] && ck >= 0 && ck < g.ncell[2]) {
       const uint32_t key = tile_key(ci, cj, ck, g.ntile);
-      const int32_t s0 = cstart[key], s1 = cstart[key + 1];
-      if (s1 > s0) {
+      s0 = cstart[key];
+      s1 = cstart[key + 1];
+    }
+    const bool has = s1 > s0;
+    if (!has && !(ntot > 0)) continue;
+    // the cell's variables: loads issued before the droplet loop, so they
+    // overlap with it
+    double W[kDeriveBase + NF];
+#pragma unroll
+    for (int q = 0; q < kDeriveBase + NF; ++q) W[q] = (q < nvar) ? V[q * mf.cstride] : 0.;
+    {
+      if (has) {
         double acc[kDeriveBase + NF];
 #pragma unroll
         for (int q = 0; q < kDeriveBase + NF; ++q) acc[q] = 0.;
+        int32_t o_next = perm[s0];
         for (int32_t s = s0; s < s1; ++s) {
-          const int32_t o = perm[s];
+          const int32_t o = o_next;
+          if (s + 1 < s1) o_next = perm[s + 1];
           double T_part, dia_part, num_ppp, film_hght, vel[3], Y[NF];
           if constexpr (sizeof(R) == 8) {
             using F = Fields<NF>;
@@ -113,19 +128,22 @@ __global__ void __launch_bounds__(256) derive_kernel(const int32_t* __restrict__
         }
 #pragma unroll
         for (int q = 0; q < kDeriveBase + NF; ++q)
-          if (q < nvar) V[q * mf.cstride] = V[q * mf.cstride] + (g.pelec ? acc[q] * g.scale[q] : acc[q]);
+          if (q < nvar) W[q] = W[q] + (g.pelec ? acc[q] * g.scale[q] : acc[q]);
       }
     }
     if (ntot > 0) {
-      if (V[num_i * mf.cstride] != 0.) V[d10_i * mf.cstride] = V[d10_i * mf.cstride] / V[num_i * mf.cstride];
-      if (V[surf_i * mf.cstride] != 0.) V[d32_i * mf.cstride] = V[d32_i * mf.cstride] / V[surf_i * mf.cstride];
-      const double mass = V[mass_i * mf.cstride];
+      if (W[num_i] != 0.) W[d10_i] = W[d10_i] / W[num_i];
+      if (W[surf_i] != 0.) W[d32_i] = W[d32_i] / W[surf_i];
+      const double mass = W[mass_i];
       if (mass != 0.) {
-        V[temp_i * mf.cstride] = V[temp_i * mf.cstride] / mass;
+        W[temp_i] = W[temp_i] / mass;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) V[(vel_i + d) * mf.cstride] = V[(vel_i + d) * mf.cstride] / mass;
+        for (int d = 0; d < 3; ++d) W[vel_i + d] = W[vel_i + d] / mass;
       }
     }
+#pragma unroll
+    for (int q = 0; q < kDeriveBase + NF; ++q)
+      if (q < nvar) V[q * mf.cstride] = W[q];
   }
 }
 
