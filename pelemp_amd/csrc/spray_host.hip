@@ -1140,6 +1140,18 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
     set_last_error("breakup needs fuel_sigma");  // SpraySetup.cpp:157-165
     return PELEMP_E_BAD_CONFIG;
   }
+  {
+    // the deposit writes each source quantity to its own component
+    // (SprayComps: momentum, density, fuel species, energy)
+    std::vector<int> sc = {comps->momSrcIndx, comps->momSrcIndx + 1, comps->momSrcIndx + 2, comps->rhoSrcIndx,
+                           comps->engSrcIndx};
+    for (int i = 0; i < sd->nfuel; ++i) sc.push_back(comps->specSrcIndx + i);
+    std::sort(sc.begin(), sc.end());
+    if (sc.front() < 0 || std::adjacent_find(sc.begin(), sc.end()) != sc.end()) {
+      set_last_error("SprayComps: the source components overlap or are negative");
+      return PELEMP_E_BAD_CONFIG;
+    }
+  }
   auto* h = new pelemp_spray_ctx();
   Ctx& c = h->c;
   c.mech = mech;

@@ -969,9 +969,14 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 
   if (t < kTileCells) a.stay_cnt[tile * kTileCells + t] = srank;
 
-  // add each cell's sums to the source FAB
+  // add each cell's sums to the source FAB: a thread's items are distinct
+  // (cell, component) words (the host checks that the source components do
+  // not overlap), so all their loads go out before the stores
+  double* dst[NITEM];
+  double old[NITEM];
 #pragma unroll
   for (int r = 0; r < NITEM; ++r) {
+    dst[r] = nullptr;
     const int item = t + r * kUpdThreads;
     if (item < kTileCells * NSRC) {
       const int c = item / NSRC, q = item % NSRC;
@@ -987,11 +992,18 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
           if (gx < 0 || gx >= a.src.n[0] || gy < 0 || gy >= a.src.n[1] || gz < 0 || gz >= a.src.n[2]) {
             if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
           } else {
-            double* dst = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
-            *dst = a.pelec ? *dst + acc[r] * a.src_scale[q] : *dst + acc[r];
+            dst[r] = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
           }
         }
       }
+    }
+    old[r] = dst[r] ? *dst[r] : 0.;
+  }
+#pragma unroll
+  for (int r = 0; r < NITEM; ++r) {
+    if (dst[r]) {
+      const int q = (t + r * kUpdThreads) % NSRC;
+      *dst[r] = a.pelec ? old[r] + acc[r] * a.src_scale[q] : old[r] + acc[r];
     }
   }
   if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
