@@ -80,3 +80,19 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
 def test_device_count_without_gpu_is_clean():
     # on a host without a GPU this returns 0 devices rather than failing
     assert C.device_count() >= 0
+
+
+def test_create_refuses_overlapping_source_components():
+    # the checks before any device call: each source quantity needs its own
+    # component (the deposit adds every quantity of a cell to its own word)
+    from pelemp_amd import problem as P
+
+    prob = P.config2(ncell=8)
+    comps = dict(prob.comps)
+    comps["engSrcIndx"] = comps["rhoSrcIndx"]
+    h = ctypes.c_void_p()
+    sd, g, cm = C.spray_data_c(prob.sdata), C.geom_c(prob.geom), C.comps_c(comps)
+    st = C.lib().pelemp_spray_create(ctypes.byref(h), int(prob.mech), 0, ctypes.byref(sd), ctypes.byref(cm),
+                                     ctypes.byref(g), 16, 1)
+    assert st == C.E_BAD_CONFIG and not h.value
+    assert b"overlap" in C.lib().pelemp_last_error()
