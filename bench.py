@@ -61,7 +61,8 @@ def run_form(prob, form: int, steps: int, warmup: int, stream, acc_steps: int = 
     h = stream.handle
 
     def step():
-        C.check(C.lib().pelemp_memset_device(src.buf.ptr, 0, src.buf.nbytes), "memset source")
+        # the caller's setVal(0.) of the source, in stream order
+        C.check(C.lib().pelemp_fab_set_zero(src.buf.ptr, src.buf.nbytes // 8, h), "zero source")
         pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
                            True, None, -1.0, stream=h, sync=False)
 
@@ -134,7 +135,8 @@ def run_branches(steps: int, warmup: int, stream, cap_factor: float = 5.0):
     n0 = prob.n
 
     def step():
-        C.check(C.lib().pelemp_memset_device(src.buf.ptr, 0, src.buf.nbytes), "memset source")
+        # the caller's setVal(0.) of the source, in stream order
+        C.check(C.lib().pelemp_fab_set_zero(src.buf.ptr, src.buf.nbytes // 8, h), "zero source")
         pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
                            True, None, -1.0, stream=h, sync=False)
 

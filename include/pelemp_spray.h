@@ -556,6 +556,11 @@ int pelemp_memcpy_htod(void* dst, const void* src, int64_t bytes);
 int pelemp_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
 int pelemp_memcpy_dtod(void* dst, const void* src, int64_t bytes);
 int pelemp_memset_device(void* ptr, int32_t value, int64_t bytes);
+/* FP64 words [ptr, ptr + n_words) set to 0 by a kernel on `stream`, in
+ * stream order (what a caller's MultiFab::setVal(0.) of the spray source
+ * does between steps); a kernel, not a memset, so it may sit in a captured
+ * graph. */
+int pelemp_fab_set_zero(double* ptr, int64_t n_words, void* stream);
 int pelemp_device_synchronize(void);
 int pelemp_stream_create(void** stream);
 int pelemp_stream_destroy(void* stream);

@@ -196,6 +196,7 @@ _PROTOS = {
     "pelemp_memcpy_dtoh": (_c_i32, [_vp, _vp, _c_i64]),
     "pelemp_memcpy_dtod": (_c_i32, [_vp, _vp, _c_i64]),
     "pelemp_memset_device": (_c_i32, [_vp, _c_i32, _c_i64]),
+    "pelemp_fab_set_zero": (_c_i32, [_vp, _c_i64, _vp]),
     "pelemp_device_synchronize": (_c_i32, []),
     "pelemp_stream_create": (_c_i32, [ctypes.POINTER(_vp)]),
     "pelemp_stream_destroy": (_c_i32, [_vp]),

@@ -2522,6 +2522,19 @@ int pelemp_memset_device(void* ptr, int32_t value, int64_t bytes) {
   return PELEMP_OK;
 }
 
+int pelemp_fab_set_zero(double* ptr, int64_t n_words, void* stream) {
+  if (n_words < 0 || (n_words > 0 && !ptr)) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  if (n_words == 0) return PELEMP_OK;
+  const unsigned grid = static_cast<unsigned>(std::min<long long>((n_words + 255) / 256, 16384));
+  hipLaunchKernelGGL(zero_fill_kernel, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), ptr,
+                     static_cast<long long>(n_words));
+  PM_HIP(hipGetLastError());
+  return PELEMP_OK;
+}
+
 int pelemp_device_synchronize(void) {
   PM_HIP(hipDeviceSynchronize());
   return PELEMP_OK;

@@ -134,3 +134,21 @@ def test_derived_vars_fp32_form_totals():
     for q in (0, 2, 3, 4, 11, 15, 16):
         assert got[q].sum() == pytest.approx(ref[q].sum(), rel=1e-12)
     pc.close()
+
+
+def test_fab_set_zero_in_stream_order():
+    # the source reset between steps: a kernel on the caller's stream that
+    # clears exactly the words asked for
+    from pelemp_amd import _capi as C
+    from pelemp_amd.spray import DeviceFab
+
+    rng = np.random.default_rng(5)
+    data = rng.normal(size=(3, 6, 5, 4)) + 10.0
+    fab = DeviceFab((0, 0, 0), (3, 4, 5), 3, data)
+    s = C.Stream()
+    n = 2 * 6 * 5 * 4 + 7  # two components and a few words of the third
+    C.check(C.lib().pelemp_fab_set_zero(fab.buf.ptr, n, s.handle), "zero")
+    s.synchronize()
+    got = fab.numpy().reshape(-1)
+    assert not got[:n].any()
+    assert np.array_equal(got[n:], data.reshape(-1)[n:])
