@@ -433,7 +433,11 @@ int oracle_inject(const pelemp_spray_data* sd_in, const pelemp_geom* g_in, pelem
   }
   std::vector<double> p, r;
   std::vector<int64_t> id;
-  const int st = oracle::spray_injection(sd, g, *jet, time, sim_dt, next_id, pelec != 0, p, r, id);
+  // the jet's state changes only when the droplets are handed out: a call
+  // that reports a too small output buffer leaves it as it was, so the
+  // caller's retry repeats the same injection
+  pelemp_spray_jet jc = *jet;
+  const int st = oracle::spray_injection(sd, g, jc, time, sim_dt, next_id, pelec != 0, p, r, id);
   if (st) return st;
   const int64_t m = static_cast<int64_t>(id.size());
   const int nsr = 10 + sd.nfuel;
@@ -448,6 +452,7 @@ int oracle_inject(const pelemp_spray_data* sd_in, const pelemp_geom* g_in, pelem
     ids[i] = id[i];
   }
   *n = m;
+  *jet = jc;
   return PELEMP_OK;
 }
 

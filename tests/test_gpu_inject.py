@@ -106,6 +106,27 @@ def test_injection_bitwise(case):
     pc.close()
 
 
+def test_injection_needing_a_second_batch_bitwise():
+    # a narrow distribution just above the smallest diameter (0.65 um): about
+    # 30% of the candidates are rejected, so one call draws ~86k candidates,
+    # more than its first batch (1.05 x the ~69k expected + 4096), and goes on
+    # in a second batch with the running mass and output offset carried over
+    from pelemp_amd.spray import SprayParticleContainer
+
+    prob = P.config2(ncell=16)
+    pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, prob.n + 80000)
+    pc.set_particles(prob.pos, prob.rdata, prob.ids)
+    jet = _make_jet(pc, prob, mass_scale=4e-3, dist_type="Normal", dist_params=(0.68e-6, 0.06e-6), inject_ppp=1.0)
+    p, r, i = _inject_steps(pc, prob, jet, 1, 1e-6)
+    assert len(i) > 50000
+    gp, gr, gi = by_id(*pc.get_particles())
+    new = gi >= i.min()
+    assert np.array_equal(gi[new], i)
+    assert np.array_equal(gp[new], p)
+    assert np.array_equal(gr[new], r)
+    pc.close()
+
+
 def test_injection_then_update_bitwise():
     # inject, update, inject, update: sources and droplets equal the oracle's
     # bit for bit, which needs the new droplets after the old ones in every

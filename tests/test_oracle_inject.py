@@ -326,3 +326,15 @@ def test_injection_log_round_trip(tmp_path):
     # adjacent duplicate names are rejected (std::unique, SprayIO.cpp:114-118)
     dn = (ctypes.c_char_p * 2)(b"jet1", b"jet1")
     assert L.pelemp_spray_read_injection_log(path.encode(), 2, dn, back) == C.E_ARG
+
+
+def test_large_injection_advances_the_jet_once():
+    # more droplets than the wrapper's first output buffer (1024): its retry
+    # with a larger buffer must repeat the same injection, so the jet's call
+    # count goes up by one and its total is the mass of the droplets returned
+    prob = _prob()
+    jet = _jet(prob, mass_flow=0.05)
+    st, pos, rd, ids = O.inject(prob.sdata, prob.geom, jet, 0.0, 1e-6, 1)
+    assert st == 0 and len(ids) > 1024
+    assert jet.n_injections == 1
+    assert jet.total_inj_mass == pytest.approx(_masses(prob.sdata, rd, prob.nf).sum(), rel=1e-12)
