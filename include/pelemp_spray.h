@@ -499,6 +499,26 @@ int pelemp_mlp_loss_grad(const float* params, const pelemp_surrogate_norm* norm,
                          const float* t, const float* w, float* grad,
                          double* loss, void* stream);
 
+/* pelemp_mlp_loss_grad without any device allocation.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight training or serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and waiting behaviour as
+ * pelemp_mlp_loss_grad, but the batch workspace (layer inputs, deltas,
+ * per-sample losses, per-chunk gradient partials, loss accumulator) is one
+ * statically sized device array of PELEMP_MLP_NOALLOC_SCRATCH_BYTES per
+ * device, held until the process ends, so n may not exceed
+ * PELEMP_MLP_NOALLOC_MAX_BATCH (PELEMP_E_ARG otherwise).  Every call on a
+ * device shares that device's workspace: calls that overlap on one device
+ * (several host threads or streams) race and must be serialised by the
+ * caller. */
+#define PELEMP_MLP_NOALLOC_MAX_BATCH 4096
+#define PELEMP_MLP_NOALLOC_SCRATCH_BYTES 10298112
+int pelemp_mlp_loss_grad_noalloc(const float* params,
+                                 const pelemp_surrogate_norm* norm,
+                                 int32_t quant, int64_t n, const float* x,
+                                 const float* t, const float* w, float* grad,
+                                 double* loss, void* stream);
+
 typedef struct pelemp_mlp_train_cfg {
   int32_t epochs;
   int32_t batch;        /* <= 65536 */

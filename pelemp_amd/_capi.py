@@ -94,6 +94,7 @@ class GeomC(ctypes.Structure):
 SURR_NPARAM = 13251
 SURR_NIN = 8
 SURR_NOUT = 3
+MLP_NOALLOC_MAX_BATCH = 4096  # PELEMP_MLP_NOALLOC_MAX_BATCH
 _c_flt = ctypes.c_float
 
 
@@ -183,6 +184,8 @@ _PROTOS = {
     "pelemp_mlp_forward": (_c_i32, [_vp, _PNORM, _c_i32, _c_i64, _vp, _vp, _vp]),
     "pelemp_mlp_loss_grad": (_c_i32, [_vp, _PNORM, _c_i32, _c_i64, _vp, _vp, _vp, _vp,
                                       ctypes.POINTER(_c_dbl), _vp]),
+    "pelemp_mlp_loss_grad_noalloc": (_c_i32, [_vp, _PNORM, _c_i32, _c_i64, _vp, _vp, _vp, _vp,
+                                              ctypes.POINTER(_c_dbl), _vp]),
     "pelemp_mlp_train": (_c_i32, [_vp, _PNORM, _c_i64, _vp, _vp, _vp, ctypes.POINTER(MlpTrainCfgC), _vp, _vp]),
     "pelemp_mlp_infer_bf16": (_c_i32, [_vp, _PNORM, _c_i64, _vp, _vp, _vp]),
     "pelemp_spray_set_surrogate": (_c_i32, [_vp, _vp, _PNORM]),

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -472,11 +473,75 @@ int launch_fwdbwd(const float* params, const MlpNorm& nm, int quant, const Batch
   return hip_check(hipGetLastError(), "mlp launch");
 }
 
+// FP64 words [p, p + n) set to zero (a kernel rather than a memset, so the
+// call can sit in a captured graph).
+__global__ void __launch_bounds__(64) zero_f64_kernel(double* __restrict__ p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.;
+}
+
 }  // namespace
 
 }  // namespace pelemp
 
 using namespace pelemp;
+
+// ---- workspace of the allocation-free loss/gradient call --------------------
+// One statically sized workspace per device, for callers that may not call
+// hipMalloc (graph capture, tight loops): the Workspace of a batch of at most
+// kNoAllocMaxBatch samples plus the loss accumulator, each array aligned to
+// 256 bytes.  Every call on a device shares that device's copy and leaves
+// its contents behind; calls that overlap on one device race.
+namespace {
+constexpr int kNoAllocMaxBatch = PELEMP_MLP_NOALLOC_MAX_BATCH;
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t noalloc_round(size_t b) { return (b + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+constexpr size_t kNoAllocChunks = (kNoAllocMaxBatch + pelemp::kDwChunk - 1) / pelemp::kDwChunk;
+constexpr size_t kNoAllocOffH = noalloc_round(sizeof(float) * kNoAllocMaxBatch * pelemp::kSurrIn);
+constexpr size_t kNoAllocOffD = kNoAllocOffH + noalloc_round(sizeof(float) * 4 * kNoAllocMaxBatch * pelemp::kSurrHid);
+constexpr size_t kNoAllocOffDout =
+    kNoAllocOffD + noalloc_round(sizeof(float) * 4 * kNoAllocMaxBatch * pelemp::kSurrHid);
+constexpr size_t kNoAllocOffLoss =
+    kNoAllocOffDout + noalloc_round(sizeof(float) * kNoAllocMaxBatch * pelemp::kSurrOut);
+constexpr size_t kNoAllocOffPart = kNoAllocOffLoss + noalloc_round(sizeof(double) * kNoAllocMaxBatch);
+constexpr size_t kNoAllocOffAcc =
+    kNoAllocOffPart + noalloc_round(sizeof(double) * kNoAllocChunks * pelemp::kSurrParams);
+constexpr size_t kNoAllocBytes = kNoAllocOffAcc + noalloc_round(sizeof(double));
+static_assert(kNoAllocBytes == PELEMP_MLP_NOALLOC_SCRATCH_BYTES, "header and workspace size disagree");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of the current device's workspace: looked up once per
+// device (with that device current) and kept; the array's own name is a
+// host-side stand-in and is never passed to a kernel.
+int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_check(e, "hipGetDevice");
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_last_error("device index beyond the workspace table");
+    return PELEMP_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+    if (!p) {
+      set_last_error("hipGetSymbolAddress(scratch) returned no address");
+      return PELEMP_E_HIP;
+    }
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(g_noalloc_ptr[dev]);
+  return PELEMP_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -542,6 +607,48 @@ int pelemp_mlp_loss_grad(const float* params, const pelemp_surrogate_norm* norm,
   const MlpNorm nm = make_mlp_norm(*norm);
   BatchMap bm{n, 1, 0, 0};
   if ((st = hip_check(hipMemsetAsync(acc, 0, sizeof(double), s), "memset")) != PELEMP_OK) return st;
+  if ((st = launch_fwdbwd(params, nm, quant, bm, nb, x, t, w, ws, s)) != PELEMP_OK) return st;
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kSurrParams + 255) / 256), dim3(256), 0, s, nullptr, grad, ws.part,
+                     (nb + kDwChunk - 1) / kDwChunk, nullptr, nullptr, 0, 0.f, 0.f, 0.f, 0.f);
+  hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(256), 0, s, ws.loss, nb, acc);
+  if ((st = hip_check(hipGetLastError(), "mlp launch")) != PELEMP_OK) return st;
+  if ((st = hip_check(hipMemcpyAsync(loss, acc, sizeof(double), hipMemcpyDeviceToHost, s), "copy")) != PELEMP_OK)
+    return st;
+  return hip_check(hipStreamSynchronize(s), "sync");
+}
+
+int pelemp_mlp_loss_grad_noalloc(const float* params, const pelemp_surrogate_norm* norm, int32_t quant, int64_t n,
+                                 const float* x, const float* t, const float* w, float* grad, double* loss,
+                                 void* stream) {
+  if (!params || !norm || n <= 0 || !x || !t || !grad || !loss) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  if (n > kNoAllocMaxBatch) {
+    set_last_error("pelemp_mlp_loss_grad_noalloc: batch of " + std::to_string(n) + " samples exceeds the " +
+                   std::to_string(kNoAllocMaxBatch) + "-sample workspace");
+    return PELEMP_E_ARG;
+  }
+  unsigned char* base = nullptr;
+  int st = noalloc_scratch(&base);
+  if (st != PELEMP_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = static_cast<int>(n);
+  // the arrays are laid out for kNoAllocMaxBatch samples; a smaller batch
+  // uses the front of each (the kernels index with nb)
+  Workspace ws;
+  ws.h0 = reinterpret_cast<float*>(base);
+  ws.h = reinterpret_cast<float*>(base + kNoAllocOffH);
+  ws.d = reinterpret_cast<float*>(base + kNoAllocOffD);
+  ws.dout = reinterpret_cast<float*>(base + kNoAllocOffDout);
+  ws.loss = reinterpret_cast<double*>(base + kNoAllocOffLoss);
+  ws.part = reinterpret_cast<double*>(base + kNoAllocOffPart);
+  double* acc = reinterpret_cast<double*>(base + kNoAllocOffAcc);
+  const MlpNorm nm = make_mlp_norm(*norm);
+  BatchMap bm{n, 1, 0, 0};
+  // the accumulator keeps the previous call's loss: clear it first
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(1), dim3(64), 0, s, acc, 1);
+  if ((st = hip_check(hipGetLastError(), "zero launch")) != PELEMP_OK) return st;
   if ((st = launch_fwdbwd(params, nm, quant, bm, nb, x, t, w, ws, s)) != PELEMP_OK) return st;
   hipLaunchKernelGGL(mlp_adam_kernel, dim3((kSurrParams + 255) / 256), dim3(256), 0, s, nullptr, grad, ws.part,
                      (nb + kDwChunk - 1) / kDwChunk, nullptr, nullptr, 0, 0.f, 0.f, 0.f, 0.f);

@@ -175,6 +175,18 @@ def loss_grad(params: np.ndarray, norm: C.SurrogateNormC, x_dev, t_dev, w_dev, n
     return float(loss.value), g.download(np.empty(C.SURR_NPARAM, np.float32))
 
 
+def loss_grad_noalloc(params_dev, norm: C.SurrogateNormC, x_dev, t_dev, w_dev, n: int, grad_dev,
+                      quant: bool = False, stream=None) -> float:
+    """pelemp_mlp_loss_grad_noalloc on caller-owned device buffers (params,
+    inputs, targets, weights, gradient); returns the loss."""
+    loss = ctypes.c_double()
+    C.check(C.lib().pelemp_mlp_loss_grad_noalloc(params_dev.ptr, ctypes.byref(norm), int(quant), n, x_dev.ptr,
+                                                 t_dev.ptr, None if w_dev is None else w_dev.ptr, grad_dev.ptr,
+                                                 ctypes.byref(loss), stream),
+            "pelemp_mlp_loss_grad_noalloc")
+    return float(loss.value)
+
+
 def train(params: np.ndarray, norm: C.SurrogateNormC, samples: Samples, *, epochs: int, batch: int = 4096,
           lr: float = 2e-3, lr_final: float = 2e-5, quant: bool = True, seed: int = 7, verbose: bool = False,
           stream=None):
