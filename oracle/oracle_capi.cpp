@@ -317,6 +317,100 @@ double oracle_spray_source_one(int mech, int nf, const pelemp_spray_data* sd, do
   return -1.0;
 }
 
+// calculateFilmSource for one wall-film particle in a prescribed gas state
+// (gas = vel[3], T, rho, Y[NS]); rdata in/out, id in/out; out_src = mom[3],
+// mass, Ydot[nf], eng (the GasPhaseVals sources); cboil_out: the boiling
+// temperatures used.
+int oracle_film_source_one(int mech, int nf, const pelemp_spray_data* sd, double flow_dt, const double* gas,
+                           double* rdata, int64_t* id, double* out_src, double* cboil_out) {
+  auto run = [&](auto tag, auto nft) -> int {
+    using M = decltype(tag);
+    constexpr int NF = decltype(nft)::value;
+    constexpr int NS = M::NS;
+    oracle::GasPhaseVals<NS, NF> gpv;
+    gpv.reset();
+    pelemp::GasProps<M>::molecular_weight(gpv.mw);
+    for (int d = 0; d < 3; ++d) gpv.vel_fluid[d] = gas[d];
+    gpv.T_fluid = gas[3];
+    gpv.rho_fluid = gas[4];
+    for (int n = 0; n < NS; ++n) gpv.Y_fluid[n] = gas[5 + n];
+    gpv.define();
+    double cB[NF];
+    oracle::calcBoilT<NS, NF>(*sd, gpv, cB);
+    oracle::Particle<NF> p;
+    for (int c = 0; c < oracle::Comps<NF>::pstateNum; ++c) p.rdata[c] = rdata[c];
+    p.id = *id;
+    for (int d = 0; d < 3; ++d) p.pos[d] = 0.0;
+    oracle::calculateFilmSource<M, NF>(flow_dt, gpv, *sd, p, cB);
+    for (int c = 0; c < oracle::Comps<NF>::pstateNum; ++c) rdata[c] = p.rdata[c];
+    *id = p.id;
+    for (int d = 0; d < 3; ++d) out_src[d] = gpv.fluid_mom_src[d];
+    out_src[3] = gpv.fluid_mass_src;
+    for (int s = 0; s < NF; ++s) out_src[4 + s] = gpv.fluid_Y_dot[s];
+    out_src[4 + NF] = gpv.fluid_eng_src;
+    for (int s = 0; s < NF; ++s) cboil_out[s] = cB[s];
+    return 0;
+  };
+  if (mech == PELEMP_MECH_DODECANE_AIR && nf == 1)
+    return run(pelemp::Mech_dodecane_air{}, std::integral_constant<int, 1>{});
+  if (mech == PELEMP_MECH_DECANE_DODECANE_AIR && nf == 2)
+    return run(pelemp::Mech_decane_dodecane_air{}, std::integral_constant<int, 2>{});
+  if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
+    return run(pelemp::Mech_alkane6_air{}, std::integral_constant<int, 6>{});
+  return PELEMP_E_ARG;
+}
+
+double oracle_lambdaL(const pelemp_spray_data* sd, double T, int spf) { return oracle::lambdaL(*sd, T, spf); }
+
+// find_tangents (AhamedSplash.H:22-48) for one velocity and wall normal.
+void oracle_find_tangents(const double* vel, const double* norm, double* tanPsi, double* tanBeta) {
+  oracle::find_tangents(vel, tanPsi, norm, tanBeta);
+}
+
+// CreateSBDroplets (SpraySB.cpp:9-207) for one record: rec = norm[3], vel[3],
+// loc[3], T0, ref_dia, num_dens, phi1, phi2, phi3, Y0[nf] (the SBPtrs
+// fields); the children (at most max_out) go to pos[3*k], rdata[(10+nf)*k],
+// ids[k] (first_id, first_id + 1, ...).  Returns the number of children, or
+// -1 when more than max_out would be made.
+int oracle_create_children(int nf, const pelemp_spray_data* sd, int type, const double* rec, int64_t parent_id,
+                           double sub_dt, double ppp_fact, uint64_t seed, int64_t first_id, int max_out,
+                           double* pos, double* rdata, int64_t* ids) {
+  auto run = [&](auto nft) -> int {
+    constexpr int NF = decltype(nft)::value;
+    constexpr int NSR = oracle::Comps<NF>::pstateNum;
+    oracle::SBRecord<NF> rf;
+    rf.type = type;
+    for (int d = 0; d < 3; ++d) {
+      rf.norm[d] = rec[d];
+      rf.vel[d] = rec[3 + d];
+      rf.loc[d] = rec[6 + d];
+    }
+    rf.T0 = rec[9];
+    rf.ref_dia = rec[10];
+    rf.num_dens = rec[11];
+    rf.phi1 = rec[12];
+    rf.phi2 = rec[13];
+    rf.phi3 = rec[14];
+    for (int s = 0; s < NF; ++s) rf.Y0[s] = rec[15 + s];
+    std::vector<oracle::Particle<NF>> out;
+    int64_t next_id = first_id;
+    oracle::create_sb_droplets<NF>(rf, parent_id, *sd, sub_dt, ppp_fact, seed, next_id, out);
+    if (static_cast<int>(out.size()) > max_out) return -1;
+    for (size_t k = 0; k < out.size(); ++k) {
+      for (int d = 0; d < 3; ++d) pos[3 * k + d] = out[k].pos[d];
+      for (int c = 0; c < NSR; ++c) rdata[NSR * k + c] = out[k].rdata[c];
+      ids[k] = out[k].id;
+    }
+    return static_cast<int>(out.size());
+  };
+  switch (nf) {
+    case 1: return run(std::integral_constant<int, 1>{});
+    case 2: return run(std::integral_constant<int, 2>{});
+    case 6: return run(std::integral_constant<int, 6>{});
+    default: return -2;
+  }
+}
+
 // updateBreakupTAB for one single-fuel droplet (mechanism 0) in a gas state
 // gas = vel[3], T, rho, Y[NS]; rdata in/out; returns the accumulated
 // tangential velocity.  cboil: boiling temperature used.

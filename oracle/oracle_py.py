@@ -52,7 +52,7 @@ def lib() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_calc_heat_coeff.restype = ctypes.c_double
         L.oracle_calc_heat_coeff.argtypes = [ctypes.c_double] * 5
-        for nm in ("oracle_psat", "oracle_rhoL", "oracle_muL"):
+        for nm in ("oracle_psat", "oracle_rhoL", "oracle_muL", "oracle_lambdaL"):
             getattr(L, nm).restype = ctypes.c_double
             getattr(L, nm).argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_int]
         L.oracle_calc_boil_t.restype = ctypes.c_double
@@ -65,6 +65,15 @@ def lib() -> ctypes.CDLL:
         L.oracle_spray_source_one.restype = ctypes.c_double
         L.oracle_spray_source_one.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(C.SprayDataC),
                                               ctypes.c_double, vp, vp, vp, vp, vp]
+        L.oracle_film_source_one.restype = ctypes.c_int
+        L.oracle_film_source_one.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(C.SprayDataC),
+                                             ctypes.c_double, vp, vp, vp, vp, vp]
+        L.oracle_find_tangents.restype = None
+        L.oracle_find_tangents.argtypes = [vp, vp, vp, vp]
+        L.oracle_create_children.restype = ctypes.c_int
+        L.oracle_create_children.argtypes = [ctypes.c_int, ctypes.POINTER(C.SprayDataC), ctypes.c_int, vp,
+                                             ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                             ctypes.c_int64, ctypes.c_int, vp, vp, vp]
         L.oracle_tab_update.restype = ctypes.c_double
         L.oracle_tab_update.argtypes = [ctypes.POINTER(C.SprayDataC), ctypes.c_double, ctypes.c_double, vp, vp, vp]
         L.oracle_khrt_update.restype = ctypes.c_int
@@ -200,6 +209,51 @@ def spray_source_one(mech: int, sdata: dict, flow_dt: float, gas, rdata, pid: in
     Re = lib().oracle_spray_source_one(mech, nf, ctypes.byref(sd), flow_dt, gas.ctypes.data, rd.ctypes.data,
                                        pid_c.ctypes.data, src.ctypes.data, cb.ctypes.data)
     return Re, rd, int(pid_c[0]), src, cb
+
+
+def film_source_one(mech: int, sdata: dict, flow_dt: float, gas, rdata, pid: int = 1):
+    """calculateFilmSource for one wall-film particle in the gas state
+    gas = [u, v, w, T, rho, Y_0..Y_NS-1].  Returns (rdata_out, id_out,
+    sources [mom3, mass, Ydot(nf), eng], cBoilT)."""
+    nf = int(sdata["nfuel"])
+    sd = C.spray_data_c(sdata)
+    gas = np.ascontiguousarray(gas, dtype=np.float64)
+    rd = np.array(rdata, dtype=np.float64, copy=True)
+    pid_c = np.array([pid], dtype=np.int64)
+    src = np.zeros(5 + nf)
+    cb = np.zeros(nf)
+    st = lib().oracle_film_source_one(mech, nf, ctypes.byref(sd), flow_dt, gas.ctypes.data, rd.ctypes.data,
+                                      pid_c.ctypes.data, src.ctypes.data, cb.ctypes.data)
+    assert st == 0, st
+    return rd, int(pid_c[0]), src, cb
+
+
+def find_tangents(vel, norm):
+    """find_tangents (AhamedSplash.H:22-48): (tanPsi, tanBeta)."""
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    n = np.ascontiguousarray(norm, dtype=np.float64)
+    tp, tb = np.zeros(3), np.zeros(3)
+    lib().oracle_find_tangents(v.ctypes.data, n.ctypes.data, tp.ctypes.data, tb.ctypes.data)
+    return tp, tb
+
+
+def create_children(sdata: dict, rec_type: int, norm, vel, loc, T0: float, ref_dia: float, num_dens: float,
+                    phi1: float, phi2: float, phi3: float, Y0, parent_id: int = 1, sub_dt: float = 1e-6,
+                    ppp_fact: float = 0.5, seed: int = 0, first_id: int = 1000, max_out: int = 4096):
+    """CreateSBDroplets (SpraySB.cpp:9-207) for one secondary-droplet record;
+    returns (pos, rdata, ids) of the children."""
+    nf = int(sdata["nfuel"])
+    sd = C.spray_data_c(sdata)
+    rec = np.concatenate([norm, vel, loc, [T0, ref_dia, num_dens, phi1, phi2, phi3], Y0]).astype(np.float64)
+    assert rec.size == 15 + nf
+    pos = np.zeros((max_out, 3))
+    rd = np.zeros((max_out, 10 + nf))
+    ids = np.zeros(max_out, dtype=np.int64)
+    m = lib().oracle_create_children(nf, ctypes.byref(sd), int(rec_type), rec.ctypes.data, parent_id, sub_dt,
+                                     ppp_fact, seed, first_id, max_out, pos.ctypes.data, rd.ctypes.data,
+                                     ids.ctypes.data)
+    assert m >= 0, m
+    return pos[:m].copy(), rd[:m].copy(), ids[:m].copy()
 
 
 def tab_update(sdata: dict, Reyn: float, dt: float, gas, rdata):
