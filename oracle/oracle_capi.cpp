@@ -27,6 +27,10 @@ extern "C" struct oracle_branch_params {
   // 1: PeleC flavour -- CGS SprayData without the latent shift, CGS geometry,
   // droplets, state and sources (converted to and from SI as the library does)
   int64_t pelec;
+  // ghost widths of the call (state_ghosts, source_ghosts); negative: the
+  // FABs' own boxes bound the gather and the deposits
+  int64_t state_ghosts;
+  int64_t source_ghosts;
 };
 
 namespace {
@@ -79,6 +83,8 @@ int run_update(const pelemp_spray_data* sd, const pelemp_spray_comps* comps, con
     bp.khrt_B1 = bpp->khrt_B1;
     bp.khrt_C3 = bpp->khrt_C3;
     bp.avg_inject_mass = bpp->avg_inject_mass;
+    bp.state_ghosts = static_cast<int>(bpp->state_ghosts);
+    bp.source_ghosts = static_cast<int>(bpp->source_ghosts);
     cap = bpp->cap;
     if (pelec) {
       bp.pelec = true;

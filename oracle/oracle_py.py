@@ -29,7 +29,7 @@ class BranchParamsC(ctypes.Structure):
     _fields_ = [("max_ppp", ctypes.c_double), ("ppp_fact", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("next_id", ctypes.c_int64), ("cap", ctypes.c_int64), ("khrt_B0", ctypes.c_double),
                 ("khrt_B1", ctypes.c_double), ("khrt_C3", ctypes.c_double), ("avg_inject_mass", ctypes.c_double),
-                ("pelec", ctypes.c_int64)]
+                ("pelec", ctypes.c_int64), ("state_ghosts", ctypes.c_int64), ("source_ghosts", ctypes.c_int64)]
 
 
 def lib() -> ctypes.CDLL:
@@ -146,18 +146,22 @@ def latent_shifted(sdata: dict, mech: int, ns: int) -> dict:
 def update(mech: int, sdata: dict, comps: dict, geom: dict, pos, rdata, ids, state: np.ndarray, state_lo,
            state_hi, source: np.ndarray, src_lo, src_hi, flow_dt: float, do_move: bool = True,
            spray_cfl: float = -1.0, is_virt: bool = False, is_ghost: bool = False, nthreads: int = 0,
-           branch: dict | None = None):
+           branch: dict | None = None, state_ghosts: int | None = None, source_ghosts: int | None = None):
     """One updateParticles call on host arrays; `source` is updated in place.
     Returns (status, pos, rdata, ids) with the surviving droplets in cell order
     (then the secondary droplets created by breakup / splash).  `branch`
     (max_ppp, ppp_fact, seed, next_id, cap, khrt_B0, khrt_B1, khrt_C3,
     avg_inject_mass, pelec) sets the breakup / splash parameters and the
     flavour (pelec=1: CGS SprayData without the latent shift, CGS geometry,
-    droplets, state and sources); its next_id is updated."""
+    droplets, state and sources); its next_id is updated.  state_ghosts /
+    source_ghosts bound the gather and the deposits to the domain grown by
+    that many cells (None: the FABs' boxes only)."""
     nf = int(sdata["nfuel"])
     n = np.asarray(ids).shape[0]
     bp = BranchParamsC(max_ppp=100.0, ppp_fact=0.5, seed=0, next_id=0, cap=n, khrt_B0=0.61, khrt_B1=7.0,
-                       khrt_C3=1.0, avg_inject_mass=0.0)
+                       khrt_C3=1.0, avg_inject_mass=0.0,
+                       state_ghosts=-1 if state_ghosts is None else int(state_ghosts),
+                       source_ghosts=-1 if source_ghosts is None else int(source_ghosts))
     if branch is not None:
         for k in ("max_ppp", "ppp_fact", "seed", "next_id", "cap", "khrt_B0", "khrt_B1", "khrt_C3",
                   "avg_inject_mass", "pelec"):

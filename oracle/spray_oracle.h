@@ -548,6 +548,23 @@ struct Fab {
   double operator()(const int c[3], int comp) const { return data[off(c, comp)]; }
 };
 
+// Box of the domain's cells grown by `ghosts` cells (a negative width: no
+// limit), the reference's pti.growntilebox(ghosts) for a one-box level.
+struct GrownBox {
+  int lo[3], hi[3];
+  bool on;
+  GrownBox(const pelemp_geom& g, int ghosts) : on(ghosts >= 0) {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = g.domain_lo[d] - ghosts;
+      hi[d] = g.domain_hi[d] + ghosts;
+    }
+  }
+  bool contains(const int c[3]) const {
+    if (!on) return true;
+    return c[0] >= lo[0] && c[0] <= hi[0] && c[1] >= lo[1] && c[1] <= hi[1] && c[2] >= lo[2] && c[2] <= hi[2];
+  }
+};
+
 // PeleC flavour of one state cell (SprayInterpolation.H:126-146 without
 // PELELM_USE_SPRAY): the state holds rho, rho*u, rho*E, rho*Y and T in CGS;
 // Y, u, the kinetic energy and the internal energy are formed in CGS as the
@@ -576,13 +593,15 @@ inline void pelec_cell(const double* S, int64_t cstride, const pelemp_spray_comp
 // or the PeleC flavour (pelec) through pelec_cell
 template <class Mech, int NF>
 bool InterpolateGasPhase(GasPhaseVals<Mech::NS, NF>& gpv, const Fab& st, const pelemp_spray_comps& SPI,
-                         const int idx[8][3], const double w[8], bool pelec = false) {
+                         const int idx[8][3], const double w[8], bool pelec = false,
+                         const GrownBox* state_box = nullptr) {
   constexpr int NS = Mech::NS;
   for (int a = 0; a < 8; ++a) {
     const double cw = w[a];
     if (cw > 0.) {
       const int* c = idx[a];
-      if (!st.contains(c)) return false;
+      // "state box too small" (SprayInterpolation.H:121-124)
+      if (!st.contains(c) || (state_box && !state_box->contains(c))) return false;
       if (pelec) {
         double rho, Y[NS], vel[3], T;
         const int64_t cstride = static_cast<int64_t>(st.n[0]) * st.n[1] * st.n[2];
@@ -680,7 +699,13 @@ struct BranchParams {
   bool pelec = false;
   double src_scale[5 + PELEMP_MAX_FUEL] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
   int64_t n_redirect = 0;  // out: deposits eb_interp sent to another cell
+  // ghost widths of the call (SprayParticles.cpp:250-251): the state box and
+  // the source box are the box of the domain's cells grown by them (then
+  // intersected with the FABs); negative: the FABs' own boxes
+  int state_ghosts = -1;
+  int source_ghosts = -1;
 };
+
 
 // per-droplet values the reference keeps in registers across sub-cycles
 template <int NF>
@@ -725,6 +750,8 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
   }
   const Fab st(state_v);
   Fab srcv(source_v);
+  const GrownBox state_box(g, bp.state_ghosts);
+  const GrownBox src_box(g, bp.source_ghosts);
   constexpr int NSRC = 5 + NF;  // mom(3) mass Y(nf) eng, contribution slots
   std::vector<Carry<NF>> carry(parts.size());
   // film thickness per cell from the film particles at the start of the call
@@ -841,7 +868,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
           for (int d = 0; d < 3; ++d) idx[a][d] += g.domain_lo[d];
       }
       gpv.reset();
-      if (!InterpolateGasPhase<Mech, NF>(gpv, st, SPI, idx, w, bp.pelec)) {
+      if (!InterpolateGasPhase<Mech, NF>(gpv, st, SPI, idx, w, bp.pelec, &state_box)) {
         err_local = std::max(err_local, PELEMP_E_STATE_BOX);
         continue;
       }
@@ -869,7 +896,10 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
       double cvol = inv_vol;
       if (eb_in_box && eb.single(ijkc[0], ijkc[1], ijkc[2])) cvol *= 1. / (eb.vf(ijkc[0], ijkc[1], ijkc[2]));
       const double cur_coef = -cvol * sub_dt / flow_dt;
-      if (!srcv.contains(ijkc)) {
+      // "source box too small" (SprayParticles.cpp:420-425); a ghost
+      // droplet's deposit outside the box is dropped (the reference writes
+      // it at an index the FAB may not hold)
+      if (!srcv.contains(ijkc) || !src_box.contains(ijkc)) {
         if (!isGhost) {
           err_local = std::max(err_local, PELEMP_E_SRC_BOX);
           continue;
@@ -888,6 +918,9 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         redir[pid] = (ijkc[0] != own_cell[0] || ijkc[1] != own_cell[1] || ijkc[2] != own_cell[2]) ? 1 : 0;
         own_off[pid] = srcv.contains(own_cell) ? srcv.off(own_cell, 0) : -1;
       }
+      // the reference's ijkc after the sub-cycle: the unclamped cell of the
+      // new position when the droplet moved, else the gather's cell
+      int ghost_cell[3] = {ijkc_rel[0] + g.domain_lo[0], ijkc_rel[1] + g.domain_lo[1], ijkc_rel[2] + g.domain_lo[2]};
       if (do_move && !fdat.fixed_parts && p.id > 0 && !is_film) {
         const double pos0[3] = {p.pos[0], p.pos[1], p.pos[2]};
         for (int dir = 0; dir < 3; ++dir) {
@@ -914,13 +947,14 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
                             eb_in_box ? &eb : nullptr, ijkc_rel);
           }
         }
+        for (int d = 0; d < 3; ++d)
+          ghost_cell[d] = static_cast<int>(std::floor((p.pos[d] - g.plo[d]) * dxi[d])) + g.domain_lo[d];
       }
+      // a ghost droplet whose cell left the source box is deleted
+      // (SprayParticles.cpp:477-479), before the periodic shift the host's
+      // Redistribute would apply
+      if (isGhost && !src_box.contains(ghost_cell)) p.id = -1;
       periodic_wrap(p.pos, g);
-      if (isGhost) {
-        int c2[3];
-        containing_cell(p.pos, g, dxi, c2);
-        if (!srcv.contains(c2)) p.id = -1;
-      }
     }
     if (err_local != PELEMP_OK) {
       status = err_local;

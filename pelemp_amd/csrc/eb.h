@@ -327,6 +327,7 @@ struct EbRedirectArgs {
   int pelec;
   double scale[5 + PELEMP_MAX_FUEL];
   int is_ghost;
+  int src_g;  // source ghost width: deposits beyond it are outside the source box (negative: no limit)
   int* err;
 };
 
@@ -360,7 +361,10 @@ __global__ void __launch_bounds__(128) eb_redirect_kernel(EbRedirectArgs r) {
         keys[j] = k;
       }
   const int gx = r.dom_lo[0] + cx - r.src_lo[0], gy = r.dom_lo[1] + cy - r.src_lo[1], gz = r.dom_lo[2] + cz - r.src_lo[2];
-  const bool in_src = gx >= 0 && gx < r.src_n[0] && gy >= 0 && gy < r.src_n[1] && gz >= 0 && gz < r.src_n[2];
+  const bool in_box = r.src_g < 0 || (cx >= -r.src_g && cx < r.ncell[0] + r.src_g && cy >= -r.src_g &&
+                                       cy < r.ncell[1] + r.src_g && cz >= -r.src_g && cz < r.ncell[2] + r.src_g);
+  const bool in_src =
+      in_box && gx >= 0 && gx < r.src_n[0] && gy >= 0 && gy < r.src_n[1] && gz >= 0 && gz < r.src_n[2];
   const long long cell = (static_cast<long long>(gz) * r.src_n[1] + gy) * r.src_n[0] + gx;
   double v[NSRC];
   bool any = false;

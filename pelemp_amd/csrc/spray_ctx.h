@@ -20,6 +20,8 @@ struct PassParams {
   double cur_coef;  // -(1/V) sub_dt / flow_dt
   int do_move;
   int is_ghost;
+  int state_ghosts;   // state box = domain grown by this many cells (negative: the state FAB's box)
+  int source_ghosts;  // source box = domain grown by this many cells (negative: the source FAB's box)
   int at_bounds;
   int branches;        // film / breakup / splash kernel variant
   int make_new_drops;  // SprayParticles.cpp:298-299

@@ -490,6 +490,7 @@ static int run_eb_redirect(Ctx& c, const PassParams& p, hipStream_t s) {
   }
   r.pelec = c.pelec ? 1 : 0;
   r.is_ghost = p.is_ghost;
+  r.src_g = p.source_ghosts;
   r.err = c.err;
   const dim3 grid(static_cast<unsigned>((c.n_eb_cells + 127) / 128)), block(128);
   switch (c.nf) {
@@ -2049,9 +2050,6 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
                         double flow_dt, double time, int32_t state_ghosts, int32_t source_ghosts, int32_t isVirt,
                         int32_t isGhost, int32_t do_move, double spray_cfl_lev, void* stream) {
   (void)time;
-  (void)state_ghosts;
-  (void)source_ghosts;
-  (void)isVirt;
   if (!h || !state || !source || !state->data || !source->data) {
     set_last_error("invalid argument");
     return PELEMP_E_ARG;
@@ -2089,6 +2087,10 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
   p.cur_coef = -inv_vol * sub_dt / flow_dt;
   p.do_move = do_move ? 1 : 0;
   p.is_ghost = isGhost ? 1 : 0;
+  // the state box and the source box: the level's box grown by the ghost
+  // widths (SprayParticles.cpp:250-251), within the FABs handed in
+  p.state_ghosts = state_ghosts;
+  p.source_ghosts = source_ghosts;
   p.at_bounds = 0;
   for (int d = 0; d < 3; ++d)
     if (c.geom.bndry_lo[d] != 0 || c.geom.bndry_hi[d] != 0) p.at_bounds = 1;

@@ -94,6 +94,8 @@ struct UpdateArgs {
   R sub_dt_dxi[3];  // FP32 form: sub_dt / dx per direction
   int do_move;
   int is_ghost;
+  int st_g;   // state ghost width: the gather may read cells up to st_g outside the domain (negative: the FAB)
+  int src_g;  // source ghost width: a ghost droplet is deleted when its cell leaves the domain grown by src_g
   int* err;
   // surrogate form
   const SurrWeights* __restrict__ surr_w;
@@ -176,6 +178,13 @@ __global__ void __launch_bounds__(256) pelec_convert_kernel(const double* __rest
 // fields of a secondary-droplet record (SBData.H:7-19): norm(3) vel(3)
 // loc(3) T0 ref_dia num_dens phi1 phi2 phi3, then Y0[NF]
 constexpr int kSBFields = 15;
+
+// Cell (rx, ry, rz), relative to the domain's lower corner, inside the
+// domain grown by g cells (a negative g: no limit).
+__device__ __attribute__((always_inline)) inline bool in_grown_box(int rx, int ry, int rz, const int* ncell, int g) {
+  if (g < 0) return true;
+  return rx >= -g && rx < ncell[0] + g && ry >= -g && ry < ncell[1] + g && rz >= -g && rz < ncell[2] + g;
+}
 
 __device__ inline int xcd_tile(int b, int ntiles) {
   // gfx950 dispatches workgroup b to XCD b % 8; give each XCD a contiguous
@@ -365,7 +374,8 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
         const int lx = a.dom_lo[0] + rx - a.st.lo[0];
         const int ly = a.dom_lo[1] + ry - a.st.lo[1];
         const int lz = a.dom_lo[2] + rz - a.st.lo[2];
-        if (!(lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2])) {
+        if (!(lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2]) ||
+            !in_grown_box(rx, ry, rz, a.ncell, a.st_g)) {
           set_err(a.err, PELEMP_E_STATE_BOX);
           return;
         }
@@ -492,6 +502,14 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
         if (dev_check_bounds(p3, a.plo, a.phi, a.dx, a.bndry_lo, a.bndry_hi, bflags)) d.alive = false;
       }
     }
+    // a ghost droplet whose (unclamped, unwrapped) cell left the source box
+    // is deleted (SprayParticles.cpp:477-479)
+    if (a.is_ghost) {
+      int gc[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) gc[q] = static_cast<int>(floor((p3[q] - a.plo[q]) * a.dxi[q]));
+      if (!in_grown_box(gc[0], gc[1], gc[2], a.ncell, a.src_g)) d.alive = false;
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       if (a.bndry_lo[q] == 0) {
@@ -517,6 +535,7 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
   } else {
     int c[3] = {L.cx, L.cy, L.cz};
     if (a.do_move && !a.fc.fixed_parts && d.alive && !(film > R(0.))) {
+      bool left_src = false;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         R xi = pos[q] + a.sub_dt_dxi[q] * d.vel[q];
@@ -525,6 +544,9 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
         if (xi >= R(1.)) xi = R(0.);  // rounding of a tiny negative xi
         int ic = c[q] + static_cast<int>(sh);
         const int n = a.ncell[q];
+        // a ghost droplet whose unwrapped cell left the source box is
+        // deleted (SprayParticles.cpp:477-479)
+        if (a.src_g >= 0 && (ic < -a.src_g || ic >= n + a.src_g)) left_src = true;
         // periodic: ((ic % n) + n) % n, with the integer division only for a
         // move of more than one period
         if (ic < 0) {
@@ -537,6 +559,7 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
         c[q] = ic;
         pos[q] = xi;
       }
+      if (a.is_ghost && left_src) d.alive = false;
     }
     if (d.alive) key = tile_key(c[0], c[1], c[2], a.ntile);
   }
@@ -692,7 +715,8 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
   }
 
   // drift, bounds and walls (SprayParticles.cpp:443-479), periodic wrap
-  if (a.do_move && !a.fc.fixed_parts && b.alive && !is_film) {
+  const bool moves = a.do_move && !a.fc.fixed_parts && b.alive && !is_film;
+  if (moves) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) b.pos[q] += a.sub_dt * b.vel[q];
     if (AT_BOUNDS || (EB && L.fe)) {
@@ -722,6 +746,20 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
                         EB ? &a.eb : nullptr, EB ? L.ijkc : nullptr);
       }
     }
+  }
+  // a ghost droplet whose cell left the source box is deleted
+  // (SprayParticles.cpp:477-479): the cell of the new position (before the
+  // periodic shift), or the gather's cell when it did not move
+  if (a.is_ghost) {
+    int gc[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      gc[q] = static_cast<int>(floor((b.pos[q] - a.plo[q]) * a.dxi[q]));
+      if constexpr (EB) {
+        if (!moves && L.fe) gc[q] = L.ijkc[q];
+      }
+    }
+    if (!in_grown_box(gc[0], gc[1], gc[2], a.ncell, a.src_g)) b.alive = false;
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q)
@@ -874,7 +912,9 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     const int gy = a.dom_lo[1] + oty + hy - 1;
     const int gz = a.dom_lo[2] + otz + hz - 1;
     const int lx = gx - a.st.lo[0], ly = gy - a.st.lo[1], lz = gz - a.st.lo[2];
-    const bool ok = lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2];
+    // the state box: inside the FAB and inside the domain grown by st_g
+    const bool ok = lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2] &&
+                    in_grown_box(otx + hx - 1, oty + hy - 1, otz + hz - 1, a.ncell, a.st_g);
     gvalid[h] = ok ? 1 : 0;
     if constexpr (sizeof(R) == 8) {
       if (ok && a.pelec) {
@@ -900,6 +940,11 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       for (int q = 0; q < 3; ++q)
         gas[(1 + NS + q) * kHaloCells + h] = static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
       gas[(4 + NS) * kHaloCells + h] = static_cast<R>(S[a.c_T * a.st.cstride + c]);
+    } else {
+      // outside the state box: a droplet reaching it is an error; the
+      // cleared cell keeps what it reads defined
+#pragma unroll
+      for (int f = 0; f < 5 + NS; ++f) gas[f * kHaloCells + h] = R(0);
     }
   }
   __syncthreads();

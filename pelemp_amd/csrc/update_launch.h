@@ -69,6 +69,8 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.cur_coef = p.cur_coef;
   a.do_move = p.do_move;
   a.is_ghost = p.is_ghost;
+  a.st_g = p.state_ghosts;
+  a.src_g = p.source_ghosts;
   a.err = c.err;
   const dim3 grid(c.ntiles), block(kUpdThreads);
   if constexpr (SURR) {

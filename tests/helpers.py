@@ -15,7 +15,8 @@ def run_oracle(prob, nsteps: int, sdata=None, spray_cfl: float = -1.0, do_move: 
         src = prob.new_source()
         st, pos, rd, ids = O.update(prob.mech, sd, prob.comps, prob.geom, pos, rd, ids, prob.state, prob.state_lo,
                                     prob.state_hi, src, prob.src_lo, prob.src_hi, prob.flow_dt, do_move=do_move,
-                                    spray_cfl=spray_cfl)
+                                    spray_cfl=spray_cfl, state_ghosts=prob.state_ghosts,
+                                    source_ghosts=prob.source_ghosts)
         assert st == 0, f"oracle status {st}"
         srcs.append(src)
     return pos, rd, ids, srcs
