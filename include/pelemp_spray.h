@@ -35,6 +35,14 @@ extern "C" {
 
 #define PELEMP_MAX_FUEL 8
 
+/* Droplets per workgroup segment of a 4x4x4-cell tile.  A tile holding more
+ * droplets than this is updated by several workgroups, one per segment of
+ * its cell-sorted droplets; each cell's source sum is then the sum, in
+ * segment order, of the per-segment sums (each in cell-sorted droplet
+ * order).  This fixes the association order of the deposition, so results
+ * do not depend on thread timing. */
+#define PELEMP_SEGMENT_DROPLETS 4096
+
 /* Status codes (reference: amrex::Abort sites listed per code). */
 #define PELEMP_OK 0
 #define PELEMP_E_STATE_BOX 1   /* SprayInterpolation.H:121-124 */

@@ -776,11 +776,14 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     std::vector<int64_t> order(np);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    // skey: the sorted keys (INT64_MAX for a deleted droplet's record)
+    std::vector<int64_t> skey;
     {
       std::vector<Particle<NF>> sorted;
       std::vector<Carry<NF>> sc;
       sorted.reserve(np);
       sc.reserve(np);
+      skey.reserve(np);
       for (size_t r = 0; r < np; ++r) {
         if (key[order[r]] == INT64_MAX) {
           // a deleted droplet keeps its record for the secondary droplets
@@ -788,6 +791,7 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
         }
         sorted.push_back(parts[order[r]]);
         sc.push_back(carry[order[r]]);
+        skey.push_back(key[order[r]]);
       }
       parts.swap(sorted);
       carry.swap(sc);
@@ -976,6 +980,23 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     }
     seg_start.push_back(n);
     const int64_t nseg = static_cast<int64_t>(seg_start.size()) - 1;
+    // the tile of every droplet (cell_key / 64): its first sorted position
+    // and its droplet count.  A tile of more than PELEMP_SEGMENT_DROPLETS
+    // droplets is summed per segment of that many droplets, and a cell's
+    // sum is the sum of its segment sums in segment order (the GPU's
+    // workgroups split such tiles so; the reference's atomics have no order).
+    std::vector<int64_t> tfirst(n, 0), tcount(n, 0);
+    for (int64_t i = 0; i < n;) {
+      int64_t j = i + 1;
+      if (skey[i] != INT64_MAX)
+        while (j < n && skey[j] / 64 == skey[i] / 64) ++j;
+      for (int64_t r = i; r < j; ++r) {
+        tfirst[r] = i;
+        tcount[r] = j - i;
+      }
+      i = j;
+    }
+    constexpr int64_t kSeg = PELEMP_SEGMENT_DROPLETS;
     auto on_comp = [&](int q) {
       // disabled couplings leave their source components untouched
       if (q < 3 && !fdat.mom_trans) return false;
@@ -986,11 +1007,23 @@ int updateParticles(std::vector<Particle<NF>>& parts, const pelemp_spray_data& f
     for (int64_t s = 0; s < nseg; ++s) {
       const int64_t a = seg_start[s], b = seg_start[s + 1];
       if (own_off[a] < 0) continue;
+      const int64_t tf = tfirst[a], tn = tcount[a];
       for (int q = 0; q < NSRC; ++q) {
         if (!on_comp(q)) continue;
         double acc = 0.;
-        for (int64_t i = a; i < b; ++i)
-          if (!redir[i]) acc += contrib[static_cast<size_t>(i) * NSRC + q];
+        if (tn <= kSeg) {
+          for (int64_t i = a; i < b; ++i)
+            if (!redir[i]) acc += contrib[static_cast<size_t>(i) * NSRC + q];
+        } else {
+          const int64_t ns = (tn + kSeg - 1) / kSeg;
+          for (int64_t k = 0; k < ns; ++k) {
+            double part = 0.;
+            const int64_t lo = std::max(a, tf + k * kSeg), hi = std::min(b, tf + (k + 1) * kSeg);
+            for (int64_t i = lo; i < hi; ++i)
+              if (!redir[i]) part += contrib[static_cast<size_t>(i) * NSRC + q];
+            acc = acc + part;
+          }
+        }
         double* dst = &source_v.data[own_off[a] + static_cast<int64_t>(comps[q]) * cstride];
         *dst = bp.pelec ? *dst + acc * bp.src_scale[q] : *dst + acc;
       }

@@ -8,8 +8,11 @@
 // sorted by key.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
+#include "pelemp_spray.h"
 #include "pm_math.h"
 
 namespace pelemp {
@@ -48,6 +51,30 @@ struct Fields32 {
   static constexpr int ND = 8;
   static constexpr int FILM = 9;
   static constexpr int NHOT = 10;
+};
+
+// Heavy tiles.  A tile with more than kSegDroplets droplets is updated by
+// several workgroups, one per segment of kSegDroplets cell-sorted droplets
+// (the first by the tile's own workgroup, the others listed as work items).
+// A stayer's rank is then local to its segment: arrival = (segment <<
+// kSegShift) | rank, and seg_stay[(slot + segment) * 64 + cell] holds the
+// stayers of the cell in the earlier segments (the split_combine kernel
+// turns the per-segment counts into these offsets).
+constexpr int32_t kSegDroplets = PELEMP_SEGMENT_DROPLETS;
+constexpr int kSegShift = 13;
+static_assert(kSegDroplets <= (1 << kSegShift), "segment rank field too narrow");
+static_assert(kSegDroplets % 256 == 0, "segments hold whole chunks of the update workgroup");
+constexpr int32_t kSegRankMask = (1 << kSegShift) - 1;
+
+// Split state of the tiles for the next update pass.  counters: [items,
+// split tiles, segment slots].
+struct SplitDev {
+  int32_t* tile_slot;  // per tile: first segment slot, or -1 (one segment)
+  int2* items;         // (tile, segment >= 1) work items
+  int4* list;          // (tile, first slot, segments, -) of every split tile
+  int32_t* counters;
+  double* part;        // per slot: 64 cells x NSRC partial source sums
+  int32_t* seg_stay;   // per slot: 64 cells: stayers, then their offsets
 };
 
 PM_HD uint32_t tile_key(int i, int j, int k, const int* ntile) {

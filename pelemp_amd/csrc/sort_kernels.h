@@ -107,8 +107,11 @@ __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __res
                                                             const int32_t* __restrict__ n_ptr,
                                                             const int32_t* __restrict__ arrival,
                                                             const int32_t* __restrict__ cstart,
-                                                            int32_t* __restrict__ perm) {
+                                                            int32_t* __restrict__ perm,
+                                                            int32_t* __restrict__ split_counters) {
   const int32_t n = *n_ptr;
+  // the split lists are rebuilt after the sort
+  if (blockIdx.x == 0 && threadIdx.x < 3) split_counters[threadIdx.x] = 0;
   for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const uint32_t k = key[s];
     if (k != kDeadKey) perm[cstart[k] + arrival[s]] = s;
@@ -129,14 +132,38 @@ __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __res
 // the two mover groups (usually empty or one droplet) need ordering.
 constexpr int32_t kArrRankMask = (1 << 30) - 1;
 
+// One thread per tile after the sort: a tile of n > kSegDroplets droplets
+// gets ceil(n / kSegDroplets) consecutive segment slots and its segments
+// 1.. as work items (the order of the lists does not matter: every entry
+// names its tile and segment).
+__global__ void __launch_bounds__(256) split_tiles_kernel(const int32_t* __restrict__ cstart, int32_t ntiles,
+                                                           SplitDev sp) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int32_t n = cstart[(t + 1) * 64] - cstart[t * 64];
+  const int32_t nseg = (n + kSegDroplets - 1) / kSegDroplets;
+  if (nseg > 1) {
+    const int32_t slot = atomicAdd(&sp.counters[2], nseg);
+    const int32_t it = atomicAdd(&sp.counters[0], nseg - 1);
+    const int32_t li = atomicAdd(&sp.counters[1], 1);
+    sp.tile_slot[t] = slot;
+    for (int32_t k = 1; k < nseg; ++k) sp.items[it + k - 1] = make_int2(t, k);
+    sp.list[li] = make_int4(t, slot, nseg, 0);
+  } else {
+    sp.tile_slot[t] = -1;
+  }
+}
+
 __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __restrict__ key,
                                                             const int32_t* __restrict__ n_ptr,
                                                             const int32_t* __restrict__ arrival,
                                                             const int32_t* __restrict__ cstart,
                                                             const int32_t* __restrict__ mov_lo,
                                                             const int32_t* __restrict__ stay_cnt,
-                                                            int32_t* __restrict__ perm) {
+                                                            int32_t* __restrict__ perm, SplitDev sp) {
   const int32_t n = *n_ptr;
+  // the split lists of the next pass are rebuilt after this kernel
+  if (blockIdx.x == 0 && threadIdx.x < 3) sp.counters[threadIdx.x] = 0;
   for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const uint32_t k = key[s];
     if (k == kDeadKey) continue;
@@ -145,7 +172,10 @@ __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __res
     const int32_t r = v & kArrRankMask;
     int32_t p = cstart[k];
     if (cls == 0) {
-      p += mov_lo[k] + r;
+      // a stayer: its rank among the stayers of its segment of a heavy tile
+      const int32_t seg = r >> kSegShift;
+      p += mov_lo[k] + (r & kSegRankMask);
+      if (seg > 0) p += sp.seg_stay[(sp.tile_slot[k / 64] + seg) * 64 + (k % 64)];
     } else if (cls == 1) {
       p += r;
     } else {

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "layout.h"
 #include "pelemp_spray.h"
 
 namespace pelemp {
@@ -85,7 +86,13 @@ struct Ctx {
   int32_t* block_sum = nullptr;
   int32_t* total = nullptr;
   int32_t* err = nullptr;
-  int32_t* scalars = nullptr;  // n_entries, total, err, n_long in one allocation
+  int32_t* scalars = nullptr;  // n_entries, total, err, n_long, split counters[3] in one allocation
+  // heavy tiles (more than kSegDroplets droplets): split lists of the next
+  // pass and the per-segment partials; capacities from the droplet capacity
+  SplitDev sp{};
+  int32_t split_items_cap = 0;
+  int32_t split_list_cap = 0;
+  int64_t split_slots_cap = 0;
   double* dscratch = nullptr;  // reduction scratch
   // cold per-slot data (indexed by slot)
   int64_t* cold_id = nullptr;

@@ -373,13 +373,14 @@ static int run_sort(Ctx& c, hipStream_t s, bool keep_entries = false) {
   hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
                      c.n_long);
   hipLaunchKernelGGL(scatter_perm_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
-                     c.perm);
+                     c.perm, c.sp.counters);
   hipLaunchKernelGGL(order_cells_kernel, dim3((c.ncells + 255) / 256), dim3(256), 0, s, c.cstart, c.ncells,
                      c.perm, c.long_list, c.n_long, c.mov_lo);
   c.counters_dirty = false;
   // arrival is free once the scatter has read it: the merge buffer
   hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
                      c.n_long, c.perm, c.arrival);
+  hipLaunchKernelGGL(split_tiles_kernel, dim3((c.ntiles + 255) / 256), dim3(256), 0, s, c.cstart, c.ntiles, c.sp);
   PM_HIP(hipGetLastError());
   ctx_time_end(c, 1, s, ev0);
   return PELEMP_OK;
@@ -517,12 +518,13 @@ static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src, int
   hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
                      c.n_long);
   hipLaunchKernelGGL(scatter_incr_kernel, dim3(2048), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
-                     c.mov_lo, c.stay_cnt, c.perm);
+                     c.mov_lo, c.stay_cnt, c.perm, c.sp);
   hipLaunchKernelGGL(order_movers_kernel, dim3(gc), dim3(256), 0, s, c.cstart, c.mov_lo, c.stay_cnt, c.mov_hi,
                      c.ncells, c.perm, c.long_list, c.n_long);
   c.counters_dirty = false;
   hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
                      c.n_long, c.perm, c.arrival);
+  hipLaunchKernelGGL(split_tiles_kernel, dim3((c.ntiles + 255) / 256), dim3(256), 0, s, c.cstart, c.ntiles, c.sp);
   PM_HIP(hipGetLastError());
   ctx_time_end(c, 1, s, ev0);
   return PELEMP_OK;
@@ -561,7 +563,7 @@ static void free_ctx(Ctx& c) {
                   c.block_sum, c.scalars, c.dscratch, c.cold_id, c.cold_n0, c.cold_bm1, c.cold_bm2,
                   c.count_log, c.surr_w, c.surr_cnt, c.wf, c.sb_type, c.sb_data, c.sb_count, c.sb_offset,
                   c.sb_block, c.sb_parent, c.sb_pblock, c.sb_ids, c.n_new, c.eb_cells, c.eb_rkey, c.eb_rval,
-                  c.pelec_conv};
+                  c.pelec_conv, c.sp.tile_slot, c.sp.items, c.sp.list, c.sp.part, c.sp.seg_stay};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pe : c.ev_pending) {
@@ -1231,6 +1233,17 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm1), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.cold_bm2), 8 * c.cap);
   st |= alloc(c, reinterpret_cast<void**>(&c.count_log), 4 * Ctx::kCountLog);
+  // heavy tiles: a tile of n > kSegDroplets droplets has ceil(n / kSegDroplets)
+  // < 2 n / kSegDroplets segments, so the droplet capacity bounds the lists
+  c.split_items_cap = static_cast<int32_t>(c.cap / kSegDroplets + 1);
+  c.split_list_cap = c.split_items_cap;
+  c.split_slots_cap = 2 * (c.cap / kSegDroplets) + 2;
+  st |= alloc(c, reinterpret_cast<void**>(&c.sp.tile_slot), 4 * static_cast<size_t>(c.ntiles));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sp.items), sizeof(int2) * static_cast<size_t>(c.split_items_cap));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sp.list), sizeof(int4) * static_cast<size_t>(c.split_list_cap));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sp.part),
+              8 * static_cast<size_t>(c.split_slots_cap) * kTileCells * (5 + c.nf));
+  st |= alloc(c, reinterpret_cast<void**>(&c.sp.seg_stay), 4 * static_cast<size_t>(c.split_slots_cap) * kTileCells);
   if (form == PELEMP_FORM_SURROGATE) {
     st |= alloc(c, &c.surr_w, sizeof(SurrWeights));
     st |= alloc(c, reinterpret_cast<void**>(&c.surr_cnt), 2 * sizeof(unsigned long long));
@@ -1246,6 +1259,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
   c.total = c.scalars + 1;
   c.err = c.scalars + 2;
   c.n_long = c.scalars + 3;
+  c.sp.counters = c.scalars + 4;
   if (c.surr_cnt && hipMemset(c.surr_cnt, 0, 2 * sizeof(unsigned long long)) != hipSuccess) st = PELEMP_E_HIP;
   if (c.sb_ids) {
     // counters of new droplets: next id, next slot, children of the last pass
@@ -1253,6 +1267,7 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
     if (hipMemcpy(c.sb_ids, ids, sizeof(ids), hipMemcpyHostToDevice) != hipSuccess) st = PELEMP_E_HIP;
   }
   if (st != PELEMP_OK || hipMemset(c.scalars, 0, 64) != hipSuccess ||
+      hipMemset(c.sp.tile_slot, 0xff, 4 * static_cast<size_t>(c.ntiles)) != hipSuccess ||
       hipMemset(c.cstart, 0, 4 * (static_cast<size_t>(c.ncells) + 1)) != hipSuccess) {
     free_ctx(c);
     delete h;

@@ -97,6 +97,8 @@ struct UpdateArgs {
   int st_g;   // state ghost width: the gather may read cells up to st_g outside the domain (negative: the FAB)
   int src_g;  // source ghost width: a ghost droplet is deleted when its cell leaves the domain grown by src_g
   int* err;
+  // heavy tiles: the split lists of this pass and the per-segment partials
+  SplitDev sp;
   // surrogate form
   const SurrWeights* __restrict__ surr_w;
   SurrKNorm sn;
@@ -880,12 +882,18 @@ __device__ __attribute__((always_inline)) inline bool surr_stage_drop(const Upda
   return use;
 }
 
+// One segment of a tile: the tile's droplets seg * kSegDroplets .. +
+// kSegDroplets (all of them when the tile holds no more).  slot >= 0 marks
+// a heavy tile: the segment's per-cell sums and stayer counts go to the
+// partials of segment slot slot + seg, which split_combine_kernel adds to
+// the source in segment order.
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false, bool EB = false>
-__device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, R* gas,
-                                                                   unsigned char* gvalid, int32_t* cs,
-                                                                   double* contrib, unsigned char* stayed,
-                                                                   const SurrWeights* sw, unsigned& n_surr,
-                                                                   unsigned& n_fb, unsigned char* redir = nullptr) {
+__device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, int seg,
+                                                                   int slot, R* gas, unsigned char* gvalid,
+                                                                   int32_t* cs, double* contrib,
+                                                                   unsigned char* stayed, const SurrWeights* sw,
+                                                                   unsigned& n_surr, unsigned& n_fb,
+                                                                   unsigned char* redir = nullptr) {
   constexpr int NS = M::NS;
   constexpr int NSRC = 5 + NF;   // mom[3], mass, Ydot[NF], eng
   constexpr int NITEM = (kTileCells * NSRC + kUpdThreads - 1) / kUpdThreads;
@@ -898,12 +906,13 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 
   if (t <= kTileCells) cs[t] = a.cstart[tile * kTileCells + t];
   __syncthreads();
-  const int32_t d0 = cs[0];
-  const int32_t d1 = cs[kTileCells];
-  if (d1 == d0) {  // uniform: empty tile
+  const int32_t d0 = cs[0] + seg * kSegDroplets;
+  const int32_t d1 = (cs[kTileCells] - d0 < kSegDroplets) ? cs[kTileCells] : d0 + kSegDroplets;
+  if (d1 <= d0) {  // uniform: empty tile
     if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
     return;
   }
+  const int32_t seg_tag = seg << kSegShift;
 
   // ---- stage gas halo ----
   for (int h = t; h < kHaloCells; h += kUpdThreads) {
@@ -987,7 +996,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       const int32_t lo = cs[t] > base ? cs[t] : base;
       const int32_t hi = cs[t + 1] < cend ? cs[t + 1] : cend;
       for (int32_t i = lo; i < hi; ++i)
-        if (stayed[i - base]) a.arrival_out[i] = srank++;
+        if (stayed[i - base]) a.arrival_out[i] = seg_tag | srank++;
     }
     // ordered per-cell reduction of this chunk
     const int32_t chunk_end = (base + kUpdThreads < d1) ? base + kUpdThreads : d1;
@@ -1012,6 +1021,18 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     __syncthreads();
   }
 
+  if (slot >= 0) {
+    // heavy tile: this segment's sums and stayer counts, combined later
+    double* P = a.sp.part + static_cast<long long>(slot + seg) * (kTileCells * NSRC);
+#pragma unroll
+    for (int r = 0; r < NITEM; ++r) {
+      const int item = t + r * kUpdThreads;
+      if (item < kTileCells * NSRC) P[item] = acc[r];
+    }
+    if (t < kTileCells) a.sp.seg_stay[(slot + seg) * kTileCells + t] = srank;
+    if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
+    return;
+  }
   if (t < kTileCells) a.stay_cnt[tile * kTileCells + t] = srank;
 
   // add each cell's sums to the source FAB: a thread's items are distinct
@@ -1067,8 +1088,68 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
   __shared__ unsigned char redir[EB ? kUpdThreads : 1];
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
-  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB>(a, xcd_tile(blockIdx.x, ntiles), gas, gvalid, cs, contrib,
-                                                      stayed, nullptr, n_surr, n_fb, redir);
+  // workgroups 0 .. ntiles-1: the first segment of every tile; the others:
+  // the further segments of the heavy tiles
+  int tile, seg = 0;
+  if (static_cast<int>(blockIdx.x) < ntiles) {
+    tile = xcd_tile(blockIdx.x, ntiles);
+  } else {
+    const int j = blockIdx.x - ntiles;
+    if (j >= a.sp.counters[0]) return;
+    const int2 it = a.sp.items[j];
+    tile = it.x;
+    seg = it.y;
+  }
+  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs,
+                                                          contrib, stayed, nullptr, n_surr, n_fb, redir);
+}
+
+// The sums of the heavy tiles (SprayParticles.cpp:419-441): per cell and
+// component, the segments' partial sums added in segment order, then added
+// to the source FAB by the cell's owner; per cell, the stayer counts of the
+// segments turned into offsets (for the re-sort) and their total.
+template <class R, int NF>
+__global__ void __launch_bounds__(kUpdThreads) split_combine_kernel(UpdateArgs<R, NF> a) {
+  constexpr int NSRC = 5 + NF;
+  const int t = threadIdx.x;
+  const int nsplit = a.sp.counters[1];
+  const int txy = a.ntile[0] * a.ntile[1];
+  for (int i = blockIdx.x; i < nsplit; i += gridDim.x) {
+    const int4 e = a.sp.list[i];
+    const int tile = e.x, slot = e.y, nseg = e.z;
+    const int otz = (tile / txy) * kTile;
+    const int oty = ((tile % txy) / a.ntile[0]) * kTile;
+    const int otx = (tile % a.ntile[0]) * kTile;
+    for (int item = t; item < kTileCells * NSRC; item += kUpdThreads) {
+      const int c = item / NSRC, q = item % NSRC;
+      if (a.cstart[tile * kTileCells + c + 1] == a.cstart[tile * kTileCells + c]) continue;
+      if (q < 3 && !a.fc.mom_trans) continue;
+      if (q >= 3 && q < 4 + NF && !a.fc.mass_trans) continue;
+      double total = 0.;
+      for (int k = 0; k < nseg; ++k) total = total + a.sp.part[static_cast<long long>(slot + k) * (kTileCells * NSRC) + item];
+      const int comp = (q < 3) ? a.c_msrc + q : (q == 3) ? a.c_rsrc : (q < 4 + NF) ? a.c_ssrc + (q - 4) : a.c_esrc;
+      const int gx = a.dom_lo[0] + otx + (c % kTile) - a.src.lo[0];
+      const int gy = a.dom_lo[1] + oty + ((c / kTile) % kTile) - a.src.lo[1];
+      const int gz = a.dom_lo[2] + otz + c / (kTile * kTile) - a.src.lo[2];
+      if (gx < 0 || gx >= a.src.n[0] || gy < 0 || gy >= a.src.n[1] || gz < 0 || gz >= a.src.n[2]) {
+        if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
+        continue;
+      }
+      double* dst = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
+      const double old = *dst;
+      *dst = a.pelec ? old + total * a.src_scale[q] : old + total;
+    }
+    if (t < kTileCells) {
+      int32_t off = 0;
+      for (int k = 0; k < nseg; ++k) {
+        int32_t* cnt = &a.sp.seg_stay[(slot + k) * kTileCells + t];
+        const int32_t v = *cnt;
+        *cnt = off;
+        off += v;
+      }
+      a.stay_cnt[tile * kTileCells + t] = off;
+    }
+  }
 }
 
 // Surrogate form: gridDim.x workgroups (a multiple of 8) loop over the
@@ -1087,9 +1168,21 @@ __global__ void __launch_bounds__(kUpdThreads, 2) spray_update_surr_kernel(Updat
   __syncthreads();
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
-  for (int b = blockIdx.x; b < ntiles; b += gridDim.x)
-    process_tile<M, NF, float, false, false, true>(a, xcd_tile(b, ntiles), gas, gvalid, cs, contrib, stayed, &sw,
-                                                   n_surr, n_fb);
+  // the first segment of every tile, then the further segments of the
+  // heavy tiles
+  const int nwork = ntiles + a.sp.counters[0];
+  for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
+    int tile, seg = 0;
+    if (b < ntiles) {
+      tile = xcd_tile(b, ntiles);
+    } else {
+      const int2 it = a.sp.items[b - ntiles];
+      tile = it.x;
+      seg = it.y;
+    }
+    process_tile<M, NF, float, false, false, true>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs, contrib,
+                                                   stayed, &sw, n_surr, n_fb);
+  }
   // one pair of counter updates per wavefront
   unsigned long long s1 = n_surr, s2 = n_fb;
 #pragma unroll

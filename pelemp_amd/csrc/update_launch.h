@@ -72,7 +72,10 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.st_g = p.state_ghosts;
   a.src_g = p.source_ghosts;
   a.err = c.err;
-  const dim3 grid(c.ntiles), block(kUpdThreads);
+  a.sp = c.sp;
+  // one workgroup per tile, then one per further segment of a heavy tile
+  // (the workgroups beyond the pass's work items return at once)
+  const dim3 grid(c.ntiles + c.split_items_cap), block(kUpdThreads);
   if constexpr (SURR) {
     const pelemp_surrogate_norm& n = c.surr_norm;
     for (int q = 0; q < kSurrIn; ++q) {
@@ -156,6 +159,9 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   } else {
     hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, false>), grid, block, 0, s, a);
   }
+  // the heavy tiles' segment sums, added to the source in segment order
+  const int cgrid = c.split_list_cap < 1024 ? c.split_list_cap : 1024;
+  hipLaunchKernelGGL((split_combine_kernel<R, NF>), dim3(cgrid), block, 0, s, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_last_error(std::string("update launch: ") + hipGetErrorString(e));
