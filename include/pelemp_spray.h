@@ -107,6 +107,14 @@ typedef struct pelemp_fab_view {
   int32_t ncomp;
 } pelemp_fab_view;
 
+/* The same FAB in single precision (a gas state kept in FP32). */
+typedef struct pelemp_fab_view_f32 {
+  float* data;
+  int32_t lo[3];
+  int32_t hi[3];
+  int32_t ncomp;
+} pelemp_fab_view_f32;
+
 /* Level geometry: physical box, cell size, valid cell-index domain and
  * boundary kinds per face (0 periodic, 1 reflective wall, -1 outflow),
  * as built at SprayParticles.cpp:183-218. */
@@ -366,6 +374,20 @@ int pelemp_spray_update(pelemp_spray_ctx* ctx, const pelemp_fab_view* state,
                         int32_t state_ghosts, int32_t source_ghosts,
                         int32_t isVirt, int32_t isGhost, int32_t do_move,
                         double spray_cfl_lev, void* stream);
+
+/* pelemp_spray_update with the gas state in single precision: the state
+ * MultiFab of a solver that keeps its fields in FP32 (updateParticles reads
+ * the state as Real; here Real = float for the state only).  The state is
+ * widened to double exactly where the droplet model computes in FP64, so the
+ * FP64 form gives what pelemp_spray_update gives for the same values stored
+ * as doubles.  The sources stay FP64.  PeleLMeX flavour. */
+int pelemp_spray_update_f32gas(pelemp_spray_ctx* ctx,
+                               const pelemp_fab_view_f32* state,
+                               pelemp_fab_view* source, double flow_dt,
+                               double time, int32_t state_ghosts,
+                               int32_t source_ghosts, int32_t isVirt,
+                               int32_t isGhost, int32_t do_move,
+                               double spray_cfl_lev, void* stream);
 
 /* Waits for `stream` and returns the sticky device-side error recorded by
  * earlier updates (PELEMP_E_STATE_BOX, _SRC_BOX, _LEFT_DOMAIN, ...), then

@@ -83,6 +83,11 @@ class FabViewC(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("lo", _c_i32 * 3), ("hi", _c_i32 * 3), ("ncomp", _c_i32)]
 
 
+class FabViewF32C(ctypes.Structure):
+    """pelemp_fab_view_f32: the same FAB with float data."""
+    _fields_ = [("data", ctypes.c_void_p), ("lo", _c_i32 * 3), ("hi", _c_i32 * 3), ("ncomp", _c_i32)]
+
+
 class GeomC(ctypes.Structure):
     _fields_ = [
         ("plo", _c_dbl * 3), ("phi", _c_dbl * 3), ("dx", _c_dbl * 3),
@@ -168,6 +173,8 @@ _PROTOS = {
     "pelemp_spray_restart": (_c_i32, [_vp, ctypes.c_char_p]),
     "pelemp_spray_update": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl, _c_dbl,
                                      _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
+    "pelemp_spray_update_f32gas": (_c_i32, [_vp, ctypes.POINTER(FabViewF32C), ctypes.POINTER(FabViewC), _c_dbl,
+                                            _c_dbl, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
     "pelemp_spray_sync": (_c_i32, [_vp, _vp]),
     "pelemp_spray_move_kick_drift": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl,
                                               _c_dbl, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
@@ -314,8 +321,8 @@ def jet_c(jet_cent, jet_norm, spread_angle: float, jet_dia: float, jet_vel: floa
     return j
 
 
-def fab_view(ptr: int, lo, hi, ncomp: int) -> FabViewC:
-    v = FabViewC()
+def fab_view(ptr: int, lo, hi, ncomp: int, f32: bool = False):
+    v = FabViewF32C() if f32 else FabViewC()
     v.data = ptr
     fill_array(v.lo, lo)
     fill_array(v.hi, hi)

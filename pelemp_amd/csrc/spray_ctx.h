@@ -27,7 +27,8 @@ struct PassParams {
   int branches;        // film / breakup / splash kernel variant
   int make_new_drops;  // SprayParticles.cpp:298-299
   int do_splash_box;   // splash on (walls present)
-  const pelemp_fab_view* state;
+  const pelemp_fab_view* state;  // box and components of the state (data is null for an FP32 state)
+  const float* state32;          // FP32 state data, or null
   pelemp_fab_view* source;
 };
 

@@ -2061,15 +2061,23 @@ int pelemp_spray_set_eb(pelemp_spray_ctx* h, const pelemp_eb_view* eb) {
   return PELEMP_OK;
 }
 
-int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,
-                        double flow_dt, double time, int32_t state_ghosts, int32_t source_ghosts, int32_t isVirt,
-                        int32_t isGhost, int32_t do_move, double spray_cfl_lev, void* stream) {
-  (void)time;
-  if (!h || !state || !source || !state->data || !source->data) {
+}  // extern "C"
+
+namespace pelemp {
+// updateParticles for a state in FP64 (state->data) or FP32 (state32, with
+// the box and components in `state`).
+static int update_impl(pelemp_spray_ctx* h, const pelemp_fab_view* state, const float* state32,
+                       pelemp_fab_view* source, double flow_dt, int32_t state_ghosts, int32_t source_ghosts,
+                       int32_t isVirt, int32_t isGhost, int32_t do_move, double spray_cfl_lev, void* stream) {
+  if (!h || !state || !source || !(state->data || state32) || !source->data) {
     set_last_error("invalid argument");
     return PELEMP_E_ARG;
   }
   Ctx& c = h->c;
+  if (state32 && c.pelec) {
+    set_last_error("an FP32 gas state is supported in the PeleLMeX flavour only");
+    return PELEMP_E_BAD_CONFIG;
+  }
   const pelemp_spray_comps& S = c.comps;
   const int st_need = std::max({S.rhoIndx, S.momIndx + 2, S.specIndx + c.ns - 1, S.utempIndx});
   const int sr_need = std::max({S.rhoSrcIndx, S.momSrcIndx + 2, S.specSrcIndx + c.nf - 1, S.engSrcIndx});
@@ -2110,6 +2118,7 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
   for (int d = 0; d < 3; ++d)
     if (c.geom.bndry_lo[d] != 0 || c.geom.bndry_hi[d] != 0) p.at_bounds = 1;
   p.state = state;
+  p.state32 = state32;
   p.source = source;
   // branch flags (SprayParticles.cpp:162-164,289-299); the splash model
   // runs where walls are (tile_at_bndry, WallFunctions.H:7-35)
@@ -2162,6 +2171,42 @@ int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelem
     if (st2 != PELEMP_OK) return st2;
   }
   return PELEMP_OK;
+}
+
+}  // namespace pelemp
+
+extern "C" {
+
+int pelemp_spray_update(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,
+                        double flow_dt, double time, int32_t state_ghosts, int32_t source_ghosts, int32_t isVirt,
+                        int32_t isGhost, int32_t do_move, double spray_cfl_lev, void* stream) {
+  (void)time;
+  if (state && !state->data) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  return update_impl(h, state, nullptr, source, flow_dt, state_ghosts, source_ghosts, isVirt, isGhost, do_move,
+                     spray_cfl_lev, stream);
+}
+
+int pelemp_spray_update_f32gas(pelemp_spray_ctx* h, const pelemp_fab_view_f32* state, pelemp_fab_view* source,
+                               double flow_dt, double time, int32_t state_ghosts, int32_t source_ghosts,
+                               int32_t isVirt, int32_t isGhost, int32_t do_move, double spray_cfl_lev,
+                               void* stream) {
+  (void)time;
+  if (!state || !state->data) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  pelemp_fab_view box{};
+  box.data = nullptr;
+  for (int d = 0; d < 3; ++d) {
+    box.lo[d] = state->lo[d];
+    box.hi[d] = state->hi[d];
+  }
+  box.ncomp = state->ncomp;
+  return update_impl(h, &box, state->data, source, flow_dt, state_ghosts, source_ghosts, isVirt, isGhost, do_move,
+                     spray_cfl_lev, stream);
 }
 
 int pelemp_spray_move_kick_drift(pelemp_spray_ctx* h, const pelemp_fab_view* state, pelemp_fab_view* source,

@@ -51,10 +51,16 @@ __device__ inline void set_err(int* err, int code) { atomicCAS(err, 0, code); }
 
 struct FabDev {
   double* data;
+  const float* data32;  // a gas state kept in FP32 (data is then unused)
   int lo[3];
   int n[3];
   long long cstride;  // cells per component
 };
+
+// Word i of a state FAB, widened to double when the FAB is FP32 (exact).
+__device__ __attribute__((always_inline)) inline double state_at(const FabDev& f, long long i) {
+  return f.data32 ? static_cast<double>(f.data32[i]) : f.data[i];
+}
 
 // Normalisation of the surrogate inputs / outputs as used in the kernel.
 struct SurrKNorm {
@@ -382,7 +388,6 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
           return;
         }
         const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
-        const double* S = a.st.data;
         if constexpr (sizeof(R) == 8) {
           if (a.pelec) {
             const double* V = a.conv + c;
@@ -396,14 +401,15 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
             return;
           }
         }
-        const double rho = S[a.c_rho * a.st.cstride + c];
+        const double rho = state_at(a.st, a.c_rho * a.st.cstride + c);
         const double inv_rho = 1. / rho;
         g.rho += cw * static_cast<R>(rho);
 #pragma unroll
-        for (int n = 0; n < NS; ++n) g.Y[n] += cw * static_cast<R>(S[(a.c_spec + n) * a.st.cstride + c] * inv_rho);
+        for (int n = 0; n < NS; ++n)
+          g.Y[n] += cw * static_cast<R>(state_at(a.st, (a.c_spec + n) * a.st.cstride + c) * inv_rho);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
-        g.T += cw * static_cast<R>(S[a.c_T * a.st.cstride + c]);
+        for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(state_at(a.st, (a.c_vel + q) * a.st.cstride + c));
+        g.T += cw * static_cast<R>(state_at(a.st, a.c_T * a.st.cstride + c));
         return;
       }
     }
@@ -938,17 +944,16 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     }
     if (ok) {
       const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
-      const double* S = a.st.data;
-      const double rho = S[a.c_rho * a.st.cstride + c];
+      const double rho = state_at(a.st, a.c_rho * a.st.cstride + c);
       const double inv_rho = 1. / rho;
       gas[0 * kHaloCells + h] = static_cast<R>(rho);
 #pragma unroll
       for (int n = 0; n < NS; ++n)
-        gas[(1 + n) * kHaloCells + h] = static_cast<R>(S[(a.c_spec + n) * a.st.cstride + c] * inv_rho);
+        gas[(1 + n) * kHaloCells + h] = static_cast<R>(state_at(a.st, (a.c_spec + n) * a.st.cstride + c) * inv_rho);
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        gas[(1 + NS + q) * kHaloCells + h] = static_cast<R>(S[(a.c_vel + q) * a.st.cstride + c]);
-      gas[(4 + NS) * kHaloCells + h] = static_cast<R>(S[a.c_T * a.st.cstride + c]);
+        gas[(1 + NS + q) * kHaloCells + h] = static_cast<R>(state_at(a.st, (a.c_vel + q) * a.st.cstride + c));
+      gas[(4 + NS) * kHaloCells + h] = static_cast<R>(state_at(a.st, a.c_T * a.st.cstride + c));
     } else {
       // outside the state box: a droplet reaching it is an error; the
       // cleared cell keeps what it reads defined

@@ -11,6 +11,7 @@ namespace pelemp {
 inline FabDev make_fab(const pelemp_fab_view& v) {
   FabDev f;
   f.data = v.data;
+  f.data32 = nullptr;
   for (int d = 0; d < 3; ++d) {
     f.lo[d] = v.lo[d];
     f.n[d] = v.hi[d] - v.lo[d] + 1;
@@ -37,6 +38,7 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.mov_hi = c.mov_hi;
   a.stay_cnt = c.stay_cnt;
   a.st = make_fab(*p.state);
+  a.st.data32 = p.state32;
   a.c_rho = c.comps.rhoIndx;
   a.c_vel = c.comps.momIndx;
   a.c_spec = c.comps.specIndx;

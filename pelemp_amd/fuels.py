@@ -51,6 +51,29 @@ FUELS = {
     ),
 }
 
+# n-alkanes of the six-component liquid (C8-C16).  The reference tree holds no
+# property set for these species (its decks cover n-dodecane, n-decane and
+# n-heptane), so these are literature values (NIST Chemistry WebBook and Yaws,
+# Thermophysical Properties of Chemicals and Hydrocarbons): critical and
+# normal boiling temperatures, heat of vaporisation at 298.15 K, liquid heat
+# capacity and density at 298 K.  The vapour pressure uses the reference's
+# Clausius-Clapeyron branch (psat_coef[3] == 0, Drag.H:78-84) from the boiling
+# point and latent heat; lambda and mu are not used by the droplet model
+# without films or breakup and are left zero.
+def _alkane(critT, boilT, dh_kj_mol, cp_j_molk, rho, mw):
+    return dict(critT=critT, boilT=boilT, latent=dh_kj_mol * 1.0e3 / mw, cp=cp_j_molk / mw,
+                lambda_coef=[0.0, 0.0, 0.0, 0.0], mu_coef=[0.0, 0.0, 0.0, 0.0], rho_coef=[rho, 0.0, 0.0, 0.0],
+                psat_coef=[0.0, 0.0, 0.0, 0.0], sigma=None, ref_T=298.15)
+
+
+FUELS["NC8H18"] = _alkane(568.7, 398.8, 41.5, 254.6, 699.0, 0.114232)
+FUELS["NC13H28"] = _alkane(675.0, 508.6, 66.7, 406.7, 753.0, 0.184367)
+FUELS["NC14H30"] = _alkane(693.0, 526.7, 71.3, 438.0, 759.0, 0.198394)
+FUELS["NC16H34"] = _alkane(723.0, 560.0, 81.4, 501.6, 770.0, 0.226448)
+
+# the six liquid species of the alkane mechanism, in its species order
+ALKANE6 = ("NC8H18", "NC10H22", "NC12H26", "NC13H28", "NC14H30", "NC16H34")
+
 
 def spray_data(fuels, species, *, mass_trans=True, mom_trans=True, fixed_parts=False, dtmod=1.0,
                body_force=(0.0, 0.0, 0.0), do_breakup=0, do_splash=False, sigma=None, wall_T=-1.0,

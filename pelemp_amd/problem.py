@@ -165,8 +165,8 @@ def build_problem(name: str, *, ncell=(32, 32, 32), dx=1.0e-4, p=PATM, T0=1000.0
         y1 = rng.uniform(lo, hi, n)
         rdata[:, 5] = y1
         rdata[:, 6] = 1.0 - y1
-        if nf > 2:
-            raise ValueError("use build_problem_multi for nf > 2")
+        if nf > 2 and n > 0:
+            raise ValueError("nf > 2: liquid compositions come from config5's Dirichlet draw")
     rdata[:, 5 + nf] = 1.0  # numDens
     rdata[:, 6 + nf] = 1.0  # N0
     ids = np.arange(1, n + 1, dtype=np.int64)
@@ -254,6 +254,89 @@ def config3(ncell=(128, 128, 256), drops_per_cell: float = 3.0e7 / (128 * 128 * 
                         ids=np.arange(1, n + 1, dtype=np.int64), flow_dt=flow_dt, nsteps=nsteps, state_ghosts=g,
                         source_ghosts=sg, meta=dict(p=p, T0=T0, wall_T=wall_T, theta_c_deg=theta_c_deg,
                                                     num_dens=num_dens, seed=seed))
+
+
+def hollow_cone_droplets(n: int, rng, L, nozzle, *, half_angle_deg: float = 10.0, jitter_deg: float = 3.0,
+                         speed: float = 200.0, speed_jitter: float = 0.1, s_range=(0.01, 0.86), s_exp: float = 1.0,
+                         d_median: float = 10e-6, sigma_ln: float = 0.5, nf: int = 6, alpha: float = 2.0,
+                         T_d: float = 300.0):
+    """Droplets of a hollow-cone jet along +z from `nozzle`: direction at
+    polar angle half_angle_deg + U(-jitter, jitter) (degrees) and a uniform
+    azimuth (the hollow-cone draw of SprayJet::get_new_particle,
+    SprayJet.cpp:148-184, with the cone's rim angle), speed speed x (1 +
+    U(-speed_jitter, speed_jitter)) along that direction, placed at a travelled
+    distance s with density ~ s^s_exp on [s_range] x the box height (s_exp =
+    1: droplets per cell fall off as 1/s along the cone, so the cells near
+    the nozzle hold thousands), lognormal diameters (median, sigma_ln),
+    liquid mass fractions from a Dirichlet(alpha) draw over the nf species,
+    temperature T_d, one droplet per parcel.  Returns (pos, rdata)."""
+    theta = np.radians(half_angle_deg + rng.uniform(-jitter_deg, jitter_deg, n))
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    d = np.stack([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)], axis=1)
+    a = s_exp + 1.0
+    s0, s1 = s_range[0] * L[2], s_range[1] * L[2]
+    sdist = (s0 ** a + rng.random(n) * (s1 ** a - s0 ** a)) ** (1.0 / a)
+    pos = np.asarray(nozzle)[None, :] + sdist[:, None] * d
+    pos = np.mod(pos, np.asarray(L))
+    pos = np.minimum(pos, np.nextafter(np.asarray(L), 0.0))
+    umag = speed * (1.0 + rng.uniform(-speed_jitter, speed_jitter, n))
+    rdata = np.zeros((n, 10 + nf))
+    rdata[:, 0:3] = umag[:, None] * d
+    rdata[:, 3] = T_d
+    rdata[:, 4] = d_median * np.exp(sigma_ln * rng.standard_normal(n))
+    rdata[:, 5:5 + nf] = rng.dirichlet(np.full(nf, alpha), n) if nf > 1 else 1.0
+    rdata[:, 5 + nf] = 1.0
+    rdata[:, 6 + nf] = 1.0
+    return np.ascontiguousarray(pos), np.ascontiguousarray(rdata)
+
+
+def config5(ncell: int = 256, n_drops: float = 2.0e8, seed: int = 5555, droplets: bool = True,
+            **jet_kw) -> SprayProblem:
+    """Config 5 (dense uneven jet): ncell^3 periodic box, dx = 1e-4 m, gas at
+    60 atm / 800 K with the sinusoidal recipe (no mean flow; the state is to
+    be handed to the update in FP32), n_drops droplets of a hollow-cone jet
+    (half-angle 10 deg +- 3 deg, 200 m/s +- 10 %, nozzle on the axis at 4 %
+    of the box height, see hollow_cone_droplets) -- 0 droplets in most cells
+    and hundreds to thousands in the cells of the cone -- of a six-component
+    C8-C16 n-alkane liquid (Dirichlet(2) mass fractions), median diameter
+    10 um (sigma_ln 0.5, recorded here), 20 steps of 2e-7 s.  With
+    droplets=False only the gas and geometry are built (jet_droplets then
+    draws the droplets)."""
+    from .fuels import ALKANE6
+
+    prob = build_problem("config5", ncell=(ncell, ncell, ncell), p=60.0 * PATM, T0=800.0, U0=0.0,
+                         mech=C.MECH_ALKANE6_AIR, fuels=ALKANE6, n_drops=0, flow_dt=2e-7, nsteps=20, seed=seed)
+    L = prob.geom["phi"]
+    nozzle = np.array([0.5 * L[0], 0.5 * L[1], 0.04 * L[2]])
+    prob.meta.update(nozzle=nozzle.tolist(), n_target=int(n_drops), jet=dict(jet_kw))
+    if droplets and n_drops > 0:
+        prob.pos, prob.rdata, prob.ids = jet_droplets(prob, int(n_drops))
+    return prob
+
+
+def jet_droplets(prob: SprayProblem, n_drops: int, threads: int | None = None):
+    """The n_drops droplets of a config5 problem as (pos, rdata, ids), ids
+    1..n_drops.  Blocks of 2^20 droplets have their own seeded generators
+    and are drawn by a thread pool (numpy's generators release the GIL), so
+    the set does not depend on the thread count."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    L = prob.geom["phi"]
+    nozzle = np.array(prob.meta["nozzle"])
+    block = 1 << 20
+    nblk = (n_drops + block - 1) // block
+    pos = np.empty((n_drops, 3))
+    rdata = np.empty((n_drops, 10 + prob.nf))
+
+    def fill(b):
+        lo, hi = b * block, min(n_drops, (b + 1) * block)
+        rng = np.random.default_rng([prob.meta["seed"], b])
+        pos[lo:hi], rdata[lo:hi] = hollow_cone_droplets(hi - lo, rng, L, nozzle, nf=prob.nf, **prob.meta["jet"])
+
+    nt = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=nt) as ex:
+        list(ex.map(fill, range(nblk)))
+    return pos, rdata, np.arange(1, n_drops + 1, dtype=np.int64)
 
 
 def config1(scale: float = 1.0, seed: int = 1234, **kw) -> SprayProblem:

@@ -40,32 +40,41 @@ from . import _capi as C
 
 class DeviceFab:
     """One FAB in device memory: box [lo, hi] (incl. ghosts), ncomp components,
-    component-major with x fastest (the MultiFab data of one box)."""
+    component-major with x fastest (the MultiFab data of one box).  dtype
+    float32 gives a single-precision FAB (a gas state kept in FP32, for
+    updateParticles)."""
 
-    def __init__(self, lo, hi, ncomp: int, data: Optional[np.ndarray] = None):
+    def __init__(self, lo, hi, ncomp: int, data: Optional[np.ndarray] = None, dtype=np.float64):
         self.lo = tuple(int(v) for v in lo)
         self.hi = tuple(int(v) for v in hi)
         self.ncomp = int(ncomp)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise ValueError("a FAB holds float64 or float32 data")
         self.shape = (self.ncomp, self.hi[2] - self.lo[2] + 1, self.hi[1] - self.lo[1] + 1,
                       self.hi[0] - self.lo[0] + 1)
-        self.buf = C.DeviceBuffer(8 * int(np.prod(self.shape)))
+        self.buf = C.DeviceBuffer(self.dtype.itemsize * int(np.prod(self.shape)))
         if data is not None:
             if tuple(data.shape) != self.shape:
                 raise ValueError(f"FAB data shape {data.shape} != {self.shape}")
-            self.buf.upload(np.ascontiguousarray(data, dtype=np.float64))
+            self.buf.upload(np.ascontiguousarray(data, dtype=self.dtype))
         else:
             self.buf.zero()
-        self._view = C.fab_view(self.buf.ptr, self.lo, self.hi, self.ncomp)
+        self._view = C.fab_view(self.buf.ptr, self.lo, self.hi, self.ncomp, f32=self.is_f32)
 
-    def view(self) -> C.FabViewC:
+    @property
+    def is_f32(self) -> bool:
+        return self.dtype == np.dtype(np.float32)
+
+    def view(self):
         return self._view
 
     def numpy(self) -> np.ndarray:
-        return self.buf.download(np.empty(self.shape, dtype=np.float64))
+        return self.buf.download(np.empty(self.shape, dtype=self.dtype))
 
     def setVal(self, v: float) -> None:
         if v != 0.0:
-            self.buf.upload(np.full(self.shape, v, dtype=np.float64))
+            self.buf.upload(np.full(self.shape, v, dtype=self.dtype))
         else:
             self.buf.zero()
 
@@ -278,9 +287,10 @@ class SprayParticleContainer:
             C.check(self._lib.pelemp_spray_set_params(self._h, ctypes.byref(p)), "pelemp_spray_set_params")
         sv, dv = state.view(), source.view()
         h = _stream_handle(stream)
-        C.check(self._lib.pelemp_spray_update(self._h, ctypes.byref(sv), ctypes.byref(dv), float(flow_dt),
-                                              float(time), int(state_ghosts), int(source_ghosts), int(isVirt),
-                                              int(isGhost), int(do_move), float(spray_cfl_lev), h),
+        # a single-precision state goes through the FP32-state entry point
+        fn = self._lib.pelemp_spray_update_f32gas if getattr(state, "is_f32", False) else self._lib.pelemp_spray_update
+        C.check(fn(self._h, ctypes.byref(sv), ctypes.byref(dv), float(flow_dt), float(time), int(state_ghosts),
+                   int(source_ghosts), int(isVirt), int(isGhost), int(do_move), float(spray_cfl_lev), h),
                 "pelemp_spray_update")
         if sync:
             self.sync(stream)

@@ -50,6 +50,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         "pelemp_spray_data": C.SprayDataC,
         "pelemp_spray_comps": C.SprayCompsC,
         "pelemp_fab_view": C.FabViewC,
+        "pelemp_fab_view_f32": C.FabViewF32C,
         "pelemp_geom": C.GeomC,
         "pelemp_spray_params": C.SprayParamsC,
         "pelemp_surrogate_norm": C.SurrogateNormC,
