@@ -171,6 +171,92 @@ def run_branches(steps: int, warmup: int, stream, cap_factor: float = 5.0):
                 branch_compaction_frac=branch_ms * steps / ms_total)
 
 
+def run_jet(steps: int, warmup: int, stream, n_drops: int, ncell: int = 256, forms=("fp32", "fp64")):
+    """Config 5 (dense uneven jet): ncell^3 box at 60 atm / 800 K, n_drops
+    droplets of a hollow-cone jet with a six-component alkane liquid, the
+    gas state in FP32, 20-step runs at dt = 2e-7 s.  Reports per form the
+    droplet-steps/s and, from the droplets' cells, the load of the update
+    workgroups: droplets per tile (one workgroup per tile) and per work item
+    once tiles above PELEMP_SEGMENT_DROPLETS are split into segments."""
+    from pelemp_amd import _capi as C
+    from pelemp_amd import problem as P
+    from pelemp_amd.spray import DeviceFab, SprayParticleContainer
+
+    t0 = time.perf_counter()
+    prob = P.config5(ncell=ncell, n_drops=n_drops, droplets=False)
+    pos, rd, ids = P.jet_droplets(prob, n_drops)
+    gen_s = time.perf_counter() - t0
+    print(f"# config5 built: {n_drops} droplets on {ncell}^3 cells in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    state32 = prob.state.astype(np.float32)
+    # workgroup load from the droplets' tiles
+    dx = prob.geom["dx"][0]
+    nt = ncell // 4
+    c = np.floor(pos * (1.0 / dx)).astype(np.int64)
+    np.clip(c, 0, ncell - 1, out=c)
+    tiles = np.bincount(((c[:, 2] // 4) * nt + c[:, 1] // 4) * nt + c[:, 0] // 4, minlength=nt ** 3)
+    cells = np.bincount((c[:, 2] * ncell + c[:, 1]) * ncell + c[:, 0], minlength=ncell ** 3)
+    del c
+    seg = C.SEGMENT_DROPLETS
+    occ = tiles[tiles > 0]
+    nseg = (occ + seg - 1) // seg
+    items = np.concatenate([np.minimum(seg, occ - k * seg) for k in range(int(nseg.max()))])
+    items = items[items > 0]
+    out = dict(workload=f"config5: {ncell}^3 periodic box dx=1e-4 m, 60 atm / 800 K gas in FP32, {n_drops} droplets "
+                        "of a hollow-cone jet (10 deg +- 3 deg, 200 m/s +- 10 %), six C8-C16 n-alkanes "
+                        "(Dirichlet(2)), median 10 um, dt=2e-7 s",
+               droplets=n_drops, generate_s=gen_s, cells_occupied=int((cells > 0).sum()),
+               max_per_cell=int(cells.max()), cells_over_500=int((cells > 500).sum()),
+               tiles_occupied=int(occ.size), max_per_tile=int(occ.max()), mean_per_tile=float(occ.mean()),
+               tile_imbalance=float(occ.max() / occ.mean()), work_items=int(items.size),
+               max_per_item=int(items.max()), mean_per_item=float(items.mean()),
+               item_imbalance=float(items.max() / items.mean()), forms={})
+    del cells
+    for name in forms:
+        form = {"fp64": C.FORM_FP64, "fp32": C.FORM_FP32}[name]
+        pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, int(n_drops * 1.01) + 1024,
+                                    form=form)
+        t1 = time.perf_counter()
+        pc.set_particles(pos, rd, ids)
+        set_s = time.perf_counter() - t1
+        state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], state32, dtype=np.float32)
+        src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+        h = stream.handle
+
+        def step():
+            C.check(C.lib().pelemp_fab_set_zero(src.buf.ptr, src.buf.nbytes // 8, h), "zero source")
+            pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False,
+                               False, True, None, -1.0, stream=h, sync=False)
+
+        for _ in range(warmup):
+            step()
+        pc.sync(h)
+        pc.pass_counts()
+        pc.set_timing(True)
+        timer = C.Timer()
+        C.device_synchronize()
+        timer.start(h)
+        for _ in range(steps):
+            step()
+        timer.stop(h)
+        stream.synchronize()
+        pc.sync(h)
+        ms = timer.elapsed_ms()
+        counts = pc.pass_counts()
+        kms, kn = pc.get_timing()
+        pc.set_timing(False)
+        ds = int(sum(counts))
+        out["forms"][name] = dict(value=ds / (ms * 1e-3), ms_per_step=ms / steps, update_kernel_ms=kms[0] / max(kn[0], 1),
+                                  sort_ms=kms[1] / max(kn[1], 1), droplet_steps=ds, droplets_end=pc.num_particles(),
+                                  set_particles_s=set_s)
+        print(f"# config5 {name}: {ds / (ms * 1e-3):.4g} droplet-steps/s, {ms / steps:.3f} ms/step, update kernel "
+              f"{kms[0] / max(kn[0], 1):.3f} ms", file=sys.stderr, flush=True)
+        pc.close()
+        del state, src
+    if "fp64" in out["forms"] and "fp32" in out["forms"]:
+        out["fp32_speedup_vs_fp64"] = out["forms"]["fp32"]["value"] / out["forms"]["fp64"]["value"]
+    return out
+
+
 def cpu_baseline(prob, target_s: float = 12.0):
     """Time the CPU oracle on a bounded sample of the workload."""
     from oracle import oracle_py as O
@@ -365,6 +451,9 @@ def main(argv=None):
     ap.add_argument("--no-branches", action="store_true", help="skip the config-3 (walls/breakup/splash) leg")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the leg timing transferSource, estTimestep, derived variables, PeleC, EB, injection")
+    ap.add_argument("--no-jet", action="store_true", help="skip the config-5 (dense hollow-cone jet) leg")
+    ap.add_argument("--jet-drops", type=float, default=2.0e8, help="droplets of the config-5 leg")
+    ap.add_argument("--jet-only", action="store_true", help="run only the config-5 leg (its own JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the 1 GiB copy kernels used to calibrate PMC byte counters")
     args = ap.parse_args(argv)
@@ -378,6 +467,10 @@ def main(argv=None):
     from pelemp_amd import _capi as C
     from pelemp_amd import problem as P
 
+    if args.jet_only:
+        out = run_jet(args.steps, args.warmup, C.Stream(), int(args.jet_drops))
+        print(json.dumps(out))
+        return 0
     t0 = time.perf_counter()
     prob = P.config2(ncell=args.ncell)
     print(f"# problem built: {prob.n} droplets on {args.ncell}^3 cells in {time.perf_counter() - t0:.1f} s",
@@ -453,6 +546,12 @@ def main(argv=None):
         print(f"# branches (config3): {br['value']:.4g} droplet-steps/s, {br['ms_per_step']:.3f} ms/step, "
               f"film pass {br['film_pass_ms']:.3f} ms, children pass {br['children_pass_ms']:.3f} ms, "
               f"droplets {br['droplets_start']} -> {br['droplets_end']}", file=sys.stderr, flush=True)
+    if not args.no_jet:
+        try:
+            out["jet"] = run_jet(min(args.steps, 20), min(args.warmup, 2), stream, int(args.jet_drops))
+        except C.PelempError as e:
+            out["jet"] = dict(error=str(e))
+            print(f"# jet (config5) failed: {e}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(prob, args.cpu_seconds)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -100,6 +100,7 @@ SURR_NPARAM = 13251
 SURR_NIN = 8
 SURR_NOUT = 3
 MLP_NOALLOC_MAX_BATCH = 4096  # PELEMP_MLP_NOALLOC_MAX_BATCH
+SEGMENT_DROPLETS = 4096  # PELEMP_SEGMENT_DROPLETS
 _c_flt = ctypes.c_float
 
 

@@ -1320,20 +1320,32 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
   if (!h || n < 0 || (n > 0 && (!pos || !rdata || !id))) return PELEMP_E_ARG;
   Ctx& c = h->c;
   const int nsr = 10 + c.nf;
-  // keep only active particles (id > 0)
-  std::vector<double> hp, hr;
-  std::vector<int64_t> hid;
-  hp.reserve(3 * n);
-  hr.reserve(nsr * n);
-  hid.reserve(n);
-  for (int64_t i = 0; i < n; ++i) {
-    if (id[i] <= 0) continue;
-    hp.insert(hp.end(), pos + 3 * i, pos + 3 * i + 3);
-    hr.insert(hr.end(), rdata + nsr * i, rdata + nsr * i + nsr);
-    hid.push_back(id[i]);
-    if (c.pelec) cgs::droplet_to_si(&hp[hp.size() - 3], &hr[hr.size() - nsr], c.nf, c.sd.do_breakup);
+  // keep only active particles (id > 0); when all are active and no unit
+  // conversion is needed, the caller's arrays are used as they are
+  bool all_active = true;
+  for (int64_t i = 0; i < n && all_active; ++i) all_active = id[i] > 0;
+  std::vector<double> hpv, hrv;
+  std::vector<int64_t> hidv;
+  const double* hp = pos;
+  const double* hr = rdata;
+  const int64_t* hid = id;
+  int64_t m = n;
+  if (!all_active || c.pelec) {
+    hpv.reserve(3 * n);
+    hrv.reserve(nsr * n);
+    hidv.reserve(n);
+    for (int64_t i = 0; i < n; ++i) {
+      if (id[i] <= 0) continue;
+      hpv.insert(hpv.end(), pos + 3 * i, pos + 3 * i + 3);
+      hrv.insert(hrv.end(), rdata + nsr * i, rdata + nsr * i + nsr);
+      hidv.push_back(id[i]);
+      if (c.pelec) cgs::droplet_to_si(&hpv[hpv.size() - 3], &hrv[hrv.size() - nsr], c.nf, c.sd.do_breakup);
+    }
+    hp = hpv.data();
+    hr = hrv.data();
+    hid = hidv.data();
+    m = static_cast<int64_t>(hidv.size());
   }
-  const int64_t m = static_cast<int64_t>(hid.size());
   if (m > c.cap) {
     set_last_error("more particles than the context capacity");
     return PELEMP_E_ARG;
@@ -1351,9 +1363,9 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
   PM_HIP(hipMalloc(&drd, 8 * std::max<int64_t>(1, nsr * m)));
   PM_HIP(hipMalloc(&did, 8 * std::max<int64_t>(1, m)));
   if (m > 0) {
-    PM_HIP(hipMemcpy(dpos, hp.data(), 8 * 3 * m, hipMemcpyHostToDevice));
-    PM_HIP(hipMemcpy(drd, hr.data(), 8 * nsr * m, hipMemcpyHostToDevice));
-    PM_HIP(hipMemcpy(did, hid.data(), 8 * m, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(dpos, hp, 8 * 3 * m, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(drd, hr, 8 * nsr * m, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(did, hid, 8 * m, hipMemcpyHostToDevice));
   }
   c.cur = 0;
   c.nslots = 0;
