@@ -5,8 +5,10 @@ T0 = 900 K with the sinusoidal recipe, ~1.6e7 droplets (Poisson ~7.63 per
 cell) of an n-decane/n-dodecane liquid, median diameter 15 um, sigma_ln 0.5,
 drag + evaporation + full momentum/mass/energy deposition, dt = 5e-7 s,
 dtmod = 1 (one call = kick + drift).  A step is one updateParticles call
-over all droplets with the source FAB zeroed first (transferSource's
-tmpSource.setVal(0), SprayParticles.H:294).
+over all droplets that leaves the step's source in the FAB: the update runs
+in the source overwrite mode, which writes what the reference's zero fill
+(transferSource's tmpSource.setVal(0), SprayParticles.H:294) followed by
+the accumulating update gives.
 
 Prints one JSON line.  `value` is the FP64 (reference method) throughput in
 droplet-steps/s; the FP32 form is reported under "forms".  The roofline entry
@@ -60,9 +62,11 @@ def run_form(prob, form: int, steps: int, warmup: int, stream, acc_steps: int = 
     src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
     h = stream.handle
 
+    # the update writes the source's cells (the caller's setVal(0.) of the
+    # reference, SprayParticles.H:294, fused into the update)
+    pc.set_source_mode(C.SOURCE_OVERWRITE)
+
     def step():
-        # the caller's setVal(0.) of the source, in stream order
-        C.check(C.lib().pelemp_fab_set_zero(src.buf.ptr, src.buf.nbytes // 8, h), "zero source")
         pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
                            True, None, -1.0, stream=h, sync=False)
 
@@ -98,6 +102,7 @@ def run_form(prob, form: int, steps: int, warmup: int, stream, acc_steps: int = 
         # untimed accuracy pass from the initial droplets: evaporated mass =
         # sum over steps of the mass source (accumulated, += semantics) * V * dt
         pc.set_particles(prob.pos, prob.rdata, prob.ids)
+        pc.set_source_mode(C.SOURCE_ACCUMULATE)
         src.setVal(0.0)
         for _ in range(acc_steps):
             pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
@@ -134,9 +139,11 @@ def run_branches(steps: int, warmup: int, stream, cap_factor: float = 5.0):
     h = stream.handle
     n0 = prob.n
 
+    # the update writes the source's cells (the caller's setVal(0.) of the
+    # reference, SprayParticles.H:294, fused into the update)
+    pc.set_source_mode(C.SOURCE_OVERWRITE)
+
     def step():
-        # the caller's setVal(0.) of the source, in stream order
-        C.check(C.lib().pelemp_fab_set_zero(src.buf.ptr, src.buf.nbytes // 8, h), "zero source")
         pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
                            True, None, -1.0, stream=h, sync=False)
 
@@ -222,8 +229,9 @@ def run_jet(steps: int, warmup: int, stream, n_drops: int, ncell: int = 256, for
         src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
         h = stream.handle
 
+        pc.set_source_mode(C.SOURCE_OVERWRITE)
+
         def step():
-            C.check(C.lib().pelemp_fab_set_zero(src.buf.ptr, src.buf.nbytes // 8, h), "zero source")
             pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False,
                                False, True, None, -1.0, stream=h, sync=False)
 
@@ -257,40 +265,107 @@ def run_jet(steps: int, warmup: int, stream, n_drops: int, ncell: int = 256, for
     return out
 
 
-def cpu_baseline(prob, target_s: float = 12.0):
-    """Time the CPU oracle on a bounded sample of the workload."""
+def host_cpu() -> dict:
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process
+    may run on, the OpenMP thread setting and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return dict(cpu_count=os.cpu_count(), affinity=aff, omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                model=model)
+
+
+def cpu_baseline(prob, target_s: float = 12.0, parity_stream=None):
+    """Time the CPU oracle on a bounded sample of the workload, on every CPU
+    the process may use (the OpenMP thread count, when set, is the CPU share
+    the job is given).  When the sample is the whole droplet set, the GPU
+    FP64 form runs the same steps from the same droplets and is compared with
+    the oracle bit for bit (full-size parity)."""
     from oracle import oracle_py as O
 
-    nthreads = min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    nthreads = host["affinity"]
+    if host["omp_num_threads"]:
+        nthreads = min(nthreads, int(host["omp_num_threads"]))
     sd = O.latent_shifted(prob.sdata, prob.mech, len(prob.species))
 
-    def timed(stride: int, nsteps: int):
+    def timed(stride: int, nsteps: int, keep=False):
         pos, rd, ids = prob.pos[::stride], prob.rdata[::stride], prob.ids[::stride]
         t0 = time.perf_counter()
         nsum = 0
+        srcs = []
         for _ in range(nsteps):
             src = prob.new_source()
             nsum += ids.shape[0]
             st, pos, rd, ids = O.update(prob.mech, sd, prob.comps, prob.geom, pos, rd, ids, prob.state,
                                         prob.state_lo, prob.state_hi, src, prob.src_lo, prob.src_hi, prob.flow_dt,
-                                        nthreads=nthreads)
+                                        nthreads=nthreads, state_ghosts=prob.state_ghosts,
+                                        source_ghosts=prob.source_ghosts)
             assert st == 0
-        return nsum, time.perf_counter() - t0
+            if keep:
+                srcs.append(src)
+        return nsum, time.perf_counter() - t0, (pos, rd, ids, srcs)
 
     # calibrate on ~1/64 of the droplets, then size the sample for ~target_s
-    n_cal, t_cal = timed(64, 1)
+    n_cal, t_cal, _ = timed(64, 1)
     per = t_cal / max(n_cal, 1)
     want = max(1, int(target_s / per))
     stride = max(1, int(prob.n // want))
     nsteps = 1
     if stride == 1:
         nsteps = max(1, min(4, int(want // prob.n)))
-    n_s, t_s = timed(stride, nsteps)
-    return dict(value=n_s / t_s, unit="droplet-steps/s", cores=nthreads, kind="port",
-                sample=f"{nsteps} step(s) of every {stride}th droplet of the workload "
-                       f"({n_s // nsteps} of {prob.n} droplets, full 128^3 gas field), "
-                       f"FP64 oracle built with g++ -O3 -march=x86-64-v3 -fopenmp, {t_s:.1f} s",
-                seconds=t_s)
+    n_s, t_s, res = timed(stride, nsteps, keep=(stride == 1 and parity_stream is not None))
+    out = dict(value=n_s / t_s, unit="droplet-steps/s", cores=nthreads, kind="port",
+               sample=f"{nsteps} step(s) of every {stride}th droplet of the workload "
+                      f"({n_s // nsteps} of {prob.n} droplets, full 128^3 gas field), "
+                      f"FP64 oracle built with g++ -O3 -march=x86-64-v3 -fopenmp, {t_s:.1f} s",
+               seconds=t_s, host=host)
+    if stride == 1 and parity_stream is not None:
+        out["full_size_parity"] = full_size_parity(prob, nsteps, res)
+    return out
+
+
+def full_size_parity(prob, nsteps: int, oracle_result) -> dict:
+    """The GPU FP64 form over the whole workload for nsteps steps, compared
+    with the oracle's run of the same steps: droplets by id and every source
+    component of every step."""
+    from pelemp_amd.spray import DeviceFab, SprayParticleContainer
+
+    opos, ord_, oids, osrcs = oracle_result
+    pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, prob.n + 1024, form=0)
+    pc.set_particles(prob.pos, prob.rdata, prob.ids)
+    state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
+    src_equal = True
+    src_err = 0.0
+    for k in range(nsteps):
+        src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+        pc.updateParticles(0, state, src, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
+                           True, None, -1.0)
+        g = src.numpy()
+        o = osrcs[k]
+        src_equal = src_equal and bool(np.array_equal(g, o))
+        scale = np.maximum(np.max(np.abs(o), axis=(1, 2, 3), keepdims=True), 1e-300)
+        src_err = max(src_err, float(np.max(np.abs(g - o) / scale)))
+        del src
+    gpos, grd, gids = pc.get_particles()
+    pc.close()
+    go, oo = np.argsort(gids), np.argsort(oids)
+    same_ids = bool(np.array_equal(gids[go], oids[oo]))
+    res = dict(steps=nsteps, droplets=int(prob.n), survivors=int(gids.size), same_droplets=same_ids,
+               sources_bitwise=src_equal, max_rel_err_sources=src_err)
+    if same_ids:
+        res["droplets_bitwise"] = bool(np.array_equal(grd[go], ord_[oo]) and np.array_equal(gpos[go], opos[oo]))
+        den = np.maximum(np.abs(ord_[oo]), 1e-3)
+        res["max_rel_err_droplets"] = float(np.max(np.abs(grd[go] - ord_[oo]) / den))
+    print(f"# full-size parity: {json.dumps(res)}", file=sys.stderr, flush=True)
+    return res
 
 
 def calibrate_copy() -> int:
@@ -553,8 +628,10 @@ def main(argv=None):
             out["jet"] = dict(error=str(e))
             print(f"# jet (config5) failed: {e}", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        cb = cpu_baseline(prob, args.cpu_seconds)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        cb = cpu_baseline(prob, args.cpu_seconds, parity_stream=stream if "fp64" in forms else None)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "host")}
+        if "full_size_parity" in cb:
+            out["full_size_parity"] = cb["full_size_parity"]
         print(f"# cpu baseline: {cb['value']:.4g} droplet-steps/s on {cb['cores']} threads ({cb['seconds']:.1f} s)",
               file=sys.stderr, flush=True)
     else:

@@ -389,6 +389,21 @@ int pelemp_spray_update_f32gas(pelemp_spray_ctx* ctx,
                                int32_t isGhost, int32_t do_move,
                                double spray_cfl_lev, void* stream);
 
+/* How pelemp_spray_update writes the spray source.  No counterpart in the
+ * reference, whose callers zero the source MultiFab (tmpSource.setVal(0.),
+ * SprayParticles.H:294) and let the update add into it:
+ *   PELEMP_SOURCE_ACCUMULATE (default): the update adds (+=) as the
+ *     reference does;
+ *   PELEMP_SOURCE_OVERWRITE: the first particle sub-cycle of a call writes
+ *     every domain cell of the source (components the coupling flags switch
+ *     on) with 0 + its sum, which is what adding into a zeroed FAB gives, so
+ *     the caller's zero fill (and the update's read of the old values) is
+ *     not needed.  Ghost cells of the source are left as they are (deposits
+ *     redirected to them by an EB still add). */
+#define PELEMP_SOURCE_ACCUMULATE 0
+#define PELEMP_SOURCE_OVERWRITE 1
+int pelemp_spray_set_source_mode(pelemp_spray_ctx* ctx, int32_t mode);
+
 /* Waits for `stream` and returns the sticky device-side error recorded by
  * earlier updates (PELEMP_E_STATE_BOX, _SRC_BOX, _LEFT_DOMAIN, ...), then
  * clears it.  pelemp_spray_update itself does not wait for the device. */

@@ -101,6 +101,8 @@ SURR_NIN = 8
 SURR_NOUT = 3
 MLP_NOALLOC_MAX_BATCH = 4096  # PELEMP_MLP_NOALLOC_MAX_BATCH
 SEGMENT_DROPLETS = 4096  # PELEMP_SEGMENT_DROPLETS
+SOURCE_ACCUMULATE = 0
+SOURCE_OVERWRITE = 1
 _c_flt = ctypes.c_float
 
 
@@ -177,6 +179,7 @@ _PROTOS = {
     "pelemp_spray_update_f32gas": (_c_i32, [_vp, ctypes.POINTER(FabViewF32C), ctypes.POINTER(FabViewC), _c_dbl,
                                             _c_dbl, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
     "pelemp_spray_sync": (_c_i32, [_vp, _vp]),
+    "pelemp_spray_set_source_mode": (_c_i32, [_vp, _c_i32]),
     "pelemp_spray_move_kick_drift": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl,
                                               _c_dbl, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
     "pelemp_spray_est_timestep": (_c_i32, [_vp, _c_dbl, _c_dbl, ctypes.POINTER(_c_dbl), _vp]),

@@ -20,6 +20,7 @@ struct PassParams {
   double sub_dt;
   double cur_coef;  // -(1/V) sub_dt / flow_dt
   int do_move;
+  int src_overwrite;  // write the domain cells of the source (first sub-cycle of an overwrite-mode call)
   int is_ghost;
   int state_ghosts;   // state box = domain grown by this many cells (negative: the state FAB's box)
   int source_ghosts;  // source box = domain grown by this many cells (negative: the source FAB's box)
@@ -113,6 +114,7 @@ struct Ctx {
   // parameters, film thickness per cell, secondary-droplet records and
   // the counters of new droplets (device scalars: next id, next slot)
   pelemp_spray_params params{};
+  int src_mode = PELEMP_SOURCE_ACCUMULATE;
   bool branches = false;
   // PeleC flavour: CGS state (rho, rho*u, rho*E, rho*Y, T), sources, droplets
   // and geometry at the boundary; SI inside (src_scale: SI -> CGS sources)

@@ -2148,6 +2148,8 @@ static int update_impl(pelemp_spray_ctx* h, const pelemp_fab_view* state, const 
     if (stc != PELEMP_OK) return stc;
   }
   for (int it = 0; it < num_iter; ++it) {
+    // overwrite mode: the first sub-cycle writes the source, later ones add
+    p.src_overwrite = (c.src_mode == PELEMP_SOURCE_OVERWRITE && it == 0) ? 1 : 0;
     int32_t* log_slot = (c.npass < Ctx::kCountLog) ? c.count_log + c.npass : nullptr;
     ++c.npass;
     // the last sort cleared the pass counters, unless a pass stopped early
@@ -2226,6 +2228,15 @@ int pelemp_spray_move_kick_drift(pelemp_spray_ctx* h, const pelemp_fab_view* sta
                                  int32_t source_ghosts, int32_t do_move, double spray_cfl_lev, void* stream) {
   return pelemp_spray_update(h, state, source, dt, time, state_ghosts, source_ghosts, isVirt, isGhost, do_move,
                              spray_cfl_lev, stream);
+}
+
+int pelemp_spray_set_source_mode(pelemp_spray_ctx* h, int32_t mode) {
+  if (!h || (mode != PELEMP_SOURCE_ACCUMULATE && mode != PELEMP_SOURCE_OVERWRITE)) {
+    set_last_error("invalid argument");
+    return PELEMP_E_ARG;
+  }
+  h->c.src_mode = mode;
+  return PELEMP_OK;
 }
 
 int pelemp_spray_set_params(pelemp_spray_ctx* h, const pelemp_spray_params* p) {

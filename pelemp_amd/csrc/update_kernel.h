@@ -57,9 +57,32 @@ struct FabDev {
   long long cstride;  // cells per component
 };
 
-// Word i of a state FAB, widened to double when the FAB is FP32 (exact).
-__device__ __attribute__((always_inline)) inline double state_at(const FabDev& f, long long i) {
-  return f.data32 ? static_cast<double>(f.data32[i]) : f.data[i];
+// The gas state of cell c of a state FAB with elements of type T (the
+// PeleLMeX layout): v = rho, Y[NS] = rho Y / rho, vel[3], T, in double (an
+// FP32 state is widened exactly).
+template <int NS, class T>
+__device__ __attribute__((always_inline)) inline void state_cell(const T* __restrict__ S, long long cs, long long c,
+                                                                 int c_rho, int c_spec, int c_vel, int c_T,
+                                                                 double* v) {
+  const double rho = static_cast<double>(S[c_rho * cs + c]);
+  const double inv_rho = 1. / rho;
+  v[0] = rho;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) v[1 + n] = static_cast<double>(S[(c_spec + n) * cs + c]) * inv_rho;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) v[1 + NS + q] = static_cast<double>(S[(c_vel + q) * cs + c]);
+  v[4 + NS] = static_cast<double>(S[c_T * cs + c]);
+}
+
+// The same for a FAB in double or in single precision (one uniform branch
+// per cell).
+template <int NS>
+__device__ __attribute__((always_inline)) inline void state_cell(const FabDev& f, long long c, int c_rho, int c_spec,
+                                                                 int c_vel, int c_T, double* v) {
+  if (f.data32)
+    state_cell<NS>(f.data32, f.cstride, c, c_rho, c_spec, c_vel, c_T, v);
+  else
+    state_cell<NS>(f.data, f.cstride, c, c_rho, c_spec, c_vel, c_T, v);
 }
 
 // Normalisation of the surrogate inputs / outputs as used in the kernel.
@@ -99,6 +122,7 @@ struct UpdateArgs {
   double cur_coef;  // -(1/V) sub_dt / flow_dt
   R sub_dt_dxi[3];  // FP32 form: sub_dt / dx per direction
   int do_move;
+  int src_overwrite;  // 1: the domain cells of the source get this pass's sums (written, not added)
   int is_ghost;
   int st_g;   // state ghost width: the gather may read cells up to st_g outside the domain (negative: the FAB)
   int src_g;  // source ghost width: a ghost droplet is deleted when its cell leaves the domain grown by src_g
@@ -237,6 +261,26 @@ template <class R, int NF>
 struct UpdOccupancy {
   static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1);
 };
+
+// Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
+// c of the tile at (otx, oty, otz): null when the component is switched off
+// (SprayParticles.cpp:426-441) or the cell lies outside the FAB (an error
+// unless the droplets are ghosts, :420-425).
+template <class R, int NF>
+__device__ __attribute__((always_inline)) inline double* src_word(const UpdateArgs<R, NF>& a, int otx, int oty,
+                                                                   int otz, int c, int q) {
+  if (q < 3 && !a.fc.mom_trans) return nullptr;
+  if (q >= 3 && q < 4 + NF && !a.fc.mass_trans) return nullptr;
+  const int comp = (q < 3) ? a.c_msrc + q : (q == 3) ? a.c_rsrc : (q < 4 + NF) ? a.c_ssrc + (q - 4) : a.c_esrc;
+  const int gx = a.dom_lo[0] + otx + (c % kTile) - a.src.lo[0];
+  const int gy = a.dom_lo[1] + oty + ((c / kTile) % kTile) - a.src.lo[1];
+  const int gz = a.dom_lo[2] + otz + c / (kTile * kTile) - a.src.lo[2];
+  if (gx < 0 || gx >= a.src.n[0] || gy < 0 || gy >= a.src.n[1] || gz < 0 || gz >= a.src.n[2]) {
+    if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
+    return nullptr;
+  }
+  return &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
+}
 
 // Per-droplet values carried from the gather to the update.
 template <class M, int NF, class R>
@@ -401,15 +445,14 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
             return;
           }
         }
-        const double rho = state_at(a.st, a.c_rho * a.st.cstride + c);
-        const double inv_rho = 1. / rho;
-        g.rho += cw * static_cast<R>(rho);
+        double v[5 + NS];
+        state_cell<NS>(a.st, c, a.c_rho, a.c_spec, a.c_vel, a.c_T, v);
+        g.rho += cw * static_cast<R>(v[0]);
 #pragma unroll
-        for (int n = 0; n < NS; ++n)
-          g.Y[n] += cw * static_cast<R>(state_at(a.st, (a.c_spec + n) * a.st.cstride + c) * inv_rho);
+        for (int n = 0; n < NS; ++n) g.Y[n] += cw * static_cast<R>(v[1 + n]);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(state_at(a.st, (a.c_vel + q) * a.st.cstride + c));
-        g.T += cw * static_cast<R>(state_at(a.st, a.c_T * a.st.cstride + c));
+        for (int q = 0; q < 3; ++q) g.vel[q] += cw * static_cast<R>(v[1 + NS + q]);
+        g.T += cw * static_cast<R>(v[4 + NS]);
         return;
       }
     }
@@ -554,7 +597,7 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
         const int n = a.ncell[q];
         // a ghost droplet whose unwrapped cell left the source box is
         // deleted (SprayParticles.cpp:477-479)
-        if (a.src_g >= 0 && (ic < -a.src_g || ic >= n + a.src_g)) left_src = true;
+        if (a.is_ghost && a.src_g >= 0 && (ic < -a.src_g || ic >= n + a.src_g)) left_src = true;
         // periodic: ((ic % n) + n) % n, with the integer division only for a
         // move of more than one period
         if (ic < 0) {
@@ -915,6 +958,13 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
   const int32_t d0 = cs[0] + seg * kSegDroplets;
   const int32_t d1 = (cs[kTileCells] - d0 < kSegDroplets) ? cs[kTileCells] : d0 + kSegDroplets;
   if (d1 <= d0) {  // uniform: empty tile
+    if (a.src_overwrite) {
+      // its cells get no spray source this pass
+      for (int item = t; item < kTileCells * NSRC; item += kUpdThreads) {
+        double* w = src_word(a, otx, oty, otz, item / NSRC, item % NSRC);
+        if (w) *w = 0.;
+      }
+    }
     if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
     return;
   }
@@ -944,16 +994,10 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     }
     if (ok) {
       const long long c = (static_cast<long long>(lz) * a.st.n[1] + ly) * a.st.n[0] + lx;
-      const double rho = state_at(a.st, a.c_rho * a.st.cstride + c);
-      const double inv_rho = 1. / rho;
-      gas[0 * kHaloCells + h] = static_cast<R>(rho);
+      double v[5 + NS];
+      state_cell<NS>(a.st, c, a.c_rho, a.c_spec, a.c_vel, a.c_T, v);
 #pragma unroll
-      for (int n = 0; n < NS; ++n)
-        gas[(1 + n) * kHaloCells + h] = static_cast<R>(state_at(a.st, (a.c_spec + n) * a.st.cstride + c) * inv_rho);
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        gas[(1 + NS + q) * kHaloCells + h] = static_cast<R>(state_at(a.st, (a.c_vel + q) * a.st.cstride + c));
-      gas[(4 + NS) * kHaloCells + h] = static_cast<R>(state_at(a.st, a.c_T * a.st.cstride + c));
+      for (int f = 0; f < 5 + NS; ++f) gas[f * kHaloCells + h] = static_cast<R>(v[f]);
     } else {
       // outside the state box: a droplet reaching it is an error; the
       // cleared cell keeps what it reads defined
@@ -1042,7 +1086,9 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 
   // add each cell's sums to the source FAB: a thread's items are distinct
   // (cell, component) words (the host checks that the source components do
-  // not overlap), so all their loads go out before the stores
+  // not overlap), so all their loads go out before the stores.  In the
+  // overwrite mode every cell gets 0 + its sum (the value an add into a
+  // zeroed FAB gives) and the FAB is not read.
   double* dst[NITEM];
   double old[NITEM];
 #pragma unroll
@@ -1051,24 +1097,9 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     const int item = t + r * kUpdThreads;
     if (item < kTileCells * NSRC) {
       const int c = item / NSRC, q = item % NSRC;
-      if (cs[c + 1] > cs[c]) {
-        bool on = true;
-        if (q < 3 && !a.fc.mom_trans) on = false;
-        if (q >= 3 && q < 4 + NF && !a.fc.mass_trans) on = false;
-        if (on) {
-          const int comp = (q < 3) ? a.c_msrc + q : (q == 3) ? a.c_rsrc : (q < 4 + NF) ? a.c_ssrc + (q - 4) : a.c_esrc;
-          const int gx = a.dom_lo[0] + otx + (c % kTile) - a.src.lo[0];
-          const int gy = a.dom_lo[1] + oty + ((c / kTile) % kTile) - a.src.lo[1];
-          const int gz = a.dom_lo[2] + otz + c / (kTile * kTile) - a.src.lo[2];
-          if (gx < 0 || gx >= a.src.n[0] || gy < 0 || gy >= a.src.n[1] || gz < 0 || gz >= a.src.n[2]) {
-            if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
-          } else {
-            dst[r] = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
-          }
-        }
-      }
+      if (a.src_overwrite || cs[c + 1] > cs[c]) dst[r] = src_word(a, otx, oty, otz, c, q);
     }
-    old[r] = dst[r] ? *dst[r] : 0.;
+    old[r] = (dst[r] && !a.src_overwrite) ? *dst[r] : 0.;
   }
 #pragma unroll
   for (int r = 0; r < NITEM; ++r) {
@@ -1094,7 +1125,8 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
   // workgroups 0 .. ntiles-1: the first segment of every tile; the others:
-  // the further segments of the heavy tiles
+  // one further segment of a heavy tile each (those beyond the pass's work
+  // items return at once)
   int tile, seg = 0;
   if (static_cast<int>(blockIdx.x) < ntiles) {
     tile = xcd_tile(blockIdx.x, ntiles);
@@ -1127,21 +1159,15 @@ __global__ void __launch_bounds__(kUpdThreads) split_combine_kernel(UpdateArgs<R
     const int otx = (tile % a.ntile[0]) * kTile;
     for (int item = t; item < kTileCells * NSRC; item += kUpdThreads) {
       const int c = item / NSRC, q = item % NSRC;
-      if (a.cstart[tile * kTileCells + c + 1] == a.cstart[tile * kTileCells + c]) continue;
-      if (q < 3 && !a.fc.mom_trans) continue;
-      if (q >= 3 && q < 4 + NF && !a.fc.mass_trans) continue;
+      const bool has = a.cstart[tile * kTileCells + c + 1] > a.cstart[tile * kTileCells + c];
+      if (!has && !a.src_overwrite) continue;
+      double* dst = src_word(a, otx, oty, otz, c, q);
+      if (!dst) continue;
       double total = 0.;
-      for (int k = 0; k < nseg; ++k) total = total + a.sp.part[static_cast<long long>(slot + k) * (kTileCells * NSRC) + item];
-      const int comp = (q < 3) ? a.c_msrc + q : (q == 3) ? a.c_rsrc : (q < 4 + NF) ? a.c_ssrc + (q - 4) : a.c_esrc;
-      const int gx = a.dom_lo[0] + otx + (c % kTile) - a.src.lo[0];
-      const int gy = a.dom_lo[1] + oty + ((c / kTile) % kTile) - a.src.lo[1];
-      const int gz = a.dom_lo[2] + otz + c / (kTile * kTile) - a.src.lo[2];
-      if (gx < 0 || gx >= a.src.n[0] || gy < 0 || gy >= a.src.n[1] || gz < 0 || gz >= a.src.n[2]) {
-        if (!a.is_ghost) set_err(a.err, PELEMP_E_SRC_BOX);
-        continue;
-      }
-      double* dst = &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
-      const double old = *dst;
+      if (has)
+        for (int k = 0; k < nseg; ++k)
+          total = total + a.sp.part[static_cast<long long>(slot + k) * (kTileCells * NSRC) + item];
+      const double old = a.src_overwrite ? 0. : *dst;
       *dst = a.pelec ? old + total * a.src_scale[q] : old + total;
     }
     if (t < kTileCells) {

@@ -70,6 +70,7 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
   a.sub_dt = static_cast<R>(p.sub_dt);
   a.cur_coef = p.cur_coef;
   a.do_move = p.do_move;
+  a.src_overwrite = p.src_overwrite;
   a.is_ghost = p.is_ghost;
   a.st_g = p.state_ghosts;
   a.src_g = p.source_ghosts;
@@ -162,7 +163,7 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
     hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, false>), grid, block, 0, s, a);
   }
   // the heavy tiles' segment sums, added to the source in segment order
-  const int cgrid = c.split_list_cap < 1024 ? c.split_list_cap : 1024;
+  const int cgrid = c.split_list_cap < 256 ? c.split_list_cap : 256;
   hipLaunchKernelGGL((split_combine_kernel<R, NF>), dim3(cgrid), block, 0, s, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

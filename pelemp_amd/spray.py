@@ -310,6 +310,11 @@ class SprayParticleContainer:
         self.updateParticles(level, state, source, dt, time, state_ghosts, source_ghosts, isVirtualPart,
                              isGhostPart, False, ltransparm, spray_cfl_lev, stream, sync)
 
+    def set_source_mode(self, mode: int) -> None:
+        """C.SOURCE_ACCUMULATE (+=, the reference's) or C.SOURCE_OVERWRITE (the
+        update writes the source's domain cells; no zero fill needed)."""
+        C.check(self._lib.pelemp_spray_set_source_mode(self._h, int(mode)), "pelemp_spray_set_source_mode")
+
     def sync(self, stream=None) -> None:
         C.check(self._lib.pelemp_spray_sync(self._h, _stream_handle(stream)), "device check")
 
