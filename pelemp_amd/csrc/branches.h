@@ -156,7 +156,9 @@ PM_HDI void film_source(double flow_dt, const GasAt<double, M::NS>& g, const Fue
                                   L_fuel, B_M, sumXVap, cp_skin, mw_skin);
   double lambda_skin = 0., mu_skin = 0.;
   const double rho_skin = mw_skin * g.p / (kRU * T_skin);
-  GasProps<M>::transport(false, true, true, T_skin, rho_skin, Y_skin, Ddiag, mu_skin, lambda_skin);
+  // only the fuels' diffusivities are used (WallFilm.H:148-152)
+  GasProps<M>::template transport<double, NS - NF>(false, true, true, T_skin, rho_skin, Y_skin, Ddiag, mu_skin,
+                                                   lambda_skin);
   if (sumXVap > 0.) {
     const double h_heat = lambda_skin / film_height;
     const double q_conv = film_area * h_heat * (g.T - T_film);

@@ -74,6 +74,7 @@ struct FuelConst {
   int dep_indx[NF];
   bool cc_psat[NF];  // psat_coef[4*spf+3] == 0: Clausius-Clapeyron
   R ref_T;
+  R ln10;  // pm::log(10) (host and device give the same bits)
   R dtmod;
   R body_force[3];
   R y_tab_coef;  // unused placeholder for alignment
@@ -105,6 +106,7 @@ inline FuelConst<R, NF> make_fuel_const(const pelemp_spray_data& f, const double
     c.cc_psat[spf] = (f.psat_coef[4 * spf + 3] == 0.);
   }
   c.ref_T = static_cast<R>(f.ref_T);
+  c.ln10 = static_cast<R>(pm::log(10.0));
   c.dtmod = static_cast<R>(f.dtmod);
   for (int d = 0; d < 3; ++d) c.body_force[d] = static_cast<R>(f.body_force[d]);
   c.mass_trans = f.mass_trans != 0;
@@ -164,7 +166,30 @@ template <class R, int NF>
 PM_HDI R psat(const FuelConst<R, NF>& f, R T, int spf) {
   const R a = f.psat_coef[4 * spf], b = f.psat_coef[4 * spf + 1];
   const R c = f.psat_coef[4 * spf + 2], d = f.psat_coef[4 * spf + 3];
-  return d * rm::pow(R(10.), a - b / (T + c));
+  if constexpr (sizeof(R) == 8) {
+    // pm::pow(10, z) is pm::exp(z pm::log(10)) for every z (z = 0 included)
+    return d * pm::exp((a - b / (T + c)) * f.ln10);
+  } else {
+    return d * rm::pow(R(10.), a - b / (T + c));
+  }
+}
+
+// pm::log1p(b) and pm::pow(1 + b, 0.7) from one logarithm of u = 1 + b.
+// pm::pow(u, 0.7) is pm::exp(0.7 pm::log(u)) for every u (u = 0, 1, inf and
+// NaN included), and pm::log1p(b) is pm::log(u) (b / (u - 1)), or b itself
+// for u == 1, NaN or infinite b; both results are bit for bit those of the
+// separate calls.  lu returns pm::log(u).
+// (FP64 only; the template lets single-precision instantiations of the
+// callers compile the branch they never take.)
+template <class R>
+PM_HDI void log1p_pow07(R b, R& l1p, R& p07, R& lu) {
+  static_assert(sizeof(R) == 8 || sizeof(R) == 4, "");
+  const double bd = static_cast<double>(b);
+  const double u = 1.0 + bd;
+  const double l = pm::log(u);
+  lu = static_cast<R>(l);
+  l1p = static_cast<R>((!(bd == bd) || bd == pm::kInf || u == 1.0) ? bd : l * (bd / (u - 1.0)));
+  p07 = static_cast<R>(pm::exp(0.7 * l));
 }
 
 // GasPhaseVals::define (SprayFuelData.H:101-111): mixture MW and pressure
@@ -188,16 +213,18 @@ PM_HDI void calc_boil_t(const FuelConst<R, NF>& f, R p, R* cBoilT) {
   }
 }
 
-// calcHeatCoeff (Drag.H:8-45)
+// calcHeatCoeff (Drag.H:8-45).  lBM1: log(1 + B_M), which the caller has
+// already taken; pm::pow(1 + B_M, phi) is pm::exp(phi lBM1) for every phi
+// when B_M > C_eps (1 + B_M > 1, so lBM1 > 0).
 template <class R>
-PM_HDI R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
+PM_HDI R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0, R lBM1) {
   if (B_M <= C_eps) return R(0);
   if constexpr (sizeof(R) == 4) {
     // single precision: (1+B_M)^phi = exp(phi ln(1+B_M)) with the logarithm
     // taken once, and (1+B_T)^0.7 = exp(0.7 log1p(B_T)) sharing log1p(B_T)
     const int maxIter = 100;
     const R NU2 = Nu_0 - R(2);
-    const R lnBM1 = rm::log1p(B_M);
+    const R lnBM1 = lBM1;
     R phi = ratio / Nu_0;
     R B_T_old = rm::exp(phi * lnBM1) - R(1);
     R logB = rm::log1p(B_T_old);
@@ -222,30 +249,33 @@ PM_HDI R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0) {
     Nu_num = R(2) + NU2 * invFT;
     return Nu_num * logB / B_T;
   }
+  // the reference's sequence (pow(1+B_M, phi), log1p(B_T), pow(1+B_T, 0.7)
+  // per step) with log(1 + B_M) taken once and log(1 + B_T) shared by
+  // log1p and the 0.7 power: the same bits with two logarithms fewer per step
   const int maxIter = 100;
   const R NU2 = Nu_0 - R(2);
-  const R BM1 = R(1) + B_M;
   R phi = ratio / Nu_0;
-  R B_T_old = rm::pow(BM1, phi) - R(1);
-  R logB = rm::log1p(B_T_old);
-  R invFT = B_T_old / (logB * rm::pow(R(1) + B_T_old, R(0.7)));
+  R B_T_old = pm::exp(phi * lBM1) - R(1);
+  R logB, p07, lu;
+  log1p_pow07(B_T_old, logB, p07, lu);
+  R invFT = B_T_old / (logB * p07);
   R Nu_num = R(2) + NU2 * invFT;
   phi = ratio / Nu_num;
-  R B_T = rm::pow(BM1, phi) - R(1);
+  R B_T = pm::exp(phi * lBM1) - R(1);
   R error = rm::fabs(B_T - B_T_old);
   int k = 0;
   while (k < maxIter && error > B_eps) {  // B_eps: see spray_source
     B_T_old = B_T;
-    logB = rm::log1p(B_T);
-    invFT = B_T / (logB * rm::pow(R(1) + B_T, R(0.7)));
+    log1p_pow07(B_T, logB, p07, lu);
+    invFT = B_T / (logB * p07);
     Nu_num = R(2) + NU2 * invFT;
     phi = ratio / Nu_num;
-    B_T = rm::pow(BM1, phi) - R(1);
+    B_T = pm::exp(phi * lBM1) - R(1);
     error = rm::fabs(B_T - B_T_old);
     k++;
   }
-  logB = rm::log1p(B_T);
-  invFT = B_T / (logB * rm::pow(R(1) + B_T, R(0.7)));
+  log1p_pow07(B_T, logB, p07, lu);
+  invFT = B_T / (logB * p07);
   Nu_num = R(2) + NU2 * invFT;
   return Nu_num * logB / B_T;
 }
@@ -395,7 +425,9 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
   R lambda_skin = R(0), mu_skin = R(0);
   R rho_skin = g.rho;
   if (mass_trans) rho_skin = mw_skin * g.p / (static_cast<R>(kRU) * T_skin);
-  GasProps<M>::transport(true, mass_trans, mass_trans, T_skin, rho_skin, Y_skin, Ddiag, mu_skin, lambda_skin);
+  // only the fuels' diffusivities are used (Drag.H:312-318)
+  GasProps<M>::template transport<R, NS - NF>(true, mass_trans, mass_trans, T_skin, rho_skin, Y_skin, Ddiag, mu_skin,
+                                             lambda_skin);
 #pragma unroll
   for (int q = 0; q < 3; ++q) o.diff_vel[q] = g.vel[q] - vel_part[q];
   const R diff_vel_mag = rm::sqrt(o.diff_vel[0] * o.diff_vel[0] + o.diff_vel[1] * o.diff_vel[1] +
@@ -441,18 +473,22 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
       }
       cp_fuel /= sumYSkin;
       const R Sc_skin = mu_skin / rhoDtotal;
-      const R logB = rm::log1p(B_M);
-      R invFM;
+      R logB, invFM, lBM1;
       if constexpr (sizeof(R) == 4) {
+        logB = rm::log1p(B_M);
         invFM = B_M / (logB * rm::exp(R(0.7) * logB));  // (1+B_M)^0.7 from log1p(B_M)
+        lBM1 = logB;
       } else {
-        invFM = B_M / (logB * rm::pow(R(1.) + B_M, R(0.7)));
+        // log1p(B_M), (1 + B_M)^0.7 and log(1 + B_M) from one logarithm
+        R p07;
+        log1p_pow07(B_M, logB, p07, lBM1);
+        invFM = B_M / (logB * p07);
       }
       const R Sh_0 = R(1.) + powR * rm::cbrt(R(1.) + Reyn * Sc_skin);
       const R Sh_num = R(2.) + (Sh_0 - R(2.)) * invFM;
       const R mdotcoeff = static_cast<R>(kPIc) * dia_part * Sh_num * logB;
       const R ratio = cp_fuel * Sh_num * rhoDtotal / lambda_skin;
-      Nu_num = calc_heat_coeff<R>(ratio, B_M, B_eps, C_eps, Nu_0);
+      Nu_num = calc_heat_coeff<R>(ratio, B_M, B_eps, C_eps, Nu_0, lBM1);
       o.ShRhoD = Sh_num * rhoDtotal;
       o.rhoD = rhoDtotal;
 #pragma unroll

@@ -142,8 +142,10 @@ struct GasProps {
     return T;
   }
 
-  // Mixture-averaged transport at (T, rho, Y).  Ddiag[k] = rho D_k,mix.
-  template <class R>
+  // Mixture-averaged transport at (T, rho, Y).  Ddiag[k] = rho D_k,mix for
+  // k >= D_FIRST (the others are left as they are; each Ddiag[k] is computed
+  // on its own, so the ones computed do not depend on D_FIRST).
+  template <class R, int D_FIRST = 0>
   static PM_HDI void transport(bool get_mu, bool get_lam, bool get_D, R T,
                               R rho, const R* Y, R* Ddiag, R& mu, R& lam) {
     const R lt = rm::log(T);
@@ -191,16 +193,23 @@ struct GasProps {
       constexpr int NP = NS * (NS - 1) / 2;
       R invD[NP > 0 ? NP : 1];
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const R f = static_cast<R>(M::fitdbin[4 * p]) + static_cast<R>(M::fitdbin[4 * p + 1]) * lt +
-                    static_cast<R>(M::fitdbin[4 * p + 2]) * lt2 +
-                    static_cast<R>(M::fitdbin[4 * p + 3]) * lt3;
-        invD[p] = rm::exp(-f);
+      for (int a = 0; a < NS; ++a) {
+#pragma unroll
+        for (int b = a + 1; b < NS; ++b) {
+          const int p = a * NS - a * (a + 1) / 2 + (b - a - 1);
+          if (b < D_FIRST) {  // the pair of two species whose Ddiag is not wanted
+            invD[p] = R(0);
+            continue;
+          }
+          const R f = static_cast<R>(M::fitdbin[4 * p]) + static_cast<R>(M::fitdbin[4 * p + 1]) * lt +
+                      static_cast<R>(M::fitdbin[4 * p + 2]) * lt2 + static_cast<R>(M::fitdbin[4 * p + 3]) * lt3;
+          invD[p] = rm::exp(-f);
+        }
       }
       // p / p_atm from the ideal gas law
       const R pratio = rho * static_cast<R>(kRU) * T / wbar / static_cast<R>(kPATM);
 #pragma unroll
-      for (int i = 0; i < NS; ++i) {
+      for (int i = D_FIRST; i < NS; ++i) {
         R term1 = R(0);
         R term2 = R(0);
 #pragma unroll
