@@ -348,18 +348,61 @@ int pelemp_spray_read_injection_log(const char* path, int32_t njets,
 
 /* The ASCII particle file written with plot files (SprayIO.cpp:35-47,
  * AMReX ParticleContainer::WriteAsciiFile): the particle count, then one
- * line per particle: position, id, cpu (0), the real components in
- * SprayComps order; 15 significant digits. */
+ * line per particle: position, id, cpu, the real components in SprayComps
+ * order; 15 significant digits. */
 int pelemp_spray_write_ascii(pelemp_spray_ctx* ctx, const char* path);
 
-/* Checkpoint and restart of the droplets in the context (SprayIO.cpp:6-34
- * writes them through AMReX's Checkpoint): a text header (format tag,
- * count, fuel count, the real-component names of SprayIO.cpp:9-33) and the
- * particles in the AoS meaning as little-endian binary (positions, ids,
- * rdata), so a restart is bit for bit. */
+/* Particle ids.  AMReX names a particle by (id, cpu): id < 2^39 and the
+ * rank that made it, cpu < 2^24.  This library holds one 64-bit id per
+ * droplet, id + cpu * 2^39, which is the plain id for droplets made on
+ * rank 0; the on-disk formats below split it back into (id, cpu). */
+
+/* Checkpoint of the droplets in the context, SprayParticleIO
+ * (SprayIO.cpp:6-34), which writes them with AMReX's
+ * ParticleContainer::Checkpoint(dir, "particles", real_comp_names) in
+ * AMReX's particle checkpoint layout: dir/particles/Header
+ * ("Version_Two_Dot_One_double", the SprayIO.cpp:9-33 component names,
+ * particle count, next id, grid list), dir/particles/Level_0/Particle_H
+ * (the box array: here the level's domain box) and
+ * dir/particles/Level_0/DATA_00000 (per grid: the packed idcpu words as
+ * int32 pairs, then position and components as doubles).  fuel_names name
+ * the spray_mf_<fuel> components (a missing name is the fuel's index). */
 int pelemp_spray_checkpoint(pelemp_spray_ctx* ctx, const char* dir,
                             int32_t nfuel_names, const char* const* fuel_names);
+/* Restart(dir, "particles") (SpraySetup.cpp:320-325): the level-0
+ * particles of such a checkpoint, from every grid and DATA file the header
+ * lists, written by this library or by AMReX (Version_Two_Dot_One or
+ * _Two_Dot_Zero, _double or _single); the next id becomes the larger of the
+ * header's and one past the largest id.  PELEMP_E_BAD_CONFIG when the
+ * component counts differ from the context's (10 + nf reals, no ints). */
 int pelemp_spray_restart(pelemp_spray_ctx* ctx, const char* dir);
+
+/* The same layout on host arrays, no device involved (a converter, or a
+ * host that holds its particles itself).  pos is [n][3], rdata [n][nreal],
+ * ids as above.  nboxes boxes (box_lo/box_hi [nboxes][3] cell indices) form
+ * the level's box array; with more than one, each particle goes to the
+ * first box that holds its cell, floor((pos - plo) / dx), and a particle in
+ * none is PELEMP_E_ARG.  plo and dx may be NULL for one box. */
+int pelemp_particles_write_checkpoint(const char* dir, const char* name,
+                                      int64_t n, const double* pos,
+                                      const double* rdata, const int64_t* ids,
+                                      int32_t nreal,
+                                      const char* const* real_names,
+                                      int64_t next_id, int32_t nboxes,
+                                      const int32_t* box_lo,
+                                      const int32_t* box_hi,
+                                      const double plo[3], const double dx[3]);
+/* Particle count of one level, real-component count and next id of a
+ * checkpoint at dir/name. */
+int pelemp_particles_checkpoint_info(const char* dir, const char* name,
+                                     int32_t level, int64_t* n,
+                                     int32_t* nreal, int64_t* next_id);
+/* Reads one level's particles (grid by grid, in file order; particles with
+ * a negative AMReX id are skipped) into arrays of `capacity` particles. */
+int pelemp_particles_read_checkpoint(const char* dir, const char* name,
+                                     int32_t level, int64_t capacity,
+                                     int64_t* n, double* pos, double* rdata,
+                                     int64_t* ids);
 
 /* The droplet update for one level, SprayParticleContainer::updateParticles
  * (SprayParticles.cpp:142-502) with the same argument meanings; `state` and

@@ -174,6 +174,11 @@ _PROTOS = {
     "pelemp_spray_write_ascii": (_c_i32, [_vp, ctypes.c_char_p]),
     "pelemp_spray_checkpoint": (_c_i32, [_vp, ctypes.c_char_p, _c_i32, _vp]),
     "pelemp_spray_restart": (_c_i32, [_vp, ctypes.c_char_p]),
+    "pelemp_particles_write_checkpoint": (_c_i32, [ctypes.c_char_p, ctypes.c_char_p, _c_i64, _vp, _vp, _vp, _c_i32,
+                                                  _vp, _c_i64, _c_i32, _vp, _vp, _vp, _vp]),
+    "pelemp_particles_checkpoint_info": (_c_i32, [ctypes.c_char_p, ctypes.c_char_p, _c_i32, _vp, _vp, _vp]),
+    "pelemp_particles_read_checkpoint": (_c_i32, [ctypes.c_char_p, ctypes.c_char_p, _c_i32, _c_i64, _vp, _vp, _vp,
+                                                 _vp]),
     "pelemp_spray_update": (_c_i32, [_vp, ctypes.POINTER(FabViewC), ctypes.POINTER(FabViewC), _c_dbl, _c_dbl,
                                      _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_dbl, _vp]),
     "pelemp_spray_update_f32gas": (_c_i32, [_vp, ctypes.POINTER(FabViewF32C), ctypes.POINTER(FabViewC), _c_dbl,
@@ -442,3 +447,51 @@ def device_count() -> int:
 
 def device_synchronize() -> None:
     check(lib().pelemp_device_synchronize(), "pelemp_device_synchronize")
+
+
+def write_particle_checkpoint(dir: str, name: str, pos, rdata, ids, real_names, next_id: int, boxes=None,
+                              plo=None, dx=None) -> None:
+    """AMReX particle checkpoint of host arrays (pelemp_particles_write_checkpoint).
+    boxes: list of ((lo), (hi)) cell-index boxes, one box by default (then plo
+    and dx are not needed)."""
+    import numpy as np
+
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    n = pos.shape[0]
+    nreal = len(real_names)
+    rdata = np.ascontiguousarray(rdata, dtype=np.float64).reshape(n, nreal)
+    ids = np.ascontiguousarray(ids, dtype=np.int64).reshape(n)
+    if boxes is None:
+        boxes = [((0, 0, 0), (0, 0, 0))]
+    lo = np.ascontiguousarray([b[0] for b in boxes], dtype=np.int32)
+    hi = np.ascontiguousarray([b[1] for b in boxes], dtype=np.int32)
+    nm = (ctypes.c_char_p * max(nreal, 1))(*[s.encode() for s in real_names])
+    p3 = None if plo is None else (ctypes.c_double * 3)(*plo)
+    d3 = None if dx is None else (ctypes.c_double * 3)(*dx)
+    check(lib().pelemp_particles_write_checkpoint(
+        dir.encode(), name.encode(), n, pos.ctypes.data, rdata.ctypes.data, ids.ctypes.data, nreal,
+        ctypes.cast(nm, ctypes.c_void_p), int(next_id), len(boxes), lo.ctypes.data, hi.ctypes.data,
+        ctypes.cast(p3, ctypes.c_void_p) if p3 is not None else None,
+        ctypes.cast(d3, ctypes.c_void_p) if d3 is not None else None), "write_particle_checkpoint")
+
+
+def read_particle_checkpoint(dir: str, name: str, level: int = 0):
+    """(pos [n,3], rdata [n,nreal], ids [n], next_id) of one level of a
+    particle checkpoint (pelemp_particles_read_checkpoint)."""
+    import numpy as np
+
+    n = ctypes.c_int64()
+    nr = ctypes.c_int32()
+    nxt = ctypes.c_int64()
+    check(lib().pelemp_particles_checkpoint_info(dir.encode(), name.encode(), level, ctypes.byref(n),
+                                                 ctypes.byref(nr), ctypes.byref(nxt)), "particle checkpoint info")
+    cap = int(n.value)
+    pos = np.zeros((max(cap, 1), 3))
+    rd = np.zeros((max(cap, 1), max(int(nr.value), 1)))
+    ids = np.zeros(max(cap, 1), dtype=np.int64)
+    m = ctypes.c_int64()
+    check(lib().pelemp_particles_read_checkpoint(dir.encode(), name.encode(), level, cap, ctypes.byref(m),
+                                                 pos.ctypes.data, rd.ctypes.data, ids.ctypes.data),
+          "read_particle_checkpoint")
+    m = int(m.value)
+    return pos[:m], rd[:m, :int(nr.value)], ids[:m], int(nxt.value)

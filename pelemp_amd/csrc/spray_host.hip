@@ -28,6 +28,7 @@
 #include "transfer_kernels.h"
 #include "units.h"
 #include "inject_kernels.h"
+#include "particle_io.h"
 
 struct pelemp_spray_ctx {
   pelemp::Ctx c;
@@ -1079,6 +1080,28 @@ static int inject_droplets(Ctx& c, const pelemp_spray_jet& j, double time, doubl
   return st;
 }
 
+// real-component names of SprayParticleIO (SprayIO.cpp:9-33)
+std::vector<std::string> spray_comp_names(const Ctx& c, int32_t nfuel_names, const char* const* fuel_names) {
+  std::vector<std::string> names = {"xvel", "yvel", "zvel", "temperature", "diam"};
+  for (int sp = 0; sp < c.nf; ++sp)
+    names.push_back("spray_mf_" + ((fuel_names && sp < nfuel_names && fuel_names[sp]) ? std::string(fuel_names[sp])
+                                                                                       : std::to_string(sp)));
+  names.push_back("number_density");
+  names.push_back("num_dens0");
+  if (c.sd.do_breakup == 1) {
+    names.push_back("Y_TAB");
+    names.push_back("Ydot_TAB");
+  } else if (c.sd.do_breakup == 2) {
+    names.push_back("shed_mass");
+    names.push_back("rt_time");
+  } else {
+    names.push_back("unused1");
+    names.push_back("unused2");
+  }
+  names.push_back("wall_film_height");
+  return names;
+}
+
 }  // namespace pelemp
 
 using namespace pelemp;
@@ -1874,7 +1897,7 @@ int pelemp_spray_write_ascii(pelemp_spray_ctx* h, const char* path) {
   file.precision(15);
   for (size_t i = 0; i < ids.size(); ++i) {
     for (int d = 0; d < 3; ++d) file << pos[3 * i + d] << ' ';
-    file << ids[i] << ' ' << 0 << ' ';
+    file << io::amrex_id(ids[i]) << ' ' << io::amrex_cpu(ids[i]) << ' ';
     for (int q = 0; q < nsr; ++q) file << rd[nsr * i + q] << ' ';
     file << '\n';
   }
@@ -1885,8 +1908,6 @@ int pelemp_spray_write_ascii(pelemp_spray_ctx* h, const char* path) {
   }
   return PELEMP_OK;
 }
-
-static const char kCkptTag[] = "pelemp_spray_particles_v1";
 
 int pelemp_spray_checkpoint(pelemp_spray_ctx* h, const char* dir, int32_t nfuel_names,
                             const char* const* fuel_names) {
@@ -1899,84 +1920,123 @@ int pelemp_spray_checkpoint(pelemp_spray_ctx* h, const char* dir, int32_t nfuel_
   int64_t next_id = 0, next_slot = 0;
   st = read_counters(c, &next_id, &next_slot);
   if (st) return st;
-  const std::string pdir = std::string(dir) + "/particles";
-  mkdir(dir, 0755);
-  mkdir(pdir.c_str(), 0755);
-  // component names of SprayParticleIO (SprayIO.cpp:9-33)
-  std::vector<std::string> names = {"xvel", "yvel", "zvel", "temperature", "diam"};
-  for (int sp = 0; sp < c.nf; ++sp)
-    names.push_back("spray_mf_" + ((fuel_names && sp < nfuel_names && fuel_names[sp]) ? std::string(fuel_names[sp])
-                                                                                       : std::to_string(sp)));
-  names.push_back("number_density");
-  names.push_back("num_dens0");
-  if (c.sd.do_breakup == 1) {
-    names.push_back("Y_TAB");
-    names.push_back("Ydot_TAB");
-  } else if (c.sd.do_breakup == 2) {
-    names.push_back("shed_mass");
-    names.push_back("rt_time");
-  } else {
-    names.push_back("unused1");
-    names.push_back("unused2");
+  io::Box dom;
+  for (int d = 0; d < 3; ++d) {
+    dom.lo[d] = c.geom.domain_lo[d];
+    dom.hi[d] = c.geom.domain_hi[d];
   }
-  names.push_back("wall_film_height");
-  {
-    std::ofstream hd(pdir + "/Header.pelemp", std::ios::out | std::ios::trunc);
-    hd << kCkptTag << '\n' << ids.size() << '\n' << next_id << '\n' << c.nf << '\n' << names.size() << '\n';
-    for (const auto& nm : names) hd << nm << '\n';
-    hd.close();
-    if (!hd.good()) {
-      set_last_error("Problem writing checkpoint header");
-      return PELEMP_E_ARG;
-    }
-  }
-  std::ofstream df(pdir + "/DATA.pelemp", std::ios::out | std::ios::trunc | std::ios::binary);
-  df.write(reinterpret_cast<const char*>(pos.data()), static_cast<std::streamsize>(8 * pos.size()));
-  df.write(reinterpret_cast<const char*>(ids.data()), static_cast<std::streamsize>(8 * ids.size()));
-  df.write(reinterpret_cast<const char*>(rd.data()), static_cast<std::streamsize>(8 * rd.size()));
-  df.close();
-  if (!df.good()) {
-    set_last_error("Problem writing checkpoint data");
-    return PELEMP_E_ARG;
-  }
-  return PELEMP_OK;
+  std::string err;
+  st = io::write_checkpoint(dir, "particles", static_cast<int64_t>(ids.size()), pos.data(), rd.data(), ids.data(),
+                            10 + c.nf, spray_comp_names(c, nfuel_names, fuel_names), next_id, {dom}, c.geom.plo,
+                            c.geom.dx, err);
+  if (st) set_last_error(err);
+  return st;
 }
 
 int pelemp_spray_restart(pelemp_spray_ctx* h, const char* dir) {
   if (!h || !dir) return PELEMP_E_ARG;
   Ctx& c = h->c;
   const std::string pdir = std::string(dir) + "/particles";
-  std::ifstream hd(pdir + "/Header.pelemp");
-  std::string tag;
-  int64_t n = 0, next_id = 1;
-  int nf = 0;
-  size_t ncomp = 0;
-  hd >> tag >> n >> next_id >> nf >> ncomp;
-  if (!hd.good() || tag != kCkptTag || n < 0) {
-    set_last_error("not a particle checkpoint: " + pdir);
-    return PELEMP_E_ARG;
+  io::HeaderInfo hi;
+  std::string err;
+  int st = io::read_header(pdir, hi, err);
+  if (st) {
+    set_last_error(err);
+    return st;
   }
-  if (nf != c.nf || ncomp != static_cast<size_t>(10 + c.nf)) {
-    set_last_error("checkpoint fuel count differs from the context's");
+  if (hi.nreal != 10 + c.nf || hi.nint != 0) {
+    set_last_error("checkpoint has " + std::to_string(hi.nreal) + " real and " + std::to_string(hi.nint) +
+                   " int components; the context's droplets have " + std::to_string(10 + c.nf) + " and 0");
     return PELEMP_E_BAD_CONFIG;
   }
-  const int nsr = 10 + c.nf;
-  std::vector<double> pos(3 * n), rd(nsr * n);
-  std::vector<int64_t> ids(n);
-  std::ifstream df(pdir + "/DATA.pelemp", std::ios::binary);
-  df.read(reinterpret_cast<char*>(pos.data()), static_cast<std::streamsize>(8 * pos.size()));
-  df.read(reinterpret_cast<char*>(ids.data()), static_cast<std::streamsize>(8 * ids.size()));
-  df.read(reinterpret_cast<char*>(rd.data()), static_cast<std::streamsize>(8 * rd.size()));
-  if (!df.good()) {
-    set_last_error("truncated checkpoint data");
-    return PELEMP_E_ARG;
+  std::vector<double> pos, rd;
+  std::vector<int64_t> ids;
+  st = io::read_level(pdir, hi, 0, pos, rd, ids, err);
+  if (st) {
+    set_last_error(err);
+    return st;
   }
-  int st = pelemp_spray_set_particles(h, n, pos.data(), rd.data(), ids.data());
+  const int64_t n = static_cast<int64_t>(ids.size());
+  st = pelemp_spray_set_particles(h, n, pos.data(), rd.data(), ids.data());
   if (st) return st;
   int64_t cur_id = 0, next_slot = 0;
   st = read_counters(c, &cur_id, &next_slot);
   if (st) return st;
-  return write_counters(c, std::max(cur_id, next_id), next_slot);
+  return write_counters(c, std::max(cur_id, hi.next_id), next_slot);
+}
+
+int pelemp_particles_write_checkpoint(const char* dir, const char* name, int64_t n, const double* pos,
+                                      const double* rdata, const int64_t* ids, int32_t nreal,
+                                      const char* const* real_names, int64_t next_id, int32_t nboxes,
+                                      const int32_t* box_lo, const int32_t* box_hi, const double plo[3],
+                                      const double dx[3]) {
+  if (!dir || !name || nreal < 0 || (nreal > 0 && !real_names) || nboxes < 1 || !box_lo || !box_hi ||
+      (nboxes > 1 && (!plo || !dx)))
+    return PELEMP_E_ARG;
+  std::vector<std::string> names;
+  for (int q = 0; q < nreal; ++q) names.push_back(real_names[q] ? real_names[q] : "");
+  std::vector<io::Box> boxes(nboxes);
+  for (int b = 0; b < nboxes; ++b)
+    for (int d = 0; d < 3; ++d) {
+      boxes[b].lo[d] = box_lo[3 * b + d];
+      boxes[b].hi[d] = box_hi[3 * b + d];
+    }
+  const double zero[3] = {0.0, 0.0, 0.0}, one[3] = {1.0, 1.0, 1.0};
+  std::string err;
+  const int st = io::write_checkpoint(dir, name, n, pos, rdata, ids, nreal, names, next_id, boxes, plo ? plo : zero,
+                                      dx ? dx : one, err);
+  if (st) set_last_error(err);
+  return st;
+}
+
+int pelemp_particles_checkpoint_info(const char* dir, const char* name, int32_t level, int64_t* n,
+                                     int32_t* nreal, int64_t* next_id) {
+  if (!dir || !name || !n || !nreal || !next_id) return PELEMP_E_ARG;
+  io::HeaderInfo hi;
+  std::string err;
+  int st = io::read_header(std::string(dir) + "/" + name, hi, err);
+  if (!st && (level < 0 || level > hi.finest)) {
+    err = "particle checkpoint: level " + std::to_string(level) + " is not in the checkpoint";
+    st = PELEMP_E_ARG;
+  }
+  if (st) {
+    set_last_error(err);
+    return st;
+  }
+  *n = io::level_count(hi, level);
+  *nreal = hi.nreal;
+  *next_id = hi.next_id;
+  return PELEMP_OK;
+}
+
+int pelemp_particles_read_checkpoint(const char* dir, const char* name, int32_t level, int64_t capacity,
+                                     int64_t* n, double* pos, double* rdata, int64_t* ids) {
+  if (!dir || !name || !n || capacity < 0) return PELEMP_E_ARG;
+  const std::string pdir = std::string(dir) + "/" + name;
+  io::HeaderInfo hi;
+  std::string err;
+  std::vector<double> p, r;
+  std::vector<int64_t> id;
+  int st = io::read_header(pdir, hi, err);
+  if (!st) st = io::read_level(pdir, hi, level, p, r, id, err);
+  if (st) {
+    set_last_error(err);
+    return st;
+  }
+  const int64_t m = static_cast<int64_t>(id.size());
+  if (m > capacity) {
+    set_last_error("particle checkpoint: " + std::to_string(m) + " particles exceed the capacity " +
+                   std::to_string(capacity));
+    return PELEMP_E_ARG;
+  }
+  if (m > 0 && (!pos || !ids || (hi.nreal > 0 && !rdata))) return PELEMP_E_ARG;
+  if (m > 0) {
+    std::memcpy(pos, p.data(), sizeof(double) * p.size());
+    if (hi.nreal > 0) std::memcpy(rdata, r.data(), sizeof(double) * r.size());
+    std::memcpy(ids, id.data(), sizeof(int64_t) * id.size());
+  }
+  *n = m;
+  return PELEMP_OK;
 }
 
 int pelemp_spray_set_eb(pelemp_spray_ctx* h, const pelemp_eb_view* eb) {

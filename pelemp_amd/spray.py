@@ -471,7 +471,9 @@ class SprayParticleContainer:
                 "write injection log")
 
     def Restart(self, dir: str) -> None:
-        """Reads the particles of a checkpoint written by SprayParticleIO."""
+        """Restart(dir, "particles") (SpraySetup.cpp:320-325): the level-0
+        particles of an AMReX particle checkpoint under dir/particles, written
+        by SprayParticleIO or by PeleMP itself."""
         C.check(self._lib.pelemp_spray_restart(self._h, dir.encode()), "Restart")
 
     def PostInitRestart(self, dir: str = "") -> None:

@@ -297,6 +297,18 @@ def test_checkpoint_restart_and_ascii(tmp_path):
     d = str(tmp_path / "chk00042")
     pc.SprayParticleIO(0, True, d)
     before = by_id(*pc.get_particles())
+    # AMReX's particle checkpoint layout, read by the independent numpy reader
+    from tests import amrex_particles as A
+
+    h = A.read(d + "/particles")
+    assert h["version"] == "Version_Two_Dot_One_double" and h["nparticles"] == len(before[2])
+    assert h["real_names"][:5] == ["xvel", "yvel", "zvel", "temperature", "diam"]
+    assert h["real_names"][-1] == "wall_film_height" and len(h["real_names"]) == before[1].shape[1]
+    gid, gcpu, gpos, grd = h["levels"][0][0]
+    assert not gcpu.any()
+    o = np.argsort(gid)
+    assert np.array_equal(gid[o], before[2]) and np.array_equal(gpos[o], before[0])
+    assert np.array_equal(grd[o], before[1])
     # the ASCII file next to the checkpoint: count, then x y z id cpu rdata
     txt = open(str(tmp_path / "spray00042.p3d")).read().split("\n")
     assert int(txt[0]) == len(before[2])
