@@ -207,6 +207,16 @@ void oracle_pm_eval(int fn, int64_t n, const double* x, const double* y, double*
       case 8: r = pm::acos(a); break;
       case 9: r = std::sqrt(a); break;
       case 10: r = a / b; break;
+      case 11: {  // sin from sincos
+        double c;
+        pm::sincos(a, r, c);
+        break;
+      }
+      case 12: {  // cos from sincos
+        double sn;
+        pm::sincos(a, sn, r);
+        break;
+      }
       default: r = 0.0;
     }
     out[i] = r;

@@ -227,8 +227,11 @@ PM_HDI double tab_cd(double Reyn) {
   return 0.;
 }
 
-template <int NS, int NF>
-PM_HDI double tab_update(double Reyn_d, double dt, const double* cBoilT, const GasAt<double, NS>& g,
+// updateBreakupTAB: of the gas state only the density rho_g and velocity
+// vel_g at the droplet enter; d.dia, d.numDens, d.bm1 (y) and d.bm2 (dy/dt)
+// are updated, the other fields read.
+template <int NF>
+PM_HDI double tab_update(double Reyn_d, double dt, const double* cBoilT, double rho_g, const double* vel_g,
                          const FuelConst<double, NF>& fd, const LiquidExtra& lx, BDrop<NF>& d) {
   const double C_k = 8.;
   const double C_d = 10.;
@@ -252,15 +255,15 @@ PM_HDI double tab_update(double Reyn_d, double dt, const double* cBoilT, const G
   if (rad_part < min_rad) return 0.;
   double dv[3];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) dv[q] = g.vel[q] - d.vel[q];
+  for (int q = 0; q < 3; ++q) dv[q] = vel_g[q] - d.vel[q];
   const double dv2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
   const double dvl = std::sqrt(dv2);
-  const double We_div_r = g.rho * dv2 / sigma;
+  const double We_div_r = rho_g * dv2 / sigma;
   const double We_crit = C_k * C_b / C_F;
   double denom = rho_part * rad_part * rad_part;
   double td = 2. * denom / (C_d * mu_part);
   double omega2 = C_k * sigma / (denom * rad_part) - 1. / (td * td);
-  const double tbconst = 2. * std::sqrt(3. * rho_part / g.rho) / dvl;
+  const double tbconst = 2. * std::sqrt(3. * rho_part / rho_g) / dvl;
   if (omega2 <= 0.) {
     d.bm1 = 0.;
     d.bm2 = 0.;
@@ -271,20 +274,31 @@ PM_HDI double tab_update(double Reyn_d, double dt, const double* cBoilT, const G
   double ydotn = d.bm2;
   double Reyn = Reyn_d;
   double C_D = tab_cd(Reyn);
-  const double tb_estconst = std::sqrt(3. * rho_part / g.rho) / dvl;
+  const double tb_estconst = std::sqrt(3. * rho_part / rho_g) / dvl;
   double tb_est = 0.1 * rad_part * tb_estconst;
   double subdt = rmin(dt, tb_est);
   double curt = 0.;
+  // exp(-subdt/td), cos and sin(omega subdt) and 1/omega change only when a
+  // breakup event or the last, shortened interval changes subdt, omega or
+  // td: kept from the previous interval otherwise (the same values)
+  double k_subdt = __builtin_nan(""), k_omega = 0., k_td = 0.;
+  double ex = 0., cs = 0., sn = 0., inv_omega = 0.;
   while (curt < dt) {
     const double We = We_div_r * rad_part;
     const double Wer = We / We_crit;
     const double A2 = (yn - Wer) * (yn - Wer) + ydotn * ydotn / omega2;
     const double A = std::sqrt(A2);
-    const double ex = rm::exp(-subdt / td);
-    const double cs = rm::cos(omega * subdt), sn = rm::sin(omega * subdt);
-    const double ynp = Wer + ex * ((yn - Wer) * cs + 1. / omega * (ydotn + (yn - Wer) / td) * sn);
+    if (!(subdt == k_subdt && omega == k_omega && td == k_td)) {
+      ex = rm::exp(-subdt / td);
+      pm::sincos(omega * subdt, sn, cs);
+      inv_omega = 1. / omega;
+      k_subdt = subdt;
+      k_omega = omega;
+      k_td = td;
+    }
+    const double ynp = Wer + ex * ((yn - Wer) * cs + inv_omega * (ydotn + (yn - Wer) / td) * sn);
     const double ydotnp =
-        (Wer - ynp) / td + omega * ex * (1. / omega * (ydotn + (yn - Wer) / td) * cs - (yn - Wer) * sn);
+        (Wer - ynp) / td + omega * ex * (inv_omega * (ydotn + (yn - Wer) / td) * cs - (yn - Wer) * sn);
     if (Wer + A <= 1.) {
       yn = ynp;
       ydotn = ydotnp;

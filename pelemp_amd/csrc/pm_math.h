@@ -240,6 +240,23 @@ PM_HD double cos(double x) {
   }
 }
 
+// sin(x) and cos(x) from one reduction: the same values as sin and cos
+PM_HD void sincos(double x, double& s, double& c) {
+  if (!(x == x) || x == kInf || x == -kInf) {
+    s = c = __builtin_nan("");
+    return;
+  }
+  int n = 0;
+  const double r = reduce_pio2(x, n);
+  const double sk = sin_kernel(r), ck = cos_kernel(r);
+  switch (n) {
+    case 0: s = sk; c = ck; break;
+    case 1: s = ck; c = -sk; break;
+    case 2: s = -sk; c = -ck; break;
+    default: s = -ck; c = sk; break;
+  }
+}
+
 constexpr double kPiSixth = 0x1.0c152382d7366p-1;
 constexpr double kInvSqrt3 = 0x1.279a74590331cp-1;
 constexpr double kTanPi12 = 0x1.126145e9ecd56p-2;  // 2 - sqrt(3)

@@ -1080,6 +1080,31 @@ static int inject_droplets(Ctx& c, const pelemp_spray_jet& j, double time, doubl
   return st;
 }
 
+// The domain check of updateParticles (SprayParticles.cpp:347-351 with
+// check_bounds, SprayInterpolation.H:65-100) on droplets handed in: a
+// droplet behind an outflow face has left the domain; one behind a
+// reflective face is kept (the wall placement of film droplets leaves them a
+// rounding error behind the wall); on a periodic axis the position must lie
+// in [plo, phi].
+int check_in_domain(const pelemp_geom& g, const double* pos, int64_t m) {
+  for (int64_t i = 0; i < m; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const double x = pos[3 * i + d];
+      if (!(x == x)) {
+        set_last_error("particle position is NaN");
+        return PELEMP_E_LEFT_DOMAIN;
+      }
+      const bool below = x < g.plo[d], above = x > g.phi[d];
+      if (!below && !above) continue;
+      const int kind = below ? g.bndry_lo[d] : g.bndry_hi[d];
+      if (kind == 1) continue;
+      set_last_error(kind == 0 ? "particle outside the domain on a periodic axis (not wrapped)"
+                               : "particle outside the domain (behind an outflow face)");
+      return PELEMP_E_LEFT_DOMAIN;
+    }
+  return PELEMP_OK;
+}
+
 // real-component names of SprayParticleIO (SprayIO.cpp:9-33)
 std::vector<std::string> spray_comp_names(const Ctx& c, int32_t nfuel_names, const char* const* fuel_names) {
   std::vector<std::string> names = {"xvel", "yvel", "zvel", "temperature", "diam"};
@@ -1373,13 +1398,7 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
     set_last_error("more particles than the context capacity");
     return PELEMP_E_ARG;
   }
-  // domain check (SprayParticles.cpp:347-351 aborts on out-of-domain particles)
-  for (int64_t i = 0; i < m; ++i)
-    for (int d = 0; d < 3; ++d)
-      if (!(hp[3 * i + d] >= c.geom.plo[d] && hp[3 * i + d] <= c.geom.phi[d])) {
-        set_last_error("particle outside the domain");
-        return PELEMP_E_LEFT_DOMAIN;
-      }
+  if (const int st = check_in_domain(c.geom, hp, m)) return st;
   double *dpos = nullptr, *drd = nullptr;
   int64_t* did = nullptr;
   PM_HIP(hipMalloc(&dpos, 8 * std::max<int64_t>(1, 3 * m)));
@@ -1528,12 +1547,7 @@ int pelemp_spray_add_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
   }
   const int64_t m = static_cast<int64_t>(hid.size());
   if (m == 0) return PELEMP_OK;
-  for (int64_t i = 0; i < m; ++i)
-    for (int d = 0; d < 3; ++d)
-      if (!(hp[3 * i + d] >= c.geom.plo[d] && hp[3 * i + d] <= c.geom.phi[d])) {
-        set_last_error("particle outside the domain");
-        return PELEMP_E_LEFT_DOMAIN;
-      }
+  if (const int st = check_in_domain(c.geom, hp.data(), m)) return st;
   double *dpos = nullptr, *drd = nullptr;
   int64_t* did = nullptr;
   PM_HIP(hipMalloc(&dpos, 8 * 3 * m));

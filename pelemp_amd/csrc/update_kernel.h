@@ -256,10 +256,14 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // FP64 form needs ~265 registers per lane at full freedom; it is compiled
 // for 2 waves per SIMD (256 registers) at the cost of a few spilled values.
 // The FP32 form with up to two liquid species fits 3 waves per SIMD (168
-// registers) without spills; with more species it keeps 1.
-template <class R, int NF>
+// registers) without spills; with more species it keeps 1.  The branch
+// kernels (breakup, walls, films) need far more than 256: at 2 waves per
+// SIMD they spilled 360-600 bytes per lane to scratch memory and waited on
+// it; at 1 wave per SIMD the overflow goes to the accumulation registers
+// (no scratch), measured 18 % faster on the wall/TAB slab.
+template <class R, int NF, bool BR = false>
 struct UpdOccupancy {
-  static constexpr int kMinBlocks = (sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1);
+  static constexpr int kMinBlocks = BR ? 1 : ((sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1));
 };
 
 // Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
@@ -669,29 +673,29 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
   }
 }
 
-// FP64 form with the branches (SprayParticles.cpp:354-479): film source or
-// droplet source, TAB update and split, deposition, drift, wall reflection /
-// splash, store of the breakup state and of the secondary-droplet record.
-template <class M, int NF, bool AT_BOUNDS, bool EB = false>
-__device__ __attribute__((always_inline)) inline void drop_update_br(const UpdateArgs<double, NF>& a, int tile,
-                                                                     int t, int32_t s, DropLocal<M, NF, double>& L,
-                                                                     double* contrib, unsigned char* stayed,
-                                                                     unsigned char* redir) {
+// FP64 form with the branches (SprayParticles.cpp:354-479), in three steps
+// around the workgroup's TAB stage: br_source (film source or droplet source
+// and the deposit contribution), tab_stage (the TAB update of the chunk's
+// droplets, reordered for wavefront coherence) and br_finish (TAB split or
+// KH-RT, drift, wall reflection / splash, store of the breakup state and of
+// the secondary-droplet record).  The deposit contribution depends only on
+// the source, so computing it before the breakup update changes nothing.
+template <class M, int NF, bool EB = false>
+__device__ __attribute__((always_inline)) inline void br_source(const UpdateArgs<double, NF>& a, int t,
+                                                                int32_t s, DropLocal<M, NF, double>& L,
+                                                                BDrop<NF>& b, double& Reyn_d, double* contrib,
+                                                                unsigned char* redir) {
   constexpr int NSRC = 5 + NF;
-  using F = Fields<NF>;
   const long long cap = a.cap;
   Drop<double, NF>& d = L.d;
   const bool is_film = L.film > 0.;
-  BDrop<NF> b;
   b.alive = true;
   b.film = L.film;
   b.numDens = d.numDens;
   b.N0 = L.n0;
   b.bm2 = L.bm2;
-  SBRec<NF> rec;
-  rec.type = kSBNone;
   DropSrc<double, NF> sr;
-  double Reyn_d = 0.;
+  Reyn_d = 0.;
   if (is_film) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -717,17 +721,6 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
     for (int spf = 0; spf < NF; ++spf) b.Y[spf] = d.Y[spf];
     b.bm1 = d.bm1;
     b.alive = d.alive;
-  }
-  // breakup (SprayParticles.cpp:398-412): the KH-RT update also runs for the
-  // TAB model when no new droplets can be made
-  if (b.alive && a.do_breakup > 0) {
-    double Utan = 0.;
-    if (a.do_breakup == 1) Utan = tab_update<M::NS, NF>(Reyn_d, a.sub_dt, L.cBoilT, L.g, a.fc, a.lx, b);
-    if (a.do_breakup == 1 && a.make_new_drops) {
-      tab_split<NF>(b, a.max_ppp, Utan, rec);
-    } else {
-      khrt_update<M::NS, NF>(Reyn_d, L.cBoilT, L.g, a.fc, a.lx, a.kh, a.make_new_drops != 0, b, rec);
-    }
   }
   // contributions cur_coef * source (SprayParticles.cpp:413-441); with an EB
   // 1/vfrac in a cut deposit cell, and eb_interp's closest cell may be a
@@ -762,6 +755,132 @@ __device__ __attribute__((always_inline)) inline void drop_update_br(const Updat
     if (to_other) {
 #pragma unroll
       for (int q = 0; q < NSRC; ++q) a.eb_rval[q * cap + s] = cb[q];
+    }
+  }
+}
+
+// TAB stage of one chunk (TABBreakup.H:12-184 per droplet).  The number of
+// TAB sub-intervals differs from droplet to droplet (0.1 r sqrt(3 rho_l /
+// rho_g) / |u_g - u_d| against dt, shorter after a breakup event), and a
+// wavefront runs as long as its slowest lane: the chunk's droplets are
+// binned by an estimate of that number (rho_l ~ 700 kg/m^3: an ordering key
+// only), their TAB inputs written to LDS in bin order, each lane updates the
+// droplet of its position in that order and the results go back to the
+// droplets' own lanes.  A droplet's result depends on its inputs only, so
+// the order changes no value.  Every thread of the workgroup takes part.
+template <int NF>
+struct TabRec {
+  static constexpr int kReyn = 0, kBoil = 1, kRho = 1 + NF, kGvel = 2 + NF, kT = 5 + NF, kDia = 6 + NF,
+                       kND = 7 + NF, kBM1 = 8 + NF, kBM2 = 9 + NF, kVel = 10 + NF, kY = 13 + NF,
+                       kUtan = 13 + 2 * NF, kN = 14 + 2 * NF;
+};
+constexpr int kTabBins = 16;
+
+template <class M, int NF>
+__device__ __attribute__((always_inline)) inline void tab_stage(const UpdateArgs<double, NF>& a, bool need, int t,
+                                                                double Reyn_d, const DropLocal<M, NF, double>& L,
+                                                                BDrop<NF>& b, double& Utan, double* rec,
+                                                                int32_t* cnt) {
+  using T = TabRec<NF>;
+  constexpr int W = kUpdThreads;
+  if (t < kTabBins) cnt[t] = 0;
+  __syncthreads();
+  int bin = 0, pos = 0;
+  if (need) {
+    double dv2 = 0.;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dv2 += (L.g.vel[q] - b.vel[q]) * (L.g.vel[q] - b.vel[q]);
+    const double est = a.sub_dt * std::sqrt(dv2 * L.g.rho) / (0.05 * b.dia * 45.8257569495584);
+    const double e = (est >= 0.) ? (est < kTabBins ? est : kTabBins) : 0.;  // NaN: bin 0
+    bin = static_cast<int>(e);
+    bin = bin < kTabBins ? bin : kTabBins - 1;
+    pos = atomicAdd(&cnt[bin], 1);
+  }
+  __syncthreads();
+  if (t == 0) {
+    int32_t sum = 0;
+    for (int k = 0; k < kTabBins; ++k) {
+      const int32_t c = cnt[k];
+      cnt[k] = sum;
+      sum += c;
+    }
+    cnt[kTabBins] = sum;
+  }
+  __syncthreads();
+  const int r = need ? cnt[bin] + pos : 0;
+  if (need) {
+    rec[T::kReyn * W + r] = Reyn_d;
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      rec[(T::kBoil + spf) * W + r] = L.cBoilT[spf];
+      rec[(T::kY + spf) * W + r] = b.Y[spf];
+    }
+    rec[T::kRho * W + r] = L.g.rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      rec[(T::kGvel + q) * W + r] = L.g.vel[q];
+      rec[(T::kVel + q) * W + r] = b.vel[q];
+    }
+    rec[T::kT * W + r] = b.T;
+    rec[T::kDia * W + r] = b.dia;
+    rec[T::kND * W + r] = b.numDens;
+    rec[T::kBM1 * W + r] = b.bm1;
+    rec[T::kBM2 * W + r] = b.bm2;
+  }
+  __syncthreads();
+  if (t < cnt[kTabBins]) {
+    BDrop<NF> d;
+    double cb[NF], vg[3];
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) {
+      cb[spf] = rec[(T::kBoil + spf) * W + t];
+      d.Y[spf] = rec[(T::kY + spf) * W + t];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      vg[q] = rec[(T::kGvel + q) * W + t];
+      d.vel[q] = rec[(T::kVel + q) * W + t];
+    }
+    d.T = rec[T::kT * W + t];
+    d.dia = rec[T::kDia * W + t];
+    d.numDens = rec[T::kND * W + t];
+    d.bm1 = rec[T::kBM1 * W + t];
+    d.bm2 = rec[T::kBM2 * W + t];
+    const double U = tab_update<NF>(rec[T::kReyn * W + t], a.sub_dt, cb, rec[T::kRho * W + t], vg, a.fc, a.lx, d);
+    rec[T::kDia * W + t] = d.dia;
+    rec[T::kND * W + t] = d.numDens;
+    rec[T::kBM1 * W + t] = d.bm1;
+    rec[T::kBM2 * W + t] = d.bm2;
+    rec[T::kUtan * W + t] = U;
+  }
+  __syncthreads();
+  if (need) {
+    b.dia = rec[T::kDia * W + r];
+    b.numDens = rec[T::kND * W + r];
+    b.bm1 = rec[T::kBM1 * W + r];
+    b.bm2 = rec[T::kBM2 * W + r];
+    Utan = rec[T::kUtan * W + r];
+  }
+}
+
+template <class M, int NF, bool AT_BOUNDS, bool EB = false>
+__device__ __attribute__((always_inline)) inline void br_finish(const UpdateArgs<double, NF>& a, int tile, int t,
+                                                                int32_t s, DropLocal<M, NF, double>& L,
+                                                                BDrop<NF>& b, double Reyn_d, double Utan,
+                                                                unsigned char* stayed) {
+  using F = Fields<NF>;
+  const long long cap = a.cap;
+  const bool is_film = L.film > 0.;
+  SBRec<NF> rec;
+  rec.type = kSBNone;
+  // breakup (SprayParticles.cpp:398-412): the TAB split, or the KH-RT update
+  // (which also runs for the TAB model when no new droplets can be made);
+  // the TAB update itself ran in the tab stage
+  if (b.alive && a.do_breakup > 0) {
+    if (a.do_breakup == 1 && a.make_new_drops) {
+      tab_split<NF>(b, a.max_ppp, Utan, rec);
+    } else {
+      khrt_update<M::NS, NF>(Reyn_d, L.cBoilT, L.g, a.fc, a.lx, a.kh, a.make_new_drops != 0, b, rec);
     }
   }
 
@@ -942,7 +1061,9 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
                                                                    int32_t* cs, double* contrib,
                                                                    unsigned char* stayed, const SurrWeights* sw,
                                                                    unsigned& n_surr, unsigned& n_fb,
-                                                                   unsigned char* redir = nullptr) {
+                                                                   unsigned char* redir = nullptr,
+                                                                   double* tabrec = nullptr,
+                                                                   int32_t* tabcnt = nullptr) {
   constexpr int NS = M::NS;
   constexpr int NSRC = 5 + NF;   // mom[3], mass, Ydot[NF], eng
   constexpr int NITEM = (kTileCells * NSRC + kUpdThreads - 1) / kUpdThreads;
@@ -1015,11 +1136,16 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
   for (int32_t base = d0; base < d1; base += kUpdThreads) {
     const int32_t s = base + t;
     if constexpr (BR) {
-      if (s < d1) {
-        DropLocal<M, NF, R> L;
+      const bool act = s < d1;
+      DropLocal<M, NF, R> L;
+      BDrop<NF> b;
+      double Reyn_d = 0., Utan = 0.;
+      if (act) {
         drop_gather<M, NF, R, true, AT_BOUNDS, true, EB>(a, gas, gvalid, cs, otx, oty, otz, s, L);
-        drop_update_br<M, NF, AT_BOUNDS, EB>(a, tile, t, s, L, contrib, stayed, redir);
+        br_source<M, NF, EB>(a, t, s, L, b, Reyn_d, contrib, redir);
       }
+      if (a.do_breakup == 1) tab_stage<M, NF>(a, act && b.alive, t, Reyn_d, L, b, Utan, tabrec, tabcnt);
+      if (act) br_finish<M, NF, AT_BOUNDS, EB>(a, tile, t, s, L, b, Reyn_d, Utan, stayed);
     } else if constexpr (!SURR) {
       if (s < d1) {
         DropLocal<M, NF, R> L;
@@ -1112,7 +1238,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 }
 
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false>
-__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks))
+__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, BR>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NG = 5 + M::NS;  // staged gas fields: rho, Y[NS], vel[3], T
   constexpr int NSRC = 5 + NF;
@@ -1122,6 +1248,9 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
   __shared__ double contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
   __shared__ unsigned char redir[EB ? kUpdThreads : 1];
+  // the TAB stage's records and bin offsets (branch kernels)
+  __shared__ double tabrec[BR ? TabRec<NF>::kN * kUpdThreads : 1];
+  __shared__ int32_t tabcnt[BR ? kTabBins + 1 : 1];
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
   // workgroups 0 .. ntiles-1: the first segment of every tile; the others:
@@ -1138,7 +1267,8 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF>::kMinBlocks)
     seg = it.y;
   }
   process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs,
-                                                          contrib, stayed, nullptr, n_surr, n_fb, redir);
+                                                          contrib, stayed, nullptr, n_surr, n_fb, redir, tabrec,
+                                                          tabcnt);
 }
 
 // The sums of the heavy tiles (SprayParticles.cpp:419-441): per cell and

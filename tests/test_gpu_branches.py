@@ -141,3 +141,50 @@ def test_config3_children_ids_and_run_to_run():
     assert all(np.array_equal(a, b) for a, b in zip(g1, g2))
     ids = g1[2]
     assert np.unique(ids).size == ids.size
+
+
+def test_restart_from_state_with_film_droplets(tmp_path):
+    """A state the update makes itself: droplets placed on the reflective
+    wall (film droplets among them) sit a rounding error behind it
+    (z < plo).  Handed back
+    through an AMReX particle checkpoint (Restart), the context accepts them
+    (check_bounds keeps droplets behind a reflective face,
+    SprayInterpolation.H:65-100) and the next update equals the oracle's
+    from the same particles, bit for bit."""
+    from oracle import oracle_py as O
+    from pelemp_amd import _capi as C
+    from pelemp_amd.spray import DeviceFab, SprayParticleContainer
+
+    prob = P.config3(ncell=(16, 16, 32), drops_per_cell=3.0)
+    sdata = prob.sdata
+    sd_o = O.latent_shifted(sdata, prob.mech, len(prob.species))
+    cap = 20 * prob.n
+    branch = dict(max_ppp=100.0, ppp_fact=0.5, seed=5, next_id=0, cap=cap)
+    pos, rd, ids = prob.pos, prob.rdata, prob.ids
+    for _ in range(2):
+        st, pos, rd, ids = O.update(prob.mech, sd_o, prob.comps, prob.geom, pos, rd, ids, prob.state,
+                                    prob.state_lo, prob.state_hi, prob.new_source(), prob.src_lo, prob.src_hi,
+                                    prob.flow_dt, branch=branch)
+        assert st == 0
+    behind = pos[:, 2] < prob.geom["plo"][2]
+    print(f"restart: {ids.size} droplets, {int(behind.sum())} behind the wall", flush=True)
+    assert behind.sum() > 0 and np.any(rd[behind, -1] > 0.0)
+    names = [f"c{q}" for q in range(rd.shape[1])]
+    C.write_particle_checkpoint(str(tmp_path), "particles", pos, rd, ids, names, branch["next_id"])
+    pc = SprayParticleContainer(prob.mech, sdata, prob.comps, prob.geom, cap, form=0)
+    pc.set_params(100.0, 0.5, 5)
+    pc.Restart(str(tmp_path))
+    state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
+    src_g = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+    pc.updateParticles(0, state, src_g, prob.flow_dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False,
+                       True, None, -1.0)
+    src_o = prob.new_source()
+    st, pos, rd, ids = O.update(prob.mech, sd_o, prob.comps, prob.geom, pos, rd, ids, prob.state, prob.state_lo,
+                                prob.state_hi, src_o, prob.src_lo, prob.src_hi, prob.flow_dt, branch=branch)
+    assert st == 0
+    g = by_id(*pc.get_particles())
+    o = by_id(pos, rd, ids)
+    pc.close()
+    assert np.array_equal(g[2], o[2])
+    assert np.array_equal(g[1], o[1]) and np.array_equal(g[0], o[0])
+    assert np.array_equal(src_g.numpy(), src_o)

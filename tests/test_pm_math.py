@@ -68,3 +68,15 @@ def test_special_values():
     assert p[0] == 0.0 and p[1] == inf and p[2] == 1.0 and p[3] == 1.0
     l1 = O.pm_eval(2, np.array([0.0, 1e-300, -1.0]))
     assert l1[0] == 0.0 and l1[1] == 1e-300 and l1[2] == -inf
+
+
+def test_sincos_is_sin_and_cos():
+    # one reduction for both: bit for bit the separate functions, in every
+    # quadrant, at large arguments and at the special values
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-10.0, 10.0, 20000), rng.uniform(-1e6, 1e6, 2000),
+                        np.array([0.0, -0.0, np.pi / 2, np.pi, 3 * np.pi / 2, 1e-300, np.inf, -np.inf, np.nan])])
+    s, c = O.pm_eval(5, x), O.pm_eval(6, x)
+    s2, c2 = O.pm_eval(11, x), O.pm_eval(12, x)
+    assert np.array_equal(s.view(np.uint64), s2.view(np.uint64))
+    assert np.array_equal(c.view(np.uint64), c2.view(np.uint64))
