@@ -257,13 +257,15 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // for 2 waves per SIMD (256 registers) at the cost of a few spilled values.
 // The FP32 form with up to two liquid species fits 3 waves per SIMD (168
 // registers) without spills; with more species it keeps 1.  The branch
-// kernels (breakup, walls, films) need far more than 256: at 2 waves per
-// SIMD they spilled 360-600 bytes per lane to scratch memory and waited on
-// it; at 1 wave per SIMD the overflow goes to the accumulation registers
-// (no scratch), measured 18 % faster on the wall/TAB slab.
-template <class R, int NF, bool BR = false>
+// kernel with the TAB stage needs far more than 256: at 2 waves per SIMD it
+// spilled ~600 bytes per lane to scratch memory and waited on it; at 1 wave
+// per SIMD the overflow goes to the accumulation registers (no scratch),
+// measured 18 % faster on the wall/TAB slab.  The branch kernels without TAB
+// (walls, films, splash, KH-RT, EB) are faster at 2 waves (9.9 against
+// 13.4 ms on the same slab without breakup, 5.5 against 7.7 ms with an EB).
+template <class R, int NF, bool TAB = false>
 struct UpdOccupancy {
-  static constexpr int kMinBlocks = BR ? 1 : ((sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1));
+  static constexpr int kMinBlocks = TAB ? 1 : ((sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1));
 };
 
 // Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
@@ -1055,7 +1057,8 @@ __device__ __attribute__((always_inline)) inline bool surr_stage_drop(const Upda
 // a heavy tile: the segment's per-cell sums and stayer counts go to the
 // partials of segment slot slot + seg, which split_combine_kernel adds to
 // the source in segment order.
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false, bool EB = false>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false, bool EB = false,
+          bool TAB = false>
 __device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, int seg,
                                                                    int slot, R* gas, unsigned char* gvalid,
                                                                    int32_t* cs, double* contrib,
@@ -1144,7 +1147,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
         drop_gather<M, NF, R, true, AT_BOUNDS, true, EB>(a, gas, gvalid, cs, otx, oty, otz, s, L);
         br_source<M, NF, EB>(a, t, s, L, b, Reyn_d, contrib, redir);
       }
-      if (a.do_breakup == 1) tab_stage<M, NF>(a, act && b.alive, t, Reyn_d, L, b, Utan, tabrec, tabcnt);
+      if constexpr (TAB) tab_stage<M, NF>(a, act && b.alive, t, Reyn_d, L, b, Utan, tabrec, tabcnt);
       if (act) br_finish<M, NF, AT_BOUNDS, EB>(a, tile, t, s, L, b, Reyn_d, Utan, stayed);
     } else if constexpr (!SURR) {
       if (s < d1) {
@@ -1237,8 +1240,10 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
   if constexpr (SURR) __syncthreads();  // cs is rewritten by the next tile
 }
 
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false>
-__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, BR>::kMinBlocks))
+// TAB: the branch kernel for the TAB breakup model (do_breakup == 1), with
+// the workgroup TAB stage
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false, bool TAB = false>
+__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NG = 5 + M::NS;  // staged gas fields: rho, Y[NS], vel[3], T
   constexpr int NSRC = 5 + NF;
@@ -1249,8 +1254,8 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, BR>::kMinBlo
   __shared__ unsigned char stayed[kUpdThreads];
   __shared__ unsigned char redir[EB ? kUpdThreads : 1];
   // the TAB stage's records and bin offsets (branch kernels)
-  __shared__ double tabrec[BR ? TabRec<NF>::kN * kUpdThreads : 1];
-  __shared__ int32_t tabcnt[BR ? kTabBins + 1 : 1];
+  __shared__ double tabrec[TAB ? TabRec<NF>::kN * kUpdThreads : 1];
+  __shared__ int32_t tabcnt[TAB ? kTabBins + 1 : 1];
   const int ntiles = a.ntile[0] * a.ntile[1] * a.ntile[2];
   unsigned n_surr = 0, n_fb = 0;
   // workgroups 0 .. ntiles-1: the first segment of every tile; the others:
@@ -1266,7 +1271,7 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, BR>::kMinBlo
     tile = it.x;
     seg = it.y;
   }
-  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs,
+  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB, TAB>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs,
                                                           contrib, stayed, nullptr, n_surr, n_fb, redir, tabrec,
                                                           tabcnt);
 }

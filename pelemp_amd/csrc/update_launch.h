@@ -137,14 +137,28 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
         a.eb_rkey = c.eb_rkey;
         a.eb_rval = c.eb_rval;
       }
+      // the TAB model runs the kernel with the workgroup TAB stage
+      const bool tab = c.sd.do_breakup == 1;
       if (c.eb_on) {
         if (p.at_bounds) {
-          hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true, true>), grid, block, 0, s, a);
+          if (tab) {
+            hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true, true, true>), grid, block, 0, s, a);
+          } else {
+            hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true, true>), grid, block, 0, s, a);
+          }
+        } else if (tab) {
+          hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true, true, true>), grid, block, 0, s, a);
         } else {
           hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true, true>), grid, block, 0, s, a);
         }
       } else if (p.at_bounds) {
-        hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true>), grid, block, 0, s, a);
+        if (tab) {
+          hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true, false, true>), grid, block, 0, s, a);
+        } else {
+          hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, true, true>), grid, block, 0, s, a);
+        }
+      } else if (tab) {
+        hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true, false, true>), grid, block, 0, s, a);
       } else {
         hipLaunchKernelGGL((spray_update_kernel<M, NF, double, true, false, true>), grid, block, 0, s, a);
       }
