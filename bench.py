@@ -115,6 +115,194 @@ def run_form(prob, form: int, steps: int, warmup: int, stream, acc_steps: int = 
     return res
 
 
+def _liquid_mass(prob, rd) -> float:
+    """Liquid mass of droplets in the AoS meaning: N pi/6 rho_L d^3 with
+    rho_L the mass-fraction-weighted harmonic mean of the fuels' cubic fits
+    at T_d (SprayFuelData.H:146-155)."""
+    nf = prob.nf
+    c = np.asarray(prob.sdata["rho_coef"], dtype=np.float64)
+    T = rd[:, 3]
+    rinv = np.zeros(rd.shape[0])
+    for k in range(nf):
+        a, b, cc, d = c[4 * k:4 * k + 4]
+        rinv += rd[:, 5 + k] / (a + T * (b + T * (cc + T * d)))
+    return float(np.sum(rd[:, 5 + nf] * np.pi / 6.0 / rinv * rd[:, 4] ** 3))
+
+
+def lifetime_study(prob, stream, forms=("fp64", "fp32", "surrogate"), max_steps: int = 6000, every: int = 10,
+                   field_steps=(100, 500, 1000)):
+    """Errors of the FP32 and surrogate forms against the FP64 form (which
+    equals the oracle bit for bit) over whole droplet lifetimes, from the same
+    droplets in the same frozen gas: each form runs until its liquid is gone
+    (or max_steps).  Every `every` steps the droplets are sampled; a droplet
+    that vanished since the last sample gets the lifetime its last two
+    samples give by the d^2 law.  Reported per form: the times at which 50,
+    90 and 99 % of the initial liquid mass has evaporated, the per-droplet
+    lifetime error against the FP64 form (median, p99, max of |dt|/t), the
+    largest difference of the evaporated-mass curves as a fraction of the
+    initial mass, and, at the steps in field_steps, the error of the
+    accumulated gas-phase source fields per component: the largest cell
+    difference relative to the FP64 field's largest value, and the relative
+    L2 difference.  A droplet that crosses a cell face one step earlier or
+    later moves that step's deposit to the neighbouring cell, which the
+    largest cell difference shows and the evaporated mass does not."""
+    from pelemp_amd import _capi as C
+    from pelemp_amd.spray import DeviceFab, SprayParticleContainer
+
+    h = stream.handle
+    V = float(np.prod(prob.geom["dx"]))
+    dt = prob.flow_dt
+    m0 = _liquid_mass(prob, prob.rdata)
+    nid = int(prob.ids.max()) + 1
+    code = {"fp64": C.FORM_FP64, "fp32": C.FORM_FP32, "surrogate": C.FORM_SURROGATE}
+    runs = {}
+    for name in forms:
+        pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, int(prob.n * 1.02) + 1024,
+                                    form=code[name])
+        if name == "surrogate":
+            from pelemp_amd import surrogate as S
+
+            S.Surrogate.load(S.default_path(prob.mech)).install(pc)
+        pc.set_particles(prob.pos, prob.rdata, prob.ids)
+        state = DeviceFab(prob.state_lo, prob.state_hi, prob.state.shape[0], prob.state)
+        src = DeviceFab(prob.src_lo, prob.src_hi, prob.src_shape[0])
+        pc.set_source_mode(C.SOURCE_ACCUMULATE)
+        life = np.full(nid, np.nan)
+        d2a = np.full(nid, np.nan)  # d^2 at the last sample
+        d2b = np.full(nid, np.nan)  # d^2 at the sample before
+        d2a[prob.ids] = prob.rdata[:, 4] ** 2
+        ta, tb = 0.0, np.nan
+        alive = np.zeros(nid, bool)
+        alive[prob.ids] = True
+        curve_t, curve_m = [0.0], [0.0]
+        fields = {}
+        step = 0
+        while step < max_steps and alive.any():
+            for _ in range(every):
+                pc.updateParticles(0, state, src, dt, 0.0, prob.state_ghosts, prob.source_ghosts, False, False, True,
+                                   None, -1.0, stream=h, sync=False)
+            step += every
+            pc.sync(h)
+            t = step * dt
+            s = src.numpy()
+            curve_t.append(t)
+            curve_m.append(float(s[prob.comps["rhoSrcIndx"]].sum()) * V * dt)
+            if step in field_steps:
+                fields[step] = s.copy()
+            _, rd, ids = pc.get_particles()
+            now = np.zeros(nid, bool)
+            now[ids] = True
+            gone = np.flatnonzero(alive & ~now)
+            if gone.size:
+                slope = (d2a[gone] - d2b[gone]) / (ta - tb) if np.isfinite(tb) else np.full(gone.size, np.nan)
+                est = ta - d2a[gone] / np.where(slope < 0.0, slope, np.nan)
+                life[gone] = np.clip(np.where(np.isfinite(est), est, t), ta, t)
+            alive = now
+            d2b[ids] = d2a[ids]
+            d2a[ids] = rd[:, 4] ** 2
+            tb, ta = ta, t
+        if name == "surrogate":
+            import ctypes
+
+            ns, nfb = ctypes.c_int64(), ctypes.c_int64()
+            C.check(C.lib().pelemp_spray_surrogate_counts(pc._h, ctypes.byref(ns), ctypes.byref(nfb)), "counts")
+            share = ns.value / max(1, ns.value + nfb.value)
+        else:
+            share = None
+        pc.close()
+        runs[name] = dict(life=life, t=np.array(curve_t), m=np.array(curve_m) / m0, fields=fields, steps=step,
+                          left=int(alive.sum()), share=share)
+
+    def t_at(r, frac):
+        k = np.flatnonzero(r["m"] >= frac)
+        return float(r["t"][k[0]]) * 1e3 if k.size else None
+
+    ref = runs.get("fp64")
+    out = dict(droplets=int(prob.n), dt=dt, every=every, initial_liquid_mass=m0)
+    for name, r in runs.items():
+        o = dict(steps=r["steps"], droplets_left=r["left"], evaporated_frac_end=float(r["m"][-1]),
+                 t50_ms=t_at(r, 0.5), t90_ms=t_at(r, 0.9), t99_ms=t_at(r, 0.99))
+        if r["share"] is not None:
+            o["surrogate_share"] = r["share"]
+        if ref is not None and name != "fp64":
+            both = np.isfinite(r["life"]) & np.isfinite(ref["life"])
+            rel = np.abs(r["life"][both] - ref["life"][both]) / ref["life"][both]
+            o["lifetime_droplets_compared"] = int(both.sum())
+            if rel.size:
+                o["lifetime_rel_err"] = dict(median=float(np.median(rel)), p99=float(np.quantile(rel, 0.99)),
+                                             max=float(rel.max()))
+            k = min(len(r["m"]), len(ref["m"]))
+            o["evaporated_frac_max_abs_err"] = float(np.max(np.abs(r["m"][:k] - ref["m"][:k])))
+            ferr = {}
+            for st_, f in r["fields"].items():
+                g = ref["fields"].get(st_)
+                if g is None:
+                    continue
+                scale = np.max(np.abs(g), axis=(1, 2, 3))
+                err = np.max(np.abs(f - g), axis=(1, 2, 3)) / np.where(scale > 0, scale, 1.0)
+                l2g = np.sqrt(np.sum(g * g, axis=(1, 2, 3)))
+                l2 = np.sqrt(np.sum((f - g) ** 2, axis=(1, 2, 3))) / np.where(l2g > 0, l2g, 1.0)
+                nfp = prob.nf
+                ferr[str(st_)] = dict(max=dict(mom=float(err[0:3].max()), mass=float(err[3]),
+                                               species=float(err[4:4 + nfp].max()), energy=float(err[4 + nfp])),
+                                      l2=dict(mom=float(l2[0:3].max()), mass=float(l2[3]),
+                                              species=float(l2[4:4 + nfp].max()), energy=float(l2[4 + nfp])))
+            o["source_field_err"] = ferr
+        out[name] = o
+    return out
+
+
+def run_lifetimes(stream, ncell: int = 16):
+    """The lifetime study on the config-4 physics (1 atm, 1000 K, n-dodecane,
+    dt 1e-6 s) and the config-2 physics (30 atm, 900 K, n-decane/n-dodecane,
+    dt 5e-7 s), each on a small box with the workload's droplet density."""
+    from pelemp_amd import problem as P
+
+    out = {}
+    p4 = P.config4(ncell=ncell)
+    out["config4"] = dict(workload=f"config4 recipe on {ncell}^3 cells",
+                          **lifetime_study(p4, stream, max_steps=16000, every=10, field_steps=(100, 1000, 4000)))
+    p2 = P.config2(ncell=ncell)
+    out["config2"] = dict(workload=f"config2 recipe on {ncell}^3 cells",
+                          **lifetime_study(p2, stream, max_steps=32000, every=20, field_steps=(200, 2000, 8000)))
+    return out
+
+
+def run_config4(steps: int, warmup: int, stream, acc_steps: int = 10):
+    """Config 4: the learned surrogate on the config-1 physics scaled to
+    1.6e7 droplets (128^3 cells, 1 atm, 1000 K, n-dodecane), beside the FP64
+    and FP32 forms on the same droplets: droplet-steps/s, the share of
+    droplet-substeps the network served, and the evaporated mass of
+    acc_steps steps against the FP64 form (errors over whole lifetimes are
+    in the lifetimes entry)."""
+    from pelemp_amd import _capi as C
+    from pelemp_amd import problem as P
+
+    t0 = time.perf_counter()
+    prob = P.config4()
+    print(f"# config4 built: {prob.n} droplets in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = dict(workload=f"config4: 128^3 periodic box dx=1e-4 m, 1 atm, T0=1000 K, {prob.n} n-dodecane droplets "
+                        f"(Poisson ~7.63/cell, median 20 um, sigma_ln 0.4, T_d 300 K), dt=1e-6 s",
+               droplets=int(prob.n), forms={})
+    code = {"fp64": C.FORM_FP64, "fp32": C.FORM_FP32, "surrogate": C.FORM_SURROGATE}
+    for name in ("fp64", "fp32", "surrogate"):
+        r = run_form(prob, code[name], steps, warmup, stream, acc_steps)
+        f = dict(value=r["rate"], ms_per_step=r["ms_total"] / steps, update_kernel_ms=r["update_ms"],
+                 sort_ms=r["sort_ms"], evaporated_mass=r.get("evaporated_mass"))
+        if "surrogate_share" in r:
+            f["surrogate_share"] = r["surrogate_share"]
+        out["forms"][name] = f
+        print(f"# config4 {name}: {r['rate']:.4g} droplet-steps/s, {f['ms_per_step']:.3f} ms/step, "
+              f"update kernel {r['update_ms']:.3f} ms", file=sys.stderr, flush=True)
+    ref = out["forms"]["fp64"]
+    for name, f in out["forms"].items():
+        f["speedup_vs_fp64"] = f["value"] / ref["value"]
+        if ref.get("evaporated_mass"):
+            f["evaporated_mass_rel_err_vs_fp64"] = abs(f["evaporated_mass"] - ref["evaporated_mass"]) / abs(
+                ref["evaporated_mass"])
+    return out
+
+
 def run_branches(steps: int, warmup: int, stream, cap_factor: float = 5.0):
     """Config 3 (FP64 with walls, TAB breakup, splash and wall films): the
     update kernel with its branches inline, plus the passes around it -- the
@@ -524,6 +712,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-branches", action="store_true", help="skip the config-3 (walls/breakup/splash) leg")
+    ap.add_argument("--no-config4", action="store_true", help="skip the config-4 (surrogate, 1.6e7 droplets) leg")
+    ap.add_argument("--no-lifetimes", action="store_true",
+                    help="skip the FP32/surrogate error study over whole droplet lifetimes")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the leg timing transferSource, estTimestep, derived variables, PeleC, EB, injection")
     ap.add_argument("--no-jet", action="store_true", help="skip the config-5 (dense hollow-cone jet) leg")
@@ -621,6 +812,24 @@ def main(argv=None):
         print(f"# branches (config3): {br['value']:.4g} droplet-steps/s, {br['ms_per_step']:.3f} ms/step, "
               f"film pass {br['film_pass_ms']:.3f} ms, children pass {br['children_pass_ms']:.3f} ms, "
               f"droplets {br['droplets_start']} -> {br['droplets_end']}", file=sys.stderr, flush=True)
+    if not args.no_config4:
+        try:
+            out["config4"] = run_config4(min(args.steps, 10), min(args.warmup, 2), stream)
+        except C.PelempError as e:
+            out["config4"] = dict(error=str(e))
+            print(f"# config4 failed: {e}", file=sys.stderr, flush=True)
+    if not args.no_lifetimes:
+        try:
+            out["lifetimes"] = lt = run_lifetimes(stream)
+            for k, v in lt.items():
+                msg = ", ".join(f"{f}: t50 {v[f]['t50_ms']} ms, lifetime err p99 "
+                                f"{v[f].get('lifetime_rel_err', {}).get('p99')}, evap err "
+                                f"{v[f].get('evaporated_frac_max_abs_err')}" for f in ("fp64", "fp32", "surrogate")
+                                if f in v)
+                print(f"# lifetimes {k}: {msg}", file=sys.stderr, flush=True)
+        except C.PelempError as e:
+            out["lifetimes"] = dict(error=str(e))
+            print(f"# lifetimes failed: {e}", file=sys.stderr, flush=True)
     if not args.no_jet:
         try:
             out["jet"] = run_jet(min(args.steps, 20), min(args.warmup, 2), stream, int(args.jet_drops))

@@ -348,6 +348,18 @@ def config1(scale: float = 1.0, seed: int = 1234, **kw) -> SprayProblem:
                          flow_dt=1e-6, nsteps=100, seed=seed, **kw)
 
 
+def config4(ncell: int = 128, drops_per_cell: float = 1.6e7 / 128 ** 3, seed: int = 2468, **kw) -> SprayProblem:
+    """Config 4: the learned surrogate on the config-1 physics scaled to
+    1.6e7 droplets: a 128^3 periodic box, dx = 1e-4 m, air at 1 atm and
+    T0 = 1000 K (the config-1 gas recipe), Poisson(~7.63) n-dodecane droplets
+    per cell (median 20 um, sigma_ln 0.4, T_d = 300 K), 1e-6 s steps.
+    `ncell` < 128 gives the same recipe on a smaller box (for tests)."""
+    n = drops_per_cell * ncell ** 3
+    return build_problem("config4", ncell=(ncell, ncell, ncell), p=PATM, T0=1000.0, mech=C.MECH_DODECANE_AIR,
+                         fuels=("NC12H26",), n_drops=n, poisson=True, d_median=20e-6, sigma_ln=0.4, flow_dt=1e-6,
+                         nsteps=100, seed=seed, **kw)
+
+
 def config2(ncell: int = 128, drops_per_cell: float = 1.6e7 / 128 ** 3, seed: int = 4321, **kw) -> SprayProblem:
     """Config 2: 128^3 periodic box at 30 atm, T0 = 900 K, Poisson(~7.63)
     droplets per cell of an n-decane/n-dodecane liquid (Y_decane ~ U[0.3,0.7]),

@@ -288,20 +288,21 @@ def test_training_reduces_loss():
 
 
 # ---- the surrogate form of the update ---------------------------------------
-def _trained():
+def _trained(mech_name="MECH_DECANE_DODECANE_AIR"):
     from pelemp_amd import _capi as C
     from pelemp_amd import surrogate as S
 
-    path = S.default_path(C.MECH_DECANE_DODECANE_AIR)
+    path = S.default_path(getattr(C, mech_name))
     assert os.path.exists(path), f"trained network missing: {path}"
     return S.Surrogate.load(path)
 
 
-def test_trained_network_error_per_output():
+@pytest.mark.parametrize("mech_name", ["MECH_DECANE_DODECANE_AIR", "MECH_DODECANE_AIR"])
+def test_trained_network_error_per_output(mech_name):
     C = _lib()
     from pelemp_amd import surrogate as S
 
-    net = _trained()
+    net = _trained(mech_name)
     val = S.Samples(net.mech, 100_000, 777)
     _, t, w = val.host()
     err = S.output_errors(S.infer_bf16(net.params, net.norm, val.x, val.n), t, w, net.norm)

@@ -13,7 +13,8 @@ computes the three film quantities -- ln(sum rho D_k), ln(lambda Nu*),
 ln(mu) -- with the CPU oracle (the plain FP64 method), the network and the
 closure, and checks that the network's error is a small fraction of the
 closure's: the accuracy it buys at the closure's cost class (one small MLP
-instead of the transport call and fixed point)."""
+instead of the transport call and fixed point).  Both committed networks:
+the two-fuel one (config 2) and the n-dodecane one (configs 1 and 4)."""
 import ctypes
 import math
 
@@ -47,7 +48,8 @@ def _film_quantities(O, mech, sd, W, fuel_idx, dia, Tg, p, urel, Y, rho, Yl, cB,
     cp_fuel = sum(Ysk[ix] * cp[ix] for ix in fuel_idx) / ysk_f
     ratio = cp_fuel * Sh * rhoD / lam
     Nu = O.calc_heat_coeff(ratio, BM, 1e-7, 1e-15, Nu0)
-    x = [math.log(dia), Td, math.log(urel + 0.01), Tg, math.log(p), ysk_f, Yl[0], Yl[-1]]
+    # the second liquid feature is 0 for a one-fuel liquid (surr_features)
+    x = [math.log(dia), Td, math.log(urel + 0.01), Tg, math.log(p), ysk_f, Yl[0], Yl[-1] if len(Yl) > 1 else 0.0]
     return (math.log(rhoD), math.log(lam * Nu), math.log(mu)), x
 
 
@@ -64,14 +66,15 @@ def _closure(O, mech, W, fuel_idx, dia, Tg, p, urel, Y, rho):
     return math.log(rhoD), math.log(lam * Nu0), math.log(mu)
 
 
-def test_network_beats_the_frozen_property_closure():
+@pytest.mark.parametrize("mech_name", ["MECH_DECANE_DODECANE_AIR", "MECH_DODECANE_AIR"])
+def test_network_beats_the_frozen_property_closure(mech_name):
     from oracle import mlp_ref as R
     from oracle import oracle_py as O
     from pelemp_amd import _capi as C
     from pelemp_amd import surrogate as S
     from pelemp_amd.problem import mech_info
 
-    mech = C.MECH_DECANE_DODECANE_AIR
+    mech = getattr(C, mech_name)
     net = S.Surrogate.load(S.default_path(mech))
     n = S.norm_dict(net.norm)
     W = np.array(mech_info(mech)["W"], dtype=np.float64)
