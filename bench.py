@@ -466,8 +466,28 @@ def host_cpu() -> dict:
     except OSError:
         pass
     aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    lscpu = {}
+    try:
+        import subprocess
+
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20)
+        keep = ("Architecture", "CPU(s)", "On-line CPU(s) list", "Model name", "Thread(s) per core",
+                "Core(s) per socket", "Socket(s)", "NUMA node(s)", "CPU max MHz", "L3 cache")
+        for line in r.stdout.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keep:
+                lscpu[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    compiler = None
+    try:
+        from oracle import oracle_py as O
+
+        compiler = O.compiler() + " -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (oracle/Makefile)"
+    except (OSError, AttributeError):
+        pass
     return dict(cpu_count=os.cpu_count(), affinity=aff, omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
-                model=model)
+                model=model, nproc=aff, lscpu=lscpu, compiler=compiler)
 
 
 def cpu_baseline(prob, target_s: float = 12.0, parity_stream=None):
@@ -583,6 +603,19 @@ def measured_traffic(form: str):
         return None
     for line in open(path):
         if line.startswith(f"traffic_bytes_per_launch[{form}]"):
+            return float(line.split("=")[1].split()[0])
+    return None
+
+
+def valu_profile(key: str, form: str = "fp64"):
+    """A per-droplet figure of the update kernel from the committed
+    rocprofv3 counter summary (profiles/spray_update_config2_valu_counters.md):
+    fp64_flop_per_droplet, valu_instr_per_droplet or valu_busy; None if absent."""
+    path = os.path.join(PROFILES, "spray_update_config2_valu_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"{key}[{form}]"):
             return float(line.split("=")[1].split()[0])
     return None
 
@@ -794,6 +827,18 @@ def main(argv=None):
                      "kernel": "spray_update_kernel", "bytes_per_droplet_step": mf["bytes_per_droplet_step"]},
         "forms": forms,
     }
+    flop = valu_profile("fp64_flop_per_droplet")
+    if "fp64" in forms and flop is not None and args.ncell == 128:
+        # FP64 arithmetic of the FP64 update kernel against the FP64 vector
+        # peak: the profiled flop count per droplet-step x droplet-steps per
+        # launch / the live mean kernel time
+        f64 = forms["fp64"]
+        tflops = flop * prob.n / (f64["update_kernel_ms"] * 1e-3) / 1e12
+        out["valu"] = {"achieved": tflops, "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": tflops / FP64_VALU_PEAK_TFLOPS, "fp64_flop_per_droplet_step": flop,
+                       "valu_instr_per_droplet_step": valu_profile("valu_instr_per_droplet"),
+                       "valu_busy": valu_profile("valu_busy"),
+                       "source": "profiles/spray_update_config2_valu_counters.md"}
     if not args.no_rows and "fp64" in forms:
         try:
             out["rows"] = run_rows(prob, stream)

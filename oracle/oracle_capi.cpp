@@ -190,6 +190,9 @@ int oracle_eb_interp(const pelemp_geom* g, double min_eb_vfrac, double* pos, int
 
 // Host evaluation of the pm:: elementary functions (fn codes as in
 // pelemp_math_eval of include/pelemp_spray.h).
+// The compiler that built the oracle (recorded with the CPU baseline).
+const char* oracle_compiler(void) { return "g++ " __VERSION__; }
+
 void oracle_pm_eval(int fn, int64_t n, const double* x, const double* y, double* out) {
   for (int64_t i = 0; i < n; ++i) {
     const double a = x[i];

@@ -46,6 +46,8 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                     ctypes.POINTER(BranchParamsC), ctypes.POINTER(ctypes.c_int64)]
         L.oracle_pm_eval.restype = None
+        L.oracle_compiler.restype = ctypes.c_char_p
+        L.oracle_compiler.argtypes = []
         L.oracle_pm_eval.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp]
         L.oracle_gas_props.restype = ctypes.c_int
         L.oracle_gas_props.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp,
@@ -114,6 +116,11 @@ def lib() -> ctypes.CDLL:
         L.oracle_inj_random.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
         _lib = L
     return _lib
+
+
+def compiler() -> str:
+    """The compiler and version that built the oracle library."""
+    return lib().oracle_compiler().decode()
 
 
 def pm_eval(fn: int, x, y=None) -> np.ndarray:
