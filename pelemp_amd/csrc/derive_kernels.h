@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) derive_kernel(const int32_t* __restrict__
             T_part = static_cast<double>(f[G::T * cap + o]);
             dia_part = static_cast<double>(f[G::DIA * cap + o]);
             num_ppp = static_cast<double>(f[G::ND * cap + o]);
-            film_hght = static_cast<double>(f[G::FILM * cap + o]);
+            film_hght = 0.;  // the FP32 form has no wall films
 #pragma unroll
             for (int d = 0; d < 3; ++d) vel[d] = static_cast<double>(f[(G::VEL + d) * cap + o]);
             double mt = 0.;

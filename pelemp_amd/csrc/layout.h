@@ -42,15 +42,16 @@ struct Fields {
 
 // FP32 form: single-precision fields plus nf double-precision liquid species
 // masses per droplet in a separate array (mass bookkeeping stays FP64; the
-// diameter is written for output only and recomputed from the mass).
+// diameter is written for output only and recomputed from the mass).  The
+// FP32 form has no walls and so no wall films: there is no film-height
+// field (droplets with a film height are refused on input).
 struct Fields32 {
   static constexpr int POS = 0;  // position inside the cell, [0,1)
   static constexpr int VEL = 3;
   static constexpr int T = 6;
   static constexpr int DIA = 7;
   static constexpr int ND = 8;
-  static constexpr int FILM = 9;
-  static constexpr int NHOT = 10;
+  static constexpr int NHOT = 9;
 };
 
 // Heavy tiles.  A tile with more than kSegDroplets droplets is updated by

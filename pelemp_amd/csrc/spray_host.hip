@@ -242,7 +242,6 @@ __global__ void __launch_bounds__(256) ingest32_kernel(int64_t n, int nf, int ns
   f[G::T * cap + i] = static_cast<float>(r[3]);
   f[G::DIA * cap + i] = static_cast<float>(r[4]);
   f[G::ND * cap + i] = static_cast<float>(r[5 + nf]);
-  f[G::FILM * cap + i] = static_cast<float>(r[9 + nf]);
   for (int s = 0; s < nf; ++s) mk[s * cap + i] = r[5 + s] * mass[i];
   const int64_t sl = slot_base + i;
   cold_n0[sl] = r[6 + nf];
@@ -284,7 +283,7 @@ __global__ void __launch_bounds__(256) gather32_kernel(const int32_t* __restrict
   for (int sp = 0; sp < nf; ++sp) m += mk[sp * cap + o];
   for (int sp = 0; sp < nf; ++sp) r[5 + sp] = (m > 0.) ? mk[sp * cap + o] / m : 0.;
   r[5 + nf] = static_cast<double>(f[G::ND * cap + o]);
-  r[9 + nf] = static_cast<double>(f[G::FILM * cap + o]);
+  r[9 + nf] = 0.;
   const int32_t sl = slot[o];
   r[6 + nf] = cold_n0[sl];
   r[7 + nf] = cold_bm1[sl];
@@ -718,6 +717,18 @@ static int compact_storage(Ctx& c) {
   if (n > 0) PM_HIP(hipMemcpy(c.key, c.arrival, 4 * static_cast<size_t>(n), hipMemcpyDeviceToDevice));
   PM_HIP(hipDeviceSynchronize());
   c.cur = nxt;
+  return PELEMP_OK;
+}
+
+// The FP32 and surrogate forms have no walls, so no wall films: a droplet
+// handed in with a film height is refused.
+static int check_no_film(const Ctx& c, const double* hr, int64_t m, int nsr) {
+  if (c.form == PELEMP_FORM_FP64) return PELEMP_OK;
+  for (int64_t i = 0; i < m; ++i)
+    if (hr[nsr * i + 9 + c.nf] > 0.) {
+      set_last_error("the FP32 and surrogate forms have no wall films (a droplet has a film height)");
+      return PELEMP_E_BAD_CONFIG;
+    }
   return PELEMP_OK;
 }
 
@@ -1399,6 +1410,7 @@ int pelemp_spray_set_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
     return PELEMP_E_ARG;
   }
   if (const int st = check_in_domain(c.geom, hp, m)) return st;
+  if (const int st = check_no_film(c, hr, m, nsr)) return st;
   double *dpos = nullptr, *drd = nullptr;
   int64_t* did = nullptr;
   PM_HIP(hipMalloc(&dpos, 8 * std::max<int64_t>(1, 3 * m)));
@@ -1548,6 +1560,7 @@ int pelemp_spray_add_particles(pelemp_spray_ctx* h, int64_t n, const double* pos
   const int64_t m = static_cast<int64_t>(hid.size());
   if (m == 0) return PELEMP_OK;
   if (const int st = check_in_domain(c.geom, hp.data(), m)) return st;
+  if (const int st = check_no_film(c, hr.data(), m, nsr)) return st;
   double *dpos = nullptr, *drd = nullptr;
   int64_t* did = nullptr;
   PM_HIP(hipMalloc(&dpos, 8 * 3 * m));

@@ -34,6 +34,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "branches.h"
 #include "droplet_physics.h"
 #include "eb.h"
@@ -288,10 +290,19 @@ __device__ __attribute__((always_inline)) inline double* src_word(const UpdateAr
   return &a.src.data[comp * a.src.cstride + (static_cast<long long>(gz) * a.src.n[1] + gy) * a.src.n[0] + gx];
 }
 
+// Type of the per-droplet source contributions staged in LDS for the ordered
+// per-cell reduction: the plain kernels stage the droplet's source in the
+// form's precision and the reduction multiplies it by the cell coefficient
+// in double; the branch kernels stage coefficient x source in double (the
+// coefficient varies per droplet in cut cells).
+template <class R, bool BR>
+using ContribT = std::conditional_t<BR, double, R>;
+
 // Per-droplet values carried from the gather to the update.
 template <class M, int NF, class R>
 struct DropLocal {
-  int32_t o;  // storage position in the input buffer
+  int32_t o;     // storage position in the input buffer
+  int32_t slot;  // its cold-data slot
   int lc, cx, cy, cz;
   Drop<R, NF> d;
   R pos[3];
@@ -305,27 +316,18 @@ struct DropLocal {
   bool fe;      // branch kernel: EB weights were used
 };
 
-// Load droplet s of the tile and gather its gas state.
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false>
-__device__ __attribute__((always_inline)) inline void drop_gather(const UpdateArgs<R, NF>& a, const R* gas,
-                                                                  const unsigned char* gvalid, const int32_t* cs,
-                                                                  int otx, int oty, int otz, int32_t s,
-                                                                  DropLocal<M, NF, R>& L) {
-  constexpr int NS = M::NS;
+// Load droplet s of the tile: its storage position, slot and fields (the
+// loads only; drop_gather does the rest).  Issued ahead of the work that
+// needs them, so their latency overlaps with the halo staging or with the
+// previous chunk's reductions.
+template <class M, int NF, class R, bool POS64, bool BR = false>
+__device__ __attribute__((always_inline)) inline void drop_load(const UpdateArgs<R, NF>& a, int32_t s,
+                                                                DropLocal<M, NF, R>& L) {
   using F = Fields<NF>;
   const long long cap = a.cap;
   const int32_t o = a.perm[s];
   L.o = o;
-  // local cell of sorted position s: largest c with cs[c] <= s
-  int lc = 0;
-#pragma unroll
-  for (int step = 32; step >= 1; step >>= 1)
-    if (lc + step < kTileCells && cs[lc + step] <= s) lc += step;
-  L.lc = lc;
-  L.cx = otx + (lc % kTile);
-  L.cy = oty + ((lc / kTile) % kTile);
-  L.cz = otz + lc / (kTile * kTile);
-
+  L.slot = a.slot_in[o];
   Drop<R, NF>& d = L.d;
   R* pos = L.pos;
   if constexpr (POS64) {
@@ -354,13 +356,35 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
     }
     d.T = a.fin[G::T * cap + o];
     d.numDens = a.fin[G::ND * cap + o];
-    L.film = a.fin[G::FILM * cap + o];
+    L.film = R(0);  // no wall films in this form
+#pragma unroll
+    for (int spf = 0; spf < NF; ++spf) L.mk[spf] = a.min[spf * cap + o];
+  }
+}
+
+// Gather the gas state of droplet s of the tile (loaded by drop_load).
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false>
+__device__ __attribute__((always_inline)) inline void drop_gather(const UpdateArgs<R, NF>& a, const R* gas,
+                                                                  const unsigned char* gvalid, const int32_t* cs,
+                                                                  int otx, int oty, int otz, int32_t s,
+                                                                  DropLocal<M, NF, R>& L) {
+  constexpr int NS = M::NS;
+  // local cell of sorted position s: largest c with cs[c] <= s
+  int lc = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1)
+    if (lc + step < kTileCells && cs[lc + step] <= s) lc += step;
+  L.lc = lc;
+  L.cx = otx + (lc % kTile);
+  L.cy = oty + ((lc / kTile) % kTile);
+  L.cz = otz + lc / (kTile * kTile);
+
+  Drop<R, NF>& d = L.d;
+  R* pos = L.pos;
+  if constexpr (!POS64) {
     double m = 0.;
 #pragma unroll
-    for (int spf = 0; spf < NF; ++spf) {
-      L.mk[spf] = a.min[spf * cap + o];
-      m += L.mk[spf];
-    }
+    for (int spf = 0; spf < NF; ++spf) m += L.mk[spf];
     const double im = 1. / m;
 #pragma unroll
     for (int spf = 0; spf < NF; ++spf) d.Y[spf] = static_cast<R>(L.mk[spf] * im);
@@ -519,7 +543,7 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
 __device__ __attribute__((always_inline)) inline void drop_update(const UpdateArgs<R, NF>& a, int tile, int t,
                                                                   int32_t s, DropLocal<M, NF, R>& L,
-                                                                  const SubRates<M, NF, R>* pre, double* contrib,
+                                                                  const SubRates<M, NF, R>* pre, R* contrib,
                                                                   unsigned char* stayed) {
   constexpr int NSRC = 5 + NF;
   using F = Fields<NF>;
@@ -539,14 +563,15 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
   } else {
     spray_source<M, NF, R, !POS64>(a.sub_dt, L.g, a.fc, d, L.cBoilT, sr, pre);
   }
-  // contributions cur_coef * source (SprayParticles.cpp:419-441)
-  double* cb = &contrib[t * NSRC];
+  // the droplet's source, in the form's precision; the reduction multiplies
+  // each by cur_coef in double (SprayParticles.cpp:419-441)
+  R* cb = &contrib[t * NSRC];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) cb[q] = a.cur_coef * static_cast<double>(sr.mom[q]);
-  cb[3] = a.cur_coef * static_cast<double>(sr.mass);
+  for (int q = 0; q < 3; ++q) cb[q] = sr.mom[q];
+  cb[3] = sr.mass;
 #pragma unroll
-  for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = a.cur_coef * static_cast<double>(sr.Ydot[spf]);
-  cb[4 + NF] = a.cur_coef * static_cast<double>(sr.eng);
+  for (int spf = 0; spf < NF; ++spf) cb[4 + spf] = sr.Ydot[spf];
+  cb[4 + NF] = sr.eng;
 
   // drift (SprayParticles.cpp:443-479) and new cell key
   uint32_t key = kDeadKey;
@@ -644,7 +669,6 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
     fo[G::T * cap + s] = d.T;
     fo[G::DIA * cap + s] = d.dia;
     fo[G::ND * cap + s] = d.numDens;
-    fo[G::FILM * cap + s] = film;
     // FP64 mass bookkeeping: accumulate the single-precision increments
     double m = 0.;
 #pragma unroll
@@ -661,7 +685,7 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
       a.mout[spf * cap + s] = v;
     }
   }
-  a.slot_out[s] = a.slot_in[L.o];
+  a.slot_out[s] = L.slot;
   a.key_out[s] = key;
   // classify for the incremental re-sort
   const uint32_t k_old = static_cast<uint32_t>(tile) * kTileCells + L.lc;
@@ -981,7 +1005,7 @@ __device__ __attribute__((always_inline)) inline void br_finish(const UpdateArgs
 #pragma unroll
     for (int spf = 0; spf < NF; ++spf) r[(kSBFields + spf) * cap + s] = rec.Y0[spf];
   }
-  a.slot_out[s] = a.slot_in[L.o];
+  a.slot_out[s] = L.slot;
   a.key_out[s] = key;
   // classify for the incremental re-sort
   const uint32_t k_old = static_cast<uint32_t>(tile) * kTileCells + L.lc;
@@ -1061,7 +1085,7 @@ template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool 
           bool TAB = false>
 __device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, int seg,
                                                                    int slot, R* gas, unsigned char* gvalid,
-                                                                   int32_t* cs, double* contrib,
+                                                                   int32_t* cs, ContribT<R, BR>* contrib,
                                                                    unsigned char* stayed, const SurrWeights* sw,
                                                                    unsigned& n_surr, unsigned& n_fb,
                                                                    unsigned char* redir = nullptr,
@@ -1093,6 +1117,15 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     return;
   }
   const int32_t seg_tag = seg << kSegShift;
+
+  // the plain and surrogate kernels load each chunk's droplets ahead: the
+  // first chunk's before the halo staging, the next chunk's before the
+  // chunk's stayer ranks and reductions (LDS work)
+  constexpr bool kAhead = !BR;
+  DropLocal<M, NF, R> Lp;
+  if constexpr (kAhead) {
+    if (d0 + t < d1) drop_load<M, NF, R, POS64>(a, d0 + t, Lp);
+  }
 
   // ---- stage gas halo ----
   for (int h = t; h < kHaloCells; h += kUpdThreads) {
@@ -1144,6 +1177,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       BDrop<NF> b;
       double Reyn_d = 0., Utan = 0.;
       if (act) {
+        drop_load<M, NF, R, true, true>(a, s, L);
         drop_gather<M, NF, R, true, AT_BOUNDS, true, EB>(a, gas, gvalid, cs, otx, oty, otz, s, L);
         br_source<M, NF, EB>(a, t, s, L, b, Reyn_d, contrib, redir);
       }
@@ -1151,21 +1185,22 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       if (act) br_finish<M, NF, AT_BOUNDS, EB>(a, tile, t, s, L, b, Reyn_d, Utan, stayed);
     } else if constexpr (!SURR) {
       if (s < d1) {
-        DropLocal<M, NF, R> L;
-        drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, L);
-        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, L, nullptr, contrib, stayed);
+        drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, Lp);
+        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, Lp, nullptr, contrib, stayed);
       }
     } else {
-      DropLocal<M, NF, R> L;
       const bool active = s < d1;
-      if (active) drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, L);
+      if (active) drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, Lp);
       SubRates<M, NF, R> o;
-      const bool use = surr_stage_drop<M, NF>(a, *sw, active, L, o);
+      const bool use = surr_stage_drop<M, NF>(a, *sw, active, Lp, o);
       if (active) {
         n_surr += use ? 1u : 0u;
         n_fb += use ? 0u : 1u;
-        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, L, use ? &o : nullptr, contrib, stayed);
+        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, Lp, use ? &o : nullptr, contrib, stayed);
       }
+    }
+    if constexpr (kAhead) {
+      if (s + kUpdThreads < d1) drop_load<M, NF, R, POS64>(a, s + kUpdThreads, Lp);
     }
     __syncthreads();
     // stayers: rank within their cell in position order (cell owner threads)
@@ -1190,8 +1225,11 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
           // deposits sent to another cell by eb_interp are left out here
           for (int32_t i = lo; i < hi; ++i)
             if (!redir[i - base]) v = v + contrib[(i - base) * NSRC + q];
-        } else {
+        } else if constexpr (BR) {
           for (int32_t i = lo; i < hi; ++i) v = v + contrib[(i - base) * NSRC + q];
+        } else {
+          const double cc = a.cur_coef;
+          for (int32_t i = lo; i < hi; ++i) v = v + cc * static_cast<double>(contrib[(i - base) * NSRC + q]);
         }
         acc[r] = v;
       }
@@ -1250,7 +1288,7 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB>::kMinBl
   __shared__ R gas[NG * kHaloCells];
   __shared__ unsigned char gvalid[kHaloCells];
   __shared__ int32_t cs[kTileCells + 1];
-  __shared__ double contrib[kUpdThreads * NSRC];
+  __shared__ ContribT<R, BR> contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
   __shared__ unsigned char redir[EB ? kUpdThreads : 1];
   // the TAB stage's records and bin offsets (branch kernels)
@@ -1327,7 +1365,7 @@ __global__ void __launch_bounds__(kUpdThreads, 2) spray_update_surr_kernel(Updat
   __shared__ float gas[NG * kHaloCells];
   __shared__ unsigned char gvalid[kHaloCells];
   __shared__ int32_t cs[kTileCells + 1];
-  __shared__ double contrib[kUpdThreads * NSRC];
+  __shared__ float contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
   __shared__ SurrWeights sw;
   surr_stage(sw, a.surr_w, threadIdx.x, kUpdThreads);
