@@ -46,6 +46,16 @@ namespace pelemp {
 
 constexpr int kHaloW = kTile + 2;
 constexpr int kHaloCells = kHaloW * kHaloW * kHaloW;
+
+// The staged gas state in LDS is cell-major: the 5 + NS values of halo cell
+// h (rho, Y[NS], vel[3], T) at gas[h * kGasStride + f], the stride padded to
+// 16 bytes so that a droplet's trilinear gather reads each corner cell with
+// a few wide LDS loads instead of one load per value.
+template <class R, int NS>
+constexpr int gas_stride() {
+  constexpr int per16 = 16 / static_cast<int>(sizeof(R));
+  return ((5 + NS + per16 - 1) / per16) * per16;
+}
 constexpr int kUpdThreads = 256;
 
 // Error codes written to the sticky error word (first failure wins).
@@ -257,8 +267,8 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // Occupancy targets (workgroups of 4 waves per CU = waves per SIMD): the
 // FP64 form needs ~265 registers per lane at full freedom; it is compiled
 // for 2 waves per SIMD (256 registers) at the cost of a few spilled values.
-// The FP32 form with up to two liquid species fits 3 waves per SIMD (168
-// registers) without spills; with more species it keeps 1.  The branch
+// The FP32 form with up to two liquid species runs at 4 waves per SIMD (128
+// registers, a few values spilled), with more species at 2.  The branch
 // kernel with the TAB stage needs far more than 256: at 2 waves per SIMD it
 // spilled ~600 bytes per lane to scratch memory and waited on it; at 1 wave
 // per SIMD the overflow goes to the accumulation registers (no scratch),
@@ -267,7 +277,7 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // 13.4 ms on the same slab without breakup, 5.5 against 7.7 ms with an EB).
 template <class R, int NF, bool TAB = false>
 struct UpdOccupancy {
-  static constexpr int kMinBlocks = TAB ? 1 : ((sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 1));
+  static constexpr int kMinBlocks = TAB ? 1 : ((sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 2));
 };
 
 // Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
@@ -491,12 +501,13 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
     hz = inr ? hz : 0;
     const int h = (hz * kHaloW + hy) * kHaloW + hx;
     if (!inr || !gvalid[h]) set_err(a.err, PELEMP_E_STATE_BOX);
-    g.rho += cw * gas[h];
+    const R* gc = gas + h * gas_stride<R, NS>();
+    g.rho += cw * gc[0];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) g.Y[n] += cw * gas[(1 + n) * kHaloCells + h];
+    for (int n = 0; n < NS; ++n) g.Y[n] += cw * gc[1 + n];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) g.vel[q] += cw * gas[(1 + NS + q) * kHaloCells + h];
-    g.T += cw * gas[(4 + NS) * kHaloCells + h];
+    for (int q = 0; q < 3; ++q) g.vel[q] += cw * gc[1 + NS + q];
+    g.T += cw * gc[4 + NS];
   };
   bool eb_weights = false;
   if constexpr (BR) {
@@ -1145,7 +1156,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
         const double* V = a.conv + c;
         const long long cs = a.st.cstride;
 #pragma unroll
-        for (int f = 0; f < 5 + NS; ++f) gas[f * kHaloCells + h] = V[f * cs];
+        for (int f = 0; f < 5 + NS; ++f) gas[h * gas_stride<R, NS>() + f] = V[f * cs];
         continue;
       }
     }
@@ -1154,12 +1165,12 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       double v[5 + NS];
       state_cell<NS>(a.st, c, a.c_rho, a.c_spec, a.c_vel, a.c_T, v);
 #pragma unroll
-      for (int f = 0; f < 5 + NS; ++f) gas[f * kHaloCells + h] = static_cast<R>(v[f]);
+      for (int f = 0; f < 5 + NS; ++f) gas[h * gas_stride<R, NS>() + f] = static_cast<R>(v[f]);
     } else {
       // outside the state box: a droplet reaching it is an error; the
       // cleared cell keeps what it reads defined
 #pragma unroll
-      for (int f = 0; f < 5 + NS; ++f) gas[f * kHaloCells + h] = R(0);
+      for (int f = 0; f < 5 + NS; ++f) gas[h * gas_stride<R, NS>() + f] = R(0);
     }
   }
   __syncthreads();
@@ -1283,9 +1294,9 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false, bool TAB = false>
 __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
-  constexpr int NG = 5 + M::NS;  // staged gas fields: rho, Y[NS], vel[3], T
+  constexpr int NG = gas_stride<R, M::NS>();  // staged gas fields: rho, Y[NS], vel[3], T (padded)
   constexpr int NSRC = 5 + NF;
-  __shared__ R gas[NG * kHaloCells];
+  __shared__ __attribute__((aligned(16))) R gas[NG * kHaloCells];
   __shared__ unsigned char gvalid[kHaloCells];
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ ContribT<R, BR> contrib[kUpdThreads * NSRC];
@@ -1360,9 +1371,9 @@ __global__ void __launch_bounds__(kUpdThreads) split_combine_kernel(UpdateArgs<R
 // tiles; the network is staged once per workgroup.
 template <class M, int NF>
 __global__ void __launch_bounds__(kUpdThreads, 2) spray_update_surr_kernel(UpdateArgs<float, NF> a) {
-  constexpr int NG = 5 + M::NS;
+  constexpr int NG = gas_stride<float, M::NS>();
   constexpr int NSRC = 5 + NF;
-  __shared__ float gas[NG * kHaloCells];
+  __shared__ __attribute__((aligned(16))) float gas[NG * kHaloCells];
   __shared__ unsigned char gvalid[kHaloCells];
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ float contrib[kUpdThreads * NSRC];
