@@ -50,15 +50,25 @@ PM_HD double acos(double x) { return pm::acos(x); }
 PM_HD double atan2(double y, double x) { return pm::atan2(y, x); }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-__device__ inline float exp(float x) { return __expf(x); }
-__device__ inline float log(float x) { return __logf(x); }
-__device__ inline float log1p(float x) { return ::log1pf(x); }
+// Single precision: the hardware base-2 exponential and logarithm
+// (v_exp_f32, v_log_f32, about 1 ulp) without the rescaling steps for
+// denormal arguments and results that the library forms add: the arguments
+// here are positive physical quantities (temperatures, pressures, mixture
+// sums, transfer numbers) far from the denormal range.
+__device__ inline float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ inline float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+// log(1 + x) for x >= 0 with the rounding of 1 + x corrected (Goldberg):
+// log(u) * x / (u - 1), u = fl(1 + x)
+__device__ inline float log1p(float x) {
+  const float u = 1.f + x;
+  return (u == 1.f) ? x : log(u) * (x / (u - 1.f));
+}
 __device__ inline float pow(float x, float y) {
-  return (x == 0.0f) ? ((y > 0.0f) ? 0.0f : __builtin_huge_valf()) : __expf(y * __logf(x));
+  return (x == 0.0f) ? ((y > 0.0f) ? 0.0f : __builtin_huge_valf()) : exp(y * log(x));
 }
 // x >= 0 on every call site (masses, Reynolds and Peclet groups, mixture
 // sums): the hardware exp2/log2 pair instead of the software cbrtf
-__device__ inline float cbrt(float x) { return __expf(__logf(x) * (1.f / 3.f)); }
+__device__ inline float cbrt(float x) { return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.f / 3.f)); }
 __device__ inline float sqrt(float x) { return __builtin_sqrtf(x); }
 __device__ inline float fabs(float x) { return __builtin_fabsf(x); }
 #else

@@ -392,13 +392,16 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
   Drop<R, NF>& d = L.d;
   R* pos = L.pos;
   if constexpr (!POS64) {
+    // the ODE's mass and mass fractions in single precision (the FP64
+    // species masses stay the bookkeeping)
     double m = 0.;
 #pragma unroll
     for (int spf = 0; spf < NF; ++spf) m += L.mk[spf];
-    const double im = 1. / m;
+    const R mf = static_cast<R>(m);
+    const R im = R(1) / mf;
 #pragma unroll
-    for (int spf = 0; spf < NF; ++spf) d.Y[spf] = static_cast<R>(L.mk[spf] * im);
-    d.mass = static_cast<R>(m);
+    for (int spf = 0; spf < NF; ++spf) d.Y[spf] = static_cast<R>(L.mk[spf]) * im;
+    d.mass = mf;
     d.dia = R(0);
   }
   if constexpr (!BR) d.bm1 = R(0);
