@@ -206,13 +206,12 @@ PM_HDI void film_source(double flow_dt, const GasAt<double, M::NS>& g, const Fue
   s.mass = mdot_total;
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
-    const int fdspec = fuel_species<NS, NF>(spf);
     const double oldY = d.Y[spf];
     const double newY = Y_film[spf];
     const double midot = (newY * film_mass - oldY * start_mass) / dt;
     s.Ydot[spf] = midot;
     d.Y[spf] = newY;
-    s.eng += midot * h_film[fdspec];
+    s.eng += midot * dep_value<NS, NF>(fd, h_film, spf);
   }
   d.dia = film_dia;
   d.film = film_height;

@@ -49,13 +49,21 @@ PM_HD R rmax(R a, R b) {
 }
 
 // Gas species index of liquid fuel spf.  Every compiled mechanism lists its
-// NF fuel species last and the vapour of each fuel is deposited into its own
-// species (indx == dep_indx), which the context checks at creation; the
-// index is then a compile-time constant and the per-species arrays stay in
-// registers.
+// NF fuel species last, which the context checks at creation; the index is
+// then a compile-time constant and the per-species arrays stay in registers.
 template <int NS, int NF>
 PM_HD constexpr int fuel_species(int spf) {
   return NS - NF + spf;
+}
+
+// v[k] of a per-species register array at a run-time species index k (a
+// chain of selects, so that the array stays in registers).
+template <int NS, class R>
+PM_HDI R species_at(const R* v, int k) {
+  R r = v[0];
+#pragma unroll
+  for (int n = 1; n < NS; ++n) r = (k == n) ? v[n] : r;
+  return r;
 }
 
 // Fuel data hoisted per launch (droplet independent).
@@ -71,7 +79,8 @@ struct FuelConst {
   R rho_coef[4 * NF];
   R psat_coef[4 * NF];
   int indx[NF];
-  int dep_indx[NF];
+  int dep_indx[NF];  // gas species that receives the vapour of fuel spf (dep_fuel_species)
+  bool dep_own;      // every fuel's vapour goes to its own species (dep_indx == indx)
   bool cc_psat[NF];  // psat_coef[4*spf+3] == 0: Clausius-Clapeyron
   R ref_T;
   R ln10;  // pm::log(10) (host and device give the same bits)
@@ -105,6 +114,8 @@ inline FuelConst<R, NF> make_fuel_const(const pelemp_spray_data& f, const double
     c.dep_indx[spf] = f.dep_indx[spf];
     c.cc_psat[spf] = (f.psat_coef[4 * spf + 3] == 0.);
   }
+  c.dep_own = true;
+  for (int spf = 0; spf < NF; ++spf) c.dep_own = c.dep_own && (f.dep_indx[spf] == f.indx[spf]);
   c.ref_T = static_cast<R>(f.ref_T);
   c.ln10 = static_cast<R>(pm::log(10.0));
   c.dtmod = static_cast<R>(f.dtmod);
@@ -114,6 +125,13 @@ inline FuelConst<R, NF> make_fuel_const(const pelemp_spray_data& f, const double
   c.fixed_parts = f.fixed_parts != 0;
   c.do_breakup = f.do_breakup;
   return c;
+}
+
+// Of the deposition species of fuel spf (dep_indx, Drag.H:76,112,426): the
+// gas mass fraction and a per-species value such as the enthalpy.
+template <int NS, int NF, class R>
+PM_HDI R dep_value(const FuelConst<R, NF>& fd, const R* v, int spf) {
+  return fd.dep_own ? v[fuel_species<NS, NF>(spf)] : species_at<NS>(v, fd.dep_indx[spf]);
 }
 
 // Interpolated gas state at the droplet (GasPhaseVals, SprayFuelData.H:71-112)
@@ -291,7 +309,6 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
     const int fspec = fuel_species<NS, NF>(spf);
-    const int fdspec = fspec;
     const R boilT_ref = fd.boilT[spf];
     const R mw_fuel = fd.mw_fuel[spf];
     const R T_part = rmin(T_in, R(0.99) * cBoilT[spf]);
@@ -304,7 +321,7 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
     } else {
       pres_sat = psat<R, NF>(fd, T_part, spf);
     }
-    const R X_fluid = g.Y[fdspec] * g.mw_mix / mw_fuel;
+    const R X_fluid = dep_value<NS, NF>(fd, g.Y, spf) * g.mw_mix / mw_fuel;
     if (pres_sat > X_fluid * g.p) {
       const R Xl = Y_l[spf] * mw_part / mw_fuel;
       const R Xv = Xl * pres_sat / g.p;
@@ -321,11 +338,10 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
   for (int spf = 0; spf < NF; ++spf) {
     if (X_vapor[spf] > R(0)) {
       const int fspec = fuel_species<NS, NF>(spf);
-      const int fdspec = fspec;
       const R mw_fuel = fd.mw_fuel[spf];
       const R Yfv = X_vapor[spf] * mw_fuel / totalmwx;
       sumYVap += Yfv;
-      const R gasY = g.Y[fdspec];
+      const R gasY = dep_value<NS, NF>(fd, g.Y, spf);
       sumYfFluid += gasY;
       const R Ysk = Yfv + rule * (gasY - Yfv);
       Y_skin[fspec] = Ysk;
@@ -688,10 +704,9 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
     }
 #pragma unroll
     for (int spf = 0; spf < NF; ++spf) {
-      const int fdspec = fuel_species<NS, NF>(spf);
       const R midot = d.dmk[spf] * inv_dt;
       s.Ydot[spf] = num_ppp * midot;
-      s.eng += num_ppp * midot * o.h_part[fdspec];
+      s.eng += num_ppp * midot * dep_value<NS, NF>(fd, o.h_part, spf);
       d.Y[spf] = Y_part[spf];
     }
 #pragma unroll
@@ -713,12 +728,11 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
   }
 #pragma unroll
   for (int spf = 0; spf < NF; ++spf) {
-    const int fdspec = fuel_species<NS, NF>(spf);
     const R oldY = d.Y[spf];
     const R newY = Y_part[spf];
     const R midot = (newY * pmass - oldY * startmass) / (fd.dtmod * flow_dt);
     s.Ydot[spf] = num_ppp * midot;
-    s.eng += num_ppp * midot * o.h_part[fdspec];
+    s.eng += num_ppp * midot * dep_value<NS, NF>(fd, o.h_part, spf);
     d.Y[spf] = Y_part[spf];
   }
 #pragma unroll

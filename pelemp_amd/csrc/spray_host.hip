@@ -1178,11 +1178,15 @@ int pelemp_spray_create(pelemp_spray_ctx** out, int32_t mech, int32_t form, cons
       set_last_error("fuel species index not in the mechanism");  // SpraySetup.cpp:272-277
       return PELEMP_E_BAD_CONFIG;
     }
-    // the kernels address fuel species at compile time: the fuels are the
-    // last nfuel species of the mechanism and deposit into themselves
-    if (sd->indx[i] != ns - sd->nfuel + i || sd->dep_indx[i] != sd->indx[i]) {
-      set_last_error("fuel species must be the last nfuel species of the mechanism, in order, "
-                     "with dep_indx == indx");
+    // the kernels address the fuel species at compile time: every compiled
+    // mechanism lists its fuels last, and the liquid's species are those
+    // fuels in order.  The vapour may go to any species (dep_fuel_species,
+    // SpraySetup.cpp:103-125), except in the surrogate form, whose network
+    // sees the fuel's own gas-phase mass fraction.
+    if (sd->indx[i] != ns - sd->nfuel + i ||
+        (form == PELEMP_FORM_SURROGATE && sd->dep_indx[i] != sd->indx[i])) {
+      set_last_error("fuel species must be the last nfuel species of the mechanism, in order "
+                     "(and, in the surrogate form, deposit into themselves)");
       return PELEMP_E_BAD_CONFIG;
     }
   }

@@ -77,11 +77,14 @@ ALKANE6 = ("NC8H18", "NC10H22", "NC12H26", "NC13H28", "NC14H30", "NC16H34")
 
 def spray_data(fuels, species, *, mass_trans=True, mom_trans=True, fixed_parts=False, dtmod=1.0,
                body_force=(0.0, 0.0, 0.0), do_breakup=0, do_splash=False, sigma=None, wall_T=-1.0,
-               theta_c_deg=-1.0) -> dict:
+               theta_c_deg=-1.0, dep_species=None) -> dict:
     """SprayData dict for liquid `fuels` in a gas mechanism with `species`.
 
     `latent` is set to the reference latent heat; the library subtracts the
     gas enthalpy at ref_T when the context is created (SpraySetup.cpp:284-290).
+    `dep_species`: the gas species that receive each fuel's vapour (the
+    reference's `dep_fuel_species` input, SpraySetup.cpp:103-125); by default
+    each fuel's own species.
     """
     nf = len(fuels)
     d = dict(mass_trans=int(mass_trans), mom_trans=int(mom_trans), fixed_parts=int(fixed_parts),
@@ -99,7 +102,12 @@ def spray_data(fuels, species, *, mass_trans=True, mom_trans=True, fixed_parts=F
     for k in ("psat_coef", "rho_coef", "lambda_coef", "mu_coef"):
         d[k] = [c for f in fuels for c in FUELS[f][k]]
     d["indx"] = [species.index(f) for f in fuels]
-    d["dep_indx"] = list(d["indx"])
+    if dep_species is None:
+        d["dep_indx"] = list(d["indx"])
+    else:
+        if len(dep_species) != nf:
+            raise ValueError("one deposition species per fuel")
+        d["dep_indx"] = [species.index(f) for f in dep_species]
     sig = sigma if sigma is not None else FUELS[fuels[0]].get("sigma")
     d["sigma"] = float(sig) if sig is not None else -1.0
     return d

@@ -54,3 +54,45 @@ def test_fp32_form_vs_fp64_oracle():
     for gs, os_ in zip(gsrc, osrc):
         scale = np.max(np.abs(os_), axis=(1, 2, 3), keepdims=True)
         assert np.max(np.abs(gs - os_) / np.maximum(scale, 1e-300)) < 1e-3
+
+
+def test_fp32_form_with_swapped_deposition_species():
+    # dep_fuel_species in the FP32 form: the decane vapour goes to NC12H26
+    # and the dodecane vapour to NC10H22, in a gas holding some of both;
+    # the same tolerances against the FP64 oracle
+    from tests.test_gpu_parity import _with_fuel_vapour
+
+    prob = _with_fuel_vapour(P.config2(ncell=16))
+    prob.sdata = dict(prob.sdata, dep_indx=[prob.sdata["indx"][1], prob.sdata["indx"][0]])
+    nsteps = 3
+    _, gpos, grd, gids, gsrc = run_gpu(prob, nsteps, form=C.FORM_FP32)
+    opos, ord_, oids, osrc = run_oracle(prob, nsteps)
+    gpos, grd, gids = by_id(gpos, grd, gids)
+    opos, ord_, oids = by_id(opos, ord_, oids)
+    assert np.array_equal(gids, oids)
+    assert np.max(np.abs(grd[:, 4] / ord_[:, 4] - 1)) < 1e-4
+    assert np.max(np.abs(grd[:, 3] / ord_[:, 3] - 1)) < 1e-4
+    cm = prob.comps
+    for gs, os_ in zip(gsrc, osrc):
+        scale = np.max(np.abs(os_), axis=(1, 2, 3), keepdims=True)
+        # a saturated species has a round-off source in the FP64 method (its
+        # mass fraction renormalised) and an exact zero in the FP32 form's
+        # mass bookkeeping: species are measured against the mass source
+        m = scale[cm["rhoSrcIndx"]]
+        for k in range(prob.nf):
+            scale[cm["specSrcIndx"] + k] = np.maximum(scale[cm["specSrcIndx"] + k], m)
+        assert np.max(np.abs(gs - os_) / np.maximum(scale, 1e-300)) < 1e-3
+
+
+def test_fp32_form_refuses_wall_films():
+    # the FP32 form has no walls and so no film-height field
+    from pelemp_amd._capi import PelempError
+    from pelemp_amd.spray import SprayParticleContainer
+
+    prob = P.config2(ncell=8)
+    rd = prob.rdata.copy()
+    rd[0, 9 + prob.nf] = 1e-6  # a film height
+    pc = SprayParticleContainer(prob.mech, prob.sdata, prob.comps, prob.geom, prob.n, form=C.FORM_FP32)
+    with pytest.raises(PelempError):
+        pc.set_particles(prob.pos, rd, prob.ids)
+    pc.close()

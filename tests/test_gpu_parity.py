@@ -77,3 +77,34 @@ def test_math_functions_bitwise_host_device():
         d = C.math_eval(fn, x, y)
         h = O.pm_eval(fn, x, y)
         assert np.array_equal(d.view(np.int64), h.view(np.int64)), f"fn {fn} differs host/device"
+
+
+def _with_fuel_vapour(prob, frac=(2e-4, 1e-5)):
+    """The problem's gas with some vapour of each fuel (taken from N2 at
+    constant density), so the deposition species' gas fractions matter: at
+    30 atm, 2e-4 of n-decane is below the n-decane surface saturation
+    fraction of a 300 K droplet and above the n-dodecane one."""
+    import copy
+
+    p = copy.deepcopy(prob)
+    cm, sp = p.comps, p.species
+    rho = p.state[cm["rhoIndx"]]
+    for k, f in enumerate(p.sdata["indx"][:p.nf]):
+        p.state[cm["specIndx"] + f] += frac[k] * rho
+        p.state[cm["specIndx"] + sp.index("N2")] -= frac[k] * rho
+    return p
+
+
+def test_deposition_species_swapped_fp64_parity():
+    # dep_fuel_species (SpraySetup.cpp:103-125): the decane vapour goes to
+    # the NC12H26 species and the dodecane vapour to NC10H22; the FP64 form
+    # still reproduces the oracle bit for bit
+    prob = _with_fuel_vapour(P.config2(ncell=16))
+    prob.sdata = dict(prob.sdata, dep_indx=[prob.sdata["indx"][1], prob.sdata["indx"][0]])
+    gpos, grd, opos, ord_, gsrc, osrc = _compare(prob, 3)
+    assert np.array_equal(grd, ord_)
+    assert all(np.array_equal(a, b) for a, b in zip(gsrc, osrc))
+    # and the run differs from the one that deposits into the fuels' own species
+    own = _with_fuel_vapour(P.config2(ncell=16))
+    _, _, rd_own, _, src_own = run_gpu(own, 3)
+    assert not all(np.array_equal(a, b) for a, b in zip(gsrc, src_own))

@@ -347,3 +347,17 @@ def test_surrogate_form_evaporated_mass_vs_fp64_oracle():
     assert evap_o > 0
     assert rel < 5e-3
     assert frac > 0.9
+
+
+def test_surrogate_form_refuses_other_deposition_species():
+    # the network sees the fuel's own gas-phase mass fraction, so the
+    # surrogate form needs each fuel's vapour in its own species
+    from pelemp_amd import _capi as C
+    from pelemp_amd import problem as P
+    from pelemp_amd._capi import PelempError
+    from pelemp_amd.spray import SprayParticleContainer
+
+    prob = P.config2(ncell=8)
+    sd = dict(prob.sdata, dep_indx=[prob.sdata["indx"][1], prob.sdata["indx"][0]])
+    with pytest.raises(PelempError):
+        SprayParticleContainer(prob.mech, sd, prob.comps, prob.geom, prob.n, form=C.FORM_SURROGATE)
