@@ -56,13 +56,22 @@ PM_HD constexpr int fuel_species(int spf) {
   return NS - NF + spf;
 }
 
-// v[k] of a per-species register array at a run-time species index k (a
-// chain of selects, so that the array stays in registers).
+// v[k] of a per-species register array at a run-time species index k, as a
+// chain of selects so that the array stays in registers.  Each element goes
+// through an empty asm statement: without it the compiler folds the selects
+// of array elements into one load at a computed address, which moves the
+// array (and everything it shares a stack slot with) to scratch memory.
 template <int NS, class R>
 PM_HDI R species_at(const R* v, int k) {
   R r = v[0];
 #pragma unroll
-  for (int n = 1; n < NS; ++n) r = (k == n) ? v[n] : r;
+  for (int n = 1; n < NS; ++n) {
+    R vn = v[n];
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(vn));
+#endif
+    r = (k == n) ? vn : r;
+  }
   return r;
 }
 
