@@ -211,7 +211,7 @@ PM_HDI void film_source(double flow_dt, const GasAt<double, M::NS>& g, const Fue
     const double midot = (newY * film_mass - oldY * start_mass) / dt;
     s.Ydot[spf] = midot;
     d.Y[spf] = newY;
-    s.eng += midot * dep_value<NS, NF>(fd, h_film, spf);
+    s.eng += midot * dep_value<NS, NF, true>(fd, h_film, spf);
   }
   d.dia = film_dia;
   d.film = film_height;

@@ -137,9 +137,12 @@ inline FuelConst<R, NF> make_fuel_const(const pelemp_spray_data& f, const double
 }
 
 // Of the deposition species of fuel spf (dep_indx, Drag.H:76,112,426): the
-// gas mass fraction and a per-species value such as the enthalpy.
-template <int NS, int NF, class R>
+// gas mass fraction and a per-species value such as the enthalpy.  DEP:
+// the deposition species may differ from the fuel's own (checked at run
+// time); kernels compiled without it assume dep_indx == indx.
+template <int NS, int NF, bool DEP, class R>
 PM_HDI R dep_value(const FuelConst<R, NF>& fd, const R* v, int spf) {
+  if constexpr (!DEP) return v[fuel_species<NS, NF>(spf)];
   return fd.dep_own ? v[fuel_species<NS, NF>(spf)] : species_at<NS>(v, fd.dep_indx[spf]);
 }
 
@@ -308,7 +311,7 @@ PM_HDI R calc_heat_coeff(R ratio, R B_M, R B_eps, R C_eps, R Nu_0, R lBM1) {
 }
 
 // calcVaporState (Drag.H:47-135)
-template <class M, int NF, class R>
+template <class M, int NF, class R, bool DEP = true>
 PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g, R rule, R T_in,
                             R C_eps, R mw_part, const R* Y_l, const R* h_part, const R* cp_n,
                             const R* cBoilT, R* Y_skin, R* X_vapor, R* L_fuel, R& B_M, R& sumXVap,
@@ -330,7 +333,7 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
     } else {
       pres_sat = psat<R, NF>(fd, T_part, spf);
     }
-    const R X_fluid = dep_value<NS, NF>(fd, g.Y, spf) * g.mw_mix / mw_fuel;
+    const R X_fluid = dep_value<NS, NF, DEP>(fd, g.Y, spf) * g.mw_mix / mw_fuel;
     if (pres_sat > X_fluid * g.p) {
       const R Xl = Y_l[spf] * mw_part / mw_fuel;
       const R Xv = Xl * pres_sat / g.p;
@@ -350,7 +353,7 @@ PM_HDI void calc_vapor_state(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& 
       const R mw_fuel = fd.mw_fuel[spf];
       const R Yfv = X_vapor[spf] * mw_fuel / totalmwx;
       sumYVap += Yfv;
-      const R gasY = dep_value<NS, NF>(fd, g.Y, spf);
+      const R gasY = dep_value<NS, NF, DEP>(fd, g.Y, spf);
       sumYfFluid += gasY;
       const R Ysk = Yfv + rule * (gasY - Yfv);
       Y_skin[fspec] = Ysk;
@@ -395,7 +398,7 @@ struct SubRates {
   R B_M;      // Spalding mass transfer number
 };
 
-template <class M, int NF, class R>
+template <class M, int NF, class R, bool DEP = true>
 PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, R T_part, R dia_part,
                           R pmass, R rho_part, const R* Y_part, const R* vel_part, R y_tab,
                           const R* cBoilT, SubRates<M, NF, R>& o) {
@@ -437,7 +440,7 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
   o.ShRhoD = R(0);
   o.rhoD = R(0);
   if (mass_trans) {
-    calc_vapor_state<M, NF, R>(fd, g, rule, T_part, C_eps, mw_part, Y_part, o.h_part, cp_n, cBoilT, Y_skin,
+    calc_vapor_state<M, NF, R, DEP>(fd, g, rule, T_part, C_eps, mw_part, Y_part, o.h_part, cp_n, cBoilT, Y_skin,
                                X_vapor, L_fuel, B_M, sumXVap, cp_skin, mw_skin, &o.sumYVap);
   } else {
 #pragma unroll
@@ -560,7 +563,7 @@ PM_HDI void substep_rates(const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd, 
 //
 // pre (surrogate form): rates of the first substep predicted by the learned
 // surrogate; when given they replace substep_rates for that substep.
-template <class M, int NF, class R, bool MASSB = false>
+template <class M, int NF, class R, bool MASSB = false, bool DEP = true>
 PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF>& fd,
                      Drop<R, NF>& d, const R* cBoilT, DropSrc<R, NF>& s,
                      const SubRates<M, NF, R>* pre = nullptr) {
@@ -609,7 +612,7 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
     if (pre != nullptr && isub == 1) {
       o = *pre;
     } else {
-      substep_rates<M, NF, R>(g, fd, T_part, dia_part, pmass, rho_part, Y_part, vel_part, y_tab, cBoilT, o);
+      substep_rates<M, NF, R, DEP>(g, fd, T_part, dia_part, pmass, rho_part, Y_part, vel_part, y_tab, cBoilT, o);
     }
     Reyn = o.Reyn;
     const R Tboil = o.Tboil;
@@ -715,7 +718,7 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
     for (int spf = 0; spf < NF; ++spf) {
       const R midot = d.dmk[spf] * inv_dt;
       s.Ydot[spf] = num_ppp * midot;
-      s.eng += num_ppp * midot * dep_value<NS, NF>(fd, o.h_part, spf);
+      s.eng += num_ppp * midot * dep_value<NS, NF, DEP>(fd, o.h_part, spf);
       d.Y[spf] = Y_part[spf];
     }
 #pragma unroll
@@ -741,7 +744,7 @@ PM_HDI R spray_source(R flow_dt, const GasAt<R, M::NS>& g, const FuelConst<R, NF
     const R newY = Y_part[spf];
     const R midot = (newY * pmass - oldY * startmass) / (fd.dtmod * flow_dt);
     s.Ydot[spf] = num_ppp * midot;
-    s.eng += num_ppp * midot * dep_value<NS, NF>(fd, o.h_part, spf);
+    s.eng += num_ppp * midot * dep_value<NS, NF, DEP>(fd, o.h_part, spf);
     d.Y[spf] = Y_part[spf];
   }
 #pragma unroll

@@ -554,7 +554,7 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
 
 // Droplet physics, source contribution, drift, store and re-sort
 // classification of droplet s (storage position s of the output buffer).
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS>
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool DEP = true>
 __device__ __attribute__((always_inline)) inline void drop_update(const UpdateArgs<R, NF>& a, int tile, int t,
                                                                   int32_t s, DropLocal<M, NF, R>& L,
                                                                   const SubRates<M, NF, R>* pre, R* contrib,
@@ -575,7 +575,7 @@ __device__ __attribute__((always_inline)) inline void drop_update(const UpdateAr
 #pragma unroll
     for (int spf = 0; spf < NF; ++spf) sr.Ydot[spf] = R(0);
   } else {
-    spray_source<M, NF, R, !POS64>(a.sub_dt, L.g, a.fc, d, L.cBoilT, sr, pre);
+    spray_source<M, NF, R, !POS64, DEP>(a.sub_dt, L.g, a.fc, d, L.cBoilT, sr, pre);
   }
   // the droplet's source, in the form's precision; the reduction multiplies
   // each by cur_coef in double (SprayParticles.cpp:419-441)
@@ -1096,7 +1096,7 @@ __device__ __attribute__((always_inline)) inline bool surr_stage_drop(const Upda
 // partials of segment slot slot + seg, which split_combine_kernel adds to
 // the source in segment order.
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool SURR, bool BR = false, bool EB = false,
-          bool TAB = false>
+          bool TAB = false, bool DEP = true>
 __device__ __attribute__((always_inline)) inline void process_tile(const UpdateArgs<R, NF>& a, int tile, int seg,
                                                                    int slot, R* gas, unsigned char* gvalid,
                                                                    int32_t* cs, ContribT<R, BR>* contrib,
@@ -1200,7 +1200,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     } else if constexpr (!SURR) {
       if (s < d1) {
         drop_gather<M, NF, R, POS64, AT_BOUNDS>(a, gas, gvalid, cs, otx, oty, otz, s, Lp);
-        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, Lp, nullptr, contrib, stayed);
+        drop_update<M, NF, R, POS64, AT_BOUNDS, DEP>(a, tile, t, s, Lp, nullptr, contrib, stayed);
       }
     } else {
       const bool active = s < d1;
@@ -1210,7 +1210,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
       if (active) {
         n_surr += use ? 1u : 0u;
         n_fb += use ? 0u : 1u;
-        drop_update<M, NF, R, POS64, AT_BOUNDS>(a, tile, t, s, Lp, use ? &o : nullptr, contrib, stayed);
+        drop_update<M, NF, R, POS64, AT_BOUNDS, false>(a, tile, t, s, Lp, use ? &o : nullptr, contrib, stayed);
       }
     }
     if constexpr (kAhead) {
@@ -1293,8 +1293,12 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 }
 
 // TAB: the branch kernel for the TAB breakup model (do_breakup == 1), with
-// the workgroup TAB stage
-template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false, bool TAB = false>
+// the workgroup TAB stage.  DEP: the deposition species are read at run time
+// (dep_fuel_species); the plain kernels are also compiled without it for
+// the usual case of every fuel depositing into its own species, which
+// leaves the run-time selects out.
+template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false, bool TAB = false,
+          bool DEP = true>
 __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NG = gas_stride<R, M::NS>();  // staged gas fields: rho, Y[NS], vel[3], T (padded)
@@ -1323,7 +1327,7 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB>::kMinBl
     tile = it.x;
     seg = it.y;
   }
-  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB, TAB>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs,
+  process_tile<M, NF, R, POS64, AT_BOUNDS, false, BR, EB, TAB, DEP>(a, tile, seg, a.sp.tile_slot[tile], gas, gvalid, cs,
                                                           contrib, stayed, nullptr, n_surr, n_fb, redir, tabrec,
                                                           tabcnt);
 }

@@ -173,6 +173,9 @@ int launch_update_t(Ctx& c, const PassParams& p, hipStream_t s) {
       set_last_error("walls are supported by the FP64 form only");
       return PELEMP_E_BAD_CONFIG;
     }
+  } else if (a.fc.dep_own) {
+    hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, false, false, false, false, false>), grid, block, 0, s,
+                       a);
   } else {
     hipLaunchKernelGGL((spray_update_kernel<M, NF, R, POS64, false>), grid, block, 0, s, a);
   }
