@@ -164,24 +164,37 @@ __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __res
   const int32_t n = *n_ptr;
   // the split lists of the next pass are rebuilt after this kernel
   if (blockIdx.x == 0 && threadIdx.x < 3) sp.counters[threadIdx.x] = 0;
-  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const uint32_t k = key[s];
-    if (k == kDeadKey) continue;
-    const int32_t v = arrival[s];
-    const int cls = v >> 30;
-    const int32_t r = v & kArrRankMask;
-    int32_t p = cstart[k];
-    if (cls == 0) {
-      // a stayer: its rank among the stayers of its segment of a heavy tile
-      const int32_t seg = r >> kSegShift;
-      p += mov_lo[k] + (r & kSegRankMask);
-      if (seg > 0) p += sp.seg_stay[(sp.tile_slot[k / 64] + seg) * 64 + (k % 64)];
-    } else if (cls == 1) {
-      p += r;
-    } else {
-      p += mov_lo[k] + stay_cnt[k] + r;
+  // kU entries per thread and round, their loads issued together: each entry
+  // is a chain of dependent loads (key and arrival, then the cell's offsets)
+  constexpr int kU = 4;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t s0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s0 < n; s0 += kU * stride) {
+    uint32_t k[kU];
+    int32_t v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t s = s0 + u * stride;
+      k[u] = (s < n) ? key[s] : kDeadKey;
+      v[u] = (s < n) ? arrival[s] : 0;
     }
-    perm[p] = s;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (k[u] == kDeadKey) continue;
+      const int cls = v[u] >> 30;
+      const int32_t r = v[u] & kArrRankMask;
+      int32_t p = cstart[k[u]];
+      if (cls == 0) {
+        // a stayer: its rank among the stayers of its segment of a heavy tile
+        const int32_t seg = r >> kSegShift;
+        p += mov_lo[k[u]] + (r & kSegRankMask);
+        if (seg > 0) p += sp.seg_stay[(sp.tile_slot[k[u] / 64] + seg) * 64 + (k[u] % 64)];
+      } else if (cls == 1) {
+        p += r;
+      } else {
+        p += mov_lo[k[u]] + stay_cnt[k[u]] + r;
+      }
+      perm[p] = static_cast<int32_t>(s0 + u * stride);
+    }
   }
 }
 
