@@ -93,38 +93,90 @@ struct GasProps {
     for (int n = 0; n < NS; ++n) mw[n] = static_cast<R>(M::W[n]);
   }
 
+  // NASA-7 coefficient a_q of species n for the range of T (lo: T < Tmid).
+  static PM_HD constexpr double coef(bool lo, int n, int q) {
+    return lo ? M::nasa_lo[7 * n + q] : M::nasa_hi[7 * n + q];
+  }
+  static PM_HD constexpr double tmid_min() {
+    double t = M::Tmid[0];
+    for (int n = 1; n < NS; ++n) t = M::Tmid[n] < t ? M::Tmid[n] : t;
+    return t;
+  }
+  static PM_HD constexpr double tmid_max() {
+    double t = M::Tmid[0];
+    for (int n = 1; n < NS; ++n) t = M::Tmid[n] > t ? M::Tmid[n] : t;
+    return t;
+  }
+
+  // cp_n of species n with the coefficients of range lo (compile-time) or of
+  // the range T falls in (a select per coefficient)
+  template <class R, int RANGE>  // RANGE: 1 lower, 0 upper, -1 by T
+  static PM_HDI R cp_one(R T, int n) {
+    const bool lo = RANGE == 1 ? true : (RANGE == 0 ? false : T < static_cast<R>(M::Tmid[n]));
+    const R a0 = static_cast<R>(coef(lo, n, 0));
+    const R a1 = static_cast<R>(coef(lo, n, 1));
+    const R a2 = static_cast<R>(coef(lo, n, 2));
+    const R a3 = static_cast<R>(coef(lo, n, 3));
+    const R a4 = static_cast<R>(coef(lo, n, 4));
+    const R rw = static_cast<R>(kRU / M::W[n]);
+    return rw * (a0 + T * (a1 + T * (a2 + T * (a3 + T * a4))));
+  }
+  template <class R, int RANGE>
+  static PM_HDI R h_one(R T, int n) {
+    const bool lo = RANGE == 1 ? true : (RANGE == 0 ? false : T < static_cast<R>(M::Tmid[n]));
+    // coefficient selects between two compile-time constants
+    const R b0 = static_cast<R>(coef(lo, n, 0));
+    const R b1 = static_cast<R>(coef(lo, n, 1) * 0.5);
+    const R b2 = static_cast<R>(coef(lo, n, 2) * (1.0 / 3.0));
+    const R b3 = static_cast<R>(coef(lo, n, 3) * 0.25);
+    const R b4 = static_cast<R>(coef(lo, n, 4) * 0.2);
+    const R b5 = static_cast<R>(coef(lo, n, 5));
+    const R rw = static_cast<R>(kRU / M::W[n]);
+    return rw * (T * (b0 + T * (b1 + T * (b2 + T * (b3 + T * b4)))) + b5);
+  }
+
+  // Range of T for the whole wavefront: 1 when every lane is below every
+  // species' Tmid, 0 when every lane is at or above it, -1 otherwise.  The
+  // per-lane values are the same either way; a uniform range only drops the
+  // per-coefficient selects.
+  template <class R>
+  static PM_HDI int wave_range(R T) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__all(T < static_cast<R>(tmid_min()))) return 1;
+    if (__all(!(T < static_cast<R>(tmid_max())))) return 0;
+#endif
+    return -1;
+  }
+
   // Species heat capacities at constant pressure [J/kg/K].
   template <class R>
   static PM_HDI void T2Cpi(R T, R* cp) {
+    const int range = wave_range(T);
+    if (range == 1) {
 #pragma unroll
-    for (int n = 0; n < NS; ++n) {
-      const bool lo = T < static_cast<R>(M::Tmid[n]);
-      const R a0 = static_cast<R>(lo ? M::nasa_lo[7 * n + 0] : M::nasa_hi[7 * n + 0]);
-      const R a1 = static_cast<R>(lo ? M::nasa_lo[7 * n + 1] : M::nasa_hi[7 * n + 1]);
-      const R a2 = static_cast<R>(lo ? M::nasa_lo[7 * n + 2] : M::nasa_hi[7 * n + 2]);
-      const R a3 = static_cast<R>(lo ? M::nasa_lo[7 * n + 3] : M::nasa_hi[7 * n + 3]);
-      const R a4 = static_cast<R>(lo ? M::nasa_lo[7 * n + 4] : M::nasa_hi[7 * n + 4]);
-      const R rw = static_cast<R>(kRU / M::W[n]);
-      cp[n] = rw * (a0 + T * (a1 + T * (a2 + T * (a3 + T * a4))));
+      for (int n = 0; n < NS; ++n) cp[n] = cp_one<R, 1>(T, n);
+    } else if (range == 0) {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) cp[n] = cp_one<R, 0>(T, n);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) cp[n] = cp_one<R, -1>(T, n);
     }
   }
 
   // Species specific enthalpies [J/kg] (incl. heat of formation).
   template <class R>
   static PM_HDI void T2Hi(R T, R* h) {
+    const int range = wave_range(T);
+    if (range == 1) {
 #pragma unroll
-    for (int n = 0; n < NS; ++n) {
-      const bool lo = T < static_cast<R>(M::Tmid[n]);
-      // coefficient selects between two compile-time constants
-      const R b0 = static_cast<R>(lo ? M::nasa_lo[7 * n + 0] : M::nasa_hi[7 * n + 0]);
-      const R b1 = static_cast<R>(lo ? M::nasa_lo[7 * n + 1] * 0.5 : M::nasa_hi[7 * n + 1] * 0.5);
-      const R b2 = static_cast<R>(lo ? M::nasa_lo[7 * n + 2] * (1.0 / 3.0)
-                                     : M::nasa_hi[7 * n + 2] * (1.0 / 3.0));
-      const R b3 = static_cast<R>(lo ? M::nasa_lo[7 * n + 3] * 0.25 : M::nasa_hi[7 * n + 3] * 0.25);
-      const R b4 = static_cast<R>(lo ? M::nasa_lo[7 * n + 4] * 0.2 : M::nasa_hi[7 * n + 4] * 0.2);
-      const R b5 = static_cast<R>(lo ? M::nasa_lo[7 * n + 5] : M::nasa_hi[7 * n + 5]);
-      const R rw = static_cast<R>(kRU / M::W[n]);
-      h[n] = rw * (T * (b0 + T * (b1 + T * (b2 + T * (b3 + T * b4)))) + b5);
+      for (int n = 0; n < NS; ++n) h[n] = h_one<R, 1>(T, n);
+    } else if (range == 0) {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) h[n] = h_one<R, 0>(T, n);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) h[n] = h_one<R, -1>(T, n);
     }
   }
 
@@ -175,9 +227,13 @@ struct GasProps {
         const R f = static_cast<R>(M::fitmu[4 * n]) + static_cast<R>(M::fitmu[4 * n + 1]) * lt +
                     static_cast<R>(M::fitmu[4 * n + 2]) * lt2 +
                     static_cast<R>(M::fitmu[4 * n + 3]) * lt3;
-        const R m = rm::exp(f);
-        const R m3 = m * m * m;
-        s += X[n] * (m3 * m3);
+        if constexpr (sizeof(R) == 4) {
+          s += X[n] * rm::exp(R(6) * f);  // mu_k^6 with one exponential
+        } else {
+          const R m = rm::exp(f);
+          const R m3 = m * m * m;
+          s += X[n] * (m3 * m3);
+        }
       }
       if constexpr (sizeof(R) == 4) {
         mu = rm::exp(rm::log(s) * R(1. / 6.));  // s^(1/6)
@@ -192,8 +248,12 @@ struct GasProps {
         const R f = static_cast<R>(M::fitlam[4 * n]) + static_cast<R>(M::fitlam[4 * n + 1]) * lt +
                     static_cast<R>(M::fitlam[4 * n + 2]) * lt2 +
                     static_cast<R>(M::fitlam[4 * n + 3]) * lt3;
-        const R l = rm::exp(f);
-        s += X[n] * rm::sqrt(rm::sqrt(l));
+        if constexpr (sizeof(R) == 4) {
+          s += X[n] * rm::exp(R(0.25) * f);  // lambda_k^(1/4) with one exponential
+        } else {
+          const R l = rm::exp(f);
+          s += X[n] * rm::sqrt(rm::sqrt(l));
+        }
       }
       const R s2 = s * s;
       lam = s2 * s2;

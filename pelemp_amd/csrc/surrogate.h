@@ -77,20 +77,7 @@ PM_HDI void vapor_lite(const FuelConst<R, NF>& fd, const GasAt<R, M::NS>& g, R T
   for (int spf = 0; spf < NF; ++spf) {
     const int fspec = fuel_species<NS, NF>(spf);
     // h_k(T_d) of the fuel species only (the plain path evaluates all NS)
-    R hk;
-    {
-      const bool lo = T_in < static_cast<R>(M::Tmid[fspec]);
-      // coefficients select between compile-time constants (as in T2Hi)
-      const R b0 = static_cast<R>(lo ? M::nasa_lo[7 * fspec + 0] : M::nasa_hi[7 * fspec + 0]);
-      const R b1 = static_cast<R>(lo ? M::nasa_lo[7 * fspec + 1] * 0.5 : M::nasa_hi[7 * fspec + 1] * 0.5);
-      const R b2 = static_cast<R>(lo ? M::nasa_lo[7 * fspec + 2] * (1.0 / 3.0)
-                                     : M::nasa_hi[7 * fspec + 2] * (1.0 / 3.0));
-      const R b3 = static_cast<R>(lo ? M::nasa_lo[7 * fspec + 3] * 0.25 : M::nasa_hi[7 * fspec + 3] * 0.25);
-      const R b4 = static_cast<R>(lo ? M::nasa_lo[7 * fspec + 4] * 0.2 : M::nasa_hi[7 * fspec + 4] * 0.2);
-      const R b5 = static_cast<R>(lo ? M::nasa_lo[7 * fspec + 5] : M::nasa_hi[7 * fspec + 5]);
-      const R rw = static_cast<R>(kRU / M::W[fspec]);
-      hk = rw * (T_in * (b0 + T_in * (b1 + T_in * (b2 + T_in * (b3 + T_in * b4)))) + b5);
-    }
+    const R hk = GasProps<M>::template h_one<R, -1>(T_in, fspec);
     v.h_fuel[spf] = hk;
     const R mw_fuel = fd.mw_fuel[spf];
     const R T_part = rmin(T_in, R(0.99) * cBoilT[spf]);
