@@ -18,7 +18,7 @@ kernel time from HIP events on the launch stream.  cpu_baseline times the
 CPU oracle (the FP64 restatement of the reference method) on a bounded
 sample of the same workload on this host's cores.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--forms fp64,fp32]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--forms fp64,fp32]
 """
 from __future__ import annotations
 
@@ -736,7 +736,10 @@ def run_rows(prob, stream, reps: int = 5):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    # default: config 2's own run length (50 steps of 5e-7 s); the timed
+    # window then covers the droplets' heat-up and early evaporation, over
+    # which the per-step cost of the forms changes
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--forms", default="fp64,fp32,surrogate")
     ap.add_argument("--acc-steps", type=int, default=10,
