@@ -132,6 +132,7 @@ typedef struct pelemp_geom {
 #define PELEMP_MECH_DODECANE_AIR 0        /* O2 N2 NC12H26 */
 #define PELEMP_MECH_DECANE_DODECANE_AIR 1 /* O2 N2 NC10H22 NC12H26 */
 #define PELEMP_MECH_ALKANE6_AIR 2         /* O2 N2 NC8..NC16 (6 fuels) */
+#define PELEMP_MECH_DECANE_AIR 3          /* O2 N2 NC10H22 (the decks' decane_3sp species) */
 
 /* Arithmetic form of the per-droplet update. */
 #define PELEMP_FORM_FP64 0      /* reference method, FP64 end to end */

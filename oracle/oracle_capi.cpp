@@ -142,6 +142,10 @@ int oracle_update(int mech, int nf, const pelemp_spray_data* sd, const pelemp_sp
     return run_update<pelemp::Mech_alkane6_air, 6>(sd, comps, g, n, pos, rdata, id, state, source,
                                                    flow_dt, isVirt, isGhost, do_move, spray_cfl,
                                                    nthreads, bp, n_out);
+  if (mech == PELEMP_MECH_DECANE_AIR && nf == 1)
+    return run_update<pelemp::Mech_decane_air, 1>(sd, comps, g, n, pos, rdata, id, state, source,
+                                                  flow_dt, isVirt, isGhost, do_move, spray_cfl,
+                                                  nthreads, bp, n_out);
   return PELEMP_E_ARG;
 }
 
@@ -240,6 +244,7 @@ int oracle_gas_props(int mech, double T, double rho, const double* Y, double* cp
     case PELEMP_MECH_DODECANE_AIR: return run(pelemp::Mech_dodecane_air{});
     case PELEMP_MECH_DECANE_DODECANE_AIR: return run(pelemp::Mech_decane_dodecane_air{});
     case PELEMP_MECH_ALKANE6_AIR: return run(pelemp::Mech_alkane6_air{});
+    case PELEMP_MECH_DECANE_AIR: return run(pelemp::Mech_decane_air{});
     default: return PELEMP_E_ARG;
   }
 }
@@ -251,6 +256,7 @@ double oracle_ey2t(int mech, double e, const double* Y, double T0) {
     case PELEMP_MECH_DODECANE_AIR: return pelemp::GasProps<pelemp::Mech_dodecane_air>::EY2T(e, Y, T0);
     case PELEMP_MECH_DECANE_DODECANE_AIR: return pelemp::GasProps<pelemp::Mech_decane_dodecane_air>::EY2T(e, Y, T0);
     case PELEMP_MECH_ALKANE6_AIR: return pelemp::GasProps<pelemp::Mech_alkane6_air>::EY2T(e, Y, T0);
+    case PELEMP_MECH_DECANE_AIR: return pelemp::GasProps<pelemp::Mech_decane_air>::EY2T(e, Y, T0);
     default: return -1.;
   }
 }
@@ -333,6 +339,8 @@ double oracle_spray_source_one(int mech, int nf, const pelemp_spray_data* sd, do
     return run(pelemp::Mech_decane_dodecane_air{}, std::integral_constant<int, 1>{});
   if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
     return run(pelemp::Mech_alkane6_air{}, std::integral_constant<int, 6>{});
+  if (mech == PELEMP_MECH_DECANE_AIR && nf == 1)
+    return run(pelemp::Mech_decane_air{}, std::integral_constant<int, 1>{});
   return -1.0;
 }
 
@@ -376,6 +384,8 @@ int oracle_film_source_one(int mech, int nf, const pelemp_spray_data* sd, double
     return run(pelemp::Mech_decane_dodecane_air{}, std::integral_constant<int, 2>{});
   if (mech == PELEMP_MECH_ALKANE6_AIR && nf == 6)
     return run(pelemp::Mech_alkane6_air{}, std::integral_constant<int, 6>{});
+  if (mech == PELEMP_MECH_DECANE_AIR && nf == 1)
+    return run(pelemp::Mech_decane_air{}, std::integral_constant<int, 1>{});
   return PELEMP_E_ARG;
 }
 

@@ -47,6 +47,7 @@ DIST_TYPES = {"Uniform": DIST_UNIFORM, "Normal": DIST_NORMAL, "LogNormal": DIST_
 MECH_DODECANE_AIR = 0
 MECH_DECANE_DODECANE_AIR = 1
 MECH_ALKANE6_AIR = 2
+MECH_DECANE_AIR = 3
 
 FORM_FP64 = 0
 FORM_FP32 = 1

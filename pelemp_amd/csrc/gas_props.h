@@ -144,6 +144,8 @@ struct GasProps {
 #if defined(__HIP_DEVICE_COMPILE__)
     if (__all(T < static_cast<R>(tmid_min()))) return 1;
     if (__all(!(T < static_cast<R>(tmid_max())))) return 0;
+#else
+    (void)T;
 #endif
     return -1;
   }

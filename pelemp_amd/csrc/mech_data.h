@@ -168,4 +168,40 @@ struct Mech_alkane6_air {
   };
 };
 
+struct Mech_decane_air {
+  static constexpr int NS = 3;
+  // species: O2, N2, NC10H22
+  static constexpr double W[3] = {
+      0x1.0620ab7132724p-5, 0x1.cafb3b7521145p-6, 0x1.2366d7a56de34p-3,
+  };
+  static constexpr double Tmid[3] = {
+      0x1.f400000000000p+9, 0x1.f400000000000p+9, 0x1.f400000000000p+9,
+  };
+  static constexpr double nasa_lo[21] = {
+      0x1.e42787ae5fa45p+1, -0x1.88c9b66c8c0dfp-9, 0x1.4a6bae1fdae75p-17, -0x1.4ca5927ada53cp-27, 0x1.c883a18b02bcdp-39, -0x1.09fc63497b741p+10, 0x1.d42eb7e3dc88dp+1,
+      0x1.a63b0c4588a05p+1, 0x1.712969da04050p-10, -0x1.09f7a2b098bd4p-18, 0x1.83ae94da0fe59p-28, -0x1.581526048e0d6p-39, -0x1.fe732fec56d5dp+9, 0x1.f9a5ca29845ddp+1,
+      -0x1.0eaf2aaf1c51ep+0, 0x1.da40affa5ba8ap-4, -0x1.08ba1228c5064p-14, 0x1.a99e0e79bf805p-27, 0x1.263e34fed170bp-41, -0x1.0cd583c9995f6p+15, 0x0.0p+0,
+  };
+  static constexpr double nasa_hi[21] = {
+      0x1.a42a332f575b8p+1, 0x1.84c851ecf574dp-10, -0x1.96ee255022a3dp-21, 0x1.cca1706e20348p-33, -0x1.8667712a69c18p-46, -0x1.101d4b48d3ae7p+10, 0x1.5d01bdd004baap+2,
+      0x1.769c23b7952d2p+1, 0x1.86106ec5d7fc6p-10, -0x1.3132bf1b89c39p-21, 0x1.bc128a9b8a724p-34, -0x1.e6a15668d1fd9p-48, -0x1.cd661b089a027p+9, 0x1.7ec0f88333fc8p+2,
+      -0x1.0eaf2aaf1c51ep+0, 0x1.da40affa5ba8ap-4, -0x1.08ba1228c5064p-14, 0x1.a99e0e79bf805p-27, 0x1.263e34fed170bp-41, -0x1.0cd583c9995f6p+15, 0x0.0p+0,
+  };
+  static constexpr double fitmu[12] = {
+      -0x1.51d857a6480d2p+4, 0x1.b4e92e840fb95p+1, -0x1.8566b3fe3c6fcp-2, 0x1.1d276cd09057bp-6,
+      -0x1.489f8aed55cb2p+4, 0x1.90f18d636097fp+1, -0x1.60179d81699c4p-2, 0x1.036a58b6f18f8p-6,
+      -0x1.442f9ca7437ebp+4, 0x1.97bcac61b1c9cp+0, -0x1.037e8f99be7a4p-7, -0x1.21611dee5e6d7p-8,
+  };
+  static constexpr double fitlam[12] = {
+      -0x1.5593478cc452ep+3, 0x1.c0f28a72b5ecep+0, -0x1.df931a09c188dp-4, 0x1.208d41685923cp-8,
+      -0x1.dfbb4be5dcf2fp+1, -0x1.323b80789f944p+0, 0x1.30a337e08db2fp-2, -0x1.eac46c6a420c9p-7,
+      -0x1.178e99150a736p+6, 0x1.a26407e330b34p+4, -0x1.c5ac8a051c681p+1, 0x1.58d6d5f6dc020p-3,
+  };
+  static constexpr double fitdbin[12] = {
+      -0x1.90ca45323fb91p+4, 0x1.cf57ce18e36bdp+1, -0x1.0d1f9243e1ba2p-2, 0x1.80153bce76c01p-7,
+      -0x1.f320c6e11f8f4p+4, 0x1.57d2f1b695bd7p+2, -0x1.e52ae3efc8788p-2, 0x1.4b039a416d2fap-6,
+      -0x1.ef57dd2fb1c33p+4, 0x1.5414866252a6fp+2, -0x1.dff77442f076fp-2, 0x1.4915e218554c0p-6,
+  };
+};
+
 }  // namespace pelemp

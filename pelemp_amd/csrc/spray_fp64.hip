@@ -14,6 +14,7 @@ int launch_pelec_convert(Ctx& c, const pelemp_fab_view& st, hipStream_t s) {
     case PELEMP_MECH_DODECANE_AIR: return launch_pelec_convert_t<Mech_dodecane_air>(c, st, s);
     case PELEMP_MECH_DECANE_DODECANE_AIR: return launch_pelec_convert_t<Mech_decane_dodecane_air>(c, st, s);
     case PELEMP_MECH_ALKANE6_AIR: return launch_pelec_convert_t<Mech_alkane6_air>(c, st, s);
+    case PELEMP_MECH_DECANE_AIR: return launch_pelec_convert_t<Mech_decane_air>(c, st, s);
     default: set_last_error("unsupported mechanism"); return PELEMP_E_BAD_CONFIG;
   }
 }

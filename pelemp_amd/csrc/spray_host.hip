@@ -453,6 +453,8 @@ static int run_children(Ctx& c, const PassParams& p, hipStream_t s) {
     run(Mech_decane_dodecane_air{}, std::integral_constant<int, 2>{});
   } else if (c.mech == PELEMP_MECH_ALKANE6_AIR && c.nf == 6) {
     run(Mech_alkane6_air{}, std::integral_constant<int, 6>{});
+  } else if (c.mech == PELEMP_MECH_DECANE_AIR && c.nf == 1) {
+    run(Mech_decane_air{}, std::integral_constant<int, 1>{});
   } else {
     set_last_error("unsupported mechanism / fuel count combination");
     return PELEMP_E_BAD_CONFIG;
@@ -578,6 +580,7 @@ int mech_ns(int mech) {
     case PELEMP_MECH_DODECANE_AIR: return Mech_dodecane_air::NS;
     case PELEMP_MECH_DECANE_DODECANE_AIR: return Mech_decane_dodecane_air::NS;
     case PELEMP_MECH_ALKANE6_AIR: return Mech_alkane6_air::NS;
+    case PELEMP_MECH_DECANE_AIR: return Mech_decane_air::NS;
     default: return -1;
   }
 }
@@ -592,6 +595,7 @@ void mech_mw(int mech, std::vector<double>& mw) {
     case PELEMP_MECH_DODECANE_AIR: fill(Mech_dodecane_air{}); break;
     case PELEMP_MECH_DECANE_DODECANE_AIR: fill(Mech_decane_dodecane_air{}); break;
     case PELEMP_MECH_ALKANE6_AIR: fill(Mech_alkane6_air{}); break;
+    case PELEMP_MECH_DECANE_AIR: fill(Mech_decane_air{}); break;
     default: break;
   }
 }
@@ -605,6 +609,7 @@ void mech_h(int mech, double T, double* h) {
     case PELEMP_MECH_DODECANE_AIR: run(Mech_dodecane_air{}); break;
     case PELEMP_MECH_DECANE_DODECANE_AIR: run(Mech_decane_dodecane_air{}); break;
     case PELEMP_MECH_ALKANE6_AIR: run(Mech_alkane6_air{}); break;
+    case PELEMP_MECH_DECANE_AIR: run(Mech_decane_air{}); break;
     default: break;
   }
 }

@@ -232,6 +232,8 @@ int dispatch_update(Ctx& c, const PassParams& p, hipStream_t s) {
     return launch_update_t<Mech_decane_dodecane_air, 2, R, POS64>(c, p, s);
   if (c.mech == PELEMP_MECH_ALKANE6_AIR && c.nf == 6)
     return launch_update_t<Mech_alkane6_air, 6, R, POS64>(c, p, s);
+  if (c.mech == PELEMP_MECH_DECANE_AIR && c.nf == 1)
+    return launch_update_t<Mech_decane_air, 1, R, POS64>(c, p, s);
   set_last_error("unsupported mechanism / fuel count combination");
   return PELEMP_E_BAD_CONFIG;
 }

@@ -165,6 +165,8 @@ MECHS = {
     "dodecane_air": ["O2", "N2", "NC12H26"],
     "decane_dodecane_air": ["O2", "N2", "NC10H22", "NC12H26"],
     "alkane6_air": ["O2", "N2", "NC8H18", "NC10H22", "NC12H26", "NC13H28", "NC14H30", "NC16H34"],
+    # the species of the reference decks' decane_3sp chemistry (n-decane in air)
+    "decane_air": ["O2", "N2", "NC10H22"],
 }
 
 

@@ -31,7 +31,7 @@ PATM = 101325.0
 
 _MECH_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mech", "mech_data.json")
 MECH_NAMES = {C.MECH_DODECANE_AIR: "dodecane_air", C.MECH_DECANE_DODECANE_AIR: "decane_dodecane_air",
-              C.MECH_ALKANE6_AIR: "alkane6_air"}
+              C.MECH_ALKANE6_AIR: "alkane6_air", C.MECH_DECANE_AIR: "decane_air"}
 
 
 def mech_info(mech: int) -> dict:
@@ -346,6 +346,20 @@ def config1(scale: float = 1.0, seed: int = 1234, **kw) -> SprayProblem:
     return build_problem("config1", ncell=(32, 32, 32), p=PATM, T0=1000.0, mech=C.MECH_DODECANE_AIR,
                          fuels=("NC12H26",), n_drops=int(round(1e5 * scale)), d_median=20e-6, sigma_ln=0.4,
                          flow_dt=1e-6, nsteps=100, seed=seed, **kw)
+
+
+def decane_case(ncell: int = 32, drops_per_cell: float = 3.0, p: float = 10.0 * PATM, T0: float = 1500.0,
+                seed: int = 777, **kw) -> SprayProblem:
+    """An n-decane spray in the gas mechanism of the reference's decane decks
+    (decane_3sp: O2, N2, NC10H22; Exec/SprayTests/PeleLMeX/abramzon_test):
+    a periodic ncell^3 box, dx = 1e-4 m, air at the Abramzon case's 10 bar
+    and 1500 K with the config-1 sinusoidal recipe, Poisson(drops_per_cell)
+    droplets per cell (median 20 um, sigma_ln 0.4, T_d = 300 K), 1e-6 s
+    steps."""
+    n = drops_per_cell * ncell ** 3
+    return build_problem("decane", ncell=(ncell, ncell, ncell), p=p, T0=T0, mech=C.MECH_DECANE_AIR,
+                         fuels=("NC10H22",), n_drops=n, poisson=True, d_median=20e-6, sigma_ln=0.4, flow_dt=1e-6,
+                         nsteps=100, seed=seed, **kw)
 
 
 def config4(ncell: int = 128, drops_per_cell: float = 1.6e7 / 128 ** 3, seed: int = 2468, **kw) -> SprayProblem:

@@ -96,3 +96,18 @@ def test_fp32_form_refuses_wall_films():
     with pytest.raises(PelempError):
         pc.set_particles(prob.pos, rd, prob.ids)
     pc.close()
+
+
+def test_fp32_form_decane_mechanism():
+    prob = P.decane_case(ncell=16)
+    nsteps = 4
+    _, gpos, grd, gids, gsrc = run_gpu(prob, nsteps, form=C.FORM_FP32)
+    opos, ord_, oids, osrc = run_oracle(prob, nsteps)
+    gpos, grd, gids = by_id(gpos, grd, gids)
+    opos, ord_, oids = by_id(opos, ord_, oids)
+    assert np.array_equal(gids, oids)
+    assert np.max(np.abs(grd[:, 4] / ord_[:, 4] - 1)) < 1e-4
+    assert np.max(np.abs(grd[:, 3] / ord_[:, 3] - 1)) < 1e-4
+    for gs, os_ in zip(gsrc, osrc):
+        scale = np.max(np.abs(os_), axis=(1, 2, 3), keepdims=True)
+        assert np.max(np.abs(gs - os_) / np.maximum(scale, 1e-300)) < 1e-3

@@ -108,3 +108,12 @@ def test_deposition_species_swapped_fp64_parity():
     own = _with_fuel_vapour(P.config2(ncell=16))
     _, _, rd_own, _, src_own = run_gpu(own, 3)
     assert not all(np.array_equal(a, b) for a, b in zip(gsrc, src_own))
+
+
+def test_decane_mechanism_fp64_parity():
+    # the decane_air mechanism (the species of the decks' decane_3sp
+    # chemistry) at the Abramzon case's 10 bar / 1500 K
+    prob = P.decane_case(ncell=16)
+    gpos, grd, opos, ord_, gsrc, osrc = _compare(prob, 4)
+    assert np.array_equal(grd, ord_)
+    assert all(np.array_equal(a, b) for a, b in zip(gsrc, osrc))
