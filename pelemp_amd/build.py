@@ -8,11 +8,15 @@ pelemp_amd/lib/libpelemp_spray.so.  The FP64 units use -ffp-contract=off
 reciprocal multiplication for division and the hardware approximations of
 the elementary functions (its results are held to stated tolerances against
 the FP64 oracle, not to bitwise agreement).  Packed-FP32 instructions are
-disabled for every unit.
+disabled for every unit.  The compiler's per-kernel register, scratch,
+occupancy and LDS figures are written to pelemp_amd/lib/kernel_resources.json
+(read by tests/test_kernel_resources.py).
 """
 from __future__ import annotations
 
+import json
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -39,21 +43,44 @@ UNITS = {
 }
 
 
-def _compile(unit: str, extra: list[str]) -> str:
+RESOURCES = os.path.join(LIBDIR, "kernel_resources.json")
+_REMARK = re.compile(r"remark:\s+(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]|"
+                     r"LDS Size \[bytes/block\]): (\S+)")
+
+
+def _resources(stderr: str) -> dict:
+    """Per-kernel register, scratch, occupancy and LDS figures from hipcc's
+    kernel-resource-usage remarks (device functions included)."""
+    out, cur = {}, None
+    for key, val in _REMARK.findall(stderr):
+        if key == "Function Name":
+            cur = out.setdefault(val, {})
+        elif cur is not None:
+            cur[key.split(" ")[0].lower()] = int(val)
+    return out
+
+
+def _compile(unit: str, extra: list[str]):
     src = os.path.join(CSRC, unit)
     obj = os.path.join(OBJDIR, unit.replace(".hip", ".o"))
-    cmd = [HIPCC, *COMMON, *extra, "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *extra, "-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError(f"hipcc failed for {unit}")
-    return obj
+    return obj, _resources(r.stderr)
 
 
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
+        res = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
+    objs = [o for o, _ in res]
+    resources = {}
+    for _, r in res:
+        resources.update(r)
+    with open(RESOURCES, "w") as f:
+        json.dump(resources, f, indent=1, sort_keys=True)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
