@@ -35,10 +35,17 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", CSRC, "-I", INC, "-Wno-unused-result",
 ]
+# The update kernels' units are compiled without machine-level loop-invariant
+# code motion: hoisting the materialised FP64 constants and uniform values
+# out of the droplet and substep loops kept them live in scalar registers,
+# which then spilled into vector-register lanes (v_writelane/v_readlane,
+# VALU instructions); without it the FP64 update kernel runs 2 % faster,
+# the FP32 one 2 %, with the same results.
+NO_MLICM = ["-mllvm", "-disable-machine-licm"]
 UNITS = {
     "spray_host.hip": ["-ffp-contract=off"],
-    "spray_fp64.hip": ["-ffp-contract=off"],
-    "spray_fp32.hip": ["-ffp-contract=fast", "-freciprocal-math", "-fapprox-func"],
+    "spray_fp64.hip": ["-ffp-contract=off", *NO_MLICM],
+    "spray_fp32.hip": ["-ffp-contract=fast", "-freciprocal-math", "-fapprox-func", *NO_MLICM],
     "spray_surrogate.hip": ["-ffp-contract=off"],
 }
 

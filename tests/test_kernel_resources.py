@@ -40,8 +40,9 @@ def test_every_update_variant_is_reported():
     assert {"Mech_dodecane_air", "Mech_decane_dodecane_air", "Mech_alkane6_air", "Mech_decane_air"} <= mechs
 
 
-# (FP32?, fuels <= 2?) -> (waves per SIMD, scratch bytes per lane) of the
-# plain kernels that config 2, 4 and 5 run (no walls, own-species deposition)
+# (FP32?, fuels <= 2?) -> (least waves per SIMD, most scratch bytes per lane)
+# of the plain kernels that config 2, 4 and 5 run (no walls, own-species
+# deposition)
 BUDGET = {(True, True): (4, 16), (True, False): (2, 32), (False, True): (2, 32), (False, False): (2, 640)}
 
 
@@ -52,7 +53,7 @@ def test_plain_update_kernels_keep_their_registers():
         if br:
             continue
         waves, scratch = BUDGET[(r == "f", nf <= 2)]
-        assert res["occupancy"] == waves, (mech, nf, flags, res)
+        assert res["occupancy"] >= waves, (mech, nf, flags, res)
         if not at_bounds and not dep:
             checked += 1
             assert res.get("scratchsize", 0) <= scratch, (mech, nf, flags, res)
@@ -67,5 +68,5 @@ def test_surrogate_kernel_keeps_its_registers():
     for name, res in _kernels().items():
         if "spray_update_surr_kernel" in name:
             n += 1
-            assert res["occupancy"] == 2 and res.get("scratchsize", 0) <= 64, (name, res)
+            assert res["occupancy"] >= 2 and res.get("scratchsize", 0) <= 64, (name, res)
     assert n == 2
