@@ -264,20 +264,20 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
   return false;
 }
 
-// Occupancy targets (workgroups of 4 waves per CU = waves per SIMD): the
-// FP64 form needs ~265 registers per lane at full freedom; it is compiled
-// for 2 waves per SIMD (256 registers) at the cost of a few spilled values.
-// The FP32 form with up to two liquid species runs at 4 waves per SIMD (128
-// registers, a few values spilled), with more species at 2.  The branch
-// kernel with the TAB stage needs far more than 256: at 2 waves per SIMD it
-// spilled ~600 bytes per lane to scratch memory and waited on it; at 1 wave
-// per SIMD the overflow goes to the accumulation registers (no scratch),
-// measured 18 % faster on the wall/TAB slab.  The branch kernels without TAB
-// (walls, films, splash, KH-RT, EB) are faster at 2 waves (9.9 against
-// 13.4 ms on the same slab without breakup, 5.5 against 7.7 ms with an EB).
-template <class R, int NF, bool TAB = false>
+// Occupancy targets (workgroups of 4 waves per CU = waves per SIMD), with
+// the units compiled without machine-level loop-invariant code motion (which
+// kept hoisted FP64 constants live and spilled scalar registers): the plain
+// FP64 kernel with up to two liquid species runs at 3 waves per SIMD (168
+// registers, about 100 bytes per lane spilled; measured 18 % faster than 2
+// waves without spills), the six-fuel one and the branch kernels at 2, the
+// TAB branch kernel too (193 registers; it needed ~390 with the hoisting).
+// The FP32 form with up to two liquid species runs at 4 waves per SIMD (120
+// registers; at 5 waves it spilled 88 bytes per lane and ran 19 % slower),
+// with more species at 2.
+template <class R, int NF, bool TAB = false, bool BR = false>
 struct UpdOccupancy {
-  static constexpr int kMinBlocks = TAB ? 1 : ((sizeof(R) == 8) ? 2 : (NF <= 2 ? 4 : 2));
+  static constexpr int kMinBlocks =
+      TAB ? 2 : ((sizeof(R) == 8) ? ((!BR && NF <= 2) ? 3 : 2) : (NF <= 2 ? 4 : 2));
 };
 
 // Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
@@ -1299,7 +1299,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
 // leaves the run-time selects out.
 template <class M, int NF, class R, bool POS64, bool AT_BOUNDS, bool BR = false, bool EB = false, bool TAB = false,
           bool DEP = true>
-__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB>::kMinBlocks))
+__global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB, BR>::kMinBlocks))
     spray_update_kernel(UpdateArgs<R, NF> a) {
   constexpr int NG = gas_stride<R, M::NS>();  // staged gas fields: rho, Y[NS], vel[3], T (padded)
   constexpr int NSRC = 5 + NF;
