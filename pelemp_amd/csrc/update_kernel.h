@@ -536,14 +536,52 @@ __device__ __attribute__((always_inline)) inline void drop_gather(const UpdateAr
     }
   }
   if (!eb_weights) {
+    if constexpr (!AT_BOUNDS) {
+      // the eight corners ijk-1 .. ijk of a droplet away from the walls lie
+      // in the staged halo: one bounds check for the block, the halo cell
+      // of corner (-1,-1,-1) and constant offsets for the others; the
+      // per-corner state-box flags are read only when the staging found a
+      // halo cell outside the state box (gvalid[kHaloCells] == 0)
+      const int hx = ijk[0] - otx, hy = ijk[1] - oty, hz = ijk[2] - otz;
+      const bool inr = hx >= 0 && hx < kHaloW - 1 && hy >= 0 && hy < kHaloW - 1 && hz >= 0 && hz < kHaloW - 1;
+      if (!inr) set_err(a.err, PELEMP_E_STATE_BOX);
+      const int h0 = inr ? (hz * kHaloW + hy) * kHaloW + hx : 0;
+      const bool chk = gvalid[kHaloCells] == 0;
+      const R* gb = gas + h0 * gas_stride<R, NS>();
 #pragma unroll
-    for (int kk = -1; kk < 1; ++kk) {
+      for (int kk = 0; kk < 2; ++kk) {
 #pragma unroll
-      for (int jj = -1; jj < 1; ++jj) {
+        for (int jj = 0; jj < 2; ++jj) {
 #pragma unroll
-        for (int ii = -1; ii < 1; ++ii) {
-          const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
-          if (cw > R(0.)) add_cell(ijk[0] + ii, ijk[1] + jj, ijk[2] + kk, cw);
+          for (int ii = 0; ii < 2; ++ii) {
+            const R cw = ssv[ii][0] * ssv[jj][1] * ssv[kk][2];
+            if (cw > R(0.)) {
+              constexpr int kOff[2][2][2] = {{{0, 1}, {kHaloW, kHaloW + 1}},
+                                             {{kHaloW * kHaloW, kHaloW * kHaloW + 1},
+                                              {kHaloW * kHaloW + kHaloW, kHaloW * kHaloW + kHaloW + 1}}};
+              const int off = kOff[kk][jj][ii];
+              if (chk && !gvalid[h0 + off]) set_err(a.err, PELEMP_E_STATE_BOX);
+              const R* gc = gb + off * gas_stride<R, NS>();
+              g.rho += cw * gc[0];
+#pragma unroll
+              for (int n = 0; n < NS; ++n) g.Y[n] += cw * gc[1 + n];
+#pragma unroll
+              for (int q = 0; q < 3; ++q) g.vel[q] += cw * gc[1 + NS + q];
+              g.T += cw * gc[4 + NS];
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int kk = -1; kk < 1; ++kk) {
+#pragma unroll
+        for (int jj = -1; jj < 1; ++jj) {
+#pragma unroll
+          for (int ii = -1; ii < 1; ++ii) {
+            const R cw = ssv[ii + 1][0] * ssv[jj + 1][1] * ssv[kk + 1][2];
+            if (cw > R(0.)) add_cell(ijk[0] + ii, ijk[1] + jj, ijk[2] + kk, cw);
+          }
         }
       }
     }
@@ -1116,6 +1154,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
   const int otx = (tile % a.ntile[0]) * kTile;
 
   if (t <= kTileCells) cs[t] = a.cstart[tile * kTileCells + t];
+  if (t == 0) gvalid[kHaloCells] = 1;  // cleared below if a halo cell is outside the state box
   __syncthreads();
   const int32_t d0 = cs[0] + seg * kSegDroplets;
   const int32_t d1 = (cs[kTileCells] - d0 < kSegDroplets) ? cs[kTileCells] : d0 + kSegDroplets;
@@ -1152,6 +1191,7 @@ __device__ __attribute__((always_inline)) inline void process_tile(const UpdateA
     const bool ok = lx >= 0 && lx < a.st.n[0] && ly >= 0 && ly < a.st.n[1] && lz >= 0 && lz < a.st.n[2] &&
                     in_grown_box(otx + hx - 1, oty + hy - 1, otz + hz - 1, a.ncell, a.st_g);
     gvalid[h] = ok ? 1 : 0;
+    if (!ok) gvalid[kHaloCells] = 0;
     if constexpr (sizeof(R) == 8) {
       if (ok && a.pelec) {
         // PeleC flavour: the cell in SI, converted before the update
@@ -1304,7 +1344,7 @@ __global__ void __launch_bounds__(kUpdThreads, (UpdOccupancy<R, NF, TAB, BR>::kM
   constexpr int NG = gas_stride<R, M::NS>();  // staged gas fields: rho, Y[NS], vel[3], T (padded)
   constexpr int NSRC = 5 + NF;
   __shared__ __attribute__((aligned(16))) R gas[NG * kHaloCells];
-  __shared__ unsigned char gvalid[kHaloCells];
+  __shared__ unsigned char gvalid[kHaloCells + 1];  // + the all-valid flag
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ ContribT<R, BR> contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
@@ -1381,7 +1421,7 @@ __global__ void __launch_bounds__(kUpdThreads, 2) spray_update_surr_kernel(Updat
   constexpr int NG = gas_stride<float, M::NS>();
   constexpr int NSRC = 5 + NF;
   __shared__ __attribute__((aligned(16))) float gas[NG * kHaloCells];
-  __shared__ unsigned char gvalid[kHaloCells];
+  __shared__ unsigned char gvalid[kHaloCells + 1];  // + the all-valid flag
   __shared__ int32_t cs[kTileCells + 1];
   __shared__ float contrib[kUpdThreads * NSRC];
   __shared__ unsigned char stayed[kUpdThreads];
