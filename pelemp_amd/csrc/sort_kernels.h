@@ -136,10 +136,7 @@ constexpr int32_t kArrRankMask = (1 << 30) - 1;
 // gets ceil(n / kSegDroplets) consecutive segment slots and its segments
 // 1.. as work items (the order of the lists does not matter: every entry
 // names its tile and segment).
-__global__ void __launch_bounds__(256) split_tiles_kernel(const int32_t* __restrict__ cstart, int32_t ntiles,
-                                                           SplitDev sp) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
+__device__ inline void split_tile(const int32_t* __restrict__ cstart, int32_t t, const SplitDev& sp) {
   const int32_t n = cstart[(t + 1) * 64] - cstart[t * 64];
   const int32_t nseg = (n + kSegDroplets - 1) / kSegDroplets;
   if (nseg > 1) {
@@ -154,6 +151,12 @@ __global__ void __launch_bounds__(256) split_tiles_kernel(const int32_t* __restr
   }
 }
 
+__global__ void __launch_bounds__(256) split_tiles_kernel(const int32_t* __restrict__ cstart, int32_t ntiles,
+                                                           SplitDev sp) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) split_tile(cstart, t, sp);
+}
+
 __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __restrict__ key,
                                                             const int32_t* __restrict__ n_ptr,
                                                             const int32_t* __restrict__ arrival,
@@ -166,7 +169,7 @@ __global__ void __launch_bounds__(256) scatter_incr_kernel(const uint32_t* __res
   if (blockIdx.x == 0 && threadIdx.x < 3) sp.counters[threadIdx.x] = 0;
   // kU entries per thread and round, their loads issued together: each entry
   // is a chain of dependent loads (key and arrival, then the cell's offsets)
-  constexpr int kU = 4;
+  constexpr int kU = 8;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t s0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s0 < n; s0 += kU * stride) {
     uint32_t k[kU];
@@ -381,8 +384,12 @@ __global__ void __launch_bounds__(256) order_movers_kernel(const int32_t* __rest
                                                             int32_t* __restrict__ stay_cnt,
                                                             int32_t* __restrict__ mov_hi, int32_t ncells,
                                                             int32_t* __restrict__ perm, int2* __restrict__ long_list,
-                                                            int32_t* __restrict__ n_long) {
+                                                            int32_t* __restrict__ n_long, int32_t ntiles,
+                                                            SplitDev sp) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  // the next pass's heavy-tile split (split_tiles_kernel's work, in this
+  // launch: cstart is final and scatter_incr has read the old lists)
+  if (c < ntiles) split_tile(cstart, c, sp);
   int32_t lo = 0, hi = 0, a = 0, h0 = 0;
   if (c < ncells) {
     lo = mov_lo[c];

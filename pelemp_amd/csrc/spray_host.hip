@@ -522,11 +522,10 @@ static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src, int
   hipLaunchKernelGGL(scatter_incr_kernel, dim3(2048), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.mov_lo, c.stay_cnt, c.perm, c.sp);
   hipLaunchKernelGGL(order_movers_kernel, dim3(gc), dim3(256), 0, s, c.cstart, c.mov_lo, c.stay_cnt, c.mov_hi,
-                     c.ncells, c.perm, c.long_list, c.n_long);
+                     c.ncells, c.perm, c.long_list, c.n_long, c.ntiles, c.sp);
   c.counters_dirty = false;
   hipLaunchKernelGGL(sort_long_groups_kernel, dim3(kLongSortGrid), dim3(kLongSortThreads), 0, s, c.long_list,
                      c.n_long, c.perm, c.arrival);
-  hipLaunchKernelGGL(split_tiles_kernel, dim3((c.ntiles + 255) / 256), dim3(256), 0, s, c.cstart, c.ntiles, c.sp);
   PM_HIP(hipGetLastError());
   ctx_time_end(c, 1, s, ev0);
   return PELEMP_OK;
