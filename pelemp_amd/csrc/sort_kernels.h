@@ -31,7 +31,8 @@
 
 namespace pelemp {
 
-constexpr int kScanBlock = 1024;
+constexpr int kScanBlock = 1024;    // cells per block of the cell-count scan
+constexpr int kScanThreads = 256;   // its threads: four cells each
 
 __global__ void __launch_bounds__(256) count_keys_kernel(const uint32_t* __restrict__ key,
                                                           const int32_t* __restrict__ n_ptr,
@@ -50,57 +51,77 @@ __global__ void __launch_bounds__(256) count_keys_kernel(const uint32_t* __restr
 // With n_entries given, thread 0 also sets the entry count of the new buffer
 // (n_src, or the previous sort's total out[n]) and logs the previous total
 // (the droplets at the pass start) into log_slot when that is given.
-__global__ void __launch_bounds__(kScanBlock) scan_block_kernel(const int32_t* __restrict__ c0,
-                                                                 const int32_t* __restrict__ c1,
-                                                                 const int32_t* __restrict__ c2, int32_t n,
-                                                                 int32_t* __restrict__ out,
-                                                                 int32_t* __restrict__ block_sum,
-                                                                 const int32_t* __restrict__ n_src,
-                                                                 int32_t* __restrict__ n_entries,
-                                                                 int32_t* __restrict__ log_slot) {
-  __shared__ int32_t sh[kScanBlock];
-  const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-  const int32_t v = (i < n) ? (c1 ? c0[i] + c1[i] + c2[i] : c0[i]) : 0;
-  if (i == 0 && n_entries) {
+__global__ void __launch_bounds__(kScanThreads) scan_block_kernel(const int32_t* __restrict__ c0,
+                                                                   const int32_t* __restrict__ c1,
+                                                                   const int32_t* __restrict__ c2, int32_t n,
+                                                                   int32_t* __restrict__ out,
+                                                                   int32_t* __restrict__ block_sum,
+                                                                   const int32_t* __restrict__ n_src,
+                                                                   int32_t* __restrict__ n_entries,
+                                                                   int32_t* __restrict__ log_slot) {
+  static_assert(kScanBlock == 4 * kScanThreads, "four cells per thread");
+  __shared__ int32_t wsum[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int32_t i0 = blockIdx.x * kScanBlock + 4 * threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n_entries) {
     const int32_t pt = out[n];
     *n_entries = n_src ? *n_src : pt;
     if (log_slot) *log_slot = pt;
   }
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    const int32_t t = (threadIdx.x >= static_cast<unsigned>(off)) ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
+  int32_t v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int32_t i = i0 + q;
+    v[q] = (i < n) ? (c1 ? c0[i] + c1[i] + c2[i] : c0[i]) : 0;
   }
-  if (i < n) out[i] = sh[threadIdx.x] - v;  // exclusive
-  if (threadIdx.x == kScanBlock - 1) block_sum[blockIdx.x] = sh[threadIdx.x];
+  const int32_t tsum = v[0] + v[1] + v[2] + v[3];
+  // inclusive scan of the thread sums over the wavefront, then over the
+  // block's wavefronts
+  int32_t x = tsum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  int32_t wbase = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) wbase += (w < wave) ? wsum[w] : 0;
+  int32_t r = wbase + x - tsum;  // exclusive prefix of this thread's first cell
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (i0 + q < n) out[i0 + q] = r;
+    r += v[q];
+  }
+  if (threadIdx.x == kScanThreads - 1) block_sum[blockIdx.x] = r;
 }
 
-// Phase 2: each block adds the sum of the totals of the blocks before it
-// (integer sums: exact in any order); the last block writes out[n] = total.
-// Also clears the count of long segments that the order step lists.
-__global__ void __launch_bounds__(kScanBlock) scan_add_kernel(int32_t* __restrict__ out, int32_t n,
-                                                               const int32_t* __restrict__ block_sum,
-                                                               int32_t nblocks, int32_t* __restrict__ total,
-                                                               int32_t* __restrict__ n_long) {
-  __shared__ int32_t red[kScanBlock / 64];
+// Phase 2: each block adds the sum of the preceding blocks' totals; the last
+// block writes the grand total to out[n] and *total.  Also clears n_long.
+__global__ void __launch_bounds__(kScanThreads) scan_add_kernel(int32_t* __restrict__ out, int32_t n,
+                                                                 const int32_t* __restrict__ block_sum,
+                                                                 int32_t nblocks, int32_t* __restrict__ total,
+                                                                 int32_t* __restrict__ n_long) {
+  __shared__ int32_t red[kScanThreads / 64];
   int32_t part = 0;
-  for (int32_t b = threadIdx.x; b < static_cast<int32_t>(blockIdx.x); b += kScanBlock) part += block_sum[b];
+  for (int32_t b = threadIdx.x; b < static_cast<int32_t>(blockIdx.x); b += kScanThreads) part += block_sum[b];
   for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
   int32_t base = 0;
-  for (int w = 0; w < kScanBlock / 64; ++w) base += red[w];
-  const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-  if (i < n) out[i] += base;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) base += red[w];
+  const int32_t i0 = blockIdx.x * kScanBlock + 4 * threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (i0 + q < n) out[i0 + q] += base;
   if (threadIdx.x == 0 && static_cast<int32_t>(blockIdx.x) == nblocks - 1) {
     const int32_t tot = base + block_sum[nblocks - 1];
     out[n] = tot;
     *total = tot;
   }
-  if (i == 0) *n_long = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *n_long = 0;
 }
 
 __global__ void __launch_bounds__(256) scatter_perm_kernel(const uint32_t* __restrict__ key,

@@ -368,9 +368,9 @@ static int run_sort(Ctx& c, hipStream_t s, bool keep_entries = false) {
   const int grid_s = 2048;
   hipLaunchKernelGGL(count_keys_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.cnt, c.arrival);
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
-  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cnt, nullptr, nullptr, c.ncells,
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanThreads), 0, s, c.cnt, nullptr, nullptr, c.ncells,
                      c.cstart, c.block_sum, nullptr, nullptr, nullptr);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanThreads), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
                      c.n_long);
   hipLaunchKernelGGL(scatter_perm_kernel, dim3(grid_s), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.perm, c.sp.counters);
@@ -515,9 +515,9 @@ static int run_sort_incremental(Ctx& c, hipStream_t s, const int32_t* n_src, int
   ctx_time_begin(c, 1, s, &ev0);
   const unsigned gc = static_cast<unsigned>((c.ncells + 255) / 256);
   const int nb = (c.ncells + kScanBlock - 1) / kScanBlock;
-  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.mov_lo, c.stay_cnt, c.mov_hi, c.ncells,
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(kScanThreads), 0, s, c.mov_lo, c.stay_cnt, c.mov_hi, c.ncells,
                      c.cstart, c.block_sum, n_src, c.n_entries, log_slot);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanBlock), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanThreads), 0, s, c.cstart, c.ncells, c.block_sum, nb, c.total,
                      c.n_long);
   hipLaunchKernelGGL(scatter_incr_kernel, dim3(2048), dim3(256), 0, s, c.key, c.n_entries, c.arrival, c.cstart,
                      c.mov_lo, c.stay_cnt, c.perm, c.sp);
