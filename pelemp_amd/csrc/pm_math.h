@@ -26,9 +26,11 @@
 //    log1p(x) = x                         if 1+x == 1
 //             = log(1+x) * x / ((1+x)-1)  otherwise.
 //  * pow(x,y), x > 0: exp(y log x).
-//  * cbrt: x = 2^(3q) m, m in [1/2, 4), quartic least-squares starting
-//    value (max rel. error 5.3e-3) and three Newton steps
-//    y <- y - (y^3 - m)/(3 y^2).
+//  * cbrt: x = 2^(3q) m, m in [1/2, 4); u ~ m^(-1/3) from a degree-5
+//    weighted least-squares fit (max rel. error 3.0e-3) and three
+//    division-free Newton steps u <- u + u (1 - m u^3)/3; then y = m u^2
+//    and one Newton step on y, y <- y + (m - y^3) u^2/3, with m - y^3
+//    formed in double-double (the fma error of y^2).
 //  * sin/cos: Cody-Waite reduction by pi/2 with a three-part pi/2 (valid for
 //    |x| < 2^20 pi/2), Taylor polynomials on |r| <= pi/4.
 //  * atan: reduction to |t| <= tan(pi/12) with atan(x) = pi/6 + atan((x -
@@ -154,15 +156,30 @@ PM_HD double cbrt(double x) {
   int q = (e >= 0) ? (e / 3) : -((-e + 2) / 3);
   int r = e - 3 * q;
   m = std::ldexp(m, r);  // m in [1/2, 4)
-  double y = -0x1.20cf54b21a4a4p-8;
-  y = std::fma(y, m, 0x1.9e5c681c62e4fp-5);
-  y = std::fma(y, m, -0x1.dcc518a871d44p-3);
-  y = std::fma(y, m, 0x1.57bb44d6f328ep-1);
-  y = std::fma(y, m, 0x1.0760836b1f0bdp-1);
+  // u ~ m^(-1/3): a degree-5 fit on [1/2, 4) (relative error < 2^-8.3),
+  // then three Newton steps u <- u + u (1 - m u^3) / 3, each squaring the
+  // relative error, with no division
+  constexpr double kThird = 0x1.5555555555555p-2;
+  double u = -0x1.7004e55df38c6p-8;
+  u = std::fma(u, m, 0x1.30ee0814c4b64p-4);
+  u = std::fma(u, m, -0x1.8e381fe895553p-2);
+  u = std::fma(u, m, 0x1.06cf19aa0aa15p+0);
+  u = std::fma(u, m, -0x1.80bf35f68a8d4p+0);
+  u = std::fma(u, m, 0x1.cb88b5e0994ebp+0);
   for (int it = 0; it < 3; ++it) {
-    const double y2 = y * y;
-    y = y - std::fma(y2, y, -m) / (3.0 * y2);
+    const double u2 = u * u;
+    const double res = std::fma(-(m * u), u2, 1.0);  // 1 - m u^3
+    u = std::fma(u * res, kThird, u);
   }
+  // y = m u^2 ~ m^(1/3), then one Newton step on y with the residual
+  // m - y^3 formed to twice the working precision (the exact error of y^2
+  // from an fma) and 1 / (3 y^2) ~ u^2 / 3
+  const double uu = u * u;
+  double y = m * uu;
+  const double y2 = y * y;
+  const double y2e = std::fma(y, y, -y2);
+  const double d = std::fma(-y2e, y, std::fma(-y2, y, m));
+  y = std::fma(d * uu, kThird, y);
   y = std::ldexp(y, q);
   return (x < 0.0) ? -y : y;
 }
