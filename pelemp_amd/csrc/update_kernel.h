@@ -271,13 +271,14 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // registers, about 100 bytes per lane spilled; measured 18 % faster than 2
 // waves without spills), the six-fuel one and the branch kernels at 2, the
 // TAB branch kernel too (193 registers; it needed ~390 with the hoisting).
-// The FP32 form with up to two liquid species runs at 4 waves per SIMD (120
-// registers; at 5 waves it spilled 88 bytes per lane and ran 19 % slower),
-// with more species at 2.
+// The FP32 form with one liquid species runs at 5 waves per SIMD (96
+// registers, 8 bytes per lane spilled; 10 % faster than 4 waves), with two
+// at 4 (120 registers; at 5 waves it spilled 88 bytes per lane and ran 19 %
+// slower), with more species at 2.
 template <class R, int NF, bool TAB = false, bool BR = false>
 struct UpdOccupancy {
   static constexpr int kMinBlocks =
-      TAB ? 2 : ((sizeof(R) == 8) ? ((!BR && NF <= 2) ? 3 : 2) : (NF <= 2 ? 4 : 2));
+      TAB ? 2 : ((sizeof(R) == 8) ? ((!BR && NF <= 2) ? 3 : 2) : (NF == 1 ? 5 : (NF <= 2 ? 4 : 2)));
 };
 
 // Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
