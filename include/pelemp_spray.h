@@ -523,7 +523,7 @@ int pelemp_spray_device_bytes(const pelemp_spray_ctx* ctx, int64_t* bytes);
  * No counterpart in the reference: the surrogate replaces the film-property
  * / Spalding-number evaluation inside the substep of calculateSpraySource
  * (Drag.H:213-347) for PELEMP_FORM_SURROGATE.  Network: 8 inputs -> 4 hidden
- * layers of 64 (SiLU) -> 3 outputs; FP32 master parameters, 13251 floats,
+ * layers of 64 (hard swish) -> 3 outputs; FP32 master parameters, 13251 floats,
  * per linear layer L = 0..4 the weights W_L[out][in] (row-major) followed by
  * the biases b_L[out]; fan-in 8,64,64,64,64, fan-out 64,64,64,64,3.
  *

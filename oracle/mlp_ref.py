@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY: used by tests/ to check the GPU forward pass, the
 GPU gradients (pelemp_mlp_loss_grad) and the bfloat16 MFMA inference.  The
 reference has no learned component; this restates the network defined in
-include/pelemp_spray.h (8 -> 4 x 64 SiLU -> 3, weights W_L[out][in] then
+include/pelemp_spray.h (8 -> 4 x 64 hard swish -> 3, weights W_L[out][in] then
 b_L per layer) and the loss
     L = (1/n) sum_i w_i |y_i - (t_i - out_mean) / out_std|^2
 with plain reverse-mode differentiation.  With quant=True the weights and
@@ -50,8 +50,13 @@ def normalise_inputs(x: np.ndarray, center, scale) -> np.ndarray:
         np.float64)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def act(z):
+    """Hard swish z clamp(z/6 + 1/2, 0, 1), the hidden layers' activation."""
+    return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+
+
+def act_grad(z):
+    return np.where(z <= -3.0, 0.0, np.where(z >= 3.0, 1.0, z / 3.0 + 0.5))
 
 
 def forward(params, xn: np.ndarray, quant: bool = False):
@@ -64,7 +69,7 @@ def forward(params, xn: np.ndarray, quant: bool = False):
         z = h @ Wq.T + b
         cache.append((h, Wq, z))
         if L < len(layers) - 1:
-            a = silu(z)
+            a = act(z)
             h = bf16_round(a) if quant else a
         else:
             h = z
@@ -84,8 +89,7 @@ def loss_grad(params, xn, t, w, out_mean, out_std, quant: bool = False):
     for L in range(len(cache) - 1, -1, -1):
         h, Wq, z = cache[L]
         if L < len(cache) - 1:
-            s = 1.0 / (1.0 + np.exp(-z))
-            delta = delta * (s * (1.0 + z * (1.0 - s)))
+            delta = delta * act_grad(z)
         grads.append((delta.T @ h, delta.sum(axis=0)))
         delta = delta @ Wq
     grads.reverse()

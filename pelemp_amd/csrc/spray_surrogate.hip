@@ -168,7 +168,8 @@ __device__ inline void stage_params(float* P, const float* __restrict__ src, boo
 }
 
 // Forward pass of one sample; if ws_h / ws_d are given, stores each linear
-// layer's input (ws_h) and the SiLU derivative at each hidden layer (ws_d).
+// layer's input (ws_h) and the activation's derivative at each hidden layer
+// (ws_d).
 __device__ inline void mlp_forward_one(const float* P, const float* xn, bool quant, float* y, float* ws_h0,
                                        float* ws_h, float* ws_d, long long j, int nb) {
   float hin[kSurrHid];
@@ -192,9 +193,11 @@ __device__ inline void mlp_forward_one(const float* P, const float* xn, bool qua
 #pragma unroll
         for (int i = 0; i < kSurrHid; ++i) z += W[o * kSurrHid + i] * hin[i];
       }
-      const float sg = 1.f / (1.f + expf(-z));
-      const float a = z * sg;
-      if (ws_d) ws_d[(static_cast<long long>(L) * nb + j) * kSurrHid + o] = sg * (1.f + z * (1.f - sg));
+      // hard swish z clamp(z/6 + 1/2, 0, 1), derivative 0 / z/3 + 1/2 / 1
+      const float a = surr_act(z);
+      if (ws_d)
+        ws_d[(static_cast<long long>(L) * nb + j) * kSurrHid + o] =
+            (z <= -3.f) ? 0.f : ((z >= 3.f) ? 1.f : z * (1.f / 3.f) + 0.5f);
       hout[o] = qbf(a, quant);
     }
 #pragma unroll
@@ -244,7 +247,7 @@ __device__ inline long long batch_row(const BatchMap& m, int j) {
 struct Workspace {
   float* h0;      // [nb][8]    input of layer 0
   float* h;       // [4][nb][64] input of layers 1..4
-  float* d;       // [4][nb][64] SiLU' then delta of hidden layers 0..3
+  float* d;       // [4][nb][64] act' then delta of hidden layers 0..3
   float* dout;    // [nb][3]    delta of the output layer
   double* loss;   // [nb]
   double* part;   // [nchunk][kSurrParams] partial gradient sums
@@ -276,7 +279,7 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_fwdbwd_kernel(const float* __
     ws.dout[static_cast<long long>(j) * kSurrOut + q] = dl[q];
   }
   ws.loss[j] = l / nb;
-  // back-propagation: g = W^T delta, delta_L = g * SiLU'(z_L)
+  // back-propagation: g = W^T delta, delta_L = g * act'(z_L)
   float g[kSurrHid];
   {
     const float* W = P + surr_w_off(4);

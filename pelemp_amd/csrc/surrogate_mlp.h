@@ -133,11 +133,15 @@ __device__ __attribute__((always_inline)) inline f32x16_t surr_bias_tile(const f
   return c;
 }
 
+// The hidden layers' activation, hard swish: z clamp(z/6 + 1/2, 0, 1) (a
+// multiply-add, a median of three and a multiply; no transcendental).  The
+// same function (one definition) in training and inference.
 __device__ __attribute__((always_inline)) inline float surr_act(float z) {
-  return z * __builtin_amdgcn_rcpf(1.f + __expf(-z));
+  const float g = __fmaf_rn(z, 1.f / 6.f, 0.5f);
+  return z * __builtin_amdgcn_fmed3f(g, 0.f, 1.f);
 }
 
-// SiLU of the two output tiles, packed as the 4 K-step B operands
+// Activation of the two output tiles, packed as the 4 K-step B operands
 __device__ __attribute__((always_inline)) inline void surr_act_pack(const f32x16_t* acc, bf16x8_t* B) {
 #pragma unroll
   for (int m = 0; m < 2; ++m)

@@ -11,7 +11,7 @@ Everything runs through the C ABI (include/pelemp_spray.h):
     python -m pelemp_amd.surrogate --mech 1 --samples 10000000 --epochs 24 \\
         --out pelemp_amd/surrogate_data/decane_dodecane_air.npz
 
-The network (8 -> 4 x 64 SiLU -> 3) predicts the film diffusivity
+The network (8 -> 4 x 64 hard swish -> 3) predicts the film diffusivity
 ln(sum rho D_k), the heat-transfer group ln(lambda Nu*) and the film
 viscosity ln(mu) from ln d, T_d, ln(|u_rel| + 0.01), T_gas, ln p, the film
 fuel mass fraction and two liquid mass fractions; dm/dt, dT_d/dt and the

@@ -38,7 +38,7 @@ def test_gradient_matches_central_differences():
 
 
 def test_gradient_of_a_planted_error_fails():
-    # a dropped SiLU' factor in the backward pass is caught by the check above
+    # a dropped activation-derivative factor in the backward pass is caught by the check above
     p, xn, t, w = _problem(seed=5)
     mean, sd = np.zeros(3), np.ones(3)
     _, g = R.loss_grad(p, xn, t, w, mean, sd)
@@ -46,7 +46,7 @@ def test_gradient_of_a_planted_error_fails():
     y, cache = R.forward(p, xn)
     delta = 2.0 * w[:, None] * (y - t) / xn.shape[0]
     h, Wq, z = cache[3]
-    wrong = (delta @ cache[4][1]).T @ h  # no SiLU'(z3)
+    wrong = (delta @ cache[4][1]).T @ h  # no act'(z3)
     k0 = sum(i * o + o for i, o in zip(R.FAN_IN[:3], R.FAN_OUT[:3]))
     right = g[k0:k0 + 64 * 64].reshape(64, 64)
     assert np.max(np.abs(wrong - right)) > 1e-3 * np.max(np.abs(right))
