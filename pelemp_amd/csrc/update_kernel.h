@@ -274,11 +274,12 @@ __device__ __attribute__((always_inline)) inline bool dev_check_bounds(const dou
 // The FP32 form with one liquid species runs at 5 waves per SIMD (96
 // registers, 8 bytes per lane spilled; 10 % faster than 4 waves), with two
 // at 4 (120 registers; at 5 waves it spilled 88 bytes per lane and ran 19 %
-// slower), with more species at 2.
+// slower), with six at 3 (168 registers, about 190 bytes per lane spilled;
+// 15 % faster than 2 waves without spills).
 template <class R, int NF, bool TAB = false, bool BR = false>
 struct UpdOccupancy {
   static constexpr int kMinBlocks =
-      TAB ? 2 : ((sizeof(R) == 8) ? ((!BR && NF <= 2) ? 3 : 2) : (NF == 1 ? 5 : (NF <= 2 ? 4 : 2)));
+      TAB ? 2 : ((sizeof(R) == 8) ? ((!BR && NF <= 2) ? 3 : 2) : (NF == 1 ? 5 : (NF <= 2 ? 4 : 3)));
 };
 
 // Word of the source FAB for component q (mom 3, mass, Y nf, energy) of cell
